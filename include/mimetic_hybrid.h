@@ -1,0 +1,196 @@
+/*
+ * mimetic_hybrid.h -- C-ABI of the MI355X mimetic hybrid face-pressure solver.
+ *
+ * This is the boundary that replaces the inside of opm-porsol's
+ *   template <class GI, class RI, class BCI, template<class,class> class IP>
+ *   class Opm::IncompFlowSolverHybrid
+ * (opm/porsol/mimetic/IncompFlowSolverHybrid.hpp:362-1902) together with its
+ * inner-product plug-in Opm::MimeticIPEvaluator
+ * (opm/porsol/mimetic/MimeticIPEvaluator.hpp:84-555) and the dune-istl linear
+ * solvers it calls.  A header-only C++ adapter with the reference's template
+ * signature sits on top (opm-porsol_amd/host/opm/porsol/mimetic/
+ * IncompFlowSolverHybrid.hpp); this header is what it, or any FFI, binds.
+ *
+ * Conventions
+ *  - Every function returns an int status (MHS_OK == 0) and never throws.
+ *    On failure mhs_last_error() describes the cause.  The reference throws
+ *    std::runtime_error for non-convergence (IncompFlowSolverHybrid.hpp:
+ *    1416-1419, 1536-1539) and for an unknown linsolver_type (:698-700); the
+ *    adapter turns MHS_ERR_NOT_CONVERGED / MHS_ERR_INVALID back into those.
+ *  - Host pointers are read during the call only (copied in); the solver
+ *    owns all device memory.  Entry points named *_device take device
+ *    pointers that must stay valid until the call returns.
+ *  - One handle per host thread.  Work is issued on the stream given to
+ *    mhs_create (NULL: the solver creates its own stream at default
+ *    priority).
+ *  - Arrays are indexed like the reference: cells in iteration order, half-
+ *    faces cell by cell in local-face order (FlowSolution::outflux(hf),
+ *    IncompFlowSolverHybrid.hpp:430-433), face degrees of freedom in the
+ *    numbering of enumerateGridDof (:956-1067).
+ *  - Batches: a grid may be the disjoint union of several independent
+ *    instances (inst_cellpos).  Each instance is solved as if by its own
+ *    IncompFlowSolverHybrid object: its own dof numbering, its own
+ *    regularisation, its own convergence test.  Face pressures of instance t
+ *    occupy a contiguous range of length nf_t, instances in order.
+ *  - nrhs right-hand sides share one matrix.  Per-rhs arrays are laid out
+ *    rhs-major: src[k*nc + c], bc_value[k*nhf + h], pressure[k*nc + c], ...
+ */
+#ifndef MIMETIC_HYBRID_H
+#define MIMETIC_HYBRID_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mhs_solver mhs_solver;
+
+enum mhs_status {
+    MHS_OK = 0,
+    MHS_ERR_INVALID = 1,        /* bad argument, unknown linsolver_type (:698-700) */
+    MHS_ERR_STATE = 2,          /* call-order precondition (the reference asserts, :538, :569) */
+    MHS_ERR_NOT_CONVERGED = 3,  /* linear solver did not converge (:1416-1419) */
+    MHS_ERR_HIP = 4,            /* HIP runtime error */
+    MHS_ERR_ALLOC = 5,          /* device or host allocation failed */
+    MHS_ERR_BREAKDOWN = 6       /* CG breakdown: operator not positive definite */
+};
+
+/* FlowBC types (opm/porsol/common/BoundaryConditions.hpp:57, 121-171).
+ * The default FlowBC is Neumann with value 0 (:125-128). */
+enum mhs_bc_type { MHS_BC_NEUMANN = 0, MHS_BC_DIRICHLET = 1, MHS_BC_PERIODIC = 2 };
+
+enum mhs_precision {
+    MHS_FP64 = 0,   /* FP64 matrix, FP64 PCG */
+    MHS_MIXED = 1   /* FP32 matrix and PCG vectors, FP64 residual replacement */
+};
+
+typedef struct {
+    double tol;                /* residual_tolerance (:662): ||r||/||r0|| bound */
+    int maxit;                 /* linsolver_maxit (:666); 0 -> number of unknowns */
+    int linsolver_type;        /* (:664) 0: Jacobi-PCG (stands in for ILU0-CG);
+                                  1..3: the AMG family, served by the same PCG */
+    int verbosity;             /* linsolver_verbosity (:663) */
+    int same_matrix;           /* (:665) accepted; the matrix is always rebuilt */
+    double prolongate_factor;  /* (:667) accepted for signature parity */
+    int smooth_steps;          /* (:668) accepted for signature parity */
+    int precision;             /* enum mhs_precision */
+    int nrhs;                  /* right-hand sides sharing the matrix, 1..4 */
+    int chunk;                 /* PCG iterations per captured graph launch */
+    double replace_factor;     /* mixed: replace the FP32 recursive residual by
+                                  the FP64 true residual once it has dropped by
+                                  this factor since the last replacement */
+    int use_graph;             /* capture PCG chunks in a hipGraph */
+} mhs_solve_options;
+
+typedef struct {
+    int iterations;            /* PCG iterations, maximum over systems */
+    int replacements;          /* FP64 residual replacements (mixed) */
+    int converged;             /* 1 when every system reached tol */
+    double reduction;          /* max over systems of the FP64 ||b-Ax||/||r0|| */
+    double assemble_ms, solve_ms, recover_ms;
+} mhs_solve_report;
+
+typedef struct {
+    int num_cells, num_half_faces, num_faces, num_internal_faces, max_ncf;
+    int num_instances, num_rows_padded;
+    long long nnz;             /* stored entries of the face matrix (incl. padding) */
+} mhs_dims;
+
+void mhs_default_options(mhs_solve_options *opt);
+
+/* Solver object (IncompFlowSolverHybrid default constructor). */
+int mhs_create(mhs_solver **out, int device, void *stream);
+int mhs_destroy(mhs_solver *s);
+const char *mhs_last_error(const mhs_solver *s);
+
+/* IncompFlowSolverHybrid::clear (:498-516): forget grid, structure, IPs. */
+int mhs_clear(mhs_solver *s);
+
+/* Grid through the GridInterfaceEuler accessors (GridInterfaceEuler.hpp:
+ * 126-212, 335-360).  facepos[num_cells+1]; neighbour[nhf] is the neighbour
+ * cell (-1 on the boundary: Face::boundary()); geometry per cell (volume,
+ * centroid xyz) and per half-face (area, centroid xyz, unit outer normal xyz).
+ * inst_cellpos[num_instances+1] splits the cells into independent instances
+ * (NULL with num_instances == 1 means one instance). */
+int mhs_set_grid(mhs_solver *s, int num_instances, const int *inst_cellpos,
+                 int num_cells, const int *facepos, const int *neighbour,
+                 const double *cell_volume, const double *cell_centroid,
+                 const double *face_area, const double *face_centroid,
+                 const double *face_normal);
+
+/* IncompFlowSolverHybrid::initSystemStructure (:534-545): dof enumeration
+ * (:945-1110) and the face-system sparsity (:1115-1319).  Only periodic
+ * information is used, as in the reference: hf_bc_type[nhf] (MHS_BC_*) and
+ * hf_periodic_partner[nhf], the partner half-face of each periodic boundary
+ * half-face (the reference resolves it via getPeriodicPartner(bid),
+ * BoundaryConditions.hpp:289-293).  Either may be NULL (no periodic faces). */
+int mhs_init_system_structure(mhs_solver *s, const int *hf_bc_type,
+                              const int *hf_periodic_partner);
+
+/* IncompFlowSolverHybrid::computeInnerProducts (:564-578) ->
+ * MimeticIPEvaluator::buildStaticContrib (MimeticIPEvaluator.hpp:201-270).
+ * perm[9*num_cells]: row-major 3x3 tensors (ReservoirPropertyCommon_impl.hpp:
+ * 403-411); gravity[3]. */
+int mhs_compute_inner_products(mhs_solver *s, const double *perm,
+                               const double *gravity);
+/* The static inverse inner products after compute_inner_products, in the
+ * layout of MimeticIPEvaluator's Binv_ table (MimeticIPEvaluator.hpp:553):
+ * cell by cell, each ncf x ncf matrix column-major (sum ncf^2 doubles), and
+ * the static gravity fluxes N*K*g per half-face (gflux_, :554).  Either
+ * pointer may be NULL. */
+int mhs_get_inner_products(mhs_solver *s, double *binv, double *gflux);
+/* Alias: the "assembleStatic" step named in commented-out reference calls
+ * (SimulatorBase.hpp:211) is computeInnerProducts. */
+int mhs_assemble_static(mhs_solver *s, const double *perm, const double *gravity);
+
+/* IncompFlowSolverHybrid::solve (:657-703): assembleDynamic (:1324-1382),
+ * the linear solve (:1387-1667) and computePressureAndFluxes (:1672-1730).
+ * The FluidInterface enters through the per-cell values it yields in
+ * MimeticIPEvaluator::computeDynamicParams (MimeticIPEvaluator.hpp:297-311):
+ *   totmob[c] = sum of phase mobilities, rhomob[c] = sum rho_i*lambda_i.
+ * src[nrhs*nc] (m^3/s, >0 injection); hf_bc_type[nhf]; hf_bc_value[nrhs*nhf]
+ * (Dirichlet pressure / Neumann outflux / periodic pressure difference,
+ * read on boundary half-faces only).  report may be NULL. */
+int mhs_solve(mhs_solver *s, const double *totmob, const double *rhomob,
+              const double *src, const int *hf_bc_type, const double *hf_bc_value,
+              const mhs_solve_options *opt, mhs_solve_report *report);
+/* Same with totmob, rhomob, src and hf_bc_value in device memory
+ * (hf_bc_type stays a host array). */
+int mhs_solve_device(mhs_solver *s, const double *totmob, const double *rhomob,
+                     const double *src, const int *hf_bc_type,
+                     const double *hf_bc_value, const mhs_solve_options *opt,
+                     mhs_solve_report *report);
+
+/* IncompFlowSolverHybrid::getSolution (:828-831) -> FlowSolution::pressure
+ * (:411-414) and ::outflux (:426-433), plus the face (contact) pressures the
+ * reference keeps in the private soln_ (:935).  Any pointer may be NULL.
+ * cell_pressure[nrhs*nc], face_pressure[nrhs*nf], outflux[nrhs*nhf]. */
+int mhs_get_solution(mhs_solver *s, double *cell_pressure, double *face_pressure,
+                     double *outflux);
+/* Device pointers to the same data (valid until the next solve/clear/destroy).
+ * face_pressure is in the padded row numbering (see mhs_get_face_rows). */
+int mhs_solution_device(mhs_solver *s, const double **cell_pressure,
+                        const double **face_pressure, const double **outflux);
+
+/* IncompFlowSolverHybrid::postProcessFluxes (:754-807). */
+int mhs_post_process_fluxes(mhs_solver *s, double *max_modification);
+
+/* Sizes (IncompFlowSolverHybrid::printStats :848-871 reports the same). */
+int mhs_get_dims(mhs_solver *s, mhs_dims *dims);
+/* cellFaces_ (the local-to-global face map, :926): hf_face[nhf] in the
+ * unpadded per-instance-contiguous numbering; inst_facepos[num_instances+1]
+ * gives each instance's face range.  Either may be NULL. */
+int mhs_get_cell_faces(mhs_solver *s, int *hf_face, int *inst_facepos);
+/* Padded row of every face (num_faces entries), for device-side access. */
+int mhs_get_face_rows(mhs_solver *s, int *face_row);
+
+/* IncompFlowSolverHybrid::printSystem (:895-905): the assembled face system
+ * of right-hand side k after the last solve, as CSR in the unpadded face
+ * numbering (explicit zeros kept).  Call with col == NULL to get *nnz. */
+int mhs_get_system(mhs_solver *s, int k, int *row_ptr, int *col, double *val,
+                   double *rhs, long long *nnz);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MIMETIC_HYBRID_H */

@@ -1,0 +1,8 @@
+"""MI355X-native mimetic hybrid pressure solver (opm-porsol IncompFlowSolverHybrid).
+
+The directory name is not a Python identifier; load it with ``load_package``
+(in __graft_entry__.py, tests/conftest.py and bench.py), which registers it as
+``opm_porsol_amd``.
+"""
+from . import fields, grids, mhs  # noqa: F401
+from .mhs import HybridSolver, MhsError  # noqa: F401

@@ -1,0 +1,769 @@
+// C-ABI of the MI355X mimetic hybrid solver (include/mimetic_hybrid.h).
+//
+// Host orchestration only: topology on the host once per initSystemStructure,
+// everything per solve on the device.  The solve loop launches chunks of PCG
+// iterations (captured once into a hipGraph) and reads back a few bytes of
+// per-system state between chunks to decide convergence and, in mixed
+// precision, when to replace the FP32 recursive residual by the FP64 true one.
+#include "../../include/mimetic_hybrid.h"
+#include "mhs_kernels.h"
+#include "mhs_structure.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+using namespace mhs;
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct GraphCache {
+    hipGraphExec_t exec = nullptr;
+    int precision = -1, nrhs = 0, chunk = 0;
+    double tol2 = -1.0, rf2 = -1.0;
+    const void *key = nullptr;
+};
+
+}  // namespace
+
+struct mhs_solver {
+    int device = 0;
+    hipStream_t stream = nullptr;       // our own stream (default priority)
+    hipStream_t user_stream = nullptr;  // caller's stream, ordered against ours
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    std::string err;
+
+    // grid (host copies)
+    bool grid_set = false;
+    int ninst = 1, nc = 0, nhf = 0;
+    std::vector<int> inst_cellpos, facepos, nb;
+    std::vector<double> cvol, ccen, farea, fcen, fnrm;
+
+    // structure
+    bool structure_valid = false, ip_valid = false, solved = false;
+    Topology T;
+    DevTopo dT{};
+    DevGeom dG{};
+    std::vector<DevBuf> topo_bufs;
+
+    // static inner products
+    double *binv = nullptr, *gflux = nullptr;
+
+    // per-solve buffers
+    int cap_nrhs = 0;
+    bool cap_mixed = false;
+    std::vector<DevBuf> solve_bufs;
+    DevCellSys C{};
+    double *val64 = nullptr, *b = nullptr, *dinv64 = nullptr, *x64 = nullptr, *r64 = nullptr,
+           *p64 = nullptr, *q64 = nullptr;
+    float *val32 = nullptr, *dinv32 = nullptr, *x32 = nullptr, *r32 = nullptr, *p32 = nullptr,
+          *q32 = nullptr;
+    double *press = nullptr, *outflux = nullptr, *outflux_tmp = nullptr;
+    double *in_totmob = nullptr, *in_rhomob = nullptr, *in_src = nullptr, *in_bcval = nullptr;
+    uint8_t *ftype = nullptr;
+    int *inst_reg = nullptr;
+    DevRed R{};
+    DevScal S{};
+    unsigned long long *maxbits = nullptr;
+    // pinned host mirrors of the small per-system state
+    double *h_sc = nullptr;
+    int *h_act = nullptr, *h_iters = nullptr, *h_upd = nullptr, *h_status = nullptr;
+    size_t h_nsys = 0;
+    std::vector<uint8_t> last_ftype;
+    int last_nrhs = 0;
+    GraphCache graph;
+
+    ~mhs_solver();
+    void free_topology();
+    void free_solve_buffers();
+};
+
+namespace {
+
+int fail(mhs_solver *s, int code, const std::string &msg) {
+    if (s) s->err = msg;
+    return code;
+}
+
+#define HIPCHECK(expr)                                                                 \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(s, MHS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+T *dalloc(mhs_solver *s, std::vector<DevBuf> &list, size_t n, hipError_t *e) {
+    void *p = nullptr;
+    size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    *e = hipMalloc(&p, bytes);
+    if (*e != hipSuccess) return nullptr;
+    list.push_back({p, bytes});
+    return static_cast<T *>(p);
+}
+
+template <class T>
+T *dupload(mhs_solver *s, std::vector<DevBuf> &list, const std::vector<T> &v, hipError_t *e) {
+    T *p = dalloc<T>(s, list, v.size(), e);
+    if (!p) return nullptr;
+    if (!v.empty()) *e = hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    return p;
+}
+
+}  // namespace
+
+void mhs_solver::free_topology() {
+    for (auto &b : topo_bufs) (void)hipFree(b.p);
+    topo_bufs.clear();
+    binv = gflux = nullptr;
+    dT = DevTopo{};
+    dG = DevGeom{};
+}
+
+void mhs_solver::free_solve_buffers() {
+    if (graph.exec) (void)hipGraphExecDestroy(graph.exec);
+    graph = GraphCache{};
+    for (auto &b : solve_bufs) (void)hipFree(b.p);
+    solve_bufs.clear();
+    cap_nrhs = 0;
+    cap_mixed = false;
+    if (h_sc) hipHostFree(h_sc);
+    if (h_act) hipHostFree(h_act);
+    if (h_iters) hipHostFree(h_iters);
+    if (h_upd) hipHostFree(h_upd);
+    if (h_status) hipHostFree(h_status);
+    h_sc = nullptr;
+    h_act = h_iters = h_upd = h_status = nullptr;
+    h_nsys = 0;
+    last_ftype.clear();
+}
+
+mhs_solver::~mhs_solver() {
+    free_solve_buffers();
+    free_topology();
+    if (ev_in) hipEventDestroy(ev_in);
+    if (ev_out) hipEventDestroy(ev_out);
+    if (stream) hipStreamDestroy(stream);
+}
+
+extern "C" {
+
+void mhs_default_options(mhs_solve_options *o) {
+    if (!o) return;
+    o->tol = 1e-8;               // residual_tolerance default (:662)
+    o->maxit = 0;
+    o->linsolver_type = 1;       // reference default (:664)
+    o->verbosity = 0;
+    o->same_matrix = 0;
+    o->prolongate_factor = 1.6;  // (:667)
+    o->smooth_steps = 1;         // (:668)
+    o->precision = MHS_FP64;
+    o->nrhs = 1;
+    o->chunk = 32;
+    o->replace_factor = 0.1;
+    o->use_graph = 1;
+}
+
+int mhs_create(mhs_solver **out, int device, void *stream) {
+    if (!out) return MHS_ERR_INVALID;
+    *out = nullptr;
+    mhs_solver *s = new (std::nothrow) mhs_solver();
+    if (!s) return MHS_ERR_ALLOC;
+    s->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_in, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_out, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        delete s;
+        return MHS_ERR_HIP;
+    }
+    s->user_stream = static_cast<hipStream_t>(stream);
+    *out = s;
+    return MHS_OK;
+}
+
+int mhs_destroy(mhs_solver *s) {
+    if (!s) return MHS_OK;
+    hipSetDevice(s->device);
+    delete s;
+    return MHS_OK;
+}
+
+const char *mhs_last_error(const mhs_solver *s) { return s ? s->err.c_str() : "null solver"; }
+
+int mhs_clear(mhs_solver *s) {
+    if (!s) return MHS_ERR_INVALID;
+    hipSetDevice(s->device);
+    hipStreamSynchronize(s->stream);
+    s->free_solve_buffers();
+    s->free_topology();
+    s->grid_set = s->structure_valid = s->ip_valid = s->solved = false;
+    s->T = Topology();
+    s->inst_cellpos.clear();
+    s->facepos.clear();
+    s->nb.clear();
+    s->cvol.clear(); s->ccen.clear(); s->farea.clear(); s->fcen.clear(); s->fnrm.clear();
+    s->nc = s->nhf = 0;
+    s->err.clear();
+    return MHS_OK;
+}
+
+int mhs_set_grid(mhs_solver *s, int num_instances, const int *inst_cellpos, int num_cells,
+                 const int *facepos, const int *neighbour, const double *cell_volume,
+                 const double *cell_centroid, const double *face_area,
+                 const double *face_centroid, const double *face_normal) {
+    if (!s) return MHS_ERR_INVALID;
+    if (num_cells < 0 || num_instances < 1 || !facepos || (num_cells > 0 && !neighbour))
+        return fail(s, MHS_ERR_INVALID, "set_grid: bad sizes or null arrays");
+    if (num_instances > 1 && !inst_cellpos)
+        return fail(s, MHS_ERR_INVALID, "set_grid: inst_cellpos required for several instances");
+    if (!cell_volume || !cell_centroid || !face_area || !face_centroid || !face_normal)
+        return fail(s, MHS_ERR_INVALID, "set_grid: geometry arrays required");
+    mhs_clear(s);
+    s->ninst = num_instances;
+    s->nc = num_cells;
+    if (facepos[0] != 0) return fail(s, MHS_ERR_INVALID, "set_grid: facepos[0] must be 0");
+    for (int c = 0; c < num_cells; ++c)
+        if (facepos[c + 1] < facepos[c]) return fail(s, MHS_ERR_INVALID, "set_grid: facepos not monotone");
+    s->nhf = facepos[num_cells];
+    s->facepos.assign(facepos, facepos + num_cells + 1);
+    s->nb.assign(neighbour, neighbour + s->nhf);
+    for (int h = 0; h < s->nhf; ++h)
+        if (s->nb[h] >= num_cells) return fail(s, MHS_ERR_INVALID, "set_grid: neighbour out of range");
+    if (inst_cellpos) s->inst_cellpos.assign(inst_cellpos, inst_cellpos + num_instances + 1);
+    else s->inst_cellpos = {0, num_cells};
+    s->cvol.assign(cell_volume, cell_volume + num_cells);
+    s->ccen.assign(cell_centroid, cell_centroid + 3 * (size_t)num_cells);
+    s->farea.assign(face_area, face_area + s->nhf);
+    s->fcen.assign(face_centroid, face_centroid + 3 * (size_t)s->nhf);
+    s->fnrm.assign(face_normal, face_normal + 3 * (size_t)s->nhf);
+    s->grid_set = true;
+    return MHS_OK;
+}
+
+int mhs_init_system_structure(mhs_solver *s, const int *hf_bc_type, const int *hf_periodic_partner) {
+    if (!s) return MHS_ERR_INVALID;
+    if (!s->grid_set) return fail(s, MHS_ERR_STATE, "init_system_structure: no grid (call set_grid)");
+    HIPCHECK(hipSetDevice(s->device));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    s->free_solve_buffers();
+    s->free_topology();
+    s->structure_valid = s->ip_valid = s->solved = false;
+    std::string e = build_topology(s->T, s->ninst, s->inst_cellpos.data(), s->nc, s->facepos.data(),
+                                   s->nb.data(), hf_bc_type, hf_periodic_partner);
+    if (!e.empty()) return fail(s, MHS_ERR_INVALID, "init_system_structure: " + e);
+    Topology &T = s->T;
+    hipError_t he = hipSuccess;
+    auto &L = s->topo_bufs;
+    DevTopo &d = s->dT;
+    d.nc = T.nc; d.nhf = T.nhf; d.nfp = T.nfp; d.ninst = T.ninst; d.nslices = T.nslices;
+    d.nblk = T.nfp / kRowBlock;
+#define UP(field, vec)                                   \
+    do {                                                 \
+        d.field = dupload(s, L, T.vec, &he);             \
+        if (he != hipSuccess) return fail(s, MHS_ERR_HIP, "upload " #vec); \
+    } while (0)
+    UP(facepos, facepos); UP(hf_dof, hf_dof); UP(hf_cell, hf_cell); UP(hf_col, hf_col);
+    UP(hf_ppart, hf_ppart); UP(hf_twin, hf_twin); UP(hf_redirect, hf_redirect);
+    UP(row_hf, row_hf); UP(dof_hf, dof_hf); UP(ptn, ptn); UP(dslot, dslot); UP(pslot, pslot);
+    UP(slice_ptr, slice_ptr); UP(col, col); UP(slotmap, slotmap); UP(sq_pos, sq_pos);
+    UP(pk_pos, pk_pos); UP(block_seg, block_seg); UP(seg_blk, seg_blk); UP(inst_rowbase, inst_rowbase);
+#undef UP
+    DevGeom &g = s->dG;
+    g.cvol = dupload(s, L, s->cvol, &he);
+    if (he == hipSuccess) g.ccen = dupload(s, L, s->ccen, &he);
+    if (he == hipSuccess) g.farea = dupload(s, L, s->farea, &he);
+    if (he == hipSuccess) g.fcen = dupload(s, L, s->fcen, &he);
+    if (he == hipSuccess) g.fnrm = dupload(s, L, s->fnrm, &he);
+    if (he == hipSuccess) s->binv = dalloc<double>(s, L, (size_t)T.pk_pos[T.nc], &he);
+    if (he == hipSuccess) s->gflux = dalloc<double>(s, L, (size_t)T.nhf, &he);
+    if (he != hipSuccess) return fail(s, MHS_ERR_ALLOC, std::string("device allocation: ") + hipGetErrorString(he));
+    s->structure_valid = true;
+    return MHS_OK;
+}
+
+int mhs_compute_inner_products(mhs_solver *s, const double *perm, const double *gravity) {
+    if (!s) return MHS_ERR_INVALID;
+    if (!s->structure_valid)
+        return fail(s, MHS_ERR_STATE, "compute_inner_products: structure not initialised");
+    if (!perm) return fail(s, MHS_ERR_INVALID, "compute_inner_products: perm is null");
+    HIPCHECK(hipSetDevice(s->device));
+    double grav[3] = {0.0, 0.0, 0.0};
+    if (gravity) { grav[0] = gravity[0]; grav[1] = gravity[1]; grav[2] = gravity[2]; }
+    double *dperm = nullptr;
+    HIPCHECK(hipMalloc(&dperm, std::max<size_t>(1, 9 * (size_t)s->nc) * sizeof(double)));
+    hipError_t e = hipMemcpyAsync(dperm, perm, 9 * (size_t)s->nc * sizeof(double),
+                                  hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess)
+        e = launch_static_ip(s->stream, s->dT, s->dG, s->T.max_ncf, dperm, grav, s->binv, s->gflux);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    hipFree(dperm);
+    if (e != hipSuccess) return fail(s, MHS_ERR_HIP, std::string("static inner products: ") + hipGetErrorString(e));
+    s->ip_valid = true;
+    s->solved = false;
+    return MHS_OK;
+}
+
+int mhs_assemble_static(mhs_solver *s, const double *perm, const double *gravity) {
+    return mhs_compute_inner_products(s, perm, gravity);
+}
+
+int mhs_get_inner_products(mhs_solver *s, double *binv, double *gflux) {
+    if (!s) return MHS_ERR_INVALID;
+    if (!s->ip_valid) return fail(s, MHS_ERR_STATE, "get_inner_products: no inner products");
+    HIPCHECK(hipSetDevice(s->device));
+    const Topology &T = s->T;
+    if (binv) {
+        std::vector<double> pk(T.pk_pos[T.nc]);
+        HIPCHECK(hipMemcpy(pk.data(), s->binv, sizeof(double) * pk.size(), hipMemcpyDeviceToHost));
+        for (int c = 0; c < T.nc; ++c) {
+            const int n = T.facepos[c + 1] - T.facepos[c];
+            double *out = binv + T.sq_pos[c];
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < n; ++j)
+                    out[i + j * n] = pk[T.pk_pos[c] + packed_index(std::min(i, j), std::max(i, j), n)];
+        }
+    }
+    if (gflux)
+        HIPCHECK(hipMemcpy(gflux, s->gflux, sizeof(double) * T.nhf, hipMemcpyDeviceToHost));
+    return MHS_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int ensure_solve_buffers(mhs_solver *s, int nrhs, bool mixed) {
+    if (s->cap_nrhs == nrhs && s->cap_mixed == mixed) return MHS_OK;
+    s->free_solve_buffers();
+    const Topology &T = s->T;
+    auto &L = s->solve_bufs;
+    hipError_t e = hipSuccess;
+    const size_t nfp = T.nfp, nv = nfp * nrhs, nnz = (size_t)T.slice_ptr[T.nslices];
+    const size_t nsys = (size_t)T.ninst * nrhs;
+    auto ok = [&]() { return e == hipSuccess; };
+    s->C.schur = dalloc<double>(s, L, (size_t)T.pk_pos[T.nc], &e);
+    if (ok()) s->C.rr = dalloc<double>(s, L, (size_t)nrhs * T.nhf, &e);
+    if (ok()) s->C.F = dalloc<double>(s, L, (size_t)T.nhf, &e);
+    if (ok()) s->C.L = dalloc<double>(s, L, (size_t)T.nc, &e);
+    if (ok()) s->C.g = dalloc<double>(s, L, (size_t)nrhs * T.nc, &e);
+    if (ok()) s->val64 = dalloc<double>(s, L, nnz, &e);
+    if (ok()) s->b = dalloc<double>(s, L, nv, &e);
+    if (ok()) s->dinv64 = dalloc<double>(s, L, nfp, &e);
+    if (ok()) s->x64 = dalloc<double>(s, L, nv, &e);
+    if (ok()) s->r64 = dalloc<double>(s, L, nv, &e);
+    if (!mixed) {
+        if (ok()) s->p64 = dalloc<double>(s, L, nv, &e);
+        if (ok()) s->q64 = dalloc<double>(s, L, nv, &e);
+        s->val32 = nullptr; s->dinv32 = s->x32 = s->r32 = s->p32 = s->q32 = nullptr;
+    } else {
+        s->p64 = s->q64 = nullptr;
+        if (ok()) s->val32 = dalloc<float>(s, L, nnz, &e);
+        if (ok()) s->dinv32 = dalloc<float>(s, L, nfp, &e);
+        if (ok()) s->x32 = dalloc<float>(s, L, nv, &e);
+        if (ok()) s->r32 = dalloc<float>(s, L, nv, &e);
+        if (ok()) s->p32 = dalloc<float>(s, L, nv, &e);
+        if (ok()) s->q32 = dalloc<float>(s, L, nv, &e);
+    }
+    if (ok()) s->press = dalloc<double>(s, L, (size_t)nrhs * T.nc, &e);
+    if (ok()) s->outflux = dalloc<double>(s, L, (size_t)nrhs * T.nhf, &e);
+    if (ok()) s->outflux_tmp = dalloc<double>(s, L, (size_t)nrhs * T.nhf, &e);
+    if (ok()) s->in_totmob = dalloc<double>(s, L, (size_t)T.nc, &e);
+    if (ok()) s->in_rhomob = dalloc<double>(s, L, (size_t)T.nc, &e);
+    if (ok()) s->in_src = dalloc<double>(s, L, (size_t)nrhs * T.nc, &e);
+    if (ok()) s->in_bcval = dalloc<double>(s, L, (size_t)nrhs * T.nhf, &e);
+    if (ok()) s->ftype = dalloc<uint8_t>(s, L, (size_t)T.nhf, &e);
+    if (ok()) s->inst_reg = dalloc<int>(s, L, (size_t)T.ninst, &e);
+    if (ok()) s->R.partials = dalloc<double>(s, L, (size_t)(nfp / kRowBlock) * 16, &e);
+    if (ok()) s->R.counters = dalloc<int>(s, L, (size_t)T.ninst, &e);
+    if (ok()) s->S.sc = dalloc<double>(s, L, nsys * SC_N, &e);
+    if (ok()) s->S.act = dalloc<int>(s, L, nsys, &e);
+    if (ok()) s->S.iters = dalloc<int>(s, L, nsys, &e);
+    if (ok()) s->S.upd = dalloc<int>(s, L, nsys, &e);
+    if (ok()) s->S.status = dalloc<int>(s, L, 4, &e);
+    if (ok()) s->maxbits = dalloc<unsigned long long>(s, L, 1, &e);
+    if (ok()) e = hipMemset(s->R.counters, 0, sizeof(int) * T.ninst);
+    if (ok()) e = hipMemset(s->x64, 0, nv * sizeof(double));
+    if (ok()) e = hipHostMalloc(reinterpret_cast<void **>(&s->h_sc), nsys * SC_N * sizeof(double), 0);
+    if (ok()) e = hipHostMalloc(reinterpret_cast<void **>(&s->h_act), nsys * sizeof(int), 0);
+    if (ok()) e = hipHostMalloc(reinterpret_cast<void **>(&s->h_iters), nsys * sizeof(int), 0);
+    if (ok()) e = hipHostMalloc(reinterpret_cast<void **>(&s->h_upd), nsys * sizeof(int), 0);
+    if (ok()) e = hipHostMalloc(reinterpret_cast<void **>(&s->h_status), 4 * sizeof(int), 0);
+    if (!ok()) {
+        s->free_solve_buffers();
+        return fail(s, MHS_ERR_ALLOC, std::string("solve buffers: ") + hipGetErrorString(e));
+    }
+    s->R.block_seg = s->dT.block_seg;
+    s->R.seg_blk = s->dT.seg_blk;
+    s->h_nsys = nsys;
+    s->cap_nrhs = nrhs;
+    s->cap_mixed = mixed;
+    return MHS_OK;
+}
+
+// Launch `chunk` PCG iterations, through a cached hipGraph when enabled.
+template <typename MT, typename VT>
+int run_chunk(mhs_solver *s, int nrhs, int chunk, bool use_graph, const MT *val, const VT *dinv,
+              VT *x, VT *r, VT *p, VT *q, int precision, double tol2) {
+    if (use_graph) {
+        GraphCache &G = s->graph;
+        if (!(G.exec && G.precision == precision && G.nrhs == nrhs && G.chunk == chunk &&
+              G.key == (const void *)val && G.tol2 == tol2 && G.rf2 == s->S.rf2)) {
+            if (G.exec) (void)hipGraphExecDestroy(G.exec);
+            G = GraphCache{};
+            hipGraph_t graph = nullptr;
+            HIPCHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = hipSuccess;
+            for (int it = 0; it < chunk && e == hipSuccess; ++it)
+                e = launch_pcg_iteration<MT, VT>(s->stream, s->dT, nrhs, val, dinv, x, r, p, q, s->R,
+                                                 s->S, s->val64, s->b, s->x64, tol2);
+            hipError_t e2 = hipStreamEndCapture(s->stream, &graph);
+            if (e != hipSuccess || e2 != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return fail(s, MHS_ERR_HIP, std::string("graph capture: ") +
+                                                hipGetErrorString(e != hipSuccess ? e : e2));
+            }
+            hipError_t e3 = hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e3 != hipSuccess) {
+                G = GraphCache{};
+                return fail(s, MHS_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e3));
+            }
+            G.precision = precision;
+            G.nrhs = nrhs;
+            G.chunk = chunk;
+            G.key = val;
+            G.tol2 = tol2;
+            G.rf2 = s->S.rf2;
+        }
+        HIPCHECK(hipGraphLaunch(G.exec, s->stream));
+    } else {
+        for (int it = 0; it < chunk; ++it)
+            HIPCHECK((launch_pcg_iteration<MT, VT>(s->stream, s->dT, nrhs, val, dinv, x, r, p, q, s->R,
+                                                   s->S, s->val64, s->b, s->x64, tol2)));
+    }
+    return MHS_OK;
+}
+
+int read_state(mhs_solver *s, size_t nsys, bool sc) {
+    HIPCHECK(hipMemcpyAsync(s->h_act, s->S.act, nsys * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIPCHECK(hipMemcpyAsync(s->h_iters, s->S.iters, nsys * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIPCHECK(hipMemcpyAsync(s->h_status, s->S.status, 4 * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    if (sc)
+        HIPCHECK(hipMemcpyAsync(s->h_sc, s->S.sc, nsys * SC_N * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    return MHS_OK;
+}
+
+int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const double *src,
+               const int *hf_bc_type, const double *bcval, const mhs_solve_options *opt_in,
+               mhs_solve_report *rep, bool device_inputs) {
+    if (!s) return MHS_ERR_INVALID;
+    mhs_solve_options opt;
+    mhs_default_options(&opt);
+    if (opt_in) opt = *opt_in;
+    if (!s->structure_valid || !s->ip_valid)
+        return fail(s, MHS_ERR_STATE, "solve: init() (structure and inner products) must come first");
+    if (opt.linsolver_type < 0 || opt.linsolver_type > 3)
+        return fail(s, MHS_ERR_INVALID, "Unknown linsolver_type: " + std::to_string(opt.linsolver_type));
+    if (opt.nrhs < 1 || opt.nrhs > 4) return fail(s, MHS_ERR_INVALID, "solve: nrhs must be 1..4");
+    if (!(opt.tol > 0.0)) return fail(s, MHS_ERR_INVALID, "solve: tol must be positive");
+    if (opt.precision != MHS_FP64 && opt.precision != MHS_MIXED)
+        return fail(s, MHS_ERR_INVALID, "solve: unknown precision");
+    if (!totmob || !rhomob || !src || !hf_bc_type || !bcval)
+        return fail(s, MHS_ERR_INVALID, "solve: null input array");
+    const int chunk = std::max(1, std::min(opt.chunk, 4096));
+    const int nrhs = opt.nrhs;
+    const bool mixed = opt.precision == MHS_MIXED;
+    HIPCHECK(hipSetDevice(s->device));
+    const Topology &T = s->T;
+    int rc = ensure_solve_buffers(s, nrhs, mixed);
+    if (rc != MHS_OK) return rc;
+    const size_t nsys = (size_t)T.ninst * nrhs;
+    const int maxit = opt.maxit > 0 ? opt.maxit : std::max(1, T.nf_total);
+
+    // Face types (FaceType, :378) and per-instance regularisation flags.
+    std::vector<uint8_t> ft(T.nhf, FT_INTERNAL);
+    std::vector<int> reg(T.ninst, 1);
+    {
+        int t = 0;
+        for (int c = 0; c < T.nc; ++c) {
+            while (c >= T.inst_cellpos[t + 1]) ++t;
+            for (int h = T.facepos[c]; h < T.facepos[c + 1]; ++h) {
+                if (T.nb[h] >= 0) continue;
+                int bt = hf_bc_type[h];
+                if (bt == MHS_BC_DIRICHLET) { ft[h] = FT_DIRICHLET; reg[t] = 0; }
+                else if (bt == MHS_BC_PERIODIC) ft[h] = FT_PERIODIC;
+                else if (bt == MHS_BC_NEUMANN) ft[h] = FT_NEUMANN;
+                else return fail(s, MHS_ERR_INVALID, "solve: unknown boundary condition type");
+                if ((bt == MHS_BC_PERIODIC) != (T.hf_periodic[h] != 0))
+                    return fail(s, MHS_ERR_INVALID,
+                                "solve: periodic faces differ from those given to init_system_structure");
+            }
+        }
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    if (s->user_stream) {
+        HIPCHECK(hipEventRecord(s->ev_in, s->user_stream));
+        HIPCHECK(hipStreamWaitEvent(s->stream, s->ev_in, 0));
+    }
+    if (ft != s->last_ftype) {
+        HIPCHECK(hipMemcpyAsync(s->ftype, ft.data(), T.nhf, hipMemcpyHostToDevice, s->stream));
+        HIPCHECK(hipStreamSynchronize(s->stream));
+        s->last_ftype = ft;
+    }
+    HIPCHECK(hipMemcpyAsync(s->inst_reg, reg.data(), sizeof(int) * T.ninst, hipMemcpyHostToDevice, s->stream));
+    const double *d_tm = totmob, *d_rm = rhomob, *d_src = src, *d_bv = bcval;
+    if (!device_inputs) {
+        auto kind = hipMemcpyHostToDevice;
+        HIPCHECK(hipMemcpyAsync(s->in_totmob, totmob, sizeof(double) * T.nc, kind, s->stream));
+        HIPCHECK(hipMemcpyAsync(s->in_rhomob, rhomob, sizeof(double) * T.nc, kind, s->stream));
+        HIPCHECK(hipMemcpyAsync(s->in_src, src, sizeof(double) * T.nc * nrhs, kind, s->stream));
+        HIPCHECK(hipMemcpyAsync(s->in_bcval, bcval, sizeof(double) * T.nhf * nrhs, kind, s->stream));
+        d_tm = s->in_totmob; d_rm = s->in_rhomob; d_src = s->in_src; d_bv = s->in_bcval;
+    }
+    // --- assembly -----------------------------------------------------------
+    HIPCHECK(launch_assemble(s->stream, s->dT, T.max_ncf, nrhs, s->binv, s->gflux, d_tm, d_rm, d_src,
+                             s->ftype, d_bv, s->C));
+    const int dguess = opt.linsolver_type != 0;  // AMG types start from the Dirichlet guess (:1521-1532)
+    HIPCHECK(launch_fill(s->stream, s->dT, nrhs, s->C, s->ftype, d_bv, s->inst_reg, dguess, s->val64,
+                         mixed ? s->val32 : nullptr, s->b, s->dinv64, s->dinv32, s->x64));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    auto t1 = std::chrono::steady_clock::now();
+
+    // --- linear solve ----------------------------------------------------------
+    const double tol2 = opt.tol * opt.tol;
+    int total_iters = 0;
+    s->S.mixed = mixed ? 1 : 0;
+    s->S.rf2 = opt.replace_factor * opt.replace_factor;
+    HIPCHECK(hipMemsetAsync(s->S.status, 0, 4 * sizeof(int), s->stream));
+    HIPCHECK(hipMemsetAsync(s->S.iters, 0, nsys * sizeof(int), s->stream));
+    HIPCHECK(hipMemsetAsync(s->S.upd, 0, nsys * sizeof(int), s->stream));
+    if (!mixed) {
+        HIPCHECK(launch_pcg_start64(s->stream, s->dT, nrhs, s->val64, s->dinv64, s->b, s->x64, s->r64,
+                                    s->p64, tol2, s->R, s->S));
+    } else {
+        HIPCHECK(hipMemsetAsync(s->x32, 0, sizeof(float) * T.nfp * nrhs, s->stream));
+        HIPCHECK(launch_reliable(s->stream, s->dT, nrhs, s->val64, s->b, s->x64, s->dinv32, s->r32,
+                                 s->p32, 1, tol2, s->R, s->S));
+    }
+    rc = read_state(s, nsys, false);
+    if (rc) return rc;
+    while (s->h_status[0] > 0) {
+        int mx = 0;
+        for (size_t i = 0; i < nsys; ++i) mx = std::max(mx, s->h_iters[i]);
+        if (mx >= maxit) break;
+        const int n = std::min(chunk, maxit - mx);
+        if (!mixed)
+            rc = run_chunk<double, double>(s, nrhs, n, opt.use_graph && n == chunk, s->val64, s->dinv64,
+                                           s->x64, s->r64, s->p64, s->q64, MHS_FP64, tol2);
+        else
+            rc = run_chunk<float, float>(s, nrhs, n, opt.use_graph && n == chunk, s->val32, s->dinv32,
+                                         s->x32, s->r32, s->p32, s->q32, MHS_MIXED, tol2);
+        if (rc) return rc;
+        rc = read_state(s, nsys, opt.verbosity >= 2);
+        if (rc) return rc;
+        if (opt.verbosity >= 2)
+            for (size_t i = 0; i < std::min<size_t>(nsys, 4); ++i) {
+                const double *sc = s->h_sc + i * SC_N;
+                std::fprintf(stderr, "[mhs] sys %zu it %d act %d |r|/|r0| %.3e last FP64 %.3e\n", i,
+                             s->h_iters[i], s->h_act[i], std::sqrt(sc[SC_RR] / sc[SC_RR0]),
+                             std::sqrt(sc[SC_RRT] / sc[SC_RR0]));
+            }
+        if (s->h_status[1] > 0)
+            return fail(s, MHS_ERR_BREAKDOWN, "PCG breakdown: face operator not positive definite");
+    }
+    const bool converged = s->h_status[0] == 0;
+    for (size_t i = 0; i < nsys; ++i) total_iters = std::max(total_iters, s->h_iters[i]);
+    if (mixed) HIPCHECK(launch_merge_all(s->stream, s->dT, nrhs, s->x64, s->x32));
+    HIPCHECK(launch_resnorm(s->stream, s->dT, nrhs, s->val64, s->b, s->x64, s->R, s->S));
+    rc = read_state(s, nsys, true);
+    if (rc) return rc;
+    const int replacements = s->h_status[2];
+    double red = 0.0;
+    for (size_t i = 0; i < nsys; ++i) {
+        const double *sc = s->h_sc + i * SC_N;
+        if (sc[SC_RR0] > 0) red = std::max(red, std::sqrt(sc[SC_RRT] / sc[SC_RR0]));
+    }
+    auto t2 = std::chrono::steady_clock::now();
+
+    // --- recovery -------------------------------------------------------------
+    HIPCHECK(launch_recover(s->stream, s->dT, T.max_ncf, nrhs, s->binv, s->gflux, d_tm, d_rm, s->C,
+                            s->x64, s->press, s->outflux));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    if (s->user_stream) {
+        HIPCHECK(hipEventRecord(s->ev_out, s->stream));
+        HIPCHECK(hipStreamWaitEvent(s->user_stream, s->ev_out, 0));
+    }
+    auto t3 = std::chrono::steady_clock::now();
+    s->solved = true;
+    s->last_nrhs = nrhs;
+    if (rep) {
+        rep->iterations = total_iters;
+        rep->replacements = replacements;
+        rep->converged = converged ? 1 : 0;
+        rep->reduction = red;
+        rep->assemble_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        rep->solve_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+        rep->recover_ms = std::chrono::duration<double, std::milli>(t3 - t2).count();
+    }
+    if (!converged)
+        return fail(s, MHS_ERR_NOT_CONVERGED,
+                    "Linear solver failed to converge in " + std::to_string(total_iters) +
+                        " iterations.\nResidual reduction achieved is " + std::to_string(red) + "\n");
+    return MHS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mhs_solve(mhs_solver *s, const double *totmob, const double *rhomob, const double *src,
+              const int *hf_bc_type, const double *hf_bc_value, const mhs_solve_options *opt,
+              mhs_solve_report *report) {
+    return solve_impl(s, totmob, rhomob, src, hf_bc_type, hf_bc_value, opt, report, false);
+}
+
+int mhs_solve_device(mhs_solver *s, const double *totmob, const double *rhomob, const double *src,
+                     const int *hf_bc_type, const double *hf_bc_value,
+                     const mhs_solve_options *opt, mhs_solve_report *report) {
+    return solve_impl(s, totmob, rhomob, src, hf_bc_type, hf_bc_value, opt, report, true);
+}
+
+int mhs_get_solution(mhs_solver *s, double *cell_pressure, double *face_pressure, double *outflux) {
+    if (!s) return MHS_ERR_INVALID;
+    if (!s->solved) return fail(s, MHS_ERR_STATE, "get_solution: no solution (call solve)");
+    HIPCHECK(hipSetDevice(s->device));
+    const Topology &T = s->T;
+    const int nrhs = s->last_nrhs;
+    if (cell_pressure)
+        HIPCHECK(hipMemcpy(cell_pressure, s->press, sizeof(double) * T.nc * nrhs, hipMemcpyDeviceToHost));
+    if (outflux)
+        HIPCHECK(hipMemcpy(outflux, s->outflux, sizeof(double) * T.nhf * nrhs, hipMemcpyDeviceToHost));
+    if (face_pressure) {
+        std::vector<double> x((size_t)T.nfp * nrhs);
+        HIPCHECK(hipMemcpy(x.data(), s->x64, sizeof(double) * x.size(), hipMemcpyDeviceToHost));
+        for (int k = 0; k < nrhs; ++k)
+            for (int f = 0; f < T.nf_total; ++f)
+                face_pressure[(size_t)k * T.nf_total + f] = x[(size_t)T.face_row[f] * nrhs + k];
+    }
+    return MHS_OK;
+}
+
+int mhs_solution_device(mhs_solver *s, const double **cell_pressure, const double **face_pressure,
+                        const double **outflux) {
+    if (!s) return MHS_ERR_INVALID;
+    if (!s->solved) return fail(s, MHS_ERR_STATE, "solution_device: no solution (call solve)");
+    if (cell_pressure) *cell_pressure = s->press;
+    if (face_pressure) *face_pressure = s->x64;
+    if (outflux) *outflux = s->outflux;
+    return MHS_OK;
+}
+
+int mhs_post_process_fluxes(mhs_solver *s, double *max_modification) {
+    if (!s) return MHS_ERR_INVALID;
+    if (!s->solved) return fail(s, MHS_ERR_STATE, "post_process_fluxes: no solution (call solve)");
+    HIPCHECK(hipSetDevice(s->device));
+    unsigned long long zero = 0, bits = 0;
+    HIPCHECK(hipMemcpyAsync(s->maxbits, &zero, sizeof(zero), hipMemcpyHostToDevice, s->stream));
+    HIPCHECK(launch_post_process(s->stream, s->dT, s->last_nrhs, s->outflux, s->outflux_tmp, s->maxbits));
+    HIPCHECK(hipMemcpyAsync(&bits, s->maxbits, sizeof(bits), hipMemcpyDeviceToHost, s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    std::swap(s->outflux, s->outflux_tmp);
+    double mm;
+    std::memcpy(&mm, &bits, sizeof(mm));
+    if (max_modification) *max_modification = mm;
+    return MHS_OK;
+}
+
+int mhs_get_dims(mhs_solver *s, mhs_dims *d) {
+    if (!s || !d) return MHS_ERR_INVALID;
+    std::memset(d, 0, sizeof(*d));
+    d->num_cells = s->nc;
+    d->num_half_faces = s->nhf;
+    d->num_instances = s->ninst;
+    if (s->structure_valid) {
+        d->num_faces = s->T.nf_total;
+        d->num_internal_faces = s->T.nint_total;
+        d->max_ncf = s->T.max_ncf;
+        d->num_rows_padded = s->T.nfp;
+        d->nnz = s->T.slice_ptr[s->T.nslices];
+    }
+    return MHS_OK;
+}
+
+int mhs_get_cell_faces(mhs_solver *s, int *hf_face, int *inst_facepos) {
+    if (!s) return MHS_ERR_INVALID;
+    if (!s->structure_valid) return fail(s, MHS_ERR_STATE, "get_cell_faces: no structure");
+    const Topology &T = s->T;
+    if (hf_face) {
+        int t = 0;
+        for (int c = 0; c < T.nc; ++c) {
+            while (c >= T.inst_cellpos[t + 1]) ++t;
+            for (int h = T.facepos[c]; h < T.facepos[c + 1]; ++h)
+                hf_face[h] = T.hf_dof[h] - T.inst_rowbase[t] + T.inst_facebase[t];
+        }
+    }
+    if (inst_facepos)
+        for (int t = 0; t <= T.ninst; ++t) inst_facepos[t] = T.inst_facebase[t];
+    return MHS_OK;
+}
+
+int mhs_get_face_rows(mhs_solver *s, int *face_row) {
+    if (!s || !face_row) return MHS_ERR_INVALID;
+    if (!s->structure_valid) return fail(s, MHS_ERR_STATE, "get_face_rows: no structure");
+    std::memcpy(face_row, s->T.face_row.data(), sizeof(int) * s->T.nf_total);
+    return MHS_OK;
+}
+
+int mhs_get_system(mhs_solver *s, int k, int *row_ptr, int *col, double *val, double *rhs,
+                   long long *nnz) {
+    if (!s) return MHS_ERR_INVALID;
+    if (!s->solved) return fail(s, MHS_ERR_STATE, "get_system: no assembled system (call solve)");
+    if (k < 0 || k >= s->last_nrhs) return fail(s, MHS_ERR_INVALID, "get_system: bad rhs index");
+    HIPCHECK(hipSetDevice(s->device));
+    const Topology &T = s->T;
+    const int nrhs = s->last_nrhs;
+    // padded row -> unpadded face
+    std::vector<int> row_face(T.nfp, -1);
+    for (int f = 0; f < T.nf_total; ++f) row_face[T.face_row[f]] = f;
+    long long count = 0;
+    for (int f = 0; f < T.nf_total; ++f) count += T.row_len[T.face_row[f]];
+    if (nnz) *nnz = count;
+    if (!col) return MHS_OK;
+    std::vector<double> v(T.slice_ptr[T.nslices]);
+    HIPCHECK(hipMemcpy(v.data(), s->val64, sizeof(double) * v.size(), hipMemcpyDeviceToHost));
+    std::vector<double> bb((size_t)T.nfp * nrhs);
+    HIPCHECK(hipMemcpy(bb.data(), s->b, sizeof(double) * bb.size(), hipMemcpyDeviceToHost));
+    long long m = 0;
+    if (row_ptr) row_ptr[0] = 0;
+    for (int f = 0; f < T.nf_total; ++f) {
+        const int r = T.face_row[f];
+        const long long base = T.slice_ptr[r / kSlice] + (r % kSlice);
+        for (int e = 0; e < T.row_len[r]; ++e) {
+            const long long o = base + (long long)e * kSlice;
+            col[m] = row_face[T.col[o]];
+            if (val) val[m] = v[o];
+            ++m;
+        }
+        if (row_ptr) row_ptr[f + 1] = (int)m;
+        if (rhs) rhs[f] = bb[(size_t)r * nrhs + k];
+    }
+    return MHS_OK;
+}
+
+}  // extern "C"

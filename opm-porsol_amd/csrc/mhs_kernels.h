@@ -1,0 +1,101 @@
+// Device data views and kernel launchers of the mimetic hybrid solver.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace mhs {
+
+// Geometry (per cell / per half-face), used by the static inner products only.
+struct DevGeom {
+    const double *cvol, *ccen, *farea, *fcen, *fnrm;
+};
+
+// Topology views (see mhs_structure.h for the meaning of each array).
+struct DevTopo {
+    int nc, nhf, nfp, ninst, nslices, nblk;
+    const int *facepos, *hf_dof, *hf_cell, *hf_col, *hf_ppart, *hf_twin;
+    const uint8_t *hf_redirect;
+    const int *row_hf, *dof_hf, *ptn;
+    const uint8_t *dslot, *pslot;
+    const long long *slice_ptr;
+    const int *col;
+    const uint8_t *slotmap;
+    const long long *sq_pos, *pk_pos;
+    const int *block_seg, *seg_blk, *inst_rowbase;
+};
+
+// Per-solve cell data produced by the assembly kernel.
+struct DevCellSys {
+    double *schur;     // packed local Schur complements (FP64)
+    double *rr;        // [nrhs][nhf] row right-hand-side contributions
+    double *F;         // [nhf]
+    double *L;         // [nc]
+    double *g;         // [nrhs][nc]
+};
+
+// Segmented-reduction workspace shared by all PCG kernels.
+struct DevRed {
+    double *partials;  // [nblk * 16]
+    int *counters;     // [ninst]
+    const int *block_seg, *seg_blk;
+};
+
+// Per-system scalars; a system is (instance s, right-hand side k), index s*nrhs+k.
+enum { SC_RZ = 0, SC_PQ, SC_RR, SC_RR0, SC_THR, SC_ALPHA, SC_BETA, SC_RRT, SC_RRL, SC_N };
+struct DevScal {
+    double *sc;        // [nsys * SC_N]
+    int *act;          // [nsys] 1 while iterating
+    int *iters;        // [nsys]
+    int *upd;          // [nsys] 1: FP64 residual replacement due, 2: also restart p
+    int *status;       // [4]: 0 nactive, 1 breakdown count
+    int mixed;         // FP32 iteration with FP64 residual replacement
+    double rf2;        // replacement trigger: |r|^2 <= rf2 * |r at last replacement|^2
+};
+
+hipError_t launch_static_ip(hipStream_t st, const DevTopo &T, const DevGeom &G, int maxn,
+                            const double *perm, const double *grav, double *binv_pk,
+                            double *gflux);
+
+hipError_t launch_assemble(hipStream_t st, const DevTopo &T, int maxn, int nrhs,
+                           const double *binv_pk, const double *gflux, const double *totmob,
+                           const double *rhomob, const double *src, const uint8_t *ftype,
+                           const double *bcval, DevCellSys C);
+
+hipError_t launch_fill(hipStream_t st, const DevTopo &T, int nrhs, const DevCellSys &C,
+                       const uint8_t *ftype, const double *bcval, const int *inst_reg,
+                       int dirichlet_guess, double *val64, float *val32, double *b,
+                       double *dinv64, float *dinv32, double *x0);
+
+// One Jacobi-PCG iteration (spmv_pq, update_xr, update_p; in FP32 also the
+// fused FP64 residual replacement, which needs val64, b and x64).
+template <typename MT, typename VT>
+hipError_t launch_pcg_iteration(hipStream_t st, const DevTopo &T, int nrhs, const MT *val,
+                                const VT *dinv, VT *x, VT *r, VT *p, VT *q, DevRed R,
+                                DevScal S, const double *val64, const double *b, double *x64,
+                                double tol2);
+
+// FP64 PCG start: r = b - A x, p = dinv r, rz, rr0, thresholds, activation.
+hipError_t launch_pcg_start64(hipStream_t st, const DevTopo &T, int nrhs, const double *val,
+                              const double *dinv, const double *b, const double *x, double *r,
+                              double *p, double tol2, DevRed R, DevScal S);
+
+// FP32 start / FP64 residual replacement (see k_reliable), final fold of x32
+// into x64, and the FP64 residual norm for the report.
+hipError_t launch_reliable(hipStream_t st, const DevTopo &T, int nrhs, const double *val64,
+                           const double *b, const double *x64, const float *dinv32, float *r32,
+                           float *p32, int init, double tol2, DevRed R, DevScal S);
+hipError_t launch_merge_all(hipStream_t st, const DevTopo &T, int nrhs, double *x64, float *x32);
+hipError_t launch_resnorm(hipStream_t st, const DevTopo &T, int nrhs, const double *val64,
+                          const double *b, const double *x64, DevRed R, DevScal S);
+
+// Recovery and post-processing.
+hipError_t launch_recover(hipStream_t st, const DevTopo &T, int maxn, int nrhs,
+                          const double *binv_pk, const double *gflux, const double *totmob,
+                          const double *rhomob, const DevCellSys &C, const double *x64,
+                          double *press, double *outflux);
+
+hipError_t launch_post_process(hipStream_t st, const DevTopo &T, int nrhs, const double *vin,
+                               double *vout, unsigned long long *maxbits);
+
+}  // namespace mhs

@@ -1,0 +1,904 @@
+// HIP kernels of the mimetic hybrid face-pressure solver (gfx950 / CDNA4).
+//
+// The path is memory bound end to end (no MFMA work): every kernel is written
+// for coalesced HBM streams, wave64 lanes and XCD-contiguous block ranges.
+//
+//  cell kernels (one lane per cell, FP64 register-resident local matrices):
+//    k_static_ip   MimeticIPEvaluator::buildStaticContrib (MimeticIPEvaluator.hpp:201-270)
+//    k_assemble    assembleDynamic + buildCellContrib + the row/column BC rules of
+//                  addCellContrib (IncompFlowSolverHybrid.hpp:1324-1382, 1785-1901)
+//    k_recover     computePressureAndFluxes (IncompFlowSolverHybrid.hpp:1672-1730)
+//  row kernels (one lane per face row, SELL-64 storage, no atomics on values):
+//    k_fill        global assembly as a gather: each face row pulls its <= 2 cell
+//                  contributions, the periodic constraint and the regularisation
+//    k_spmv_pq / k_update_xr / k_update_p   one Jacobi-PCG iteration, with
+//                  per-(instance, rhs) reductions finished by the last workgroup
+//                  of the instance (deterministic order, no host round trip)
+//    k_start64     FP64 start (r = b - A x, p = M^-1 r)
+//    k_reliable    FP64 true residual + residual replacement for the FP32
+//                  iteration (runs every iteration, idle unless a system asked)
+//    k_post        postProcessFluxes (IncompFlowSolverHybrid.hpp:754-807)
+#include "mhs_kernels.h"
+#include "mhs_structure.h"
+
+#include <type_traits>
+
+namespace mhs {
+
+namespace {
+
+__host__ __device__ constexpr int pidx(int i, int j, int n) {  // i <= j
+    return i * n - (i * (i - 1)) / 2 + (j - i);
+}
+
+// Hardware workgroup ids are dealt round-robin over the 8 XCDs; give every XCD
+// a contiguous range of logical rows so its private L2 sees neighbouring rows.
+__device__ __forceinline__ int xcd_block(int nblk) {
+    const int b = blockIdx.x;
+    const int q = nblk >> 3, r = nblk & 7;
+    const int x = b & 7, i = b >> 3;
+    return (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+template <int NQ>
+__device__ __forceinline__ void block_sum256(double (&v)[NQ], double *lds) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        double t = wave_sum(v[q]);
+        if (lane == 0) lds[w * NQ + q] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+            v[q] = (lds[q] + lds[NQ + q]) + (lds[2 * NQ + q] + lds[3 * NQ + q]);
+    }
+    __syncthreads();
+}
+
+// Sum NQ values over all workgroups of instance s.  Every workgroup of the
+// instance must call this; the last one to arrive reduces the per-workgroup
+// partials in index order and runs fin(totals) on thread 0.
+template <int NQ, class Fin>
+__device__ __forceinline__ void seg_reduce(double (&v)[NQ], int b, int s, const DevRed &R,
+                                           Fin fin) {
+    __shared__ double lds[4 * NQ];
+    __shared__ int last;
+    block_sum256<NQ>(v, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) R.partials[(size_t)b * 16 + q] = v[q];
+        __threadfence();
+        const int nb = R.seg_blk[s + 1] - R.seg_blk[s];
+        const int prev = atomicAdd(&R.counters[s], 1);
+        last = (prev == nb - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+    for (int i = R.seg_blk[s] + threadIdx.x; i < R.seg_blk[s + 1]; i += 256) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+            acc[q] += __hip_atomic_load(&R.partials[(size_t)i * 16 + q], __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+    }
+    block_sum256<NQ>(acc, lds);
+    if (threadIdx.x == 0) {
+        fin(acc);
+        __hip_atomic_store(&R.counters[s], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+template <int NRHS>
+__device__ __forceinline__ bool load_act(const int *act, int s, int (&a)[NRHS]) {
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) {
+        a[k] = act[s * NRHS + k];
+        any |= (a[k] != 0);
+    }
+    return any;
+}
+
+// ---------------------------------------------------------------------------
+// Static inverse inner products (once per computeInnerProducts).
+// B^-1 = (N K N' + t A (I - Q Q') A) / |c|,  t = 6 tr(K)/3, Q = orth(C),
+// N rows area*normal, C rows area*(x_f - x_c)  (MimeticIPEvaluator.hpp:236-269).
+// Q comes from Gram-Schmidt applied twice (only QQ' is used, so any orthonormal
+// basis of range(C) gives the same B^-1).
+template <int MAXN>
+__global__ __launch_bounds__(64) void k_static_ip(DevTopo T, DevGeom G,
+                                                  const double *__restrict__ perm, double gx,
+                                                  double gy, double gz,
+                                                  double *__restrict__ binv,
+                                                  double *__restrict__ gflux) {
+    const int c = blockIdx.x * 64 + threadIdx.x;
+    if (c >= T.nc) return;
+    const int f0 = T.facepos[c], n = T.facepos[c + 1] - f0;
+    double K[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) K[i] = perm[9 * (size_t)c + i];
+    const double Kg0 = K[0] * gx + K[1] * gy + K[2] * gz;
+    const double Kg1 = K[3] * gx + K[4] * gy + K[5] * gz;
+    const double Kg2 = K[6] * gx + K[7] * gy + K[8] * gz;
+    const double vol = G.cvol[c];
+    const double cc0 = G.ccen[3 * (size_t)c], cc1 = G.ccen[3 * (size_t)c + 1],
+                 cc2 = G.ccen[3 * (size_t)c + 2];
+    double N[MAXN][3], Q[MAXN][3], a[MAXN];
+#pragma unroll
+    for (int i = 0; i < MAXN; ++i) {
+        if (i < n) {
+            const size_t h = f0 + i;
+            const double ar = G.farea[h];
+            a[i] = ar;
+            N[i][0] = G.fnrm[3 * h] * ar;
+            N[i][1] = G.fnrm[3 * h + 1] * ar;
+            N[i][2] = G.fnrm[3 * h + 2] * ar;
+            Q[i][0] = (G.fcen[3 * h] - cc0) * ar;
+            Q[i][1] = (G.fcen[3 * h + 1] - cc1) * ar;
+            Q[i][2] = (G.fcen[3 * h + 2] - cc2) * ar;
+            gflux[h] = N[i][0] * Kg0 + N[i][1] * Kg1 + N[i][2] * Kg2;
+        } else {
+            a[i] = 0.0;
+            N[i][0] = N[i][1] = N[i][2] = 0.0;
+            Q[i][0] = Q[i][1] = Q[i][2] = 0.0;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+#pragma unroll
+        for (int pass = 0; pass < 2; ++pass) {
+#pragma unroll
+            for (int l = 0; l < j; ++l) {
+                double d = 0.0;
+#pragma unroll
+                for (int i = 0; i < MAXN; ++i) d += Q[i][l] * Q[i][j];
+#pragma unroll
+                for (int i = 0; i < MAXN; ++i) Q[i][j] -= d * Q[i][l];
+            }
+        }
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < MAXN; ++i) s += Q[i][j] * Q[i][j];
+        const double inv = 1.0 / sqrt(s);
+#pragma unroll
+        for (int i = 0; i < MAXN; ++i) Q[i][j] *= inv;
+    }
+    double NK[MAXN][3];
+#pragma unroll
+    for (int i = 0; i < MAXN; ++i)
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+            NK[i][l] = N[i][0] * K[l] + N[i][1] * K[3 + l] + N[i][2] * K[6 + l];
+    const double t = 6.0 * (K[0] + K[4] + K[8]) / 3.0;
+    const double rv = 1.0 / vol, tv = t / vol;
+    double *out = binv + T.pk_pos[c];
+#pragma unroll
+    for (int i = 0; i < MAXN; ++i) {
+#pragma unroll
+        for (int j = i; j < MAXN; ++j) {
+            if (j < n) {
+                const double s = NK[i][0] * N[j][0] + NK[i][1] * N[j][1] + NK[i][2] * N[j][2];
+                const double qq = Q[i][0] * Q[j][0] + Q[i][1] * Q[j][1] + Q[i][2] * Q[j][2];
+                const double B = a[i] * ((i == j ? 1.0 : 0.0) - qq) * a[j];
+                out[pidx(i, j, n)] = rv * s + tv * B;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Dynamic assembly, per cell.  S = totmob*B^-1; F = S'1; L = sum F;
+// g = src - sum gflux; local rhs = gflux - h + (g/L) F; S <- S - F F'/L
+// (buildCellContrib, :1785-1811).  The per-row right-hand-side contribution
+// folds in the column rules of addCellContrib (:1880-1896): Dirichlet columns
+// and periodic columns redirected to the smaller partner dof move S(r,c)*value
+// to the right-hand side.  Dirichlet rows carry S(r,r)*p (:1832-1835).
+template <int MAXN, int NRHS>
+__global__ __launch_bounds__(64) void k_assemble(DevTopo T, const double *__restrict__ binv,
+                                                 const double *__restrict__ gfs,
+                                                 const double *__restrict__ totmob,
+                                                 const double *__restrict__ rhomob,
+                                                 const double *__restrict__ src,
+                                                 const uint8_t *__restrict__ ftype,
+                                                 const double *__restrict__ bcval,
+                                                 DevCellSys C) {
+    constexpr int NP = MAXN * (MAXN + 1) / 2;
+    const int c = blockIdx.x * 64 + threadIdx.x;
+    if (c >= T.nc) return;
+    const int f0 = T.facepos[c], n = T.facepos[c + 1] - f0;
+    const double lam = totmob[c], rl = rhomob[c];
+    const long long pk = T.pk_pos[c];
+    double S[NP];
+#pragma unroll
+    for (int i = 0; i < MAXN; ++i)
+#pragma unroll
+        for (int j = i; j < MAXN; ++j)
+            S[pidx(i, j, MAXN)] = (j < n) ? lam * binv[pk + pidx(i, j, n)] : 0.0;
+    double F[MAXN], gfl[MAXN];
+    double L = 0.0, gsum = 0.0;
+#pragma unroll
+    for (int i = 0; i < MAXN; ++i) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < MAXN; ++j) s += (i <= j) ? S[pidx(i, j, MAXN)] : S[pidx(j, i, MAXN)];
+        F[i] = (i < n) ? s : 0.0;
+        gfl[i] = (i < n) ? gfs[f0 + i] * rl : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < MAXN; ++i) L += F[i];
+#pragma unroll
+    for (int i = 0; i < MAXN; ++i) gsum += gfl[i];
+    const double rL = 1.0 / L;
+#pragma unroll
+    for (int i = 0; i < MAXN; ++i)
+#pragma unroll
+        for (int j = i; j < MAXN; ++j) {
+            double v = -rL * F[i] * F[j] + S[pidx(i, j, MAXN)];
+            S[pidx(i, j, MAXN)] = v;
+            if (j < n) C.schur[pk + pidx(i, j, n)] = v;
+        }
+    uint8_t ft[MAXN], rd[MAXN];
+#pragma unroll
+    for (int i = 0; i < MAXN; ++i) {
+        ft[i] = (i < n) ? ftype[f0 + i] : (uint8_t)FT_INTERNAL;
+        rd[i] = (i < n) ? T.hf_redirect[f0 + i] : (uint8_t)0;
+        if (i < n) C.F[f0 + i] = F[i];
+    }
+    C.L[c] = L;
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) {
+        const double gk = src[(size_t)k * T.nc + c] - gsum;
+        C.g[(size_t)k * T.nc + c] = gk;
+        const double *bv = bcval + (size_t)k * T.nhf + f0;
+        double val[MAXN];
+#pragma unroll
+        for (int i = 0; i < MAXN; ++i) val[i] = (i < n && ft[i] != FT_INTERNAL) ? bv[i] : 0.0;
+#pragma unroll
+        for (int i = 0; i < MAXN; ++i) {
+            if (i >= n) continue;
+            double rr;
+            if (ft[i] == FT_DIRICHLET) {
+                rr = S[pidx(i, i, MAXN)] * val[i];
+            } else {
+                double lr = gfl[i] - (ft[i] == FT_NEUMANN ? val[i] : 0.0);
+                lr = 1.0 * lr + (gk * rL) * F[i];
+                rr = lr;
+#pragma unroll
+                for (int j = 0; j < MAXN; ++j) {
+                    if (j < n && (ft[j] == FT_DIRICHLET || rd[j]))
+                        rr -= ((i <= j) ? S[pidx(i, j, MAXN)] : S[pidx(j, i, MAXN)]) * val[j];
+                }
+            }
+            C.rr[(size_t)k * T.nhf + f0 + i] = rr;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Face-row gather: fills SELL values, right-hand sides and the Jacobi diagonal.
+template <int NRHS>
+__global__ __launch_bounds__(256) void k_fill(DevTopo T, DevCellSys C,
+                                              const uint8_t *__restrict__ ftype,
+                                              const double *__restrict__ bcval,
+                                              const int *__restrict__ inst_reg, int dguess,
+                                              double *__restrict__ val64,
+                                              float *__restrict__ val32,
+                                              double *__restrict__ b,
+                                              double *__restrict__ dinv64,
+                                              float *__restrict__ dinv32,
+                                              double *__restrict__ x0) {
+    const int f = blockIdx.x * 256 + threadIdx.x;
+    if (f >= T.nfp) return;
+    const int slice = f >> 6, lane = f & 63;
+    const long long base = T.slice_ptr[slice] + lane;
+    const int w = (int)((T.slice_ptr[slice + 1] - T.slice_ptr[slice]) >> 6);
+    for (int e = 0; e < w; ++e) val64[base + 64LL * e] = 0.0;
+    const int h0 = T.dof_hf[f];
+    const long long dpos = base + 64LL * T.dslot[f];
+    double bk[NRHS], xk[NRHS];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) bk[k] = xk[k] = 0.0;
+    double diag;
+    if (h0 < 0) {  // padding row: identity
+        val64[dpos] = 1.0;
+        diag = 1.0;
+    } else if (ftype[h0] == FT_DIRICHLET) {
+        const int c = T.hf_cell[h0], f0 = T.facepos[c], n = T.facepos[c + 1] - f0, r = h0 - f0;
+        const double a = C.schur[T.pk_pos[c] + pidx(r, r, n)];
+        val64[dpos] = a;
+        diag = a;
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) {
+            bk[k] = C.rr[(size_t)k * T.nhf + h0];
+            xk[k] = dguess ? bk[k] / a : 0.0;
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int h = T.row_hf[2 * (size_t)f + t];
+            if (h < 0) continue;
+            const int c = T.hf_cell[h], f0 = T.facepos[c], n = T.facepos[c + 1] - f0, r = h - f0;
+            const long long pk = T.pk_pos[c];
+            const uint8_t *sm = T.slotmap + T.sq_pos[c] + (long long)r * n;
+            for (int j = 0; j < n; ++j) {
+                if (ftype[f0 + j] == FT_DIRICHLET) continue;
+                const double v = C.schur[pk + (r <= j ? pidx(r, j, n) : pidx(j, r, n))];
+                val64[base + 64LL * sm[j]] += v;
+            }
+#pragma unroll
+            for (int k = 0; k < NRHS; ++k) bk[k] += C.rr[(size_t)k * T.nhf + h];
+        }
+        const int fp = T.ptn[f];
+        if (fp >= 0) {  // periodic constraint a(x_f - x_partner) = a_f pd_f - a_p pd_p (:1838-1869)
+            const int hp = T.dof_hf[fp];
+            const int c0 = T.hf_cell[h0], f00 = T.facepos[c0], n0 = T.facepos[c0 + 1] - f00;
+            const int c1 = T.hf_cell[hp], f01 = T.facepos[c1], n1 = T.facepos[c1 + 1] - f01;
+            const double a0 = C.schur[T.pk_pos[c0] + pidx(h0 - f00, h0 - f00, n0)];
+            const double a1 = C.schur[T.pk_pos[c1] + pidx(hp - f01, hp - f01, n1)];
+            val64[dpos] += a0 + a1;
+            val64[base + 64LL * T.pslot[f]] -= a0 + a1;
+#pragma unroll
+            for (int k = 0; k < NRHS; ++k)
+                bk[k] += a0 * bcval[(size_t)k * T.nhf + h0] - a1 * bcval[(size_t)k * T.nhf + hp];
+        }
+        const int s = T.block_seg[f >> 8];
+        if (inst_reg[s] && f == T.inst_rowbase[s]) val64[dpos] *= 2.0;  // S[0][0] *= 2 (:1398-1400)
+        diag = val64[dpos];
+    }
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) {
+        b[(size_t)f * NRHS + k] = bk[k];
+        x0[(size_t)f * NRHS + k] = xk[k];
+    }
+    const double di = 1.0 / diag;
+    if (dinv64) dinv64[f] = di;
+    if (val32) {
+        for (int e = 0; e < w; ++e) val32[base + 64LL * e] = (float)val64[base + 64LL * e];
+        dinv32[f] = (float)di;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// PCG iteration kernels.
+template <typename MT, typename VT, int NRHS>
+__global__ __launch_bounds__(256) void k_spmv_pq(DevTopo T, const MT *__restrict__ val,
+                                                 const VT *__restrict__ p, VT *__restrict__ q,
+                                                 DevRed R, DevScal S) {
+    const int b = xcd_block(gridDim.x);
+    const int s = T.block_seg[b];
+    int act[NRHS];
+    if (!load_act<NRHS>(S.act, s, act)) return;
+    const int f = b * 256 + threadIdx.x;
+    const int slice = f >> 6, lane = f & 63;
+    const long long s0 = T.slice_ptr[slice], s1 = T.slice_ptr[slice + 1];
+    VT acc[NRHS];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) acc[k] = VT(0);
+    for (long long o = s0 + lane; o < s1; o += 64) {
+        const int cidx = T.col[o];
+        const VT a = (VT)val[o];
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) acc[k] += a * p[(size_t)cidx * NRHS + k];
+    }
+    double part[NRHS];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) {
+        q[(size_t)f * NRHS + k] = acc[k];
+        part[k] = (double)p[(size_t)f * NRHS + k] * (double)acc[k];
+    }
+    seg_reduce<NRHS>(part, b, s, R, [&](double *tot) {
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) {
+            const int i = s * NRHS + k;
+            if (!act[k]) continue;
+            double *sc = S.sc + (size_t)i * SC_N;
+            const double pq = tot[k];
+            sc[SC_PQ] = pq;
+            if (!(pq > 0.0)) {
+                sc[SC_ALPHA] = 0.0;
+                if (S.mixed) {          // FP32 breakdown: restart from the FP64 residual
+                    S.upd[i] = 2;
+                } else {
+                    S.act[i] = 0;
+                    atomicSub(&S.status[0], 1);
+                    atomicAdd(&S.status[1], 1);
+                }
+            } else {
+                sc[SC_ALPHA] = sc[SC_RZ] / pq;
+            }
+        }
+    });
+}
+
+template <typename VT, int NRHS>
+__global__ __launch_bounds__(256) void k_update_xr(DevTopo T, const VT *__restrict__ dinv,
+                                                   VT *__restrict__ x, VT *__restrict__ r,
+                                                   const VT *__restrict__ p,
+                                                   const VT *__restrict__ q, DevRed R,
+                                                   DevScal S) {
+    const int b = xcd_block(gridDim.x);
+    const int s = T.block_seg[b];
+    int act[NRHS];
+    if (!load_act<NRHS>(S.act, s, act)) return;
+    const int f = b * 256 + threadIdx.x;
+    const VT d = dinv[f];
+    double part[2 * NRHS];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) {
+        const size_t i = (size_t)f * NRHS + k;
+        const VT al = act[k] ? (VT)S.sc[(size_t)(s * NRHS + k) * SC_N + SC_ALPHA] : VT(0);
+        const VT xi = x[i] + al * p[i];
+        const VT ri = r[i] - al * q[i];
+        x[i] = xi;
+        r[i] = ri;
+        part[2 * k] = (double)ri * (double)(d * ri);
+        part[2 * k + 1] = (double)ri * (double)ri;
+    }
+    seg_reduce<2 * NRHS>(part, b, s, R, [&](double *tot) {
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) {
+            const int i = s * NRHS + k;
+            if (!act[k]) continue;
+            double *sc = S.sc + (size_t)i * SC_N;
+            S.iters[i] += 1;
+            const double rz = tot[2 * k], rr = tot[2 * k + 1];
+            sc[SC_RR] = rr;
+            if (S.mixed) {
+                // FP32 recursion: ask for the FP64 true residual once the recursive
+                // one has dropped by replace_factor (or reached the tolerance).
+                if (S.upd[i] == 0 && (rr <= sc[SC_THR] || rr <= S.rf2 * sc[SC_RRL])) S.upd[i] = 1;
+                sc[SC_BETA] = rz / sc[SC_RZ];
+                sc[SC_RZ] = rz;
+            } else if (rr <= sc[SC_THR]) {
+                S.act[i] = 0;
+                atomicSub(&S.status[0], 1);
+            } else {
+                sc[SC_BETA] = rz / sc[SC_RZ];
+                sc[SC_RZ] = rz;
+            }
+        }
+    });
+}
+
+template <typename VT, int NRHS>
+__global__ __launch_bounds__(256) void k_update_p(DevTopo T, const VT *__restrict__ dinv,
+                                                  const VT *__restrict__ r,
+                                                  VT *__restrict__ p, VT *__restrict__ x,
+                                                  double *__restrict__ x64, DevScal S) {
+    const int b = xcd_block(gridDim.x);
+    const int s = T.block_seg[b];
+    int act[NRHS];
+    if (!load_act<NRHS>(S.act, s, act)) return;
+    const int f = b * 256 + threadIdx.x;
+    const VT d = dinv[f];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) {
+        if (!act[k]) continue;
+        const size_t i = (size_t)f * NRHS + k;
+        const int sys = s * NRHS + k;
+        const VT be = (VT)S.sc[(size_t)sys * SC_N + SC_BETA];
+        p[i] = d * r[i] + be * p[i];
+        if (x64 && S.upd[sys]) {  // fold the low-precision correction into x64
+            x64[i] += (double)x[i];
+            x[i] = VT(0);
+        }
+    }
+}
+
+// r = b - A x (FP64), p = dinv r; rz = r.dinv.r, rr0 = r.r; activate systems.
+template <int NRHS>
+__global__ __launch_bounds__(256) void k_start64(DevTopo T, const double *__restrict__ val,
+                                                 const double *__restrict__ dinv,
+                                                 const double *__restrict__ bb,
+                                                 const double *__restrict__ x,
+                                                 double *__restrict__ r, double *__restrict__ p,
+                                                 double tol2, DevRed R, DevScal S) {
+    const int b = xcd_block(gridDim.x);
+    const int s = T.block_seg[b];
+    const int f = b * 256 + threadIdx.x;
+    const int slice = f >> 6, lane = f & 63;
+    const long long s0 = T.slice_ptr[slice], s1 = T.slice_ptr[slice + 1];
+    double acc[NRHS];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) acc[k] = 0.0;
+    for (long long o = s0 + lane; o < s1; o += 64) {
+        const int cidx = T.col[o];
+        const double a = val[o];
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) acc[k] += a * x[(size_t)cidx * NRHS + k];
+    }
+    const double d = dinv[f];
+    double part[2 * NRHS];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) {
+        const size_t i = (size_t)f * NRHS + k;
+        const double ri = bb[i] - acc[k];
+        r[i] = ri;
+        p[i] = d * ri;
+        part[2 * k] = ri * (d * ri);
+        part[2 * k + 1] = ri * ri;
+    }
+    seg_reduce<2 * NRHS>(part, b, s, R, [&](double *tot) {
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) {
+            const int i = s * NRHS + k;
+            double *sc = S.sc + (size_t)i * SC_N;
+            const double rr0 = tot[2 * k + 1];
+            sc[SC_RZ] = tot[2 * k];
+            sc[SC_RR] = rr0;
+            sc[SC_RR0] = rr0;
+            sc[SC_RRT] = rr0;
+            sc[SC_THR] = tol2 * rr0;
+            S.iters[i] = 0;
+            const int a = (rr0 >= 1e-60) ? 1 : 0;  // dune CGSolver: def0 < 1e-30 -> done
+            S.act[i] = a;
+            if (a) atomicAdd(&S.status[0], 1);
+        }
+    });
+}
+
+// FP64 residual replacement for the FP32 iteration, fused into one pass.
+// For systems with upd != 0 (all systems when init): r64 = b - A64 x64 row by
+// row, r32 = (float)r64, and on restart (upd == 2, or init) p32 = dinv r32.
+// The last workgroup of the instance decides: |r64| <= tol |r0| ends the system,
+// otherwise it continues from the replaced residual.
+template <int NRHS>
+__global__ __launch_bounds__(256) void k_reliable(DevTopo T, const double *__restrict__ val,
+                                                  const double *__restrict__ bb,
+                                                  const double *__restrict__ x,
+                                                  const float *__restrict__ dinv,
+                                                  float *__restrict__ r32, float *__restrict__ p32,
+                                                  int init, double tol2, DevRed R, DevScal S) {
+    const int b = xcd_block(gridDim.x);
+    const int s = T.block_seg[b];
+    int u[NRHS];
+    if (init) {
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) u[k] = 2;
+    } else if (!load_act<NRHS>(S.upd, s, u)) {
+        return;
+    }
+    const int f = b * 256 + threadIdx.x;
+    const int slice = f >> 6, lane = f & 63;
+    const long long s0 = T.slice_ptr[slice], s1 = T.slice_ptr[slice + 1];
+    double acc[NRHS];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) acc[k] = 0.0;
+    for (long long o = s0 + lane; o < s1; o += 64) {
+        const int cidx = T.col[o];
+        const double a = val[o];
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) acc[k] += a * x[(size_t)cidx * NRHS + k];
+    }
+    const float d = dinv[f];
+    double part[3 * NRHS];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) {
+        const size_t i = (size_t)f * NRHS + k;
+        part[3 * k] = part[3 * k + 1] = part[3 * k + 2] = 0.0;
+        if (!u[k]) continue;
+        const double ri = bb[i] - acc[k];
+        const float rf = (float)ri;
+        r32[i] = rf;
+        if (u[k] == 2) p32[i] = d * rf;
+        part[3 * k] = ri * ri;
+        part[3 * k + 1] = (double)rf * (double)(d * rf);
+        part[3 * k + 2] = (double)rf * (double)rf;
+    }
+    seg_reduce<3 * NRHS>(part, b, s, R, [&](double *tot) {
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) {
+            if (!u[k]) continue;
+            const int i = s * NRHS + k;
+            double *sc = S.sc + (size_t)i * SC_N;
+            const double rrt = tot[3 * k];
+            if (init) {
+                sc[SC_RR0] = rrt;
+                sc[SC_THR] = tol2 * rrt;
+                S.iters[i] = 0;
+            }
+            sc[SC_RRT] = rrt;
+            sc[SC_RRL] = rrt;
+            sc[SC_RZ] = tot[3 * k + 1];
+            sc[SC_RR] = tot[3 * k + 2];
+            S.upd[i] = 0;
+            if (!init) atomicAdd(&S.status[2], 1);
+            const int was = init ? 0 : S.act[i];
+            const int now = (rrt <= sc[SC_THR] || sc[SC_RR0] < 1e-60) ? 0 : 1;
+            S.act[i] = now;
+            if (now != was) atomicAdd(&S.status[0], now - was);
+        }
+    });
+}
+
+// x64 += x32, x32 = 0 for every system (final fold after the iteration).
+template <int NRHS>
+__global__ __launch_bounds__(256) void k_merge_all(DevTopo T, double *__restrict__ x64,
+                                                   float *__restrict__ x32) {
+    const int f = blockIdx.x * 256 + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) {
+        const size_t i = (size_t)f * NRHS + k;
+        x64[i] += (double)x32[i];
+        x32[i] = 0.0f;
+    }
+}
+
+// |b - A x|^2 per system into sc[RRT] (report only, no vector writes).
+template <int NRHS>
+__global__ __launch_bounds__(256) void k_resnorm(DevTopo T, const double *__restrict__ val,
+                                                 const double *__restrict__ bb,
+                                                 const double *__restrict__ x, DevRed R,
+                                                 DevScal S) {
+    const int b = xcd_block(gridDim.x);
+    const int s = T.block_seg[b];
+    const int f = b * 256 + threadIdx.x;
+    const int slice = f >> 6, lane = f & 63;
+    const long long s0 = T.slice_ptr[slice], s1 = T.slice_ptr[slice + 1];
+    double acc[NRHS];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) acc[k] = 0.0;
+    for (long long o = s0 + lane; o < s1; o += 64) {
+        const int cidx = T.col[o];
+        const double a = val[o];
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) acc[k] += a * x[(size_t)cidx * NRHS + k];
+    }
+    double part[NRHS];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) {
+        const double ri = bb[(size_t)f * NRHS + k] - acc[k];
+        part[k] = ri * ri;
+    }
+    seg_reduce<NRHS>(part, b, s, R, [&](double *tot) {
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) S.sc[(size_t)(s * NRHS + k) * SC_N + SC_RRT] = tot[k];
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Recovery, per cell: p = (g + F.pi)/L;  v = totmob B^-1 (p - pi) + rhomob gflux.
+template <int MAXN, int NRHS>
+__global__ __launch_bounds__(64) void k_recover(DevTopo T, const double *__restrict__ binv,
+                                                const double *__restrict__ gfs,
+                                                const double *__restrict__ totmob,
+                                                const double *__restrict__ rhomob,
+                                                DevCellSys C, const double *__restrict__ x,
+                                                double *__restrict__ press,
+                                                double *__restrict__ outflux) {
+    constexpr int NP = MAXN * (MAXN + 1) / 2;
+    const int c = blockIdx.x * 64 + threadIdx.x;
+    if (c >= T.nc) return;
+    const int f0 = T.facepos[c], n = T.facepos[c + 1] - f0;
+    const double lam = totmob[c], rl = rhomob[c];
+    const long long pk = T.pk_pos[c];
+    double B[NP];
+#pragma unroll
+    for (int i = 0; i < MAXN; ++i)
+#pragma unroll
+        for (int j = i; j < MAXN; ++j) B[pidx(i, j, MAXN)] = (j < n) ? binv[pk + pidx(i, j, n)] : 0.0;
+    double F[MAXN];
+    int dof[MAXN];
+#pragma unroll
+    for (int i = 0; i < MAXN; ++i) {
+        F[i] = (i < n) ? C.F[f0 + i] : 0.0;
+        dof[i] = (i < n) ? T.hf_dof[f0 + i] : 0;
+    }
+    const double L = C.L[c];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) {
+        double pi[MAXN], s = 0.0;
+#pragma unroll
+        for (int i = 0; i < MAXN; ++i) pi[i] = (i < n) ? x[(size_t)dof[i] * NRHS + k] : 0.0;
+#pragma unroll
+        for (int i = 0; i < MAXN; ++i) s += F[i] * pi[i];
+        const double p = (C.g[(size_t)k * T.nc + c] + s) / L;
+        press[(size_t)k * T.nc + c] = p;
+#pragma unroll
+        for (int i = 0; i < MAXN; ++i) pi[i] = (i < n) ? p - pi[i] : 0.0;
+#pragma unroll
+        for (int i = 0; i < MAXN; ++i) {
+            if (i >= n) continue;
+            double v = 0.0;
+#pragma unroll
+            for (int j = 0; j < MAXN; ++j)
+                v += lam * ((i <= j) ? B[pidx(i, j, MAXN)] : B[pidx(j, i, MAXN)]) * pi[j];
+            outflux[(size_t)k * T.nhf + f0 + i] = gfs[f0 + i] * rl + v;
+        }
+    }
+}
+
+// Twin half-face averaging: v <- (v - v_twin)/2 for internal and periodic
+// pairs; the largest change is folded into *maxbits (non-negative doubles
+// order like their bit patterns).
+__global__ __launch_bounds__(256) void k_post(DevTopo T, int nrhs, const double *__restrict__ vin,
+                                              double *__restrict__ vout,
+                                              unsigned long long *__restrict__ maxbits) {
+    const long long gid = (long long)blockIdx.x * 256 + threadIdx.x;
+    const long long tot = (long long)nrhs * T.nhf;
+    double d = 0.0;
+    if (gid < tot) {
+        const int k = (int)(gid / T.nhf), h = (int)(gid % T.nhf);
+        const int tw = T.hf_twin[h];
+        const double v = vin[gid];
+        double nv = v;
+        if (tw >= 0) nv = 0.5 * (v - vin[(size_t)k * T.nhf + tw]);
+        vout[gid] = nv;
+        d = fabs(v - nv);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) d = fmax(d, __shfl_down(d, off, 64));
+    if ((threadIdx.x & 63) == 0 && d > 0.0)
+        atomicMax(maxbits, (unsigned long long)__double_as_longlong(d));
+}
+
+}  // namespace
+
+// ===========================================================================
+// Launchers.
+template <int V>
+using IC = std::integral_constant<int, V>;
+
+template <class Fn>
+static hipError_t with_nrhs(int nrhs, Fn fn) {
+    switch (nrhs) {
+        case 1: fn(IC<1>{}); break;
+        case 2: fn(IC<2>{}); break;
+        case 3: fn(IC<3>{}); break;
+        case 4: fn(IC<4>{}); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <class Fn>
+static hipError_t with_maxn(int maxn, Fn fn) {
+    if (maxn <= 6) fn(IC<6>{});
+    else if (maxn <= 8) fn(IC<8>{});
+    else if (maxn <= 10) fn(IC<10>{});
+    else if (maxn <= 16) fn(IC<16>{});
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+static inline unsigned cell_blocks(int nc) { return (unsigned)((nc + 63) / 64); }
+
+hipError_t launch_static_ip(hipStream_t st, const DevTopo &T, const DevGeom &G, int maxn,
+                            const double *perm, const double *grav, double *binv_pk,
+                            double *gflux) {
+    if (T.nc == 0) return hipSuccess;
+    const double gx = grav[0], gy = grav[1], gz = grav[2];
+    return with_maxn(maxn, [&](auto mn) {
+        constexpr int MN = decltype(mn)::value;
+        k_static_ip<MN><<<cell_blocks(T.nc), 64, 0, st>>>(T, G, perm, gx, gy, gz, binv_pk, gflux);
+    });
+}
+
+hipError_t launch_assemble(hipStream_t st, const DevTopo &T, int maxn, int nrhs,
+                           const double *binv_pk, const double *gflux, const double *totmob,
+                           const double *rhomob, const double *src, const uint8_t *ftype,
+                           const double *bcval, DevCellSys C) {
+    if (T.nc == 0) return hipSuccess;
+    hipError_t e = hipSuccess;
+    hipError_t e2 = with_maxn(maxn, [&](auto mn) {
+        constexpr int MN = decltype(mn)::value;
+        e = with_nrhs(nrhs, [&](auto nr) {
+            constexpr int NR = decltype(nr)::value;
+            k_assemble<MN, NR><<<cell_blocks(T.nc), 64, 0, st>>>(T, binv_pk, gflux, totmob, rhomob,
+                                                                 src, ftype, bcval, C);
+        });
+    });
+    return e != hipSuccess ? e : e2;
+}
+
+hipError_t launch_fill(hipStream_t st, const DevTopo &T, int nrhs, const DevCellSys &C,
+                       const uint8_t *ftype, const double *bcval, const int *inst_reg,
+                       int dguess, double *val64, float *val32, double *b, double *dinv64,
+                       float *dinv32, double *x0) {
+    return with_nrhs(nrhs, [&](auto nr) {
+        constexpr int NR = decltype(nr)::value;
+        k_fill<NR><<<T.nblk, 256, 0, st>>>(T, C, ftype, bcval, inst_reg, dguess, val64, val32, b,
+                                           dinv64, dinv32, x0);
+    });
+}
+
+template <typename MT, typename VT>
+hipError_t launch_pcg_iteration(hipStream_t st, const DevTopo &T, int nrhs, const MT *val,
+                                const VT *dinv, VT *x, VT *r, VT *p, VT *q, DevRed R,
+                                DevScal S, const double *val64, const double *b, double *x64,
+                                double tol2) {
+    return with_nrhs(nrhs, [&](auto nr) {
+        constexpr int NR = decltype(nr)::value;
+        k_spmv_pq<MT, VT, NR><<<T.nblk, 256, 0, st>>>(T, val, p, q, R, S);
+        k_update_xr<VT, NR><<<T.nblk, 256, 0, st>>>(T, dinv, x, r, p, q, R, S);
+        if constexpr (std::is_same<VT, float>::value) {
+            k_update_p<VT, NR><<<T.nblk, 256, 0, st>>>(T, dinv, r, p, x, x64, S);
+            k_reliable<NR><<<T.nblk, 256, 0, st>>>(T, val64, b, x64, dinv, r, p, 0, tol2, R, S);
+        } else {
+            k_update_p<VT, NR><<<T.nblk, 256, 0, st>>>(T, dinv, r, p, x, nullptr, S);
+        }
+    });
+}
+
+template hipError_t launch_pcg_iteration<double, double>(hipStream_t, const DevTopo &, int,
+                                                         const double *, const double *,
+                                                         double *, double *, double *,
+                                                         double *, DevRed, DevScal,
+                                                         const double *, const double *,
+                                                         double *, double);
+template hipError_t launch_pcg_iteration<float, float>(hipStream_t, const DevTopo &, int,
+                                                       const float *, const float *, float *,
+                                                       float *, float *, float *, DevRed,
+                                                       DevScal, const double *, const double *,
+                                                       double *, double);
+
+hipError_t launch_pcg_start64(hipStream_t st, const DevTopo &T, int nrhs, const double *val,
+                              const double *dinv, const double *b, const double *x, double *r,
+                              double *p, double tol2, DevRed R, DevScal S) {
+    return with_nrhs(nrhs, [&](auto nr) {
+        constexpr int NR = decltype(nr)::value;
+        k_start64<NR><<<T.nblk, 256, 0, st>>>(T, val, dinv, b, x, r, p, tol2, R, S);
+    });
+}
+
+hipError_t launch_reliable(hipStream_t st, const DevTopo &T, int nrhs, const double *val64,
+                           const double *b, const double *x64, const float *dinv32, float *r32,
+                           float *p32, int init, double tol2, DevRed R, DevScal S) {
+    return with_nrhs(nrhs, [&](auto nr) {
+        constexpr int NR = decltype(nr)::value;
+        k_reliable<NR><<<T.nblk, 256, 0, st>>>(T, val64, b, x64, dinv32, r32, p32, init, tol2, R, S);
+    });
+}
+
+hipError_t launch_merge_all(hipStream_t st, const DevTopo &T, int nrhs, double *x64, float *x32) {
+    return with_nrhs(nrhs, [&](auto nr) {
+        constexpr int NR = decltype(nr)::value;
+        k_merge_all<NR><<<T.nblk, 256, 0, st>>>(T, x64, x32);
+    });
+}
+
+hipError_t launch_resnorm(hipStream_t st, const DevTopo &T, int nrhs, const double *val64,
+                          const double *b, const double *x64, DevRed R, DevScal S) {
+    return with_nrhs(nrhs, [&](auto nr) {
+        constexpr int NR = decltype(nr)::value;
+        k_resnorm<NR><<<T.nblk, 256, 0, st>>>(T, val64, b, x64, R, S);
+    });
+}
+
+hipError_t launch_recover(hipStream_t st, const DevTopo &T, int maxn, int nrhs,
+                          const double *binv_pk, const double *gflux, const double *totmob,
+                          const double *rhomob, const DevCellSys &C, const double *x64,
+                          double *press, double *outflux) {
+    if (T.nc == 0) return hipSuccess;
+    hipError_t e = hipSuccess;
+    hipError_t e2 = with_maxn(maxn, [&](auto mn) {
+        constexpr int MN = decltype(mn)::value;
+        e = with_nrhs(nrhs, [&](auto nr) {
+            constexpr int NR = decltype(nr)::value;
+            k_recover<MN, NR><<<cell_blocks(T.nc), 64, 0, st>>>(T, binv_pk, gflux, totmob, rhomob, C,
+                                                                x64, press, outflux);
+        });
+    });
+    return e != hipSuccess ? e : e2;
+}
+
+hipError_t launch_post_process(hipStream_t st, const DevTopo &T, int nrhs, const double *vin,
+                               double *vout, unsigned long long *maxbits) {
+    const long long tot = (long long)nrhs * T.nhf;
+    if (tot == 0) return hipSuccess;
+    k_post<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(T, nrhs, vin, vout, maxbits);
+    return hipGetLastError();
+}
+
+}  // namespace mhs

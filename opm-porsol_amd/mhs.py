@@ -1,0 +1,310 @@
+"""ctypes binding of libmhs.so (include/mimetic_hybrid.h).
+
+``HybridSolver`` mirrors the public methods of opm-porsol's
+IncompFlowSolverHybrid (opm/porsol/mimetic/IncompFlowSolverHybrid.hpp):
+init :476, clear :498, initSystemStructure :534, computeInnerProducts :564,
+solve :657, postProcessFluxes :754, getSolution :828.  Errors keep the
+reference's behaviour: non-convergence and an unknown linsolver_type raise
+RuntimeError (:1416-1419, :698-700).
+
+There is no CPU fallback: if libmhs.so is missing or no GPU is present the
+calls fail loudly.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libmhs.so")
+
+MHS_OK, MHS_ERR_INVALID, MHS_ERR_STATE, MHS_ERR_NOT_CONVERGED, MHS_ERR_HIP, MHS_ERR_ALLOC, \
+    MHS_ERR_BREAKDOWN = range(7)
+NEUMANN, DIRICHLET, PERIODIC = 0, 1, 2
+FP64, MIXED = 0, 1
+
+EXPORTS = (
+    "mhs_default_options", "mhs_create", "mhs_destroy", "mhs_last_error", "mhs_clear",
+    "mhs_set_grid", "mhs_init_system_structure", "mhs_compute_inner_products",
+    "mhs_assemble_static", "mhs_get_inner_products", "mhs_solve", "mhs_solve_device", "mhs_get_solution",
+    "mhs_solution_device", "mhs_post_process_fluxes", "mhs_get_dims", "mhs_get_cell_faces",
+    "mhs_get_face_rows", "mhs_get_system",
+)
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("tol", ctypes.c_double), ("maxit", ctypes.c_int),
+                ("linsolver_type", ctypes.c_int), ("verbosity", ctypes.c_int),
+                ("same_matrix", ctypes.c_int), ("prolongate_factor", ctypes.c_double),
+                ("smooth_steps", ctypes.c_int), ("precision", ctypes.c_int),
+                ("nrhs", ctypes.c_int), ("chunk", ctypes.c_int),
+                ("replace_factor", ctypes.c_double), ("use_graph", ctypes.c_int)]
+
+
+class Report(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("replacements", ctypes.c_int),
+                ("converged", ctypes.c_int), ("reduction", ctypes.c_double),
+                ("assemble_ms", ctypes.c_double), ("solve_ms", ctypes.c_double),
+                ("recover_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Dims(ctypes.Structure):
+    _fields_ = [("num_cells", ctypes.c_int), ("num_half_faces", ctypes.c_int),
+                ("num_faces", ctypes.c_int), ("num_internal_faces", ctypes.c_int),
+                ("max_ncf", ctypes.c_int), ("num_instances", ctypes.c_int),
+                ("num_rows_padded", ctypes.c_int), ("nnz", ctypes.c_longlong)]
+
+
+_LIB = None
+_P = ctypes.c_void_p
+_I = ctypes.POINTER(ctypes.c_int)
+_D = ctypes.POINTER(ctypes.c_double)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIBPATH):
+            raise ImportError(f"{LIBPATH} is missing: build it with __graft_entry__.build()")
+        L = ctypes.CDLL(LIBPATH)
+        L.mhs_default_options.argtypes = [ctypes.POINTER(Options)]
+        L.mhs_default_options.restype = None
+        L.mhs_create.argtypes = [ctypes.POINTER(_P), ctypes.c_int, _P]
+        L.mhs_destroy.argtypes = [_P]
+        L.mhs_last_error.argtypes = [_P]
+        L.mhs_last_error.restype = ctypes.c_char_p
+        L.mhs_clear.argtypes = [_P]
+        L.mhs_set_grid.argtypes = [_P, ctypes.c_int, _I, ctypes.c_int, _I, _I, _D, _D, _D, _D, _D]
+        L.mhs_init_system_structure.argtypes = [_P, _I, _I]
+        L.mhs_compute_inner_products.argtypes = [_P, _D, _D]
+        L.mhs_assemble_static.argtypes = [_P, _D, _D]
+        L.mhs_get_inner_products.argtypes = [_P, _D, _D]
+        L.mhs_solve.argtypes = [_P, _D, _D, _D, _I, _D, ctypes.POINTER(Options),
+                                ctypes.POINTER(Report)]
+        L.mhs_solve_device.argtypes = [_P, _P, _P, _P, _I, _P, ctypes.POINTER(Options),
+                                       ctypes.POINTER(Report)]
+        L.mhs_get_solution.argtypes = [_P, _D, _D, _D]
+        L.mhs_solution_device.argtypes = [_P, ctypes.POINTER(_P), ctypes.POINTER(_P),
+                                          ctypes.POINTER(_P)]
+        L.mhs_post_process_fluxes.argtypes = [_P, _D]
+        L.mhs_get_dims.argtypes = [_P, ctypes.POINTER(Dims)]
+        L.mhs_get_cell_faces.argtypes = [_P, _I, _I]
+        L.mhs_get_face_rows.argtypes = [_P, _I]
+        L.mhs_get_system.argtypes = [_P, ctypes.c_int, _I, _I, _D, _D,
+                                     ctypes.POINTER(ctypes.c_longlong)]
+        for name in EXPORTS:
+            if name not in ("mhs_default_options", "mhs_last_error"):
+                getattr(L, name).restype = ctypes.c_int
+        _LIB = L
+    return _LIB
+
+
+class MhsError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(message)
+        self.status = status
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_I)
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_D)
+
+
+class HybridSolver:
+    """Mimetic hybrid face-pressure solver on one GPU (a handle over the C-ABI)."""
+
+    def __init__(self, device: int = 0, stream: int | None = None):
+        self._L = lib()
+        h = _P()
+        rc = self._L.mhs_create(ctypes.byref(h), int(device), _P(stream or 0))
+        if rc != MHS_OK:
+            raise MhsError(rc, "mhs_create failed (no usable GPU?)")
+        self._h = h
+        self.nrhs = 1
+        self.grid = None
+        self.report = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                self._L.mhs_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != MHS_OK:
+            msg = self._L.mhs_last_error(self._h).decode()
+            raise MhsError(rc, msg)
+
+    # --- set-up -------------------------------------------------------------------
+    def clear(self):
+        self._check(self._L.mhs_clear(self._h))
+        self.grid = None
+
+    def set_grid(self, g, inst_cellpos=None):
+        keep = []
+        fp, pfp = _i(g.facepos); nb, pnb = _i(g.nb)
+        cv, pcv = _d(g.cell_volume); cc, pcc = _d(g.cell_centroid)
+        fa, pfa = _d(g.face_area); fc, pfc = _d(g.face_centroid); fn, pfn = _d(g.face_normal)
+        keep += [fp, nb, cv, cc, fa, fc, fn]
+        if inst_cellpos is None:
+            ninst, pic = 1, None
+        else:
+            ic, pic = _i(inst_cellpos)
+            keep.append(ic)
+            ninst = len(ic) - 1
+        self._check(self._L.mhs_set_grid(self._h, ninst, pic, g.num_cells, pfp, pnb, pcv, pcc, pfa,
+                                         pfc, pfn))
+        self.grid = g
+        self.ninst = ninst
+
+    def initSystemStructure(self, bc_type=None, periodic_partner=None):
+        if bc_type is None or periodic_partner is None:
+            self._check(self._L.mhs_init_system_structure(self._h, None, None))
+        else:
+            bt, pbt = _i(bc_type); pp, ppp = _i(periodic_partner)
+            self._check(self._L.mhs_init_system_structure(self._h, pbt, ppp))
+
+    def computeInnerProducts(self, perm, gravity=(0.0, 0.0, 0.0)):
+        K, pK = _d(np.asarray(perm).reshape(-1))
+        gr, pg = _d(gravity)
+        self._check(self._L.mhs_compute_inner_products(self._h, pK, pg))
+
+    assembleStatic = computeInnerProducts
+
+    def inner_products(self):
+        """(binv, gflux): per-cell B^-1 column-major, concatenated; N K g per half-face."""
+        g = self.grid
+        n = g.ncf().astype(np.int64)
+        binv = np.empty(int(np.sum(n * n)))
+        gfl = np.empty(g.num_half_faces)
+        self._check(self._L.mhs_get_inner_products(self._h, binv.ctypes.data_as(_D),
+                                                   gfl.ctypes.data_as(_D)))
+        return binv, gfl
+
+    def init(self, g, perm, gravity=(0.0, 0.0, 0.0), bc_type=None, periodic_partner=None,
+             inst_cellpos=None):
+        """IncompFlowSolverHybrid::init (:476-488)."""
+        self.set_grid(g, inst_cellpos)
+        if g.num_cells > 0:
+            self.initSystemStructure(bc_type, periodic_partner)
+            self.computeInnerProducts(perm, gravity)
+
+    # --- solve ----------------------------------------------------------------------
+    def options(self, tol=1e-8, linsolver_verbosity=0, linsolver_type=1, same_matrix=False,
+                linsolver_maxit=0, prolongate_factor=1.6, smooth_steps=1, precision=FP64,
+                nrhs=1, chunk=32, replace_factor=0.1, use_graph=True):
+        o = Options()
+        self._L.mhs_default_options(ctypes.byref(o))
+        o.tol, o.maxit, o.linsolver_type, o.verbosity = tol, linsolver_maxit, linsolver_type, \
+            linsolver_verbosity
+        o.same_matrix, o.prolongate_factor, o.smooth_steps = int(same_matrix), \
+            prolongate_factor, smooth_steps
+        o.precision = {"fp64": FP64, "mixed": MIXED}.get(precision, precision)
+        o.nrhs, o.chunk, o.replace_factor, o.use_graph = nrhs, chunk, replace_factor, \
+            int(use_graph)
+        return o
+
+    def solve(self, totmob, rhomob, src, bc_type, bc_value, residual_tolerance=1e-8,
+              linsolver_verbosity=0, linsolver_type=1, same_matrix=False, linsolver_maxit=0,
+              prolongate_factor=1.6, smooth_steps=1, precision="fp64", chunk=32,
+              replace_factor=0.1, use_graph=True, raise_on_failure=True):
+        """IncompFlowSolverHybrid::solve (:657-703) with per-cell mobilities."""
+        g = self.grid
+        src = np.asarray(src, dtype=np.float64)
+        nrhs = 1 if src.ndim == 1 else src.shape[0]
+        bv = np.asarray(bc_value, dtype=np.float64).reshape(nrhs, g.num_half_faces)
+        tm, ptm = _d(totmob); rm, prm = _d(rhomob); sr, psr = _d(src.reshape(nrhs, -1))
+        bt, pbt = _i(bc_type); bvv, pbv = _d(bv)
+        o = self.options(residual_tolerance, linsolver_verbosity, linsolver_type, same_matrix,
+                         linsolver_maxit, prolongate_factor, smooth_steps, precision, nrhs, chunk,
+                         replace_factor, use_graph)
+        rep = Report()
+        rc = self._L.mhs_solve(self._h, ptm, prm, psr, pbt, pbv, ctypes.byref(o),
+                               ctypes.byref(rep))
+        self.report = rep
+        self.nrhs = nrhs
+        if rc == MHS_ERR_NOT_CONVERGED and not raise_on_failure:
+            return rep
+        self._check(rc)
+        return rep
+
+    def solve_device(self, totmob_ptr, rhomob_ptr, src_ptr, bc_type, bcval_ptr, nrhs=1, **kw):
+        """Same as solve with device pointers (e.g. torch tensor data_ptr())."""
+        bt, pbt = _i(bc_type)
+        o = self.options(nrhs=nrhs, **kw)
+        rep = Report()
+        rc = self._L.mhs_solve_device(self._h, _P(totmob_ptr), _P(rhomob_ptr), _P(src_ptr), pbt,
+                                      _P(bcval_ptr), ctypes.byref(o), ctypes.byref(rep))
+        self.report = rep
+        self.nrhs = nrhs
+        self._check(rc)
+        return rep
+
+    def getSolution(self):
+        """(cell_pressure[nrhs,nc], face_pressure[nrhs,nf], outflux[nrhs,nhf])."""
+        d = self.dims()
+        p = np.empty((self.nrhs, d.num_cells))
+        pi = np.empty((self.nrhs, d.num_faces))
+        v = np.empty((self.nrhs, d.num_half_faces))
+        self._check(self._L.mhs_get_solution(self._h, p.ctypes.data_as(_D), pi.ctypes.data_as(_D),
+                                             v.ctypes.data_as(_D)))
+        if self.nrhs == 1:
+            return p[0], pi[0], v[0]
+        return p, pi, v
+
+    def solution_device(self):
+        a, b, c = _P(), _P(), _P()
+        self._check(self._L.mhs_solution_device(self._h, ctypes.byref(a), ctypes.byref(b),
+                                                ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def postProcessFluxes(self) -> float:
+        mm = ctypes.c_double(0.0)
+        self._check(self._L.mhs_post_process_fluxes(self._h, ctypes.byref(mm)))
+        return mm.value
+
+    # --- introspection --------------------------------------------------------------------
+    def dims(self) -> Dims:
+        d = Dims()
+        self._check(self._L.mhs_get_dims(self._h, ctypes.byref(d)))
+        return d
+
+    def cell_faces(self):
+        d = self.dims()
+        hf = np.empty(d.num_half_faces, dtype=np.int32)
+        ifp = np.empty(d.num_instances + 1, dtype=np.int32)
+        self._check(self._L.mhs_get_cell_faces(self._h, hf.ctypes.data_as(_I),
+                                               ifp.ctypes.data_as(_I)))
+        return hf, ifp
+
+    def face_rows(self):
+        d = self.dims()
+        fr = np.empty(d.num_faces, dtype=np.int32)
+        self._check(self._L.mhs_get_face_rows(self._h, fr.ctypes.data_as(_I)))
+        return fr
+
+    def system(self, k=0):
+        """(row_ptr, col, val, rhs) of the assembled face system (printSystem :895-905)."""
+        d = self.dims()
+        nnz = ctypes.c_longlong(0)
+        self._check(self._L.mhs_get_system(self._h, k, None, None, None, None, ctypes.byref(nnz)))
+        rp = np.empty(d.num_faces + 1, dtype=np.int32)
+        col = np.empty(nnz.value, dtype=np.int32)
+        val = np.empty(nnz.value)
+        rhs = np.empty(d.num_faces)
+        self._check(self._L.mhs_get_system(self._h, k, rp.ctypes.data_as(_I),
+                                           col.ctypes.data_as(_I), val.ctypes.data_as(_D),
+                                           rhs.ctypes.data_as(_D), ctypes.byref(nnz)))
+        return rp, col, val, rhs

@@ -1,0 +1,50 @@
+"""Problem set-ups shared by the CPU and GPU tests (synthetic, seeded)."""
+import sys
+
+import numpy as np
+
+pkg = sys.modules["opm_porsol_amd"]
+grids, fields = pkg.grids, pkg.fields
+
+NEU, DIR, PER = 0, 1, 2
+
+
+def xdrop_bc(g, p_left=1.0, p_right=0.0):
+    """Dirichlet p_left on x- and p_right on x+, no-flow elsewhere."""
+    bt = np.zeros(g.num_half_faces, np.int32)
+    bv = np.zeros(g.num_half_faces)
+    bt[g.side == 0] = DIR
+    bv[g.side == 0] = p_left
+    bt[g.side == 1] = DIR
+    bv[g.side == 1] = p_right
+    return bt, bv
+
+
+def periodic_bc(g, dirs=(0, 1, 2), drop_dir=0, pdrop=1.0):
+    """Periodic in `dirs` with pressure drop pdrop along drop_dir (setupUpscalingConditions
+    bct=2: x- side pressureDifference +pdrop, x+ side -pdrop); Neumann 0 elsewhere."""
+    partner = grids.periodic_partners_cartesian(g, dirs)
+    bt = np.zeros(g.num_half_faces, np.int32)
+    bv = np.zeros(g.num_half_faces)
+    for d in dirs:
+        for sgn, side in ((1.0, 2 * d), (-1.0, 2 * d + 1)):
+            m = g.side == side
+            bt[m] = PER
+            bv[m] = sgn * pdrop if d == drop_dir else 0.0
+    return bt, bv, partner
+
+
+def unit_fluid(nc):
+    return np.ones(nc), np.zeros(nc)
+
+
+def case_c1(seed=1):
+    g = grids.cartesian(20, 20, 10)
+    perm = fields.perm_c1(g.num_cells, seed)
+    bt, bv = xdrop_bc(g)
+    tm, rm = unit_fluid(g.num_cells)
+    return g, perm, tm, rm, np.zeros(g.num_cells), bt, bv
+
+
+def rel_err(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / max(np.max(np.abs(b)), 1e-300))
