@@ -1,0 +1,215 @@
+#!/usr/bin/env python3
+"""Face-pressure solves/sec and HBM GB/s of the mimetic hybrid solve at 128x128x64.
+
+One step = one IncompFlowSolverHybrid::solve(): dynamic assembly, PCG to a
+relative residual of 1e-10 and flux / cell-pressure recovery, on the 128x128x64
+Cartesian grid with anisotropic Gaussian-filtered permeability (kz/kx = 0.1,
+ln kx ~ ln 100 mD + 1.5 z, correlation length 8 cells), Dirichlet pressure
+drop along x, no-flow elsewhere.  Inputs (mobilities, sources, boundary values)
+are resident in HBM before the timed region.
+
+Prints one JSON line.  --gpus N > 1 is rejected: this build targets one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+DIMS = (128, 128, 64)
+TOL = 1e-10
+
+
+def _problem(pkg, seed):
+    g = pkg.grids.cartesian(*DIMS)
+    perm = pkg.fields.perm_c2(DIMS, seed=seed)
+    nhf = g.num_half_faces
+    bt = np.zeros(nhf, np.int32)
+    bv = np.zeros(nhf)
+    bt[g.side == 0] = 1
+    bv[g.side == 0] = 1.0e5          # 1 bar on x-
+    bt[g.side == 1] = 1
+    bv[g.side == 1] = 0.0
+    return g, perm, bt, bv
+
+
+def _alg_bytes(g, nnz, nf, nrhs=1):
+    """Algorithmic HBM bytes per unit of work (each array counted once at its
+    stored precision, 4-byte indices)."""
+    nc, nhf = g.num_cells, g.num_half_faces
+    sq = int(np.sum(g.ncf().astype(np.int64) ** 2))
+    pk = int(np.sum(g.ncf().astype(np.int64) * (g.ncf() + 1) // 2))
+    return {
+        # FP32 SpMV + <p,Ap>: values + column indices, p gathered once, q written
+        "spmv32": nnz * (4 + 4) + nf * 4 * 2 * nrhs,
+        "spmv64": nnz * (8 + 4) + nf * 8 * 2 * nrhs,
+        # one Jacobi-PCG iteration: SpMV + 13 vector passes (p,q,x,r,dinv,...)
+        "iter32": nnz * (4 + 4) + 4 * (nf + 1) + 13 * nf * 4 * nrhs,
+        "iter64": nnz * (8 + 4) + 4 * (nf + 1) + 13 * nf * 8 * nrhs,
+        # FP64 residual replacement: FP64 operator + x, b read, r written
+        "refine64": nnz * (8 + 4) + 4 * nf * 8 * nrhs,
+        # assembly (static B^-1 read packed, values written in FP64 + FP32, rhs, F, L, g)
+        "assemble": pk * 8 + nc * 16 + nnz * (8 + 4) + nf * 8 + nhf * 8 + nc * 16 + sq,
+        # recovery: pi, F, L, g, B^-1 (packed), mobility, p, v
+        "recover": nf * 8 + nhf * 8 + nc * 16 + pk * 8 + nc * 8 + nc * 8 + nhf * 8,
+    }
+
+
+def _cpu_baseline(pkg, g, perm, bt, bv, iters_needed, sample_iters):
+    """FP64 C restatement (the oracle) on one host core: full assembly and
+    recovery plus `sample_iters` Jacobi-PCG iterations, extrapolated to the
+    iteration count the identical FP64 Jacobi-PCG needs (measured on the GPU)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    nc = g.num_cells
+    tm, rm, src = np.ones(nc), np.zeros(nc), np.zeros(nc)
+    hf, nint, nf, _ = oracle.enumerate_dof(g)
+    pp, _ = oracle.periodic_dof(g, hf, nf, bt, np.full(g.num_half_faces, -1, np.int32))
+    binv, pos, gfl = oracle.static_ip(g, perm)
+    t0 = time.perf_counter()
+    S = oracle.assemble(g, hf, nf, pp, binv, gfl, tm, rm, src, bt, bv)
+    t1 = time.perf_counter()
+    x, rc, its, red = oracle.cg(S, 1e-30, maxit=sample_iters, precond=1, dirichlet_guess=True)
+    t2 = time.perf_counter()
+    oracle.recover(g, hf, binv, gfl, tm, rm, S, x)
+    t3 = time.perf_counter()
+    per_it = (t2 - t1) / max(its, 1)
+    t_solve = (t1 - t0) + (t3 - t2) + per_it * iters_needed
+    return {
+        "value": 1.0 / t_solve, "unit": "solves/s", "cores": 1, "kind": "port",
+        "sample": (f"oracle/oracle.c FP64 on 1 host core ({platform.processor() or 'cpu'}, "
+                   f"{os.cpu_count()} visible): full assembly {t1 - t0:.2f} s + recovery "
+                   f"{t3 - t2:.2f} s + {its} Jacobi-PCG iterations at {per_it * 1e3:.1f} ms, "
+                   f"extrapolated to the {iters_needed} iterations the same FP64 Jacobi-PCG takes"),
+        "seconds_per_solve": t_solve,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--chunk", type=int, default=32)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-iters", type=int, default=40)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
+              "(see DESIGN.md for how the solve would shard).", file=sys.stderr)
+        sys.exit(2)
+
+    import __graft_entry__
+    pkg = __graft_entry__.load_package()
+    import torch
+    dev = torch.device("cuda:0")
+    g, perm, bt, bv = _problem(pkg, args.seed)
+    nc = g.num_cells
+
+    s = pkg.HybridSolver(0)
+    t0 = time.perf_counter()
+    s.init(g, perm, (0.0, 0.0, 0.0), bt, None)
+    t_init = time.perf_counter() - t0
+    d = s.dims()
+    nf = d.num_faces
+
+    tm = torch.ones(nc, dtype=torch.float64, device=dev)
+    rm = torch.zeros(nc, dtype=torch.float64, device=dev)
+    src = torch.zeros(nc, dtype=torch.float64, device=dev)
+    bvd = torch.from_numpy(bv).to(dev)
+    torch.cuda.synchronize()
+
+    def step(precision):
+        return s.solve_device(tm.data_ptr(), rm.data_ptr(), src.data_ptr(), bt, bvd.data_ptr(),
+                              nrhs=1, tol=TOL, precision=precision, chunk=args.chunk)
+
+    # plain FP64 GPU solve: the comparison point, and its result for the error report
+    step("fp64")
+    rep64 = s.report.as_dict()
+    p64, pi64, v64 = s.getSolution()
+    t0 = time.perf_counter()
+    step("fp64")
+    torch.cuda.synchronize()
+    t_fp64 = time.perf_counter() - t0
+    nnz = int(s.system(0)[0][-1])
+
+    for _ in range(args.warmup):
+        step(args.precision)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(args.precision)
+    torch.cuda.synchronize()
+    t_tot = time.perf_counter() - t0
+    rep = s.report.as_dict()
+    ms_step = t_tot / args.steps * 1e3
+    p, pi, v = s.getSolution()
+
+    def rel(a, b):
+        return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+    cell = g.hf_cell()
+    mass = float(np.max(np.abs(np.bincount(cell, weights=v, minlength=nc))) / np.max(np.abs(v)))
+
+    B = _alg_bytes(g, nnz, nf)
+    it_key, sp_key = ("iter32", "spmv32") if args.precision == "mixed" else ("iter64", "spmv64")
+    n_ref = rep["replacements"]
+    bytes_solve = B["assemble"] + rep["iterations"] * B[it_key] + n_ref * B["refine64"] + B["recover"]
+    gbps = bytes_solve / (ms_step * 1e-3) / 1e9
+
+    kern_ms = s.time_kernel(0, reps=50)
+    ach = B[sp_key] / (kern_ms * 1e-3) / 1e9
+    roof = {"kernel": f"k_spmv_pq<{'float' if args.precision == 'mixed' else 'double'}>",
+            "bound": "hbm", "achieved": round(ach, 1), "peak": 8000.0, "unit": "GB/s",
+            "frac": round(ach / 8000.0, 4), "traffic": None,
+            "alg_bytes_per_launch": B[sp_key], "avg_launch_us": round(kern_ms * 1e3, 2)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = _cpu_baseline(pkg, g, perm, bt, bv, rep64["iterations"], args.cpu_sample_iters)
+
+    out = {
+        "metric": "face-pressure solves/sec + HBM GB/s at 128x128x64, rel. residual 1e-10",
+        "value": round(1000.0 / ms_step, 4),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 PCG / f64 residual, assembly, recovery" if args.precision == "mixed" else "f64",
+        "data": "synthetic (seeded Gaussian-filtered lognormal permeability)",
+        "config": {
+            "workload": "mimetic hybrid solve, 128x128x64 Cartesian, anisotropic K (kz/kx=0.1, "
+                        "sigma ln k 1.5, l=8), x pressure drop, Jacobi-PCG to 1e-10",
+            "cells": nc, "faces": nf, "nnz": nnz, "precision": args.precision,
+            "iterations": rep["iterations"], "fp64_replacements": n_ref,
+            "reduction": rep["reduction"], "init_s": round(t_init, 3),
+            "hbm_gbps_per_solve": round(gbps, 1),
+            "fp64_gpu": {"solves_per_s": round(1.0 / t_fp64, 4), "iterations": rep64["iterations"],
+                         "reduction": rep64["reduction"]},
+            "speedup_vs_fp64_gpu": round((1000.0 / ms_step) * t_fp64, 3),
+            "err_vs_fp64_gpu": {"face_pressure": rel(pi, pi64), "cell_pressure": rel(p, p64),
+                                "outflux": rel(v, v64)},
+            "mass_balance": mass,
+        },
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

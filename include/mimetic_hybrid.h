@@ -189,6 +189,12 @@ int mhs_get_face_rows(mhs_solver *s, int *face_row);
 int mhs_get_system(mhs_solver *s, int k, int *row_ptr, int *col, double *val,
                    double *rhs, long long *nnz);
 
+/* Profiling hook, no counterpart in the reference: average device time (ms)
+ * of `reps` back-to-back launches of kernel `which` (0: the PCG SpMV with
+ * <p, Ap>, in the precision of the last solve) on the solver's own stream,
+ * timed with HIP events, on the data of the last solve. */
+int mhs_time_kernel(mhs_solver *s, int which, int reps, double *avg_ms);
+
 #ifdef __cplusplus
 }
 #endif

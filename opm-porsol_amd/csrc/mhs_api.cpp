@@ -407,6 +407,7 @@ int ensure_solve_buffers(mhs_solver *s, int nrhs, bool mixed) {
         s->free_solve_buffers();
         return fail(s, MHS_ERR_ALLOC, std::string("solve buffers: ") + hipGetErrorString(e));
     }
+    s->R.nblk = T.nfp / kRowBlock;
     s->R.block_seg = s->dT.block_seg;
     s->R.seg_blk = s->dT.seg_blk;
     s->h_nsys = nsys;
@@ -727,6 +728,40 @@ int mhs_get_face_rows(mhs_solver *s, int *face_row) {
     if (!s || !face_row) return MHS_ERR_INVALID;
     if (!s->structure_valid) return fail(s, MHS_ERR_STATE, "get_face_rows: no structure");
     std::memcpy(face_row, s->T.face_row.data(), sizeof(int) * s->T.nf_total);
+    return MHS_OK;
+}
+
+int mhs_time_kernel(mhs_solver *s, int which, int reps, double *avg_ms) {
+    if (!s || !avg_ms || reps < 1) return MHS_ERR_INVALID;
+    if (!s->solved) return fail(s, MHS_ERR_STATE, "time_kernel: call solve first");
+    if (which != 0) return fail(s, MHS_ERR_INVALID, "time_kernel: unknown kernel id");
+    HIPCHECK(hipSetDevice(s->device));
+    const size_t nsys = (size_t)s->T.ninst * s->cap_nrhs;
+    std::vector<int> ones(nsys, 1);
+    HIPCHECK(hipMemcpyAsync(s->S.act, ones.data(), nsys * sizeof(int), hipMemcpyHostToDevice, s->stream));
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    hipError_t e = hipSuccess;
+    // one untimed launch, then reps timed launches between two events on our stream
+    for (int it = 0; it <= reps && e == hipSuccess; ++it) {
+        if (it == 1) e = hipEventRecord(e0, s->stream);
+        if (e != hipSuccess) break;
+        if (s->cap_mixed)
+            e = launch_spmv<float, float>(s->stream, s->dT, s->cap_nrhs, s->val32, s->p32, s->q32, s->R, s->S);
+        else
+            e = launch_spmv<double, double>(s->stream, s->dT, s->cap_nrhs, s->val64, s->p64, s->q64, s->R, s->S);
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, s->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::vector<int> zeros(nsys, 0);
+    HIPCHECK(hipMemcpy(s->S.act, zeros.data(), nsys * sizeof(int), hipMemcpyHostToDevice));
+    if (e != hipSuccess) return fail(s, MHS_ERR_HIP, std::string("time_kernel: ") + hipGetErrorString(e));
+    *avg_ms = ms / reps;
     return MHS_OK;
 }
 

@@ -36,7 +36,8 @@ struct DevCellSys {
 
 // Segmented-reduction workspace shared by all PCG kernels.
 struct DevRed {
-    double *partials;  // [nblk * 16]
+    double *partials;  // [16][nblk]
+    int nblk;
     int *counters;     // [ninst]
     const int *block_seg, *seg_blk;
 };
@@ -74,6 +75,11 @@ hipError_t launch_pcg_iteration(hipStream_t st, const DevTopo &T, int nrhs, cons
                                 const VT *dinv, VT *x, VT *r, VT *p, VT *q, DevRed R,
                                 DevScal S, const double *val64, const double *b, double *x64,
                                 double tol2);
+
+// The SpMV + <p, Ap> kernel alone (profiling).
+template <typename MT, typename VT>
+hipError_t launch_spmv(hipStream_t st, const DevTopo &T, int nrhs, const MT *val, const VT *p,
+                       VT *q, DevRed R, DevScal S);
 
 // FP64 PCG start: r = b - A x, p = dinv r, rz, rr0, thresholds, activation.
 hipError_t launch_pcg_start64(hipStream_t st, const DevTopo &T, int nrhs, const double *val,

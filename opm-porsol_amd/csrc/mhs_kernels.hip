@@ -31,6 +31,18 @@ __host__ __device__ constexpr int pidx(int i, int j, int n) {  // i <= j
     return i * n - (i * (i - 1)) / 2 + (j - i);
 }
 
+struct FinArgs {
+    double tol2;
+    int init;
+};
+
+// Operands of the FP64 residual replacement fused into the FP32 SpMV.
+struct RelArgs {
+    const double *val64, *x64, *b;
+    const float *dinv;
+    float *r32;
+};
+
 // Hardware workgroup ids are dealt round-robin over the 8 XCDs; give every XCD
 // a contiguous range of logical rows so its private L2 sees neighbouring rows.
 __device__ __forceinline__ int xcd_block(int nblk) {
@@ -63,39 +75,81 @@ __device__ __forceinline__ void block_sum256(double (&v)[NQ], double *lds) {
     __syncthreads();
 }
 
-// Sum NQ values over all workgroups of instance s.  Every workgroup of the
-// instance must call this; the last one to arrive reduces the per-workgroup
-// partials in index order and runs fin(totals) on thread 0.
-template <int NQ, class Fin>
-__device__ __forceinline__ void seg_reduce(double (&v)[NQ], int b, int s, const DevRed &R,
-                                           Fin fin) {
+// Per-workgroup partial sums: thread 0 stores them with plain stores.  The
+// matching k_finish kernel (launched next on the stream) reduces the partials
+// of each instance in index order.  Visibility across XCDs comes from the
+// kernel boundary; no per-workgroup fences (on CDNA4 an agent-scope fence
+// writes back and invalidates the whole L2 of the XCD).
+template <int NQ>
+__device__ __forceinline__ void block_partials(double (&v)[NQ], int b, const DevRed &R) {
     __shared__ double lds[4 * NQ];
-    __shared__ int last;
     block_sum256<NQ>(v, lds);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) R.partials[(size_t)b * 16 + q] = v[q];
-        __threadfence();
-        const int nb = R.seg_blk[s + 1] - R.seg_blk[s];
-        const int prev = atomicAdd(&R.counters[s], 1);
-        last = (prev == nb - 1);
+        for (int q = 0; q < NQ; ++q) R.partials[(size_t)q * R.nblk + b] = v[q];
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
+}
+
+// Same, reducing only the first M values (the finisher reads no more).
+template <int NQ, int M>
+__device__ __forceinline__ void block_partials_n(double (&v)[NQ], int b, const DevRed &R) {
+    double w[M];
+#pragma unroll
+    for (int q = 0; q < M; ++q) w[q] = v[q];
+    block_partials<M>(w, b, R);
+}
+
+// One workgroup per instance: tot[q] = sum of the instance's partials, then
+// Fin::run(s, tot) on thread 0.  Fin::skip(s) must mirror the condition under
+// which the producing kernel's workgroups returned early.
+template <int NQ, class Fin>
+__global__ __launch_bounds__(256) void k_finish(DevRed R, DevScal S, FinArgs A) {
+    const int s = blockIdx.x;
+    if (Fin::skip(s, S, A)) return;
+    const int nq = Fin::nq(s, S);
+    const int b0 = R.seg_blk[s], b1 = R.seg_blk[s + 1];
     double acc[NQ];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
-    for (int i = R.seg_blk[s] + threadIdx.x; i < R.seg_blk[s + 1]; i += 256) {
+    for (int q = 0; q < NQ; ++q) {
+        acc[q] = 0.0;
+        if (q >= nq) continue;
+        const double *pq = R.partials + (size_t)q * R.nblk;
+        for (int base = b0; base < b1; base += 16 * 256) {
+            double a[16];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q)
-            acc[q] += __hip_atomic_load(&R.partials[(size_t)i * 16 + q], __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_AGENT);
+            for (int u = 0; u < 16; ++u) {   // 16 independent loads in flight per lane
+                const int i = base + u * 256 + threadIdx.x;
+                a[u] = (i < b1) ? pq[i] : 0.0;
+            }
+#pragma unroll
+            for (int w = 8; w > 0; w >>= 1)
+#pragma unroll
+                for (int u = 0; u < w; ++u) a[u] += a[u + w];
+            acc[q] += a[0];
+        }
     }
+    __shared__ double lds[4 * NQ];
     block_sum256<NQ>(acc, lds);
-    if (threadIdx.x == 0) {
-        fin(acc);
-        __hip_atomic_store(&R.counters[s], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) Fin::run(s, acc, S, A);
+}
+
+// 16-byte vector access to a thread's contiguous chunk of N elements.
+template <typename VT, int N>
+__device__ __forceinline__ void vload(const VT *__restrict__ p, VT (&v)[N]) {
+    static_assert((N * sizeof(VT)) % 16 == 0, "chunk must be a multiple of 16 bytes");
+#pragma unroll
+    for (int i = 0; i < (int)(N * sizeof(VT) / 16); ++i) {
+        const uint4 t = reinterpret_cast<const uint4 *>(p)[i];
+        __builtin_memcpy(&v[i * 16 / sizeof(VT)], &t, 16);
+    }
+}
+template <typename VT, int N>
+__device__ __forceinline__ void vstore(VT *__restrict__ p, const VT (&v)[N]) {
+#pragma unroll
+    for (int i = 0; i < (int)(N * sizeof(VT) / 16); ++i) {
+        uint4 t;
+        __builtin_memcpy(&t, &v[i * 16 / sizeof(VT)], 16);
+        reinterpret_cast<uint4 *>(p)[i] = t;
     }
 }
 
@@ -108,6 +162,46 @@ __device__ __forceinline__ bool load_act(const int *act, int s, int (&a)[NRHS]) 
         any |= (a[k] != 0);
     }
     return any;
+}
+
+// One SELL-64 row times NRHS vectors.  The slice width is wave-uniform; rows
+// up to 16 stored entries load every index and value first (non-temporal: the
+// matrix is streamed once, the gathered vector should stay in L2), then issue
+// all gathers, so each lane has up to 32 independent loads in flight.
+template <typename MT, typename XT, typename AT, int NRHS>
+__device__ __forceinline__ void sell_row(const DevTopo &T, const MT *__restrict__ val,
+                                         const XT *__restrict__ x, int f, AT (&acc)[NRHS]) {
+    const int slice = f >> 6, lane = f & 63;
+    const long long s0 = T.slice_ptr[slice];
+    const int w = (int)((T.slice_ptr[slice + 1] - s0) >> 6);
+    const long long base = s0 + lane;
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) acc[k] = AT(0);
+    if (w <= 16) {
+        int ci[16];
+        MT a[16];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            if (e < w) {
+                ci[e] = __builtin_nontemporal_load(T.col + base + 64LL * e);
+                a[e] = __builtin_nontemporal_load(val + base + 64LL * e);
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            if (e < w) {
+#pragma unroll
+                for (int k = 0; k < NRHS; ++k) acc[k] += (AT)a[e] * (AT)x[(size_t)ci[e] * NRHS + k];
+            }
+        }
+    } else {
+        for (int e = 0; e < w; ++e) {
+            const int c = __builtin_nontemporal_load(T.col + base + 64LL * e);
+            const AT a = (AT)__builtin_nontemporal_load(val + base + 64LL * e);
+#pragma unroll
+            for (int k = 0; k < NRHS; ++k) acc[k] += a * (AT)x[(size_t)c * NRHS + k];
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -352,7 +446,7 @@ __global__ __launch_bounds__(256) void k_fill(DevTopo T, DevCellSys C,
             for (int k = 0; k < NRHS; ++k)
                 bk[k] += a0 * bcval[(size_t)k * T.nhf + h0] - a1 * bcval[(size_t)k * T.nhf + hp];
         }
-        const int s = T.block_seg[f >> 8];
+        const int s = T.block_seg[f / kRowBlock];
         if (inst_reg[s] && f == T.inst_rowbase[s]) val64[dpos] *= 2.0;  // S[0][0] *= 2 (:1398-1400)
         diag = val64[dpos];
     }
@@ -371,54 +465,68 @@ __global__ __launch_bounds__(256) void k_fill(DevTopo T, DevCellSys C,
 
 // ---------------------------------------------------------------------------
 // PCG iteration kernels.
-template <typename MT, typename VT, int NRHS>
+// Row kernels handle 1024 rows per workgroup (kRowBlock): SpMV-type kernels
+// take rows b*1024 + j*256 + t (j = 0..3, coalesced per j), vector kernels a
+// contiguous quad of rows per thread with 16-byte loads.
+
+// q = A p and <p, q>.  With REL (FP32 iteration), systems whose FP64 residual
+// replacement is due (upd != 0) also get r = b - A64 x64 in the same pass:
+// the row's FP64 residual replaces the FP32 recursive one, and the partials
+// carry |r64|^2, r.M^-1 r and |r|^2 of the replaced residual.
+template <typename MT, typename VT, int NRHS, bool REL>
 __global__ __launch_bounds__(256) void k_spmv_pq(DevTopo T, const MT *__restrict__ val,
                                                  const VT *__restrict__ p, VT *__restrict__ q,
-                                                 DevRed R, DevScal S) {
+                                                 DevRed R, DevScal S, RelArgs L) {
     const int b = xcd_block(gridDim.x);
     const int s = T.block_seg[b];
     int act[NRHS];
     if (!load_act<NRHS>(S.act, s, act)) return;
-    const int f = b * 256 + threadIdx.x;
-    const int slice = f >> 6, lane = f & 63;
-    const long long s0 = T.slice_ptr[slice], s1 = T.slice_ptr[slice + 1];
-    VT acc[NRHS];
-#pragma unroll
-    for (int k = 0; k < NRHS; ++k) acc[k] = VT(0);
-    for (long long o = s0 + lane; o < s1; o += 64) {
-        const int cidx = T.col[o];
-        const VT a = (VT)val[o];
-#pragma unroll
-        for (int k = 0; k < NRHS; ++k) acc[k] += a * p[(size_t)cidx * NRHS + k];
-    }
-    double part[NRHS];
+    int u[NRHS];
+    bool rel = false;
 #pragma unroll
     for (int k = 0; k < NRHS; ++k) {
-        q[(size_t)f * NRHS + k] = acc[k];
-        part[k] = (double)p[(size_t)f * NRHS + k] * (double)acc[k];
+        u[k] = (REL && act[k]) ? S.upd[s * NRHS + k] : 0;
+        rel |= u[k] != 0;
     }
-    seg_reduce<NRHS>(part, b, s, R, [&](double *tot) {
+    constexpr int NQ = REL ? 4 * NRHS : NRHS;
+    double part[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) part[q] = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int f = b * kRowBlock + j * 256 + threadIdx.x;
+        VT acc[NRHS];
+        sell_row<MT, VT, VT, NRHS>(T, val, p, f, acc);
 #pragma unroll
         for (int k = 0; k < NRHS; ++k) {
-            const int i = s * NRHS + k;
-            if (!act[k]) continue;
-            double *sc = S.sc + (size_t)i * SC_N;
-            const double pq = tot[k];
-            sc[SC_PQ] = pq;
-            if (!(pq > 0.0)) {
-                sc[SC_ALPHA] = 0.0;
-                if (S.mixed) {          // FP32 breakdown: restart from the FP64 residual
-                    S.upd[i] = 2;
-                } else {
-                    S.act[i] = 0;
-                    atomicSub(&S.status[0], 1);
-                    atomicAdd(&S.status[1], 1);
+            q[(size_t)f * NRHS + k] = acc[k];
+            part[k] += (double)p[(size_t)f * NRHS + k] * (double)acc[k];
+        }
+        if constexpr (REL) {
+            if (rel) {  // block-uniform: the flags are per instance
+                double a64[NRHS];
+                sell_row<double, double, double, NRHS>(T, L.val64, L.x64, f, a64);
+                const float d = L.dinv[f];
+#pragma unroll
+                for (int k = 0; k < NRHS; ++k) {
+                    if (!u[k]) continue;
+                    const size_t i = (size_t)f * NRHS + k;
+                    const double ri = L.b[i] - a64[k];
+                    const float rf = (float)ri;
+                    L.r32[i] = rf;
+                    part[NRHS + 3 * k] += ri * ri;
+                    part[NRHS + 3 * k + 1] += (double)rf * (double)(d * rf);
+                    part[NRHS + 3 * k + 2] += (double)rf * (double)rf;
                 }
-            } else {
-                sc[SC_ALPHA] = sc[SC_RZ] / pq;
             }
         }
-    });
+    }
+    if constexpr (REL) {
+        if (rel) block_partials<NQ>(part, b, R);
+        else block_partials_n<NQ, NRHS>(part, b, R);
+    } else {
+        block_partials<NQ>(part, b, R);
+    }
 }
 
 template <typename VT, int NRHS>
@@ -431,44 +539,34 @@ __global__ __launch_bounds__(256) void k_update_xr(DevTopo T, const VT *__restri
     const int s = T.block_seg[b];
     int act[NRHS];
     if (!load_act<NRHS>(S.act, s, act)) return;
-    const int f = b * 256 + threadIdx.x;
-    const VT d = dinv[f];
+    VT al[NRHS];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k)
+        al[k] = act[k] ? (VT)S.sc[(size_t)(s * NRHS + k) * SC_N + SC_ALPHA] : VT(0);
+    const int f0 = b * kRowBlock + 4 * threadIdx.x;
+    const size_t e0 = (size_t)f0 * NRHS;
+    VT d[4], xv[4 * NRHS], rv[4 * NRHS], pv[4 * NRHS], qv[4 * NRHS];
+    vload<VT, 4>(dinv + f0, d);
+    vload<VT, 4 * NRHS>(x + e0, xv);
+    vload<VT, 4 * NRHS>(r + e0, rv);
+    vload<VT, 4 * NRHS>(p + e0, pv);
+    vload<VT, 4 * NRHS>(q + e0, qv);
     double part[2 * NRHS];
 #pragma unroll
-    for (int k = 0; k < NRHS; ++k) {
-        const size_t i = (size_t)f * NRHS + k;
-        const VT al = act[k] ? (VT)S.sc[(size_t)(s * NRHS + k) * SC_N + SC_ALPHA] : VT(0);
-        const VT xi = x[i] + al * p[i];
-        const VT ri = r[i] - al * q[i];
-        x[i] = xi;
-        r[i] = ri;
-        part[2 * k] = (double)ri * (double)(d * ri);
-        part[2 * k + 1] = (double)ri * (double)ri;
-    }
-    seg_reduce<2 * NRHS>(part, b, s, R, [&](double *tot) {
+    for (int k = 0; k < 2 * NRHS; ++k) part[k] = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
 #pragma unroll
         for (int k = 0; k < NRHS; ++k) {
-            const int i = s * NRHS + k;
-            if (!act[k]) continue;
-            double *sc = S.sc + (size_t)i * SC_N;
-            S.iters[i] += 1;
-            const double rz = tot[2 * k], rr = tot[2 * k + 1];
-            sc[SC_RR] = rr;
-            if (S.mixed) {
-                // FP32 recursion: ask for the FP64 true residual once the recursive
-                // one has dropped by replace_factor (or reached the tolerance).
-                if (S.upd[i] == 0 && (rr <= sc[SC_THR] || rr <= S.rf2 * sc[SC_RRL])) S.upd[i] = 1;
-                sc[SC_BETA] = rz / sc[SC_RZ];
-                sc[SC_RZ] = rz;
-            } else if (rr <= sc[SC_THR]) {
-                S.act[i] = 0;
-                atomicSub(&S.status[0], 1);
-            } else {
-                sc[SC_BETA] = rz / sc[SC_RZ];
-                sc[SC_RZ] = rz;
-            }
+            const int i = j * NRHS + k;
+            xv[i] = xv[i] + al[k] * pv[i];
+            rv[i] = rv[i] - al[k] * qv[i];
+            part[2 * k] += (double)rv[i] * (double)(d[j] * rv[i]);
+            part[2 * k + 1] += (double)rv[i] * (double)rv[i];
         }
-    });
+    vstore<VT, 4 * NRHS>(x + e0, xv);
+    vstore<VT, 4 * NRHS>(r + e0, rv);
+    block_partials<2 * NRHS>(part, b, R);
 }
 
 template <typename VT, int NRHS>
@@ -480,19 +578,40 @@ __global__ __launch_bounds__(256) void k_update_p(DevTopo T, const VT *__restric
     const int s = T.block_seg[b];
     int act[NRHS];
     if (!load_act<NRHS>(S.act, s, act)) return;
-    const int f = b * 256 + threadIdx.x;
-    const VT d = dinv[f];
+    VT be[NRHS];
+    int u[NRHS];
+    bool any_u = false;
 #pragma unroll
     for (int k = 0; k < NRHS; ++k) {
-        if (!act[k]) continue;
-        const size_t i = (size_t)f * NRHS + k;
         const int sys = s * NRHS + k;
-        const VT be = (VT)S.sc[(size_t)sys * SC_N + SC_BETA];
-        p[i] = d * r[i] + be * p[i];
-        if (x64 && S.upd[sys]) {  // fold the low-precision correction into x64
-            x64[i] += (double)x[i];
-            x[i] = VT(0);
+        u[k] = (x64 && act[k]) ? S.upd[sys] : 0;
+        any_u |= u[k] != 0;
+        be[k] = (u[k] == 2) ? VT(0) : (VT)S.sc[(size_t)sys * SC_N + SC_BETA];
+    }
+    const int f0 = b * kRowBlock + 4 * threadIdx.x;
+    const size_t e0 = (size_t)f0 * NRHS;
+    VT d[4], rv[4 * NRHS], pv[4 * NRHS];
+    vload<VT, 4>(dinv + f0, d);
+    vload<VT, 4 * NRHS>(r + e0, rv);
+    vload<VT, 4 * NRHS>(p + e0, pv);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) {
+            const int i = j * NRHS + k;
+            if (act[k]) pv[i] = d[j] * rv[i] + be[k] * pv[i];
         }
+    vstore<VT, 4 * NRHS>(p + e0, pv);
+    if (any_u) {  // fold the low-precision correction into x64 before the replacement
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int k = 0; k < NRHS; ++k) {
+                if (!u[k]) continue;
+                const size_t i = e0 + j * NRHS + k;
+                x64[i] += (double)x[i];
+                x[i] = VT(0);
+            }
     }
 }
 
@@ -505,47 +624,26 @@ __global__ __launch_bounds__(256) void k_start64(DevTopo T, const double *__rest
                                                  double *__restrict__ r, double *__restrict__ p,
                                                  double tol2, DevRed R, DevScal S) {
     const int b = xcd_block(gridDim.x);
-    const int s = T.block_seg[b];
-    const int f = b * 256 + threadIdx.x;
-    const int slice = f >> 6, lane = f & 63;
-    const long long s0 = T.slice_ptr[slice], s1 = T.slice_ptr[slice + 1];
-    double acc[NRHS];
-#pragma unroll
-    for (int k = 0; k < NRHS; ++k) acc[k] = 0.0;
-    for (long long o = s0 + lane; o < s1; o += 64) {
-        const int cidx = T.col[o];
-        const double a = val[o];
-#pragma unroll
-        for (int k = 0; k < NRHS; ++k) acc[k] += a * x[(size_t)cidx * NRHS + k];
-    }
-    const double d = dinv[f];
     double part[2 * NRHS];
 #pragma unroll
-    for (int k = 0; k < NRHS; ++k) {
-        const size_t i = (size_t)f * NRHS + k;
-        const double ri = bb[i] - acc[k];
-        r[i] = ri;
-        p[i] = d * ri;
-        part[2 * k] = ri * (d * ri);
-        part[2 * k + 1] = ri * ri;
-    }
-    seg_reduce<2 * NRHS>(part, b, s, R, [&](double *tot) {
+    for (int k = 0; k < 2 * NRHS; ++k) part[k] = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int f = b * kRowBlock + j * 256 + threadIdx.x;
+        double acc[NRHS];
+        sell_row<double, double, double, NRHS>(T, val, x, f, acc);
+        const double d = dinv[f];
 #pragma unroll
         for (int k = 0; k < NRHS; ++k) {
-            const int i = s * NRHS + k;
-            double *sc = S.sc + (size_t)i * SC_N;
-            const double rr0 = tot[2 * k + 1];
-            sc[SC_RZ] = tot[2 * k];
-            sc[SC_RR] = rr0;
-            sc[SC_RR0] = rr0;
-            sc[SC_RRT] = rr0;
-            sc[SC_THR] = tol2 * rr0;
-            S.iters[i] = 0;
-            const int a = (rr0 >= 1e-60) ? 1 : 0;  // dune CGSolver: def0 < 1e-30 -> done
-            S.act[i] = a;
-            if (a) atomicAdd(&S.status[0], 1);
+            const size_t i = (size_t)f * NRHS + k;
+            const double ri = bb[i] - acc[k];
+            r[i] = ri;
+            p[i] = d * ri;
+            part[2 * k] += ri * (d * ri);
+            part[2 * k + 1] += ri * ri;
         }
-    });
+    }
+    block_partials<2 * NRHS>(part, b, R);
 }
 
 // FP64 residual replacement for the FP32 iteration, fused into one pass.
@@ -569,69 +667,44 @@ __global__ __launch_bounds__(256) void k_reliable(DevTopo T, const double *__res
     } else if (!load_act<NRHS>(S.upd, s, u)) {
         return;
     }
-    const int f = b * 256 + threadIdx.x;
-    const int slice = f >> 6, lane = f & 63;
-    const long long s0 = T.slice_ptr[slice], s1 = T.slice_ptr[slice + 1];
-    double acc[NRHS];
-#pragma unroll
-    for (int k = 0; k < NRHS; ++k) acc[k] = 0.0;
-    for (long long o = s0 + lane; o < s1; o += 64) {
-        const int cidx = T.col[o];
-        const double a = val[o];
-#pragma unroll
-        for (int k = 0; k < NRHS; ++k) acc[k] += a * x[(size_t)cidx * NRHS + k];
-    }
-    const float d = dinv[f];
     double part[3 * NRHS];
 #pragma unroll
-    for (int k = 0; k < NRHS; ++k) {
-        const size_t i = (size_t)f * NRHS + k;
-        part[3 * k] = part[3 * k + 1] = part[3 * k + 2] = 0.0;
-        if (!u[k]) continue;
-        const double ri = bb[i] - acc[k];
-        const float rf = (float)ri;
-        r32[i] = rf;
-        if (u[k] == 2) p32[i] = d * rf;
-        part[3 * k] = ri * ri;
-        part[3 * k + 1] = (double)rf * (double)(d * rf);
-        part[3 * k + 2] = (double)rf * (double)rf;
-    }
-    seg_reduce<3 * NRHS>(part, b, s, R, [&](double *tot) {
+    for (int k = 0; k < 3 * NRHS; ++k) part[k] = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int f = b * kRowBlock + j * 256 + threadIdx.x;
+        double acc[NRHS];
+        sell_row<double, double, double, NRHS>(T, val, x, f, acc);
+        const float d = dinv[f];
 #pragma unroll
         for (int k = 0; k < NRHS; ++k) {
             if (!u[k]) continue;
-            const int i = s * NRHS + k;
-            double *sc = S.sc + (size_t)i * SC_N;
-            const double rrt = tot[3 * k];
-            if (init) {
-                sc[SC_RR0] = rrt;
-                sc[SC_THR] = tol2 * rrt;
-                S.iters[i] = 0;
-            }
-            sc[SC_RRT] = rrt;
-            sc[SC_RRL] = rrt;
-            sc[SC_RZ] = tot[3 * k + 1];
-            sc[SC_RR] = tot[3 * k + 2];
-            S.upd[i] = 0;
-            if (!init) atomicAdd(&S.status[2], 1);
-            const int was = init ? 0 : S.act[i];
-            const int now = (rrt <= sc[SC_THR] || sc[SC_RR0] < 1e-60) ? 0 : 1;
-            S.act[i] = now;
-            if (now != was) atomicAdd(&S.status[0], now - was);
+            const size_t i = (size_t)f * NRHS + k;
+            const double ri = bb[i] - acc[k];
+            const float rf = (float)ri;
+            r32[i] = rf;
+            if (u[k] == 2) p32[i] = d * rf;
+            part[3 * k] += ri * ri;
+            part[3 * k + 1] += (double)rf * (double)(d * rf);
+            part[3 * k + 2] += (double)rf * (double)rf;
         }
-    });
+    }
+    block_partials<3 * NRHS>(part, b, R);
 }
 
 // x64 += x32, x32 = 0 for every system (final fold after the iteration).
 template <int NRHS>
 __global__ __launch_bounds__(256) void k_merge_all(DevTopo T, double *__restrict__ x64,
                                                    float *__restrict__ x32) {
-    const int f = blockIdx.x * 256 + threadIdx.x;
 #pragma unroll
-    for (int k = 0; k < NRHS; ++k) {
-        const size_t i = (size_t)f * NRHS + k;
-        x64[i] += (double)x32[i];
-        x32[i] = 0.0f;
+    for (int j = 0; j < 4; ++j) {
+        const int f = blockIdx.x * kRowBlock + j * 256 + threadIdx.x;
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) {
+            const size_t i = (size_t)f * NRHS + k;
+            x64[i] += (double)x32[i];
+            x32[i] = 0.0f;
+        }
     }
 }
 
@@ -642,29 +715,21 @@ __global__ __launch_bounds__(256) void k_resnorm(DevTopo T, const double *__rest
                                                  const double *__restrict__ x, DevRed R,
                                                  DevScal S) {
     const int b = xcd_block(gridDim.x);
-    const int s = T.block_seg[b];
-    const int f = b * 256 + threadIdx.x;
-    const int slice = f >> 6, lane = f & 63;
-    const long long s0 = T.slice_ptr[slice], s1 = T.slice_ptr[slice + 1];
-    double acc[NRHS];
-#pragma unroll
-    for (int k = 0; k < NRHS; ++k) acc[k] = 0.0;
-    for (long long o = s0 + lane; o < s1; o += 64) {
-        const int cidx = T.col[o];
-        const double a = val[o];
-#pragma unroll
-        for (int k = 0; k < NRHS; ++k) acc[k] += a * x[(size_t)cidx * NRHS + k];
-    }
     double part[NRHS];
 #pragma unroll
-    for (int k = 0; k < NRHS; ++k) {
-        const double ri = bb[(size_t)f * NRHS + k] - acc[k];
-        part[k] = ri * ri;
-    }
-    seg_reduce<NRHS>(part, b, s, R, [&](double *tot) {
+    for (int k = 0; k < NRHS; ++k) part[k] = 0.0;
 #pragma unroll
-        for (int k = 0; k < NRHS; ++k) S.sc[(size_t)(s * NRHS + k) * SC_N + SC_RRT] = tot[k];
-    });
+    for (int j = 0; j < 4; ++j) {
+        const int f = b * kRowBlock + j * 256 + threadIdx.x;
+        double acc[NRHS];
+        sell_row<double, double, double, NRHS>(T, val, x, f, acc);
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) {
+            const double ri = bb[(size_t)f * NRHS + k] - acc[k];
+            part[k] += ri * ri;
+        }
+    }
+    block_partials<NRHS>(part, b, R);
 }
 
 // ---------------------------------------------------------------------------
@@ -743,6 +808,166 @@ __global__ __launch_bounds__(256) void k_post(DevTopo T, int nrhs, const double 
         atomicMax(maxbits, (unsigned long long)__double_as_longlong(d));
 }
 
+// ---------------------------------------------------------------------------
+// Finishers of the per-instance reductions (run by k_finish on thread 0).
+template <int NRHS>
+__device__ __forceinline__ bool none_set(const int *flags, int s) {
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) any |= flags[s * NRHS + k] != 0;
+    return !any;
+}
+
+// after k_spmv_pq: alpha = rz / <p, Ap>; with REL also the FP64 residual
+// replacement (the true residual decides convergence of the FP32 iteration).
+template <int NRHS, bool REL>
+struct FinPQ {
+    __device__ static bool skip(int s, const DevScal &S, const FinArgs &) { return none_set<NRHS>(S.act, s); }
+    __device__ static int nq(int s, const DevScal &S) {
+        if (!REL) return NRHS;
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k)
+            if (S.act[s * NRHS + k] && S.upd[s * NRHS + k]) return 4 * NRHS;
+        return NRHS;
+    }
+    __device__ static void run(int s, const double *tot, DevScal &S, const FinArgs &) {
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) {
+            const int i = s * NRHS + k;
+            if (!S.act[i]) continue;
+            double *sc = S.sc + (size_t)i * SC_N;
+            if (REL && S.upd[i]) {
+                const double rrt = tot[NRHS + 3 * k];
+                sc[SC_RRT] = rrt;
+                sc[SC_RRL] = rrt;
+                sc[SC_RZ] = tot[NRHS + 3 * k + 1];
+                sc[SC_RR] = tot[NRHS + 3 * k + 2];
+                S.upd[i] = 0;
+                atomicAdd(&S.status[2], 1);
+                if (rrt <= sc[SC_THR]) {
+                    S.act[i] = 0;
+                    sc[SC_ALPHA] = 0.0;
+                    atomicSub(&S.status[0], 1);
+                    continue;
+                }
+            }
+            const double pq = tot[k];
+            sc[SC_PQ] = pq;
+            if (!(pq > 0.0)) {
+                sc[SC_ALPHA] = 0.0;
+                if (REL) {          // FP32 breakdown: restart p, then replace the residual
+                    S.upd[i] = 2;
+                } else {
+                    S.act[i] = 0;
+                    atomicSub(&S.status[0], 1);
+                    atomicAdd(&S.status[1], 1);
+                }
+            } else {
+                sc[SC_ALPHA] = sc[SC_RZ] / pq;
+            }
+        }
+    }
+};
+
+// after k_update_xr: convergence test (FP64) or replacement request (FP32), beta
+template <int NRHS>
+struct FinXR {
+    __device__ static bool skip(int s, const DevScal &S, const FinArgs &) { return none_set<NRHS>(S.act, s); }
+    __device__ static int nq(int, const DevScal &) { return 2 * NRHS; }
+    __device__ static void run(int s, const double *tot, DevScal &S, const FinArgs &) {
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) {
+            const int i = s * NRHS + k;
+            if (!S.act[i]) continue;
+            double *sc = S.sc + (size_t)i * SC_N;
+            S.iters[i] += 1;
+            const double rz = tot[2 * k], rr = tot[2 * k + 1];
+            sc[SC_RR] = rr;
+            if (S.mixed) {
+                // FP32 recursion: ask for the FP64 true residual once the recursive
+                // one has dropped by replace_factor (or reached the tolerance).
+                if (S.upd[i] == 0 && (rr <= sc[SC_THR] || rr <= S.rf2 * sc[SC_RRL])) S.upd[i] = 1;
+                sc[SC_BETA] = rz / sc[SC_RZ];
+                sc[SC_RZ] = rz;
+            } else if (rr <= sc[SC_THR]) {
+                S.act[i] = 0;
+                atomicSub(&S.status[0], 1);
+            } else {
+                sc[SC_BETA] = rz / sc[SC_RZ];
+                sc[SC_RZ] = rz;
+            }
+        }
+    }
+};
+
+// after k_start64: |r0|, thresholds, activation (dune CGSolver: def0 < 1e-30 -> done)
+template <int NRHS>
+struct FinStart {
+    __device__ static bool skip(int, const DevScal &, const FinArgs &) { return false; }
+    __device__ static int nq(int, const DevScal &) { return 2 * NRHS; }
+    __device__ static void run(int s, const double *tot, DevScal &S, const FinArgs &A) {
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) {
+            const int i = s * NRHS + k;
+            double *sc = S.sc + (size_t)i * SC_N;
+            const double rr0 = tot[2 * k + 1];
+            sc[SC_RZ] = tot[2 * k];
+            sc[SC_RR] = rr0;
+            sc[SC_RR0] = rr0;
+            sc[SC_RRT] = rr0;
+            sc[SC_THR] = A.tol2 * rr0;
+            S.iters[i] = 0;
+            const int a = (rr0 >= 1e-60) ? 1 : 0;
+            S.act[i] = a;
+            if (a) atomicAdd(&S.status[0], 1);
+        }
+    }
+};
+
+// after k_reliable: the FP64 true residual decides convergence
+template <int NRHS>
+struct FinRel {
+    __device__ static bool skip(int s, const DevScal &S, const FinArgs &A) {
+        return !A.init && none_set<NRHS>(S.upd, s);
+    }
+    __device__ static int nq(int, const DevScal &) { return 3 * NRHS; }
+    __device__ static void run(int s, const double *tot, DevScal &S, const FinArgs &A) {
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) {
+            const int i = s * NRHS + k;
+            if (!A.init && !S.upd[i]) continue;
+            double *sc = S.sc + (size_t)i * SC_N;
+            const double rrt = tot[3 * k];
+            if (A.init) {
+                sc[SC_RR0] = rrt;
+                sc[SC_THR] = A.tol2 * rrt;
+                S.iters[i] = 0;
+            }
+            sc[SC_RRT] = rrt;
+            sc[SC_RRL] = rrt;
+            sc[SC_RZ] = tot[3 * k + 1];
+            sc[SC_RR] = tot[3 * k + 2];
+            S.upd[i] = 0;
+            if (!A.init) atomicAdd(&S.status[2], 1);
+            const int was = A.init ? 0 : S.act[i];
+            const int now = (rrt <= sc[SC_THR] || sc[SC_RR0] < 1e-60) ? 0 : 1;
+            S.act[i] = now;
+            if (now != was) atomicAdd(&S.status[0], now - was);
+        }
+    }
+};
+
+// after k_resnorm
+template <int NRHS>
+struct FinRes {
+    __device__ static bool skip(int, const DevScal &, const FinArgs &) { return false; }
+    __device__ static int nq(int, const DevScal &) { return NRHS; }
+    __device__ static void run(int s, const double *tot, DevScal &S, const FinArgs &) {
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) S.sc[(size_t)(s * NRHS + k) * SC_N + SC_RRT] = tot[k];
+    }
+};
+
 }  // namespace
 
 // ===========================================================================
@@ -808,7 +1033,7 @@ hipError_t launch_fill(hipStream_t st, const DevTopo &T, int nrhs, const DevCell
                        float *dinv32, double *x0) {
     return with_nrhs(nrhs, [&](auto nr) {
         constexpr int NR = decltype(nr)::value;
-        k_fill<NR><<<T.nblk, 256, 0, st>>>(T, C, ftype, bcval, inst_reg, dguess, val64, val32, b,
+        k_fill<NR><<<T.nfp / 256, 256, 0, st>>>(T, C, ftype, bcval, inst_reg, dguess, val64, val32, b,
                                            dinv64, dinv32, x0);
     });
 }
@@ -820,14 +1045,15 @@ hipError_t launch_pcg_iteration(hipStream_t st, const DevTopo &T, int nrhs, cons
                                 double tol2) {
     return with_nrhs(nrhs, [&](auto nr) {
         constexpr int NR = decltype(nr)::value;
-        k_spmv_pq<MT, VT, NR><<<T.nblk, 256, 0, st>>>(T, val, p, q, R, S);
+        const FinArgs A{tol2, 0};
+        constexpr bool REL = std::is_same<VT, float>::value;
+        RelArgs L{};
+        if constexpr (REL) L = RelArgs{val64, x64, b, dinv, r};
+        k_spmv_pq<MT, VT, NR, REL><<<T.nblk, 256, 0, st>>>(T, val, p, q, R, S, L);
+        k_finish<REL ? 4 * NR : NR, FinPQ<NR, REL>><<<T.ninst, 256, 0, st>>>(R, S, A);
         k_update_xr<VT, NR><<<T.nblk, 256, 0, st>>>(T, dinv, x, r, p, q, R, S);
-        if constexpr (std::is_same<VT, float>::value) {
-            k_update_p<VT, NR><<<T.nblk, 256, 0, st>>>(T, dinv, r, p, x, x64, S);
-            k_reliable<NR><<<T.nblk, 256, 0, st>>>(T, val64, b, x64, dinv, r, p, 0, tol2, R, S);
-        } else {
-            k_update_p<VT, NR><<<T.nblk, 256, 0, st>>>(T, dinv, r, p, x, nullptr, S);
-        }
+        k_finish<2 * NR, FinXR<NR>><<<T.ninst, 256, 0, st>>>(R, S, A);
+        k_update_p<VT, NR><<<T.nblk, 256, 0, st>>>(T, dinv, r, p, x, REL ? x64 : nullptr, S);
     });
 }
 
@@ -843,12 +1069,26 @@ template hipError_t launch_pcg_iteration<float, float>(hipStream_t, const DevTop
                                                        DevScal, const double *, const double *,
                                                        double *, double);
 
+template <typename MT, typename VT>
+hipError_t launch_spmv(hipStream_t st, const DevTopo &T, int nrhs, const MT *val, const VT *p,
+                       VT *q, DevRed R, DevScal S) {
+    return with_nrhs(nrhs, [&](auto nr) {
+        constexpr int NR = decltype(nr)::value;
+        k_spmv_pq<MT, VT, NR, false><<<T.nblk, 256, 0, st>>>(T, val, p, q, R, S, RelArgs{});
+    });
+}
+template hipError_t launch_spmv<double, double>(hipStream_t, const DevTopo &, int, const double *,
+                                                const double *, double *, DevRed, DevScal);
+template hipError_t launch_spmv<float, float>(hipStream_t, const DevTopo &, int, const float *,
+                                              const float *, float *, DevRed, DevScal);
+
 hipError_t launch_pcg_start64(hipStream_t st, const DevTopo &T, int nrhs, const double *val,
                               const double *dinv, const double *b, const double *x, double *r,
                               double *p, double tol2, DevRed R, DevScal S) {
     return with_nrhs(nrhs, [&](auto nr) {
         constexpr int NR = decltype(nr)::value;
         k_start64<NR><<<T.nblk, 256, 0, st>>>(T, val, dinv, b, x, r, p, tol2, R, S);
+        k_finish<2 * NR, FinStart<NR>><<<T.ninst, 256, 0, st>>>(R, S, FinArgs{tol2, 0});
     });
 }
 
@@ -858,6 +1098,7 @@ hipError_t launch_reliable(hipStream_t st, const DevTopo &T, int nrhs, const dou
     return with_nrhs(nrhs, [&](auto nr) {
         constexpr int NR = decltype(nr)::value;
         k_reliable<NR><<<T.nblk, 256, 0, st>>>(T, val64, b, x64, dinv32, r32, p32, init, tol2, R, S);
+        k_finish<3 * NR, FinRel<NR>><<<T.ninst, 256, 0, st>>>(R, S, FinArgs{tol2, init});
     });
 }
 
@@ -873,6 +1114,7 @@ hipError_t launch_resnorm(hipStream_t st, const DevTopo &T, int nrhs, const doub
     return with_nrhs(nrhs, [&](auto nr) {
         constexpr int NR = decltype(nr)::value;
         k_resnorm<NR><<<T.nblk, 256, 0, st>>>(T, val64, b, x64, R, S);
+        k_finish<NR, FinRes<NR>><<<T.ninst, 256, 0, st>>>(R, S, FinArgs{0.0, 0});
     });
 }
 

@@ -13,7 +13,7 @@ namespace mhs {
 enum FaceType : uint8_t { FT_INTERNAL = 0, FT_DIRICHLET = 1, FT_NEUMANN = 2, FT_PERIODIC = 3 };
 
 constexpr int kSlice = 64;     // SELL slice height = one wavefront
-constexpr int kRowBlock = 256; // rows per PCG workgroup; instances start on a block
+constexpr int kRowBlock = 1024; // rows per PCG workgroup; instances start on a block
 constexpr int kMaxFaces = 16;  // faces per cell supported by the cell kernels
 constexpr int kMaxRowLen = 255;
 

@@ -30,7 +30,7 @@ EXPORTS = (
     "mhs_set_grid", "mhs_init_system_structure", "mhs_compute_inner_products",
     "mhs_assemble_static", "mhs_get_inner_products", "mhs_solve", "mhs_solve_device", "mhs_get_solution",
     "mhs_solution_device", "mhs_post_process_fluxes", "mhs_get_dims", "mhs_get_cell_faces",
-    "mhs_get_face_rows", "mhs_get_system",
+    "mhs_get_face_rows", "mhs_get_system", "mhs_time_kernel",
 )
 
 
@@ -97,6 +97,7 @@ def lib():
         L.mhs_get_face_rows.argtypes = [_P, _I]
         L.mhs_get_system.argtypes = [_P, ctypes.c_int, _I, _I, _D, _D,
                                      ctypes.POINTER(ctypes.c_longlong)]
+        L.mhs_time_kernel.argtypes = [_P, ctypes.c_int, ctypes.c_int, _D]
         for name in EXPORTS:
             if name not in ("mhs_default_options", "mhs_last_error"):
                 getattr(L, name).restype = ctypes.c_int
@@ -294,6 +295,12 @@ class HybridSolver:
         fr = np.empty(d.num_faces, dtype=np.int32)
         self._check(self._L.mhs_get_face_rows(self._h, fr.ctypes.data_as(_I)))
         return fr
+
+    def time_kernel(self, which=0, reps=50) -> float:
+        """Average device milliseconds of one launch of kernel `which` (0: PCG SpMV)."""
+        ms = ctypes.c_double(0.0)
+        self._check(self._L.mhs_time_kernel(self._h, which, reps, ctypes.byref(ms)))
+        return ms.value
 
     def system(self, k=0):
         """(row_ptr, col, val, rhs) of the assembled face system (printSystem :895-905)."""
