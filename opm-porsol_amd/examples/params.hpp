@@ -1,0 +1,26 @@
+// key=value command-line parameters (the style of Opm::parameter::ParameterGroup).
+#ifndef MHS_EXAMPLE_PARAMS_HPP
+#define MHS_EXAMPLE_PARAMS_HPP
+#include <cstdlib>
+#include <map>
+#include <string>
+
+struct Params {
+    std::map<std::string, std::string> kv;
+    Params(int argc, char **argv) {
+        for (int i = 1; i < argc; ++i) {
+            std::string a(argv[i]);
+            const auto eq = a.find('=');
+            if (eq != std::string::npos) kv[a.substr(0, eq)] = a.substr(eq + 1);
+        }
+    }
+    double get(const std::string &k, double def) const {
+        auto it = kv.find(k);
+        return it == kv.end() ? def : std::atof(it->second.c_str());
+    }
+    int geti(const std::string &k, int def) const {
+        auto it = kv.find(k);
+        return it == kv.end() ? def : std::atoi(it->second.c_str());
+    }
+};
+#endif
