@@ -59,8 +59,9 @@ class Grid:
         nb = self.nb.astype(np.int64)
         tw = np.full(self.num_half_faces, -1, dtype=np.int64)
         inside = nb >= 0
-        key_self = hc * nc + np.where(inside, nb, 0)
-        key_twin = np.where(inside, nb, 0) * nc + hc
+        # boundary half-faces get distinct negative keys so they never match
+        key_self = np.where(inside, hc * nc + nb, -1 - np.arange(nb.shape[0]))
+        key_twin = np.where(inside, nb * nc + hc, -2 - nb.shape[0])
         order = np.argsort(key_self, kind="stable")
         idx = np.searchsorted(key_self[order], key_twin)
         idx = np.clip(idx, 0, len(order) - 1)

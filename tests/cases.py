@@ -1,7 +1,14 @@
 """Problem set-ups shared by the CPU and GPU tests (synthetic, seeded)."""
+import os
 import sys
 
 import numpy as np
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def golden(name):
+    return os.path.join(GOLDEN, name)
 
 pkg = sys.modules["opm_porsol_amd"]
 grids, fields = pkg.grids, pkg.fields

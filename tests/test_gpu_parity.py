@@ -189,3 +189,52 @@ def test_not_converged_raises(pkg):
         s.solve(tm, rm, src, bt, bv, residual_tolerance=1e-12, linsolver_maxit=3)
     assert e.value.status == pkg.mhs.MHS_ERR_NOT_CONVERGED
     assert "failed to converge" in str(e.value)
+
+
+def test_golden_fixtures(pkg):
+    d = np.load(cases.golden("c1_20x20x10.npz"))
+    g, perm, tm, rm, src, bt, bv = cases.case_c1()
+    s = _solver(pkg, g, perm)
+    s.solve(tm, rm, src, bt, bv, residual_tolerance=1e-12, precision="mixed")
+    p, pi, v = s.getSolution()
+    assert rel_err(pi, d["face_pressure"]) < TOL_SOL
+    assert rel_err(p, d["cell_pressure"]) < TOL_SOL
+    assert rel_err(v, d["outflux"]) < TOL_SOL
+    d = np.load(cases.golden("periodic_4x4x4.npz"))
+    g = pkg.grids.cartesian(4, 4, 4)
+    perm = cases.fields.perm_c1(g.num_cells, seed=7)
+    bt, bv, partner = cases.periodic_bc(g, (0, 1, 2), drop_dir=0, pdrop=1.0)
+    s = _solver(pkg, g, perm, bt=bt, partner=partner)
+    s.solve(*cases.unit_fluid(g.num_cells), np.zeros(g.num_cells), bt, bv,
+            residual_tolerance=1e-13, precision="mixed")
+    p, pi, v = s.getSolution()
+    assert rel_err(v, d["outflux"]) < TOL_SOL
+    assert rel_err(p, d["cell_pressure"]) < TOL_SOL
+
+
+def test_full_size_properties(pkg):
+    """128x128x64 (the benchmark size), size-independent checks: with uniform
+    K the discrete solution is exactly linear; with the anisotropic lognormal
+    field every cell balances its fluxes and the mixed-precision solve agrees
+    with the FP64 solve."""
+    dims = (128, 128, 64)
+    g = pkg.grids.cartesian(*dims)
+    nc = g.num_cells
+    bt, bv = cases.xdrop_bc(g, 1.0, 0.0)
+    tm, rm = cases.unit_fluid(nc)
+    src = np.zeros(nc)
+    s = _solver(pkg, g, pkg.fields.diag_perm(np.full(nc, 1e-13)))
+    s.solve(tm, rm, src, bt, bv, residual_tolerance=1e-10, precision="mixed")
+    p, pi, v = s.getSolution()
+    x = g.cell_centroid[:, 0]
+    assert np.max(np.abs(p - (1.0 - x / 128.0))) < 1e-7
+    perm = pkg.fields.perm_c2(dims, seed=2)
+    s = _solver(pkg, g, perm)
+    s.solve(tm, rm, src, bt, bv, residual_tolerance=1e-10, precision="fp64")
+    p64, pi64, v64 = s.getSolution()
+    s.solve(tm, rm, src, bt, bv, residual_tolerance=1e-10, precision="mixed")
+    p, pi, v = s.getSolution()
+    assert s.report.reduction <= 1e-10
+    assert _mass_balance(g, v, src) < 1e-12
+    assert rel_err(pi, pi64) < 1e-6
+    assert rel_err(v, v64) < 1e-6
