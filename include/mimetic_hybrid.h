@@ -79,6 +79,10 @@ typedef struct {
                                   the FP64 true residual once it has dropped by
                                   this factor since the last replacement */
     int use_graph;             /* capture PCG chunks in a hipGraph */
+    int precond;               /* -1: from linsolver_type (0 -> Jacobi, 1..3 -> multigrid
+                                  when logical cell indices were given); 0: Jacobi;
+                                  1: aggregation multigrid V-cycle */
+    double smoother_weight;    /* Jacobi weight of the multigrid smoother */
 } mhs_solve_options;
 
 typedef struct {
@@ -86,6 +90,8 @@ typedef struct {
     int replacements;          /* FP64 residual replacements (mixed) */
     int converged;             /* 1 when every system reached tol */
     double reduction;          /* max over systems of the FP64 ||b-Ax||/||r0|| */
+    int precond;               /* preconditioner used (0 Jacobi, 1 multigrid) */
+    int levels;                /* multigrid levels (0 with Jacobi) */
     double assemble_ms, solve_ms, recover_ms;
 } mhs_solve_report;
 
@@ -116,6 +122,14 @@ int mhs_set_grid(mhs_solver *s, int num_instances, const int *inst_cellpos,
                  const double *cell_volume, const double *cell_centroid,
                  const double *face_area, const double *face_centroid,
                  const double *face_normal);
+
+/* Logical (i, j, k) index of every cell within its instance, 0-based
+ * (ijk[3*c + d]).  Optional, no counterpart in the reference: it enables the
+ * aggregation multigrid preconditioner, whose aggregates are 2x2x2 blocks of
+ * cells (the reference's AMG, IncompFlowSolverHybrid.hpp:1481-1541, builds
+ * its aggregates algebraically inside dune-istl).  Call after mhs_set_grid and
+ * before mhs_init_system_structure. */
+int mhs_set_cell_ijk(mhs_solver *s, const int *ijk);
 
 /* IncompFlowSolverHybrid::initSystemStructure (:534-545): dof enumeration
  * (:945-1110) and the face-system sparsity (:1115-1319).  Only periodic

@@ -1,0 +1,153 @@
+// Aggregation multigrid hierarchy (host-side set-up); see mhs_amg.h.
+#include "mhs_amg.h"
+
+#include <algorithm>
+#include <numeric>
+
+namespace mhs {
+
+namespace {
+
+void children(AmgLevel &C, int nf) {
+    C.child_ptr.assign(C.n + 1, 0);
+    for (int i = 0; i < nf; ++i)
+        if (C.agg[i] >= 0) C.child_ptr[C.agg[i] + 1]++;
+    for (int I = 0; I < C.n; ++I) C.child_ptr[I + 1] += C.child_ptr[I];
+    C.child.assign(C.child_ptr[C.n], 0);
+    std::vector<int> fill(C.child_ptr.begin(), C.child_ptr.end() - 1);
+    for (int i = 0; i < nf; ++i)
+        if (C.agg[i] >= 0) C.child[fill[C.agg[i]]++] = i;
+}
+
+// Sparsity of P'AP and the gather lists, coarse row by coarse row: the
+// entries (agg(j), pos) of the coarse row's children are sorted locally.
+// row_entries(i, emit) must call emit(j, pos) for every stored entry of
+// finer row i.  Requires children() to have run.
+template <class RowEntries>
+void galerkin_pattern(AmgLevel &C, RowEntries row_entries) {
+    C.row_ptr.assign(C.n + 1, 0);
+    C.col.clear();
+    C.src_ptr.assign(1, 0);
+    C.src.clear();
+    C.diag.assign(C.n, -1);
+    std::vector<std::pair<int, int>> buf;
+    for (int I = 0; I < C.n; ++I) {
+        buf.clear();
+        for (int k = C.child_ptr[I]; k < C.child_ptr[I + 1]; ++k)
+            row_entries(C.child[k], [&](int j, int pos) {
+                const int J = C.agg[j];
+                if (J >= 0) buf.emplace_back(J, pos);
+            });
+        std::sort(buf.begin(), buf.end());
+        for (size_t k = 0; k < buf.size(); ++k) {
+            if (k == 0 || buf[k].first != buf[k - 1].first) {
+                if (k > 0) C.src_ptr.push_back(int(C.src.size()));
+                if (buf[k].first == I) C.diag[I] = int(C.col.size());
+                C.col.push_back(buf[k].first);
+            }
+            C.src.push_back(buf[k].second);
+        }
+        if (!buf.empty()) C.src_ptr.push_back(int(C.src.size()));
+        C.row_ptr[I + 1] = int(C.col.size());
+    }
+}
+
+}  // namespace
+
+std::string build_amg(AmgHierarchy &H, const Topology &T, const std::vector<int> &ijk) {
+    H = AmgHierarchy();
+    const int ninst = T.ninst;
+    if ((int)ijk.size() != 3 * T.nc) return "logical cell indices: need 3 per cell";
+    // instance dimensions from the logical indices
+    std::vector<int> dims(3 * ninst, 0);
+    for (int t = 0; t < ninst; ++t)
+        for (int c = T.inst_cellpos[t]; c < T.inst_cellpos[t + 1]; ++c)
+            for (int d = 0; d < 3; ++d) {
+                if (ijk[3 * c + d] < 0) return "logical cell indices must be non-negative";
+                dims[3 * t + d] = std::max(dims[3 * t + d], ijk[3 * c + d] + 1);
+            }
+    H.lev.emplace_back();  // level 0 is the SELL face matrix
+    // ---- level 1: faces -> 2x2x2 blocks of their owner cells
+    {
+        AmgLevel L1;
+        L1.inst_ptr.assign(ninst + 1, 0);
+        L1.dims.resize(3 * ninst);
+        for (int t = 0; t < ninst; ++t) {
+            long long m = 1;
+            for (int d = 0; d < 3; ++d) {
+                L1.dims[3 * t + d] = (dims[3 * t + d] + 1) / 2;
+                m *= L1.dims[3 * t + d];
+            }
+            L1.inst_ptr[t + 1] = L1.inst_ptr[t] + int(m);
+        }
+        L1.n = L1.inst_ptr[ninst];
+        L1.agg.assign(T.nfp, -1);
+        std::vector<int> inst_of_cell(T.nc);
+        for (int t = 0; t < ninst; ++t)
+            for (int c = T.inst_cellpos[t]; c < T.inst_cellpos[t + 1]; ++c) inst_of_cell[c] = t;
+        for (int f = 0; f < T.nfp; ++f) {
+            const int h = T.dof_hf[f];
+            if (h < 0) continue;
+            const int c = T.hf_cell[h], t = inst_of_cell[c];
+            const int *bd = &L1.dims[3 * t];
+            const int I = ijk[3 * c] / 2, J = ijk[3 * c + 1] / 2, K = ijk[3 * c + 2] / 2;
+            L1.agg[f] = L1.inst_ptr[t] + I + bd[0] * (J + bd[1] * K);
+        }
+        children(L1, T.nfp);
+        galerkin_pattern(L1, [&](int f, auto emit) {
+            const long long base = T.slice_ptr[f / kSlice] + f % kSlice;
+            for (int e = 0; e < T.row_len[f]; ++e) {
+                const long long pos = base + (long long)e * kSlice;
+                emit(T.col[pos], int(pos));
+            }
+        });
+        H.lev.push_back(std::move(L1));
+    }
+    // ---- coarser levels: halve block indices until every instance is small
+    for (;;) {
+        const AmgLevel &F = H.lev.back();
+        int mx = 0;
+        for (int t = 0; t < ninst; ++t) mx = std::max(mx, F.inst_ptr[t + 1] - F.inst_ptr[t]);
+        if (mx <= kMaxCoarsest) break;
+        AmgLevel C;
+        C.inst_ptr.assign(ninst + 1, 0);
+        C.dims.resize(3 * ninst);
+        for (int t = 0; t < ninst; ++t) {
+            long long m = 1;
+            for (int d = 0; d < 3; ++d) {
+                C.dims[3 * t + d] = (F.dims[3 * t + d] + 1) / 2;
+                m *= C.dims[3 * t + d];
+            }
+            C.inst_ptr[t + 1] = C.inst_ptr[t] + int(m);
+        }
+        C.n = C.inst_ptr[ninst];
+        C.agg.assign(F.n, -1);
+        for (int t = 0; t < ninst; ++t) {
+            const int *fd = &F.dims[3 * t], *cd = &C.dims[3 * t];
+            for (int r = F.inst_ptr[t]; r < F.inst_ptr[t + 1]; ++r) {
+                const int loc = r - F.inst_ptr[t];
+                const int i = loc % fd[0], j = (loc / fd[0]) % fd[1], k = loc / (fd[0] * fd[1]);
+                C.agg[r] = C.inst_ptr[t] + i / 2 + cd[0] * (j / 2 + cd[1] * (k / 2));
+            }
+        }
+        const AmgLevel &Fr = H.lev.back();
+        children(C, F.n);
+        galerkin_pattern(C, [&](int r, auto emit) {
+            for (int k = Fr.row_ptr[r]; k < Fr.row_ptr[r + 1]; ++k) emit(Fr.col[k], k);
+        });
+        H.lev.push_back(std::move(C));
+        if (H.lev.size() > 40) return "multigrid hierarchy does not coarsen";
+    }
+    const AmgLevel &Lc = H.lev.back();
+    H.coarse_off.assign(ninst + 1, 0);
+    for (int t = 0; t < ninst; ++t) {
+        const int m = Lc.inst_ptr[t + 1] - Lc.inst_ptr[t];
+        H.coarse_off[t + 1] = H.coarse_off[t] + m * m;
+    }
+    for (size_t l = 1; l < H.lev.size(); ++l)
+        for (int I = 0; I < H.lev[l].n; ++I)
+            if (H.lev[l].diag[I] < 0) return "coarse row without a diagonal entry";
+    return "";
+}
+
+}  // namespace mhs

@@ -1,0 +1,44 @@
+// Aggregation multigrid hierarchy for the face system (host-side set-up).
+//
+// Aggregates are geometric: a face belongs to its owner cell (the cell of its
+// first half-face), and cells are grouped in 2x2x2 blocks of their logical
+// (i, j, k) indices; every coarser level halves the block indices again.  The
+// coarse operators are algebraic, A_{l+1} = P' A_l P with the piecewise-
+// constant P of the aggregates.  Their sparsity and, for every coarse entry,
+// the list of finer entries that sum into it are built here once per
+// structure, so that each solve forms the Galerkin products with a pure gather.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "mhs_structure.h"
+
+namespace mhs {
+
+constexpr int kMaxCoarsest = 64;  // largest coarsest system per instance (dense inverse)
+
+struct AmgLevel {
+    int n = 0;                      // rows (all instances, unpadded)
+    std::vector<int> inst_ptr;      // [ninst+1] first row of each instance
+    std::vector<int> dims;          // [3 * ninst] block dimensions per instance
+    std::vector<int> row_ptr, col;  // CSR pattern (columns sorted)
+    std::vector<int> diag;          // position of the diagonal in each row
+    std::vector<int> src_ptr;       // [nnz+1] gather list ranges
+    std::vector<int> src;           // positions in the finer level's value array
+    std::vector<int> agg;           // [n_finer] row of this level each finer row feeds (-1: none)
+    std::vector<int> child_ptr;     // [n+1]
+    std::vector<int> child;         // finer rows of each row
+};
+
+struct AmgHierarchy {
+    std::vector<AmgLevel> lev;      // lev[0] unused placeholder, lev[1..] coarse levels
+    std::vector<int> coarse_off;    // [ninst+1] offsets of the dense coarsest inverses
+    int nlev() const { return int(lev.size()); }
+};
+
+// Builds the hierarchy from logical cell indices ijk[3*nc] (instance-local,
+// 0-based).  Returns an empty string on success.
+std::string build_amg(AmgHierarchy &H, const Topology &T, const std::vector<int> &ijk);
+
+}  // namespace mhs

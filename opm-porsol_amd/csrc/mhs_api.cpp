@@ -6,6 +6,7 @@
 // per-system state between chunks to decide convergence and, in mixed
 // precision, when to replace the FP32 recursive residual by the FP64 true one.
 #include "../../include/mimetic_hybrid.h"
+#include "mhs_amg.h"
 #include "mhs_kernels.h"
 #include "mhs_structure.h"
 
@@ -32,8 +33,8 @@ struct DevBuf {
 
 struct GraphCache {
     hipGraphExec_t exec = nullptr;
-    int precision = -1, nrhs = 0, chunk = 0;
-    double tol2 = -1.0, rf2 = -1.0;
+    int precision = -1, nrhs = 0, chunk = 0, amg = -1, nu = -1;
+    double tol2 = -1.0, rf2 = -1.0, omega = -1.0, alpha = -1.0;
     const void *key = nullptr;
 };
 
@@ -61,6 +62,14 @@ struct mhs_solver {
 
     // static inner products
     double *binv = nullptr, *gflux = nullptr;
+
+    // multigrid hierarchy
+    std::vector<int> ijk;
+    bool has_amg = false;
+    AmgHierarchy H;
+    DevAmg dA{};
+    double *z64 = nullptr;
+    float *z32 = nullptr;
 
     // per-solve buffers
     int cap_nrhs = 0;
@@ -131,6 +140,9 @@ void mhs_solver::free_topology() {
     binv = gflux = nullptr;
     dT = DevTopo{};
     dG = DevGeom{};
+    dA = DevAmg{};
+    has_amg = false;
+    H = AmgHierarchy();
 }
 
 void mhs_solver::free_solve_buffers() {
@@ -175,6 +187,8 @@ void mhs_default_options(mhs_solve_options *o) {
     o->chunk = 32;
     o->replace_factor = 0.1;
     o->use_graph = 1;
+    o->precond = -1;
+    o->smoother_weight = 0.6;
 }
 
 int mhs_create(mhs_solver **out, int device, void *stream) {
@@ -217,6 +231,7 @@ int mhs_clear(mhs_solver *s) {
     s->facepos.clear();
     s->nb.clear();
     s->cvol.clear(); s->ccen.clear(); s->farea.clear(); s->fcen.clear(); s->fnrm.clear();
+    s->ijk.clear();
     s->nc = s->nhf = 0;
     s->err.clear();
     return MHS_OK;
@@ -252,6 +267,17 @@ int mhs_set_grid(mhs_solver *s, int num_instances, const int *inst_cellpos, int 
     s->fcen.assign(face_centroid, face_centroid + 3 * (size_t)s->nhf);
     s->fnrm.assign(face_normal, face_normal + 3 * (size_t)s->nhf);
     s->grid_set = true;
+    return MHS_OK;
+}
+
+int mhs_set_cell_ijk(mhs_solver *s, const int *ijk) {
+    if (!s) return MHS_ERR_INVALID;
+    if (!s->grid_set) return fail(s, MHS_ERR_STATE, "set_cell_ijk: no grid (call set_grid)");
+    if (!ijk) {
+        s->ijk.clear();
+        return MHS_OK;
+    }
+    s->ijk.assign(ijk, ijk + 3 * (size_t)s->nc);
     return MHS_OK;
 }
 
@@ -292,6 +318,38 @@ int mhs_init_system_structure(mhs_solver *s, const int *hf_bc_type, const int *h
     if (he == hipSuccess) s->binv = dalloc<double>(s, L, (size_t)T.pk_pos[T.nc], &he);
     if (he == hipSuccess) s->gflux = dalloc<double>(s, L, (size_t)T.nhf, &he);
     if (he != hipSuccess) return fail(s, MHS_ERR_ALLOC, std::string("device allocation: ") + hipGetErrorString(he));
+    if (!s->ijk.empty()) {
+        std::string ea = build_amg(s->H, T, s->ijk);
+        if (!ea.empty()) return fail(s, MHS_ERR_INVALID, "init_system_structure: " + ea);
+        if (s->H.nlev() > kMaxLevels) return fail(s, MHS_ERR_INVALID, "too many multigrid levels");
+        DevAmg &A = s->dA;
+        A.nlev = s->H.nlev();
+        A.ninst = T.ninst;
+        for (int l = 1; l < A.nlev && he == hipSuccess; ++l) {
+            const AmgLevel &H = s->H.lev[l];
+            DevAmgLevel &D = A.lev[l];
+            D.n = H.n;
+            D.nnz = H.row_ptr.empty() ? 0 : H.row_ptr.back();
+            D.inst_ptr = dupload(s, L, H.inst_ptr, &he);
+            if (he == hipSuccess) D.row_ptr = dupload(s, L, H.row_ptr, &he);
+            if (he == hipSuccess) D.col = dupload(s, L, H.col, &he);
+            if (he == hipSuccess) D.diag = dupload(s, L, H.diag, &he);
+            if (he == hipSuccess) D.src_ptr = dupload(s, L, H.src_ptr, &he);
+            if (he == hipSuccess) D.src = dupload(s, L, H.src, &he);
+            if (he == hipSuccess) D.agg = dupload(s, L, H.agg, &he);
+            if (he == hipSuccess) D.child_ptr = dupload(s, L, H.child_ptr, &he);
+            if (he == hipSuccess) D.child = dupload(s, L, H.child, &he);
+            if (he == hipSuccess) D.val64 = dalloc<double>(s, L, (size_t)D.nnz, &he);
+            if (he == hipSuccess) D.val32 = dalloc<float>(s, L, (size_t)D.nnz, &he);
+            if (he == hipSuccess) D.dinv64 = dalloc<double>(s, L, (size_t)D.n, &he);
+            if (he == hipSuccess) D.dinv32 = dalloc<float>(s, L, (size_t)D.n, &he);
+        }
+        if (he == hipSuccess) A.coarse_off = dupload(s, L, s->H.coarse_off, &he);
+        if (he == hipSuccess) A.cinv64 = dalloc<double>(s, L, (size_t)s->H.coarse_off.back(), &he);
+        if (he == hipSuccess) A.cinv32 = dalloc<float>(s, L, (size_t)s->H.coarse_off.back(), &he);
+        if (he != hipSuccess) return fail(s, MHS_ERR_ALLOC, std::string("multigrid allocation: ") + hipGetErrorString(he));
+        s->has_amg = true;
+    }
     s->structure_valid = true;
     return MHS_OK;
 }
@@ -396,6 +454,28 @@ int ensure_solve_buffers(mhs_solver *s, int nrhs, bool mixed) {
     if (ok()) s->S.upd = dalloc<int>(s, L, nsys, &e);
     if (ok()) s->S.status = dalloc<int>(s, L, 4, &e);
     if (ok()) s->maxbits = dalloc<unsigned long long>(s, L, 1, &e);
+    s->z64 = nullptr;
+    s->z32 = nullptr;
+    if (s->has_amg) {
+        const size_t vs = mixed ? sizeof(float) : sizeof(double);
+        DevAmg &A = s->dA;
+        auto vec = [&](size_t n) -> void * {
+            void *p = nullptr;
+            if (ok()) p = dalloc<unsigned char>(s, L, std::max<size_t>(n, 1) * nrhs * vs, &e);
+            return p;
+        };
+        A.x0 = vec(nfp);
+        A.res0 = vec(nfp);
+        A.xt0 = vec(nfp);
+        for (int l = 1; l < A.nlev; ++l) {
+            A.lev[l].b = vec(A.lev[l].n);
+            A.lev[l].x = vec(A.lev[l].n);
+            A.lev[l].res = vec(A.lev[l].n);
+            A.lev[l].xt = vec(A.lev[l].n);
+        }
+        if (mixed) { if (ok()) s->z32 = dalloc<float>(s, L, nv, &e); }
+        else { if (ok()) s->z64 = dalloc<double>(s, L, nv, &e); }
+    }
     if (ok()) e = hipMemset(s->R.counters, 0, sizeof(int) * T.ninst);
     if (ok()) e = hipMemset(s->x64, 0, nv * sizeof(double));
     if (ok()) e = hipHostMalloc(reinterpret_cast<void **>(&s->h_sc), nsys * SC_N * sizeof(double), 0);
@@ -418,20 +498,30 @@ int ensure_solve_buffers(mhs_solver *s, int nrhs, bool mixed) {
 
 // Launch `chunk` PCG iterations, through a cached hipGraph when enabled.
 template <typename MT, typename VT>
+hipError_t one_iteration(mhs_solver *s, int nrhs, bool amg, const MT *val, const VT *dinv, VT *x, VT *r,
+                         VT *p, VT *q, VT *z, double tol2) {
+    if (amg)
+        return launch_pcg_iteration_amg<MT, VT>(s->stream, s->dT, s->dA, nrhs, val, dinv, x, r, p, q, z, s->R,
+                                                s->S, s->val64, s->b, s->x64, tol2);
+    return launch_pcg_iteration<MT, VT>(s->stream, s->dT, nrhs, val, dinv, x, r, p, q, s->R, s->S, s->val64,
+                                        s->b, s->x64, tol2);
+}
+
+template <typename MT, typename VT>
 int run_chunk(mhs_solver *s, int nrhs, int chunk, bool use_graph, const MT *val, const VT *dinv,
-              VT *x, VT *r, VT *p, VT *q, int precision, double tol2) {
+              VT *x, VT *r, VT *p, VT *q, VT *z, bool amg, int precision, double tol2) {
     if (use_graph) {
         GraphCache &G = s->graph;
         if (!(G.exec && G.precision == precision && G.nrhs == nrhs && G.chunk == chunk &&
-              G.key == (const void *)val && G.tol2 == tol2 && G.rf2 == s->S.rf2)) {
+              G.key == (const void *)val && G.tol2 == tol2 && G.rf2 == s->S.rf2 && G.amg == int(amg) &&
+              G.omega == s->dA.omega && G.alpha == s->dA.alpha && G.nu == s->dA.nu)) {
             if (G.exec) (void)hipGraphExecDestroy(G.exec);
             G = GraphCache{};
             hipGraph_t graph = nullptr;
             HIPCHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
             hipError_t e = hipSuccess;
             for (int it = 0; it < chunk && e == hipSuccess; ++it)
-                e = launch_pcg_iteration<MT, VT>(s->stream, s->dT, nrhs, val, dinv, x, r, p, q, s->R,
-                                                 s->S, s->val64, s->b, s->x64, tol2);
+                e = one_iteration<MT, VT>(s, nrhs, amg, val, dinv, x, r, p, q, z, tol2);
             hipError_t e2 = hipStreamEndCapture(s->stream, &graph);
             if (e != hipSuccess || e2 != hipSuccess) {
                 if (graph) (void)hipGraphDestroy(graph);
@@ -450,12 +540,15 @@ int run_chunk(mhs_solver *s, int nrhs, int chunk, bool use_graph, const MT *val,
             G.key = val;
             G.tol2 = tol2;
             G.rf2 = s->S.rf2;
+            G.amg = int(amg);
+            G.omega = s->dA.omega;
+            G.alpha = s->dA.alpha;
+            G.nu = s->dA.nu;
         }
         HIPCHECK(hipGraphLaunch(G.exec, s->stream));
     } else {
         for (int it = 0; it < chunk; ++it)
-            HIPCHECK((launch_pcg_iteration<MT, VT>(s->stream, s->dT, nrhs, val, dinv, x, r, p, q, s->R,
-                                                   s->S, s->val64, s->b, s->x64, tol2)));
+            HIPCHECK((one_iteration<MT, VT>(s, nrhs, amg, val, dinv, x, r, p, q, z, tol2)));
     }
     return MHS_OK;
 }
@@ -546,10 +639,29 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
     HIPCHECK(hipStreamSynchronize(s->stream));
     auto t1 = std::chrono::steady_clock::now();
 
+    // --- preconditioner ---------------------------------------------------------
+    bool amg = false;
+    if (opt.precond == 1) {
+        if (!s->has_amg)
+            return fail(s, MHS_ERR_INVALID, "solve: multigrid needs logical cell indices (mhs_set_cell_ijk)");
+        amg = true;
+    } else if (opt.precond == -1) {
+        amg = s->has_amg && opt.linsolver_type != 0;
+    } else if (opt.precond != 0) {
+        return fail(s, MHS_ERR_INVALID, "solve: unknown preconditioner");
+    }
+    if (amg) {
+        s->dA.omega = opt.smoother_weight;
+        s->dA.alpha = opt.prolongate_factor;
+        s->dA.nu = std::max(1, opt.smooth_steps);
+        HIPCHECK(launch_amg_setup(s->stream, s->dT, s->dA, s->val64, mixed));
+    }
+
     // --- linear solve ----------------------------------------------------------
     const double tol2 = opt.tol * opt.tol;
     int total_iters = 0;
     s->S.mixed = mixed ? 1 : 0;
+    s->S.amg = amg ? 1 : 0;
     s->S.rf2 = opt.replace_factor * opt.replace_factor;
     HIPCHECK(hipMemsetAsync(s->S.status, 0, 4 * sizeof(int), s->stream));
     HIPCHECK(hipMemsetAsync(s->S.iters, 0, nsys * sizeof(int), s->stream));
@@ -562,6 +674,14 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
         HIPCHECK(launch_reliable(s->stream, s->dT, nrhs, s->val64, s->b, s->x64, s->dinv32, s->r32,
                                  s->p32, 1, tol2, s->R, s->S));
     }
+    if (amg) {
+        if (!mixed)
+            HIPCHECK(launch_amg_start<double>(s->stream, s->dT, s->dA, nrhs, s->val64, s->dinv64, s->r64,
+                                              s->z64, s->p64, s->R, s->S, nullptr, s->x64));
+        else
+            HIPCHECK(launch_amg_start<float>(s->stream, s->dT, s->dA, nrhs, s->val32, s->dinv32, s->r32,
+                                             s->z32, s->p32, s->R, s->S, nullptr, s->x32));
+    }
     rc = read_state(s, nsys, false);
     if (rc) return rc;
     while (s->h_status[0] > 0) {
@@ -571,10 +691,10 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
         const int n = std::min(chunk, maxit - mx);
         if (!mixed)
             rc = run_chunk<double, double>(s, nrhs, n, opt.use_graph && n == chunk, s->val64, s->dinv64,
-                                           s->x64, s->r64, s->p64, s->q64, MHS_FP64, tol2);
+                                           s->x64, s->r64, s->p64, s->q64, s->z64, amg, MHS_FP64, tol2);
         else
             rc = run_chunk<float, float>(s, nrhs, n, opt.use_graph && n == chunk, s->val32, s->dinv32,
-                                         s->x32, s->r32, s->p32, s->q32, MHS_MIXED, tol2);
+                                         s->x32, s->r32, s->p32, s->q32, s->z32, amg, MHS_MIXED, tol2);
         if (rc) return rc;
         rc = read_state(s, nsys, opt.verbosity >= 2);
         if (rc) return rc;
@@ -618,6 +738,8 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
         rep->replacements = replacements;
         rep->converged = converged ? 1 : 0;
         rep->reduction = red;
+        rep->precond = amg ? 1 : 0;
+        rep->levels = amg ? s->dA.nlev : 0;
         rep->assemble_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
         rep->solve_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
         rep->recover_ms = std::chrono::duration<double, std::milli>(t3 - t2).count();

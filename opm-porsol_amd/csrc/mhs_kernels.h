@@ -51,7 +51,32 @@ struct DevScal {
     int *upd;          // [nsys] 1: FP64 residual replacement due, 2: also restart p
     int *status;       // [4]: 0 nactive, 1 breakdown count
     int mixed;         // FP32 iteration with FP64 residual replacement
+    int amg;           // multigrid preconditioner (z stored, beta from <r, z>)
     double rf2;        // replacement trigger: |r|^2 <= rf2 * |r at last replacement|^2
+};
+
+// Multigrid hierarchy on the device (level 0 is the SELL face matrix).
+constexpr int kMaxLevels = 24;
+struct DevAmgLevel {
+    int n, nnz;
+    const int *inst_ptr;            // [ninst+1] first row of each instance
+    const int *row_ptr, *col, *diag, *src_ptr, *src;
+    const int *agg;                 // rows of the finer level -> rows of this level
+    const int *child_ptr, *child;   // rows of this level -> rows of the finer level
+    double *val64, *dinv64;
+    float *val32, *dinv32;
+    void *b, *x, *res, *xt;         // work vectors (PCG precision, n * nrhs)
+};
+struct DevAmg {
+    int nlev;                       // levels including level 0
+    int ninst;
+    DevAmgLevel lev[kMaxLevels];
+    void *x0, *res0, *xt0;          // level-0 work vectors (nfp * nrhs)
+    const int *coarse_off;          // [ninst+1] offsets of the dense coarsest inverses
+    double *cinv64;
+    float *cinv32;
+    double omega, alpha;            // Jacobi weight, coarse-correction (prolongation) factor
+    int nu;                         // smoothing sweeps before and after the coarse correction
 };
 
 hipError_t launch_static_ip(hipStream_t st, const DevTopo &T, const DevGeom &G, int maxn,
@@ -94,6 +119,25 @@ hipError_t launch_reliable(hipStream_t st, const DevTopo &T, int nrhs, const dou
 hipError_t launch_merge_all(hipStream_t st, const DevTopo &T, int nrhs, double *x64, float *x32);
 hipError_t launch_resnorm(hipStream_t st, const DevTopo &T, int nrhs, const double *val64,
                           const double *b, const double *x64, DevRed R, DevScal S);
+
+// Multigrid: per-solve set-up (Galerkin products, diagonals, coarsest inverses)
+// and the preconditioned iteration pieces.
+hipError_t launch_amg_setup(hipStream_t st, const DevTopo &T, const DevAmg &A, const double *val64,
+                            bool mixed);
+template <typename VT>
+hipError_t launch_vcycle(hipStream_t st, const DevTopo &T, const DevAmg &A, int nrhs, const VT *val0,
+                         const VT *dinv0, const VT *r, VT *z, DevRed R);
+template <typename MT, typename VT>
+hipError_t launch_pcg_iteration_amg(hipStream_t st, const DevTopo &T, const DevAmg &A, int nrhs,
+                                    const MT *val, const VT *dinv, VT *x, VT *r, VT *p, VT *q, VT *z,
+                                    DevRed R, DevScal S, const double *val64, const double *b,
+                                    double *x64, double tol2);
+// z = M^-1 r, rz, p = z (start of the preconditioned iteration after
+// launch_pcg_start64 / launch_reliable(init)).
+template <typename VT>
+hipError_t launch_amg_start(hipStream_t st, const DevTopo &T, const DevAmg &A, int nrhs, const VT *val0,
+                            const VT *dinv0, const VT *r, VT *z, VT *p, DevRed R, DevScal S,
+                            double *x64, VT *x);
 
 // Recovery and post-processing.
 hipError_t launch_recover(hipStream_t st, const DevTopo &T, int maxn, int nrhs,

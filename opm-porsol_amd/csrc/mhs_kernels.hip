@@ -20,8 +20,10 @@
 //    k_post        postProcessFluxes (IncompFlowSolverHybrid.hpp:754-807)
 #include "mhs_kernels.h"
 #include "mhs_structure.h"
+#include "mhs_amg.h"
 
 #include <type_traits>
+#include <utility>
 
 namespace mhs {
 
@@ -732,6 +734,342 @@ __global__ __launch_bounds__(256) void k_resnorm(DevTopo T, const double *__rest
     block_partials<NRHS>(part, b, R);
 }
 
+
+// ---------------------------------------------------------------------------
+// Multigrid kernels.
+//
+// V-cycle on level l (b given, x from zero):
+//   down: x = w D^-1 b (nu-1 more Jacobi sweeps); res = b - A x; b_{l+1} = P' res
+//   up:   x~ = x + alpha P x_{l+1};  x = x~ + w D^-1 (b - A x~)  (nu sweeps)
+// Level 0 is the SELL face matrix (rows in the PCG's padded numbering), the
+// coarse levels are CSR.  The coarsest level is solved with a dense inverse
+// per instance.  With symmetric smoothing the V-cycle is a fixed symmetric
+// operator, as PCG requires.
+
+// Galerkin product A_c = P' A P with piecewise-constant P: every coarse entry
+// is the sum of the finer entries listed for it (no atomics).
+__global__ __launch_bounds__(256) void k_galerkin(int nnz, const int *__restrict__ src_ptr,
+                                                  const int *__restrict__ src,
+                                                  const double *__restrict__ fine,
+                                                  double *__restrict__ out64,
+                                                  float *__restrict__ out32) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= nnz) return;
+    double s = 0.0;
+    for (int k = src_ptr[e]; k < src_ptr[e + 1]; ++k) s += fine[src[k]];
+    out64[e] = s;
+    if (out32) out32[e] = (float)s;
+}
+
+__global__ __launch_bounds__(256) void k_coarse_diag(int n, const int *__restrict__ diag,
+                                                     const double *__restrict__ val,
+                                                     double *__restrict__ dinv64,
+                                                     float *__restrict__ dinv32) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double d = 1.0 / val[diag[i]];
+    dinv64[i] = d;
+    if (dinv32) dinv32[i] = (float)d;
+}
+
+// Dense inverse of each instance's coarsest operator (Gauss-Jordan, no
+// pivoting: the Galerkin operators are symmetric positive definite).
+__global__ __launch_bounds__(256) void k_coarse_inverse(DevAmgLevel L, const int *__restrict__ inst_ptr,
+                                                        const int *__restrict__ off,
+                                                        double *__restrict__ inv64,
+                                                        float *__restrict__ inv32) {
+    constexpr int M = kMaxCoarsest;
+    __shared__ double A[M][M + 1];
+    __shared__ double colk[M];
+    const int t = blockIdx.x;
+    const int r0 = inst_ptr[t], m = inst_ptr[t + 1] - r0;
+    for (int e = threadIdx.x; e < M * M; e += 256) A[e / M][e % M] = 0.0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < m; i += 256)
+        for (int k = L.row_ptr[r0 + i]; k < L.row_ptr[r0 + i + 1]; ++k)
+            A[i][L.col[k] - r0] = L.val64[k];
+    __syncthreads();
+    for (int k = 0; k < m; ++k) {
+        const double piv = A[k][k];
+        __syncthreads();
+        for (int j = threadIdx.x; j < m; j += 256) A[k][j] = (j == k ? 1.0 : A[k][j]) / piv;
+        __syncthreads();
+        for (int i = threadIdx.x; i < m; i += 256) colk[i] = A[i][k];
+        __syncthreads();
+        for (int e = threadIdx.x; e < m * m; e += 256) {
+            const int i = e / m, j = e % m;
+            if (i == k) continue;
+            const double base = (j == k) ? 0.0 : A[i][j];
+            A[i][j] = base - colk[i] * A[k][j];
+        }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < m * m; e += 256) {
+        const double v = A[e / m][e % m];
+        inv64[off[t] + e] = v;
+        if (inv32) inv32[off[t] + e] = (float)v;
+    }
+}
+
+template <typename VT>
+__device__ __forceinline__ const VT *lvl_val(const DevAmgLevel &L) {
+    if constexpr (std::is_same<VT, float>::value) return L.val32; else return L.val64;
+}
+template <typename VT>
+__device__ __forceinline__ const VT *lvl_dinv(const DevAmgLevel &L) {
+    if constexpr (std::is_same<VT, float>::value) return L.dinv32; else return L.dinv64;
+}
+
+// Level 0, down: x0 = w D^-1 b and res = b - A x0, with x0 formed on the fly.
+template <typename VT, int NRHS>
+__global__ __launch_bounds__(256) void k_vc0_down(DevTopo T, const VT *__restrict__ val,
+                                                  const VT *__restrict__ dinv,
+                                                  const VT *__restrict__ b, VT *__restrict__ x,
+                                                  VT *__restrict__ res, VT w) {
+    const int f = blockIdx.x * 256 + threadIdx.x;
+    const int slice = f >> 6, lane = f & 63;
+    const long long s0 = T.slice_ptr[slice];
+    const int wd = (int)((T.slice_ptr[slice + 1] - s0) >> 6);
+    VT acc[NRHS];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) acc[k] = VT(0);
+    for (int e = 0; e < wd; ++e) {
+        const long long o = s0 + lane + 64LL * e;
+        const int c = __builtin_nontemporal_load(T.col + o);
+        const VT a = __builtin_nontemporal_load(val + o) * (w * dinv[c]);
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) acc[k] += a * b[(size_t)c * NRHS + k];
+    }
+    const VT d = w * dinv[f];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) {
+        const size_t i = (size_t)f * NRHS + k;
+        x[i] = d * b[i];
+        res[i] = b[i] - acc[k];
+    }
+}
+
+// Level 0, one more Jacobi sweep: xo = x + w D^-1 (b - A x~), x~ = x + alpha P xc
+// (xc == nullptr: no coarse correction).  With z/partials: the final sweep of
+// the V-cycle writes the preconditioned residual and <r, z> partials.
+template <typename VT, int NRHS, bool LAST>
+__global__ __launch_bounds__(256) void k_vc0_up(DevTopo T, const VT *__restrict__ val,
+                                                const VT *__restrict__ dinv,
+                                                const int *__restrict__ agg,
+                                                const VT *__restrict__ b, const VT *__restrict__ x,
+                                                const VT *__restrict__ xc, VT *__restrict__ xo,
+                                                VT w, VT alpha, DevRed R) {
+    const int blk = xcd_block(gridDim.x);
+    double part[NRHS];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) part[k] = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int f = blk * kRowBlock + j * 256 + threadIdx.x;
+        const int slice = f >> 6, lane = f & 63;
+        const long long s0 = T.slice_ptr[slice];
+        const int wd = (int)((T.slice_ptr[slice + 1] - s0) >> 6);
+        VT acc[NRHS];
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) acc[k] = VT(0);
+        for (int e = 0; e < wd; ++e) {
+            const long long o = s0 + lane + 64LL * e;
+            const int c = __builtin_nontemporal_load(T.col + o);
+            const VT a = __builtin_nontemporal_load(val + o);
+            const int gc = xc ? agg[c] : -1;
+#pragma unroll
+            for (int k = 0; k < NRHS; ++k) {
+                VT xv = x[(size_t)c * NRHS + k];
+                if (gc >= 0) xv += alpha * xc[(size_t)gc * NRHS + k];
+                acc[k] += a * xv;
+            }
+        }
+        const VT d = w * dinv[f];
+        const int g = xc ? agg[f] : -1;
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) {
+            const size_t i = (size_t)f * NRHS + k;
+            VT xv = x[i];
+            if (g >= 0) xv += alpha * xc[(size_t)g * NRHS + k];
+            const VT out = xv + d * (b[i] - acc[k]);
+            xo[i] = out;
+            if (LAST) part[k] += (double)b[i] * (double)out;
+        }
+    }
+    if (LAST) block_partials<NRHS>(part, blk, R);
+}
+
+// Level 0: res = b - A x (after extra down sweeps).
+template <typename VT, int NRHS>
+__global__ __launch_bounds__(256) void k_vc0_res(DevTopo T, const VT *__restrict__ val,
+                                                 const VT *__restrict__ b, const VT *__restrict__ x,
+                                                 VT *__restrict__ res) {
+    const int f = blockIdx.x * 256 + threadIdx.x;
+    VT acc[NRHS];
+    sell_row<VT, VT, VT, NRHS>(T, val, x, f, acc);
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) res[(size_t)f * NRHS + k] = b[(size_t)f * NRHS + k] - acc[k];
+}
+
+// Coarse levels (CSR).
+template <typename VT, int NRHS>
+__global__ __launch_bounds__(256) void k_vcc_down(DevAmgLevel L, const VT *__restrict__ b,
+                                                  VT *__restrict__ x, VT *__restrict__ res, VT w) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= L.n) return;
+    const VT *val = lvl_val<VT>(L);
+    const VT *dinv = lvl_dinv<VT>(L);
+    VT acc[NRHS];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) acc[k] = VT(0);
+    for (int e = L.row_ptr[i]; e < L.row_ptr[i + 1]; ++e) {
+        const int c = L.col[e];
+        const VT a = val[e] * (w * dinv[c]);
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) acc[k] += a * b[(size_t)c * NRHS + k];
+    }
+    const VT d = w * dinv[i];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) {
+        const size_t q = (size_t)i * NRHS + k;
+        x[q] = d * b[q];
+        res[q] = b[q] - acc[k];
+    }
+}
+
+template <typename VT, int NRHS>
+__global__ __launch_bounds__(256) void k_vcc_res(DevAmgLevel L, const VT *__restrict__ b,
+                                                 const VT *__restrict__ x, VT *__restrict__ res) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= L.n) return;
+    const VT *val = lvl_val<VT>(L);
+    VT acc[NRHS];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) acc[k] = VT(0);
+    for (int e = L.row_ptr[i]; e < L.row_ptr[i + 1]; ++e) {
+        const int c = L.col[e];
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) acc[k] += val[e] * x[(size_t)c * NRHS + k];
+    }
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) res[(size_t)i * NRHS + k] = b[(size_t)i * NRHS + k] - acc[k];
+}
+
+template <typename VT, int NRHS>
+__global__ __launch_bounds__(256) void k_vcc_up(DevAmgLevel L, const int *__restrict__ agg,
+                                                const VT *__restrict__ b, const VT *__restrict__ x,
+                                                const VT *__restrict__ xc, VT *__restrict__ xo, VT w,
+                                                VT alpha) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= L.n) return;
+    const VT *val = lvl_val<VT>(L);
+    const VT *dinv = lvl_dinv<VT>(L);
+    VT acc[NRHS];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) acc[k] = VT(0);
+    for (int e = L.row_ptr[i]; e < L.row_ptr[i + 1]; ++e) {
+        const int c = L.col[e];
+        const int gc = xc ? agg[c] : -1;
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) {
+            VT xv = x[(size_t)c * NRHS + k];
+            if (gc >= 0) xv += alpha * xc[(size_t)gc * NRHS + k];
+            acc[k] += val[e] * xv;
+        }
+    }
+    const VT d = w * dinv[i];
+    const int g = xc ? agg[i] : -1;
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) {
+        const size_t q = (size_t)i * NRHS + k;
+        VT xv = x[q];
+        if (g >= 0) xv += alpha * xc[(size_t)g * NRHS + k];
+        xo[q] = xv + d * (b[q] - acc[k]);
+    }
+}
+
+// Restriction b_c = P' res: sum over the coarse row's children.
+template <typename VT, int NRHS>
+__global__ __launch_bounds__(256) void k_restrict(int n, const int *__restrict__ child_ptr,
+                                                  const int *__restrict__ child,
+                                                  const VT *__restrict__ res, VT *__restrict__ bc) {
+    const int I = blockIdx.x * 256 + threadIdx.x;
+    if (I >= n) return;
+    VT acc[NRHS];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) acc[k] = VT(0);
+    for (int e = child_ptr[I]; e < child_ptr[I + 1]; ++e) {
+        const int c = child[e];
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) acc[k] += res[(size_t)c * NRHS + k];
+    }
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) bc[(size_t)I * NRHS + k] = acc[k];
+}
+
+// Coarsest level: x = A^-1 b with the per-instance dense inverse.
+template <typename VT, int NRHS>
+__global__ __launch_bounds__(256) void k_coarse_solve(const int *__restrict__ inst_ptr,
+                                                      const int *__restrict__ off,
+                                                      const VT *__restrict__ inv,
+                                                      const VT *__restrict__ b, VT *__restrict__ x) {
+    const int t = blockIdx.x;
+    const int r0 = inst_ptr[t], m = inst_ptr[t + 1] - r0;
+    const VT *A = inv + off[t];
+    for (int e = threadIdx.x; e < m * NRHS; e += 256) {
+        const int i = e / NRHS, k = e % NRHS;
+        VT s = VT(0);
+        for (int j = 0; j < m; ++j) s += A[i * m + j] * b[(size_t)(r0 + j) * NRHS + k];
+        x[(size_t)(r0 + i) * NRHS + k] = s;
+    }
+}
+
+// p = z + beta p (beta = 0 on a restart); in FP32 also fold x32 into x64 for
+// systems whose residual replacement is due.
+template <typename VT, int NRHS>
+__global__ __launch_bounds__(256) void k_update_pz(DevTopo T, const VT *__restrict__ z,
+                                                   VT *__restrict__ p, VT *__restrict__ x,
+                                                   double *__restrict__ x64, DevScal S) {
+    const int b = xcd_block(gridDim.x);
+    const int s = T.block_seg[b];
+    int act[NRHS];
+    if (!load_act<NRHS>(S.act, s, act)) return;
+    VT be[NRHS];
+    int u[NRHS];
+    bool any_u = false;
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) {
+        const int sys = s * NRHS + k;
+        u[k] = (x64 && act[k]) ? S.upd[sys] : 0;
+        any_u |= u[k] != 0;
+        be[k] = (VT)S.sc[(size_t)sys * SC_N + SC_BETA];
+    }
+    const int f0 = b * kRowBlock + 4 * threadIdx.x;
+    const size_t e0 = (size_t)f0 * NRHS;
+    VT zv[4 * NRHS], pv[4 * NRHS];
+    vload<VT, 4 * NRHS>(z + e0, zv);
+    vload<VT, 4 * NRHS>(p + e0, pv);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) {
+            const int i = j * NRHS + k;
+            if (act[k]) pv[i] = zv[i] + be[k] * pv[i];
+        }
+    vstore<VT, 4 * NRHS>(p + e0, pv);
+    if (any_u) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int k = 0; k < NRHS; ++k) {
+                if (!u[k]) continue;
+                const size_t i = e0 + j * NRHS + k;
+                x64[i] += (double)x[i];
+                x[i] = VT(0);
+            }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Recovery, per cell: p = (g + F.pi)/L;  v = totmob B^-1 (p - pi) + rhomob gflux.
 template <int MAXN, int NRHS>
@@ -840,7 +1178,7 @@ struct FinPQ {
                 const double rrt = tot[NRHS + 3 * k];
                 sc[SC_RRT] = rrt;
                 sc[SC_RRL] = rrt;
-                sc[SC_RZ] = tot[NRHS + 3 * k + 1];
+                if (!S.amg) sc[SC_RZ] = tot[NRHS + 3 * k + 1];
                 sc[SC_RR] = tot[NRHS + 3 * k + 2];
                 S.upd[i] = 0;
                 atomicAdd(&S.status[2], 1);
@@ -887,15 +1225,35 @@ struct FinXR {
                 // FP32 recursion: ask for the FP64 true residual once the recursive
                 // one has dropped by replace_factor (or reached the tolerance).
                 if (S.upd[i] == 0 && (rr <= sc[SC_THR] || rr <= S.rf2 * sc[SC_RRL])) S.upd[i] = 1;
-                sc[SC_BETA] = rz / sc[SC_RZ];
-                sc[SC_RZ] = rz;
+                if (!S.amg) {
+                    sc[SC_BETA] = rz / sc[SC_RZ];
+                    sc[SC_RZ] = rz;
+                }
             } else if (rr <= sc[SC_THR]) {
                 S.act[i] = 0;
                 atomicSub(&S.status[0], 1);
-            } else {
+            } else if (!S.amg) {
                 sc[SC_BETA] = rz / sc[SC_RZ];
                 sc[SC_RZ] = rz;
             }
+        }
+    }
+};
+
+// after the multigrid V-cycle: beta = <r, z> / <r, z>_old (init: p = z)
+template <int NRHS>
+struct FinRZ {
+    __device__ static bool skip(int s, const DevScal &S, const FinArgs &) { return none_set<NRHS>(S.act, s); }
+    __device__ static int nq(int, const DevScal &) { return NRHS; }
+    __device__ static void run(int s, const double *tot, DevScal &S, const FinArgs &A) {
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) {
+            const int i = s * NRHS + k;
+            if (!S.act[i]) continue;
+            double *sc = S.sc + (size_t)i * SC_N;
+            const double rz = tot[k];
+            sc[SC_BETA] = (A.init || S.upd[i] == 2) ? 0.0 : rz / sc[SC_RZ];
+            sc[SC_RZ] = rz;
         }
     }
 };
@@ -1117,6 +1475,166 @@ hipError_t launch_resnorm(hipStream_t st, const DevTopo &T, int nrhs, const doub
         k_finish<NR, FinRes<NR>><<<T.ninst, 256, 0, st>>>(R, S, FinArgs{0.0, 0});
     });
 }
+
+
+hipError_t launch_amg_setup(hipStream_t st, const DevTopo &T, const DevAmg &A, const double *val64,
+                            bool mixed) {
+    for (int l = 1; l < A.nlev; ++l) {
+        const DevAmgLevel &L = A.lev[l];
+        const double *fine = (l == 1) ? val64 : A.lev[l - 1].val64;
+        if (L.nnz > 0)
+            k_galerkin<<<(L.nnz + 255) / 256, 256, 0, st>>>(L.nnz, L.src_ptr, L.src, fine, L.val64,
+                                                           mixed ? L.val32 : nullptr);
+        if (L.n > 0)
+            k_coarse_diag<<<(L.n + 255) / 256, 256, 0, st>>>(L.n, L.diag, L.val64, L.dinv64,
+                                                            mixed ? L.dinv32 : nullptr);
+    }
+    const DevAmgLevel &C = A.lev[A.nlev - 1];
+    k_coarse_inverse<<<A.ninst, 256, 0, st>>>(C, C.inst_ptr, A.coarse_off, A.cinv64,
+                                              mixed ? A.cinv32 : nullptr);
+    return hipGetLastError();
+}
+
+template <typename VT, int NR>
+static void vcycle_impl(hipStream_t st, const DevTopo &T, const DevAmg &A, const VT *val0,
+                        const VT *dinv0, const VT *r, VT *z, DevRed R) {
+    const VT w = (VT)A.omega, al = (VT)A.alpha;
+    const int nu = A.nu < 1 ? 1 : A.nu;
+    const int rows_blocks = T.nfp / 256;
+    VT *x0 = (VT *)A.x0, *xt0 = (VT *)A.xt0, *res0 = (VT *)A.res0;
+    // ---- level 0 down
+    k_vc0_down<VT, NR><<<rows_blocks, 256, 0, st>>>(T, val0, dinv0, r, x0, res0, w);
+    if (nu > 1) {
+        for (int it = 1; it < nu; ++it) {
+            k_vc0_up<VT, NR, false><<<T.nblk, 256, 0, st>>>(T, val0, dinv0, nullptr, r, x0, nullptr, xt0, w,
+                                                            VT(0), R);
+            std::swap(x0, xt0);
+        }
+        k_vc0_res<VT, NR><<<rows_blocks, 256, 0, st>>>(T, val0, r, x0, res0);
+    }
+    const int L = A.nlev;
+    VT *xcur[kMaxLevels], *xalt[kMaxLevels];
+    for (int l = 1; l < L; ++l) {
+        xcur[l] = (VT *)A.lev[l].x;
+        xalt[l] = (VT *)A.lev[l].xt;
+    }
+    {
+        const DevAmgLevel &C = A.lev[1];
+        k_restrict<VT, NR><<<(C.n + 255) / 256, 256, 0, st>>>(C.n, C.child_ptr, C.child, res0, (VT *)C.b);
+    }
+    for (int l = 1; l < L - 1; ++l) {
+        const DevAmgLevel &F = A.lev[l];
+        const unsigned nb = (F.n + 255) / 256;
+        k_vcc_down<VT, NR><<<nb, 256, 0, st>>>(F, (const VT *)F.b, xcur[l], (VT *)F.res, w);
+        if (nu > 1) {
+            for (int it = 1; it < nu; ++it) {
+                k_vcc_up<VT, NR><<<nb, 256, 0, st>>>(F, nullptr, (const VT *)F.b, xcur[l], nullptr, xalt[l],
+                                                     w, VT(0));
+                std::swap(xcur[l], xalt[l]);
+            }
+            k_vcc_res<VT, NR><<<nb, 256, 0, st>>>(F, (const VT *)F.b, xcur[l], (VT *)F.res);
+        }
+        const DevAmgLevel &C = A.lev[l + 1];
+        k_restrict<VT, NR><<<(C.n + 255) / 256, 256, 0, st>>>(C.n, C.child_ptr, C.child, (const VT *)F.res,
+                                                              (VT *)C.b);
+    }
+    {
+        const DevAmgLevel &C = A.lev[L - 1];
+        const VT *inv;
+        if constexpr (std::is_same<VT, float>::value) inv = A.cinv32; else inv = A.cinv64;
+        k_coarse_solve<VT, NR><<<A.ninst, 256, 0, st>>>(C.inst_ptr, A.coarse_off, inv, (const VT *)C.b,
+                                                        xcur[L - 1]);
+    }
+    for (int l = L - 2; l >= 1; --l) {
+        const DevAmgLevel &F = A.lev[l];
+        const unsigned nb = (F.n + 255) / 256;
+        k_vcc_up<VT, NR><<<nb, 256, 0, st>>>(F, A.lev[l + 1].agg, (const VT *)F.b, xcur[l], xcur[l + 1],
+                                             xalt[l], w, al);
+        std::swap(xcur[l], xalt[l]);
+        for (int it = 1; it < nu; ++it) {
+            k_vcc_up<VT, NR><<<nb, 256, 0, st>>>(F, nullptr, (const VT *)F.b, xcur[l], nullptr, xalt[l], w,
+                                                 VT(0));
+            std::swap(xcur[l], xalt[l]);
+        }
+    }
+    // ---- level 0 up: coarse correction + nu sweeps, the last one writes z
+    const VT *xc1 = (L > 1) ? xcur[1] : nullptr;
+    if (nu == 1) {
+        k_vc0_up<VT, NR, true><<<T.nblk, 256, 0, st>>>(T, val0, dinv0, A.lev[1].agg, r, x0, xc1, z, w, al, R);
+    } else {
+        k_vc0_up<VT, NR, false><<<T.nblk, 256, 0, st>>>(T, val0, dinv0, A.lev[1].agg, r, x0, xc1, xt0, w, al,
+                                                        R);
+        std::swap(x0, xt0);
+        for (int it = 1; it < nu - 1; ++it) {
+            k_vc0_up<VT, NR, false><<<T.nblk, 256, 0, st>>>(T, val0, dinv0, nullptr, r, x0, nullptr, xt0, w,
+                                                            VT(0), R);
+            std::swap(x0, xt0);
+        }
+        k_vc0_up<VT, NR, true><<<T.nblk, 256, 0, st>>>(T, val0, dinv0, nullptr, r, x0, nullptr, z, w, VT(0),
+                                                       R);
+    }
+}
+
+template <typename VT>
+hipError_t launch_vcycle(hipStream_t st, const DevTopo &T, const DevAmg &A, int nrhs, const VT *val0,
+                         const VT *dinv0, const VT *r, VT *z, DevRed R) {
+    return with_nrhs(nrhs, [&](auto nr) {
+        constexpr int NR = decltype(nr)::value;
+        vcycle_impl<VT, NR>(st, T, A, val0, dinv0, r, z, R);
+    });
+}
+template hipError_t launch_vcycle<double>(hipStream_t, const DevTopo &, const DevAmg &, int, const double *,
+                                          const double *, const double *, double *, DevRed);
+template hipError_t launch_vcycle<float>(hipStream_t, const DevTopo &, const DevAmg &, int, const float *,
+                                         const float *, const float *, float *, DevRed);
+
+template <typename MT, typename VT>
+hipError_t launch_pcg_iteration_amg(hipStream_t st, const DevTopo &T, const DevAmg &A, int nrhs,
+                                    const MT *val, const VT *dinv, VT *x, VT *r, VT *p, VT *q, VT *z,
+                                    DevRed R, DevScal S, const double *val64, const double *b,
+                                    double *x64, double tol2) {
+    return with_nrhs(nrhs, [&](auto nr) {
+        constexpr int NR = decltype(nr)::value;
+        const FinArgs Af{tol2, 0};
+        constexpr bool REL = std::is_same<VT, float>::value;
+        RelArgs L{};
+        if constexpr (REL) L = RelArgs{val64, x64, b, dinv, r};
+        k_spmv_pq<MT, VT, NR, REL><<<T.nblk, 256, 0, st>>>(T, val, p, q, R, S, L);
+        k_finish<REL ? 4 * NR : NR, FinPQ<NR, REL>><<<T.ninst, 256, 0, st>>>(R, S, Af);
+        k_update_xr<VT, NR><<<T.nblk, 256, 0, st>>>(T, dinv, x, r, p, q, R, S);
+        k_finish<2 * NR, FinXR<NR>><<<T.ninst, 256, 0, st>>>(R, S, Af);
+        vcycle_impl<VT, NR>(st, T, A, val, dinv, r, z, R);
+        k_finish<NR, FinRZ<NR>><<<T.ninst, 256, 0, st>>>(R, S, Af);
+        k_update_pz<VT, NR><<<T.nblk, 256, 0, st>>>(T, z, p, x, REL ? x64 : nullptr, S);
+    });
+}
+template hipError_t launch_pcg_iteration_amg<double, double>(hipStream_t, const DevTopo &, const DevAmg &,
+                                                             int, const double *, const double *, double *,
+                                                             double *, double *, double *, double *, DevRed,
+                                                             DevScal, const double *, const double *,
+                                                             double *, double);
+template hipError_t launch_pcg_iteration_amg<float, float>(hipStream_t, const DevTopo &, const DevAmg &, int,
+                                                           const float *, const float *, float *, float *,
+                                                           float *, float *, float *, DevRed, DevScal,
+                                                           const double *, const double *, double *, double);
+
+template <typename VT>
+hipError_t launch_amg_start(hipStream_t st, const DevTopo &T, const DevAmg &A, int nrhs, const VT *val0,
+                            const VT *dinv0, const VT *r, VT *z, VT *p, DevRed R, DevScal S,
+                            double *x64, VT *x) {
+    return with_nrhs(nrhs, [&](auto nr) {
+        constexpr int NR = decltype(nr)::value;
+        vcycle_impl<VT, NR>(st, T, A, val0, dinv0, r, z, R);
+        k_finish<NR, FinRZ<NR>><<<T.ninst, 256, 0, st>>>(R, S, FinArgs{0.0, 1});
+        k_update_pz<VT, NR><<<T.nblk, 256, 0, st>>>(T, z, p, x, nullptr, S);
+    });
+}
+template hipError_t launch_amg_start<double>(hipStream_t, const DevTopo &, const DevAmg &, int, const double *,
+                                             const double *, const double *, double *, double *, DevRed,
+                                             DevScal, double *, double *);
+template hipError_t launch_amg_start<float>(hipStream_t, const DevTopo &, const DevAmg &, int, const float *,
+                                            const float *, const float *, float *, float *, DevRed, DevScal,
+                                            double *, float *);
 
 hipError_t launch_recover(hipStream_t st, const DevTopo &T, int maxn, int nrhs,
                           const double *binv_pk, const double *gflux, const double *totmob,

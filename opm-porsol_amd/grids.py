@@ -37,6 +37,7 @@ class Grid:
     face_normal: np.ndarray      # f64 [nhf,3] unit outer normal
     side: np.ndarray             # int8 [nhf] canonical side 0..5 of boundary half-faces, -1 inside
     dims: tuple | None = None    # (nx, ny, nz) for logically Cartesian grids
+    ijk: np.ndarray | None = None  # int32 [nc,3] logical cell indices (instance-local)
 
     @property
     def num_cells(self) -> int:
@@ -112,6 +113,7 @@ def cartesian(nx: int, ny: int, nz: int, dx: float = 1.0, dy: float = 1.0,
         face_normal=np.tile(nrm, (nc, 1)),
         side=side.reshape(-1).astype(np.int8),
         dims=(nx, ny, nz),
+        ijk=np.stack([i, j, k], axis=1).astype(np.int32),
     )
 
 
@@ -222,5 +224,7 @@ def concat(grids) -> tuple[Grid, np.ndarray]:
         face_normal=np.concatenate([g.face_normal for g in grids]),
         side=np.concatenate([g.side for g in grids]).astype(np.int8),
         dims=None,
+        ijk=(np.concatenate([g.ijk for g in grids]).astype(np.int32)
+             if all(g.ijk is not None for g in grids) else None),
     )
     return u, np.asarray(cell_off, dtype=np.int32)

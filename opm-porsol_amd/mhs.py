@@ -27,7 +27,7 @@ FP64, MIXED = 0, 1
 
 EXPORTS = (
     "mhs_default_options", "mhs_create", "mhs_destroy", "mhs_last_error", "mhs_clear",
-    "mhs_set_grid", "mhs_init_system_structure", "mhs_compute_inner_products",
+    "mhs_set_grid", "mhs_set_cell_ijk", "mhs_init_system_structure", "mhs_compute_inner_products",
     "mhs_assemble_static", "mhs_get_inner_products", "mhs_solve", "mhs_solve_device", "mhs_get_solution",
     "mhs_solution_device", "mhs_post_process_fluxes", "mhs_get_dims", "mhs_get_cell_faces",
     "mhs_get_face_rows", "mhs_get_system", "mhs_time_kernel",
@@ -40,12 +40,14 @@ class Options(ctypes.Structure):
                 ("same_matrix", ctypes.c_int), ("prolongate_factor", ctypes.c_double),
                 ("smooth_steps", ctypes.c_int), ("precision", ctypes.c_int),
                 ("nrhs", ctypes.c_int), ("chunk", ctypes.c_int),
-                ("replace_factor", ctypes.c_double), ("use_graph", ctypes.c_int)]
+                ("replace_factor", ctypes.c_double), ("use_graph", ctypes.c_int),
+                ("precond", ctypes.c_int), ("smoother_weight", ctypes.c_double)]
 
 
 class Report(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("replacements", ctypes.c_int),
                 ("converged", ctypes.c_int), ("reduction", ctypes.c_double),
+                ("precond", ctypes.c_int), ("levels", ctypes.c_int),
                 ("assemble_ms", ctypes.c_double), ("solve_ms", ctypes.c_double),
                 ("recover_ms", ctypes.c_double)]
 
@@ -80,6 +82,7 @@ def lib():
         L.mhs_last_error.restype = ctypes.c_char_p
         L.mhs_clear.argtypes = [_P]
         L.mhs_set_grid.argtypes = [_P, ctypes.c_int, _I, ctypes.c_int, _I, _I, _D, _D, _D, _D, _D]
+        L.mhs_set_cell_ijk.argtypes = [_P, _I]
         L.mhs_init_system_structure.argtypes = [_P, _I, _I]
         L.mhs_compute_inner_products.argtypes = [_P, _D, _D]
         L.mhs_assemble_static.argtypes = [_P, _D, _D]
@@ -170,6 +173,14 @@ class HybridSolver:
         self.grid = g
         self.ninst = ninst
 
+    def set_cell_ijk(self, ijk):
+        """Logical cell indices [nc, 3] (enables the multigrid preconditioner)."""
+        if ijk is None:
+            self._check(self._L.mhs_set_cell_ijk(self._h, None))
+            return
+        a, pa = _i(np.asarray(ijk).reshape(-1))
+        self._check(self._L.mhs_set_cell_ijk(self._h, pa))
+
     def initSystemStructure(self, bc_type=None, periodic_partner=None):
         if bc_type is None or periodic_partner is None:
             self._check(self._L.mhs_init_system_structure(self._h, None, None))
@@ -195,9 +206,14 @@ class HybridSolver:
         return binv, gfl
 
     def init(self, g, perm, gravity=(0.0, 0.0, 0.0), bc_type=None, periodic_partner=None,
-             inst_cellpos=None):
-        """IncompFlowSolverHybrid::init (:476-488)."""
+             inst_cellpos=None, ijk="auto"):
+        """IncompFlowSolverHybrid::init (:476-488).  ijk: logical cell indices
+        [nc, 3] for the multigrid preconditioner ("auto": from the grid, if any)."""
         self.set_grid(g, inst_cellpos)
+        if isinstance(ijk, str):
+            ijk = g.ijk if getattr(g, "ijk", None) is not None else None
+        if ijk is not None:
+            self.set_cell_ijk(ijk)
         if g.num_cells > 0:
             self.initSystemStructure(bc_type, periodic_partner)
             self.computeInnerProducts(perm, gravity)
@@ -205,7 +221,8 @@ class HybridSolver:
     # --- solve ----------------------------------------------------------------------
     def options(self, tol=1e-8, linsolver_verbosity=0, linsolver_type=1, same_matrix=False,
                 linsolver_maxit=0, prolongate_factor=1.6, smooth_steps=1, precision=FP64,
-                nrhs=1, chunk=32, replace_factor=0.1, use_graph=True):
+                nrhs=1, chunk=32, replace_factor=0.1, use_graph=True, precond=-1,
+                smoother_weight=None):
         o = Options()
         self._L.mhs_default_options(ctypes.byref(o))
         o.tol, o.maxit, o.linsolver_type, o.verbosity = tol, linsolver_maxit, linsolver_type, \
@@ -215,12 +232,16 @@ class HybridSolver:
         o.precision = {"fp64": FP64, "mixed": MIXED}.get(precision, precision)
         o.nrhs, o.chunk, o.replace_factor, o.use_graph = nrhs, chunk, replace_factor, \
             int(use_graph)
+        o.precond = {"auto": -1, "jacobi": 0, "amg": 1}.get(precond, precond)
+        if smoother_weight is not None:
+            o.smoother_weight = smoother_weight
         return o
 
     def solve(self, totmob, rhomob, src, bc_type, bc_value, residual_tolerance=1e-8,
               linsolver_verbosity=0, linsolver_type=1, same_matrix=False, linsolver_maxit=0,
               prolongate_factor=1.6, smooth_steps=1, precision="fp64", chunk=32,
-              replace_factor=0.1, use_graph=True, raise_on_failure=True):
+              replace_factor=0.1, use_graph=True, raise_on_failure=True, precond=-1,
+              smoother_weight=None):
         """IncompFlowSolverHybrid::solve (:657-703) with per-cell mobilities."""
         g = self.grid
         src = np.asarray(src, dtype=np.float64)
@@ -230,7 +251,7 @@ class HybridSolver:
         bt, pbt = _i(bc_type); bvv, pbv = _d(bv)
         o = self.options(residual_tolerance, linsolver_verbosity, linsolver_type, same_matrix,
                          linsolver_maxit, prolongate_factor, smooth_steps, precision, nrhs, chunk,
-                         replace_factor, use_graph)
+                         replace_factor, use_graph, precond, smoother_weight)
         rep = Report()
         rc = self._L.mhs_solve(self._h, ptm, prm, psr, pbt, pbv, ctypes.byref(o),
                                ctypes.byref(rep))
