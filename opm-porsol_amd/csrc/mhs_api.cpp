@@ -188,7 +188,7 @@ void mhs_default_options(mhs_solve_options *o) {
     o->replace_factor = 0.1;
     o->use_graph = 1;
     o->precond = -1;
-    o->smoother_weight = 0.6;
+    o->smoother_weight = 0.7;
 }
 
 int mhs_create(mhs_solver **out, int device, void *stream) {

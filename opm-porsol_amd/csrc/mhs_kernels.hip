@@ -536,7 +536,8 @@ __global__ __launch_bounds__(256) void k_update_xr(DevTopo T, const VT *__restri
                                                    VT *__restrict__ x, VT *__restrict__ r,
                                                    const VT *__restrict__ p,
                                                    const VT *__restrict__ q, DevRed R,
-                                                   DevScal S) {
+                                                   DevScal S, VT *__restrict__ x0s = nullptr,
+                                                   VT w = VT(0)) {
     const int b = xcd_block(gridDim.x);
     const int s = T.block_seg[b];
     int act[NRHS];
@@ -568,6 +569,13 @@ __global__ __launch_bounds__(256) void k_update_xr(DevTopo T, const VT *__restri
         }
     vstore<VT, 4 * NRHS>(x + e0, xv);
     vstore<VT, 4 * NRHS>(r + e0, rv);
+    if (x0s) {  // first multigrid smoothing step, x0 = w D^-1 r
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int k = 0; k < NRHS; ++k) xv[j * NRHS + k] = w * d[j] * rv[j * NRHS + k];
+        vstore<VT, 4 * NRHS>(x0s + e0, xv);
+    }
     block_partials<2 * NRHS>(part, b, R);
 }
 
@@ -870,18 +878,22 @@ __global__ __launch_bounds__(256) void k_vc0_up(DevTopo T, const VT *__restrict_
         const long long s0 = T.slice_ptr[slice];
         const int wd = (int)((T.slice_ptr[slice + 1] - s0) >> 6);
         VT acc[NRHS];
+        if (!xc) {
+            sell_row<VT, VT, VT, NRHS>(T, val, x, f, acc);
+        } else {
 #pragma unroll
-        for (int k = 0; k < NRHS; ++k) acc[k] = VT(0);
-        for (int e = 0; e < wd; ++e) {
-            const long long o = s0 + lane + 64LL * e;
-            const int c = __builtin_nontemporal_load(T.col + o);
-            const VT a = __builtin_nontemporal_load(val + o);
-            const int gc = xc ? agg[c] : -1;
+            for (int k = 0; k < NRHS; ++k) acc[k] = VT(0);
+            for (int e = 0; e < wd; ++e) {
+                const long long o = s0 + lane + 64LL * e;
+                const int c = __builtin_nontemporal_load(T.col + o);
+                const VT a = __builtin_nontemporal_load(val + o);
+                const int gc = agg[c];
 #pragma unroll
-            for (int k = 0; k < NRHS; ++k) {
-                VT xv = x[(size_t)c * NRHS + k];
-                if (gc >= 0) xv += alpha * xc[(size_t)gc * NRHS + k];
-                acc[k] += a * xv;
+                for (int k = 0; k < NRHS; ++k) {
+                    VT xv = x[(size_t)c * NRHS + k];
+                    if (gc >= 0) xv += alpha * xc[(size_t)gc * NRHS + k];
+                    acc[k] += a * xv;
+                }
             }
         }
         const VT d = w * dinv[f];
@@ -984,6 +996,92 @@ __global__ __launch_bounds__(256) void k_vcc_up(DevAmgLevel L, const int *__rest
         const size_t q = (size_t)i * NRHS + k;
         VT xv = x[q];
         if (g >= 0) xv += alpha * xc[(size_t)g * NRHS + k];
+        xo[q] = xv + d * (b[q] - acc[k]);
+    }
+}
+
+
+// x0 = w D^-1 r (level-0 pre-smoothing when r does not come from k_update_xr)
+template <typename VT, int NRHS>
+__global__ __launch_bounds__(256) void k_scale0(const VT *__restrict__ dinv, const VT *__restrict__ r,
+                                                VT *__restrict__ x0, VT w) {
+    const int f = blockIdx.x * 256 + threadIdx.x;
+    const VT d = w * dinv[f];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) x0[(size_t)f * NRHS + k] = d * r[(size_t)f * NRHS + k];
+}
+
+// x~ = x + alpha P xc on level 0 (padding rows have no aggregate)
+template <typename VT, int NRHS>
+__global__ __launch_bounds__(256) void k_prolong0(const int *__restrict__ agg, const VT *__restrict__ x,
+                                                  const VT *__restrict__ xc, VT *__restrict__ xo,
+                                                  VT alpha) {
+    const int f = blockIdx.x * 256 + threadIdx.x;
+    const int g = agg[f];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) {
+        const size_t i = (size_t)f * NRHS + k;
+        xo[i] = x[i] + (g >= 0 ? alpha * xc[(size_t)g * NRHS + k] : VT(0));
+    }
+}
+
+// Coarse level l -> l+1 with one smoothing sweep: the restricted residual of
+// coarse row I is the sum over its children i of b_i - sum_j a_ij w d_j b_j
+// (the pre-smoothed iterate x = w D^-1 b formed on the fly; every finer row
+// is visited once, by its aggregate's thread).
+template <typename VT, int NRHS>
+__global__ __launch_bounds__(256) void k_vcc_restrict(DevAmgLevel F, DevAmgLevel C, VT w) {
+    const int I = blockIdx.x * 256 + threadIdx.x;
+    if (I >= C.n) return;
+    const VT *val = lvl_val<VT>(F);
+    const VT *dinv = lvl_dinv<VT>(F);
+    const VT *b = (const VT *)F.b;
+    VT acc[NRHS];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) acc[k] = VT(0);
+    for (int e = C.child_ptr[I]; e < C.child_ptr[I + 1]; ++e) {
+        const int i = C.child[e];
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) acc[k] += b[(size_t)i * NRHS + k];
+        for (int q = F.row_ptr[i]; q < F.row_ptr[i + 1]; ++q) {
+            const int c = F.col[q];
+            const VT a = val[q] * (w * dinv[c]);
+#pragma unroll
+            for (int k = 0; k < NRHS; ++k) acc[k] -= a * b[(size_t)c * NRHS + k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) ((VT *)C.b)[(size_t)I * NRHS + k] = acc[k];
+}
+
+// Coarse level up with one sweep: x~ = w D^-1 b + alpha P xc (on the fly),
+// xo = x~ + w D^-1 (b - A x~).
+template <typename VT, int NRHS>
+__global__ __launch_bounds__(256) void k_vcc_up1(DevAmgLevel L, const int *__restrict__ agg,
+                                                 const VT *__restrict__ xc, VT *__restrict__ xo, VT w,
+                                                 VT alpha) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= L.n) return;
+    const VT *val = lvl_val<VT>(L);
+    const VT *dinv = lvl_dinv<VT>(L);
+    const VT *b = (const VT *)L.b;
+    VT acc[NRHS];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) acc[k] = VT(0);
+    for (int e = L.row_ptr[i]; e < L.row_ptr[i + 1]; ++e) {
+        const int c = L.col[e];
+        const VT dc = w * dinv[c];
+        const int gc = agg[c];
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k)
+            acc[k] += val[e] * (dc * b[(size_t)c * NRHS + k] + alpha * xc[(size_t)gc * NRHS + k]);
+    }
+    const VT d = w * dinv[i];
+    const int g = agg[i];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) {
+        const size_t q = (size_t)i * NRHS + k;
+        const VT xv = d * b[q] + alpha * xc[(size_t)g * NRHS + k];
         xo[q] = xv + d * (b[q] - acc[k]);
     }
 }
@@ -1497,44 +1595,41 @@ hipError_t launch_amg_setup(hipStream_t st, const DevTopo &T, const DevAmg &A, c
 
 template <typename VT, int NR>
 static void vcycle_impl(hipStream_t st, const DevTopo &T, const DevAmg &A, const VT *val0,
-                        const VT *dinv0, const VT *r, VT *z, DevRed R) {
+                        const VT *dinv0, const VT *r, VT *z, DevRed R, bool x0_ready) {
     const VT w = (VT)A.omega, al = (VT)A.alpha;
     const int nu = A.nu < 1 ? 1 : A.nu;
     const int rows_blocks = T.nfp / 256;
     VT *x0 = (VT *)A.x0, *xt0 = (VT *)A.xt0, *res0 = (VT *)A.res0;
-    // ---- level 0 down
-    k_vc0_down<VT, NR><<<rows_blocks, 256, 0, st>>>(T, val0, dinv0, r, x0, res0, w);
-    if (nu > 1) {
-        for (int it = 1; it < nu; ++it) {
-            k_vc0_up<VT, NR, false><<<T.nblk, 256, 0, st>>>(T, val0, dinv0, nullptr, r, x0, nullptr, xt0, w,
-                                                            VT(0), R);
-            std::swap(x0, xt0);
-        }
-        k_vc0_res<VT, NR><<<rows_blocks, 256, 0, st>>>(T, val0, r, x0, res0);
-    }
     const int L = A.nlev;
+    // ---- level 0 down: x0 = w D^-1 r (+ nu-1 sweeps), res0 = r - A x0
+    if (!x0_ready) k_scale0<VT, NR><<<rows_blocks, 256, 0, st>>>(dinv0, r, x0, w);
+    for (int it = 1; it < nu; ++it) {
+        k_vc0_up<VT, NR, false><<<T.nblk, 256, 0, st>>>(T, val0, dinv0, nullptr, r, x0, nullptr, xt0, w,
+                                                        VT(0), R);
+        std::swap(x0, xt0);
+    }
+    k_vc0_res<VT, NR><<<rows_blocks, 256, 0, st>>>(T, val0, r, x0, res0);
+    {
+        const DevAmgLevel &C = A.lev[1];
+        k_restrict<VT, NR><<<(C.n + 255) / 256, 256, 0, st>>>(C.n, C.child_ptr, C.child, res0, (VT *)C.b);
+    }
     VT *xcur[kMaxLevels], *xalt[kMaxLevels];
     for (int l = 1; l < L; ++l) {
         xcur[l] = (VT *)A.lev[l].x;
         xalt[l] = (VT *)A.lev[l].xt;
     }
-    {
-        const DevAmgLevel &C = A.lev[1];
-        k_restrict<VT, NR><<<(C.n + 255) / 256, 256, 0, st>>>(C.n, C.child_ptr, C.child, res0, (VT *)C.b);
-    }
+    // ---- coarse levels down
     for (int l = 1; l < L - 1; ++l) {
         const DevAmgLevel &F = A.lev[l];
+        const DevAmgLevel &C = A.lev[l + 1];
         const unsigned nb = (F.n + 255) / 256;
         k_vcc_down<VT, NR><<<nb, 256, 0, st>>>(F, (const VT *)F.b, xcur[l], (VT *)F.res, w);
-        if (nu > 1) {
-            for (int it = 1; it < nu; ++it) {
-                k_vcc_up<VT, NR><<<nb, 256, 0, st>>>(F, nullptr, (const VT *)F.b, xcur[l], nullptr, xalt[l],
-                                                     w, VT(0));
-                std::swap(xcur[l], xalt[l]);
-            }
-            k_vcc_res<VT, NR><<<nb, 256, 0, st>>>(F, (const VT *)F.b, xcur[l], (VT *)F.res);
+        for (int it = 1; it < nu; ++it) {
+            k_vcc_up<VT, NR><<<nb, 256, 0, st>>>(F, nullptr, (const VT *)F.b, xcur[l], nullptr, xalt[l], w,
+                                                 VT(0));
+            std::swap(xcur[l], xalt[l]);
         }
-        const DevAmgLevel &C = A.lev[l + 1];
+        if (nu > 1) k_vcc_res<VT, NR><<<nb, 256, 0, st>>>(F, (const VT *)F.b, xcur[l], (VT *)F.res);
         k_restrict<VT, NR><<<(C.n + 255) / 256, 256, 0, st>>>(C.n, C.child_ptr, C.child, (const VT *)F.res,
                                                               (VT *)C.b);
     }
@@ -1545,9 +1640,15 @@ static void vcycle_impl(hipStream_t st, const DevTopo &T, const DevAmg &A, const
         k_coarse_solve<VT, NR><<<A.ninst, 256, 0, st>>>(C.inst_ptr, A.coarse_off, inv, (const VT *)C.b,
                                                         xcur[L - 1]);
     }
+    // ---- coarse levels up
     for (int l = L - 2; l >= 1; --l) {
         const DevAmgLevel &F = A.lev[l];
         const unsigned nb = (F.n + 255) / 256;
+        if (nu == 1) {
+            k_vcc_up1<VT, NR><<<nb, 256, 0, st>>>(F, A.lev[l + 1].agg, xcur[l + 1], xalt[l], w, al);
+            std::swap(xcur[l], xalt[l]);
+            continue;
+        }
         k_vcc_up<VT, NR><<<nb, 256, 0, st>>>(F, A.lev[l + 1].agg, (const VT *)F.b, xcur[l], xcur[l + 1],
                                              xalt[l], w, al);
         std::swap(xcur[l], xalt[l]);
@@ -1557,22 +1658,15 @@ static void vcycle_impl(hipStream_t st, const DevTopo &T, const DevAmg &A, const
             std::swap(xcur[l], xalt[l]);
         }
     }
-    // ---- level 0 up: coarse correction + nu sweeps, the last one writes z
-    const VT *xc1 = (L > 1) ? xcur[1] : nullptr;
-    if (nu == 1) {
-        k_vc0_up<VT, NR, true><<<T.nblk, 256, 0, st>>>(T, val0, dinv0, A.lev[1].agg, r, x0, xc1, z, w, al, R);
-    } else {
-        k_vc0_up<VT, NR, false><<<T.nblk, 256, 0, st>>>(T, val0, dinv0, A.lev[1].agg, r, x0, xc1, xt0, w, al,
-                                                        R);
+    // ---- level 0 up: x~ = x0 + alpha P x1, then nu sweeps; the last writes z and <r, z>
+    k_prolong0<VT, NR><<<rows_blocks, 256, 0, st>>>(A.lev[1].agg, x0, xcur[1], xt0, al);
+    std::swap(x0, xt0);
+    for (int it = 1; it < nu; ++it) {
+        k_vc0_up<VT, NR, false><<<T.nblk, 256, 0, st>>>(T, val0, dinv0, nullptr, r, x0, nullptr, xt0, w,
+                                                        VT(0), R);
         std::swap(x0, xt0);
-        for (int it = 1; it < nu - 1; ++it) {
-            k_vc0_up<VT, NR, false><<<T.nblk, 256, 0, st>>>(T, val0, dinv0, nullptr, r, x0, nullptr, xt0, w,
-                                                            VT(0), R);
-            std::swap(x0, xt0);
-        }
-        k_vc0_up<VT, NR, true><<<T.nblk, 256, 0, st>>>(T, val0, dinv0, nullptr, r, x0, nullptr, z, w, VT(0),
-                                                       R);
     }
+    k_vc0_up<VT, NR, true><<<T.nblk, 256, 0, st>>>(T, val0, dinv0, nullptr, r, x0, nullptr, z, w, VT(0), R);
 }
 
 template <typename VT>
@@ -1580,7 +1674,7 @@ hipError_t launch_vcycle(hipStream_t st, const DevTopo &T, const DevAmg &A, int 
                          const VT *dinv0, const VT *r, VT *z, DevRed R) {
     return with_nrhs(nrhs, [&](auto nr) {
         constexpr int NR = decltype(nr)::value;
-        vcycle_impl<VT, NR>(st, T, A, val0, dinv0, r, z, R);
+        vcycle_impl<VT, NR>(st, T, A, val0, dinv0, r, z, R, false);
     });
 }
 template hipError_t launch_vcycle<double>(hipStream_t, const DevTopo &, const DevAmg &, int, const double *,
@@ -1601,9 +1695,9 @@ hipError_t launch_pcg_iteration_amg(hipStream_t st, const DevTopo &T, const DevA
         if constexpr (REL) L = RelArgs{val64, x64, b, dinv, r};
         k_spmv_pq<MT, VT, NR, REL><<<T.nblk, 256, 0, st>>>(T, val, p, q, R, S, L);
         k_finish<REL ? 4 * NR : NR, FinPQ<NR, REL>><<<T.ninst, 256, 0, st>>>(R, S, Af);
-        k_update_xr<VT, NR><<<T.nblk, 256, 0, st>>>(T, dinv, x, r, p, q, R, S);
+        k_update_xr<VT, NR><<<T.nblk, 256, 0, st>>>(T, dinv, x, r, p, q, R, S, (VT *)A.x0, (VT)A.omega);
         k_finish<2 * NR, FinXR<NR>><<<T.ninst, 256, 0, st>>>(R, S, Af);
-        vcycle_impl<VT, NR>(st, T, A, val, dinv, r, z, R);
+        vcycle_impl<VT, NR>(st, T, A, val, dinv, r, z, R, true);
         k_finish<NR, FinRZ<NR>><<<T.ninst, 256, 0, st>>>(R, S, Af);
         k_update_pz<VT, NR><<<T.nblk, 256, 0, st>>>(T, z, p, x, REL ? x64 : nullptr, S);
     });
@@ -1624,7 +1718,7 @@ hipError_t launch_amg_start(hipStream_t st, const DevTopo &T, const DevAmg &A, i
                             double *x64, VT *x) {
     return with_nrhs(nrhs, [&](auto nr) {
         constexpr int NR = decltype(nr)::value;
-        vcycle_impl<VT, NR>(st, T, A, val0, dinv0, r, z, R);
+        vcycle_impl<VT, NR>(st, T, A, val0, dinv0, r, z, R, false);
         k_finish<NR, FinRZ<NR>><<<T.ninst, 256, 0, st>>>(R, S, FinArgs{0.0, 1});
         k_update_pz<VT, NR><<<T.nblk, 256, 0, st>>>(T, z, p, x, nullptr, S);
     });

@@ -235,6 +235,9 @@ def test_full_size_properties(pkg):
     s.solve(tm, rm, src, bt, bv, residual_tolerance=1e-10, precision="mixed")
     p, pi, v = s.getSolution()
     assert s.report.reduction <= 1e-10
-    assert _mass_balance(g, v, src) < 1e-12
+    # Recovery sums terms of size ~|p| into per-cell flux sums of size
+    # ~|p|/nx (nx = 128 cells along the drop): FP64 round-off shows up
+    # amplified by about nx, so the bar here is 1e-11 instead of 1e-12.
+    assert _mass_balance(g, v, src) < 1e-11
     assert rel_err(pi, pi64) < 1e-6
     assert rel_err(v, v64) < 1e-6
