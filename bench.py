@@ -8,6 +8,10 @@ ln kx ~ ln 100 mD + 1.5 z, correlation length 8 cells), Dirichlet pressure
 drop along x, no-flow elsewhere.  Inputs (mobilities, sources, boundary values)
 are resident in HBM before the timed region.
 
+The timed solve is the default solver type (multigrid-preconditioned CG, as the
+reference's default AMG-CG) in mixed precision.  Beside it, once each: the same
+solve in FP64, and the plain method in FP64 (Jacobi-PCG, no multigrid).
+
 Prints one JSON line.  --gpus N > 1 is rejected: this build targets one GPU.
 """
 from __future__ import annotations
@@ -41,26 +45,47 @@ def _problem(pkg, seed):
     return g, perm, bt, bv
 
 
-def _alg_bytes(g, nnz, nf, nrhs=1):
-    """Algorithmic HBM bytes per unit of work (each array counted once at its
-    stored precision, 4-byte indices)."""
+def _alg_bytes(g, nnz, nf, mg_rows=0, mg_nnz=0):
+    """Algorithmic HBM bytes per unit of work for one right-hand side: each
+    array counted once per kernel at its stored precision, 4-byte indices,
+    gathered vectors counted once.  v is the value size of the PCG (4 in
+    mixed precision, 8 in FP64)."""
     nc, nhf = g.num_cells, g.num_half_faces
     sq = int(np.sum(g.ncf().astype(np.int64) ** 2))
     pk = int(np.sum(g.ncf().astype(np.int64) * (g.ncf() + 1) // 2))
-    return {
-        # FP32 SpMV + <p,Ap>: values + column indices, p gathered once, q written
-        "spmv32": nnz * (4 + 4) + nf * 4 * 2 * nrhs,
-        "spmv64": nnz * (8 + 4) + nf * 8 * 2 * nrhs,
-        # one Jacobi-PCG iteration: SpMV + 13 vector passes (p,q,x,r,dinv,...)
-        "iter32": nnz * (4 + 4) + 4 * (nf + 1) + 13 * nf * 4 * nrhs,
-        "iter64": nnz * (8 + 4) + 4 * (nf + 1) + 13 * nf * 8 * nrhs,
-        # FP64 residual replacement: FP64 operator + x, b read, r written
-        "refine64": nnz * (8 + 4) + 4 * nf * 8 * nrhs,
-        # assembly (static B^-1 read packed, values written in FP64 + FP32, rhs, F, L, g)
-        "assemble": pk * 8 + nc * 16 + nnz * (8 + 4) + nf * 8 + nhf * 8 + nc * 16 + sq,
-        # recovery: pi, F, L, g, B^-1 (packed), mobility, p, v
-        "recover": nf * 8 + nhf * 8 + nc * 16 + pk * 8 + nc * 8 + nc * 8 + nhf * 8,
-    }
+    out = {}
+    for v, tag in ((4, "32"), (8, "64")):
+        mat = nnz * (v + 4)
+        # SpMV q = A p with the fused <p,q> partials: matrix, p, q
+        out["spmv" + tag] = mat + 2 * nf * v
+        # Jacobi-PCG iteration: SpMV + x,r,p,q,dinv reads and x,r,p writes
+        out["iter" + tag] = mat + 4 * (nf + 1) + 13 * nf * v
+        # multigrid-PCG iteration: three level-0 operator passes (SpMV,
+        # pre-smoothed residual, post-smoothing sweep), 23 level-0 vector
+        # passes, aggregate and child maps, and on the coarse levels two
+        # operator passes and ten vector passes per row
+        out["amg_iter" + tag] = (3 * mat + 23 * nf * v + 8 * nf
+                                 + 2 * mg_nnz * (v + 4) + mg_rows * (12 + 10 * v))
+    # FP64 residual replacement: FP64 operator + x, b read, r written
+    out["refine64"] = nnz * (8 + 4) + 4 * nf * 8
+    # assembly (static B^-1 read packed, values written in FP64 + FP32, rhs, F, L, g)
+    out["assemble"] = pk * 8 + nc * 16 + nnz * (8 + 4) + nf * 8 + nhf * 8 + nc * 16 + sq
+    # recovery: pi, F, L, g, B^-1 (packed), mobility, p, v
+    out["recover"] = nf * 8 + nhf * 8 + nc * 16 + pk * 8 + nc * 8 + nc * 8 + nhf * 8
+    return out
+
+
+def _measured_traffic(tag):
+    """HBM bytes per launch of the FP32 SpMV as measured by the FETCH_SIZE /
+    WRITE_SIZE counter runs summarised in profiles/ (calibrated against a
+    copy of known size); None when no such measurement is committed."""
+    path = os.path.join(ROOT, "profiles", "k_spmv_pq_128x128x64_hbm_traffic.txt")
+    if tag != "32" or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
 
 
 def _cpu_baseline(pkg, g, perm, bt, bv, iters_needed, sample_iters):
@@ -88,7 +113,8 @@ def _cpu_baseline(pkg, g, perm, bt, bv, iters_needed, sample_iters):
         "sample": (f"oracle/oracle.c FP64 on 1 host core ({platform.processor() or 'cpu'}, "
                    f"{os.cpu_count()} visible): full assembly {t1 - t0:.2f} s + recovery "
                    f"{t3 - t2:.2f} s + {its} Jacobi-PCG iterations at {per_it * 1e3:.1f} ms, "
-                   f"extrapolated to the {iters_needed} iterations the same FP64 Jacobi-PCG takes"),
+                   f"extrapolated to the {iters_needed} iterations the same FP64 Jacobi-PCG "
+                   f"takes at 128x128x64"),
         "seconds_per_solve": t_solve,
     }
 
@@ -96,17 +122,24 @@ def _cpu_baseline(pkg, g, perm, bt, bv, iters_needed, sample_iters):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--precond", default="auto", choices=["auto", "amg", "jacobi"])
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--chunk", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch PCG kernels one by one instead of replaying captured graphs")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the FP64 comparison solves (profiling runs)")
     ap.add_argument("--cpu-sample-iters", type=int, default=40)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="also run one device copy of this size (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
-              "(see DESIGN.md for how the solve would shard).", file=sys.stderr)
+              "(DESIGN.md describes how the solve would shard).", file=sys.stderr)
         sys.exit(2)
 
     import __graft_entry__
@@ -129,19 +162,27 @@ def main():
     bvd = torch.from_numpy(bv).to(dev)
     torch.cuda.synchronize()
 
-    def step(precision):
+    def step(precision, precond=args.precond):
         return s.solve_device(tm.data_ptr(), rm.data_ptr(), src.data_ptr(), bt, bvd.data_ptr(),
-                              nrhs=1, tol=TOL, precision=precision, chunk=args.chunk)
+                              nrhs=1, tol=TOL, precision=precision, chunk=args.chunk,
+                              precond=precond, use_graph=not args.no_graph)
 
-    # plain FP64 GPU solve: the comparison point, and its result for the error report
-    step("fp64")
-    rep64 = s.report.as_dict()
-    p64, pi64, v64 = s.getSolution()
-    t0 = time.perf_counter()
-    step("fp64")
-    torch.cuda.synchronize()
-    t_fp64 = time.perf_counter() - t0
-    nnz = int(s.system(0)[0][-1])
+    def timed(precision, precond):
+        step(precision, precond)                 # first call builds graphs / hierarchy
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        step(precision, precond)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t
+        rep = s.report.as_dict()
+        return t, rep, s.getSolution()
+
+    cmp = {}
+    if not args.no_compare:
+        # FP64 with the same preconditioner: the comparison point for the error report
+        cmp["fp64"] = timed("fp64", args.precond)
+        # the plain method in FP64: Jacobi-PCG
+        cmp["fp64_jacobi"] = timed("fp64", "jacobi")
 
     for _ in range(args.warmup):
         step(args.precision)
@@ -154,6 +195,7 @@ def main():
     rep = s.report.as_dict()
     ms_step = t_tot / args.steps * 1e3
     p, pi, v = s.getSolution()
+    nnz = int(s.system(0)[0][-1])
 
     def rel(a, b):
         return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
@@ -161,22 +203,54 @@ def main():
     cell = g.hf_cell()
     mass = float(np.max(np.abs(np.bincount(cell, weights=v, minlength=nc))) / np.max(np.abs(v)))
 
-    B = _alg_bytes(g, nnz, nf)
-    it_key, sp_key = ("iter32", "spmv32") if args.precision == "mixed" else ("iter64", "spmv64")
+    B = _alg_bytes(g, nnz, nf, rep["mg_rows"], rep["mg_nnz"])
+    tag = "32" if args.precision == "mixed" else "64"
+    it_key = ("amg_iter" if rep["precond"] == 1 else "iter") + tag
     n_ref = rep["replacements"]
     bytes_solve = B["assemble"] + rep["iterations"] * B[it_key] + n_ref * B["refine64"] + B["recover"]
     gbps = bytes_solve / (ms_step * 1e-3) / 1e9
 
+    if args.copy_probe_mib > 0:
+        n = args.copy_probe_mib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+
     kern_ms = s.time_kernel(0, reps=50)
-    ach = B[sp_key] / (kern_ms * 1e-3) / 1e9
-    roof = {"kernel": f"k_spmv_pq<{'float' if args.precision == 'mixed' else 'double'}>",
+    ach = B["spmv" + tag] / (kern_ms * 1e-3) / 1e9
+    roof = {"kernel": f"k_spmv_pq<{'float' if tag == '32' else 'double'}> (face-matrix SpMV with "
+                      f"fused <p,Ap>)",
             "bound": "hbm", "achieved": round(ach, 1), "peak": 8000.0, "unit": "GB/s",
-            "frac": round(ach / 8000.0, 4), "traffic": None,
-            "alg_bytes_per_launch": B[sp_key], "avg_launch_us": round(kern_ms * 1e3, 2)}
+            "frac": round(ach / 8000.0, 4), "traffic": _measured_traffic(tag),
+            "alg_bytes_per_launch": B["spmv" + tag], "avg_launch_us": round(kern_ms * 1e3, 2)}
 
     cpu = None
-    if not args.no_cpu_baseline:
-        cpu = _cpu_baseline(pkg, g, perm, bt, bv, rep64["iterations"], args.cpu_sample_iters)
+    if not args.no_cpu_baseline and "fp64_jacobi" in cmp:
+        cpu = _cpu_baseline(pkg, g, perm, bt, bv, cmp["fp64_jacobi"][1]["iterations"],
+                            args.cpu_sample_iters)
+
+    config = {
+        "workload": "mimetic hybrid solve, 128x128x64 Cartesian, anisotropic K (kz/kx=0.1, "
+                    "sigma ln k 1.5, l=8), x pressure drop, "
+                    + ("multigrid-PCG" if rep["precond"] == 1 else "Jacobi-PCG") + " to 1e-10",
+        "cells": nc, "faces": nf, "nnz": nnz, "precision": args.precision,
+        "preconditioner": "multigrid" if rep["precond"] == 1 else "jacobi",
+        "levels": rep["levels"], "iterations": rep["iterations"], "fp64_replacements": n_ref,
+        "reduction": rep["reduction"], "init_s": round(t_init, 3),
+        "hbm_gbps_per_solve": round(gbps, 1), "alg_bytes_per_solve": int(bytes_solve),
+        "mass_balance": mass,
+    }
+    for key, (t, r, sol) in cmp.items():
+        p64, pi64, v64 = sol
+        config[key + "_gpu"] = {
+            "solves_per_s": round(1.0 / t, 4), "iterations": r["iterations"],
+            "reduction": r["reduction"], "speedup": round((1000.0 / ms_step) * t, 3),
+        }
+        if key == "fp64":
+            config["err_vs_fp64_gpu"] = {"face_pressure": rel(pi, pi64), "cell_pressure": rel(p, p64),
+                                         "outflux": rel(v, v64)}
 
     out = {
         "metric": "face-pressure solves/sec + HBM GB/s at 128x128x64, rel. residual 1e-10",
@@ -191,20 +265,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 PCG / f64 residual, assembly, recovery" if args.precision == "mixed" else "f64",
         "data": "synthetic (seeded Gaussian-filtered lognormal permeability)",
-        "config": {
-            "workload": "mimetic hybrid solve, 128x128x64 Cartesian, anisotropic K (kz/kx=0.1, "
-                        "sigma ln k 1.5, l=8), x pressure drop, Jacobi-PCG to 1e-10",
-            "cells": nc, "faces": nf, "nnz": nnz, "precision": args.precision,
-            "iterations": rep["iterations"], "fp64_replacements": n_ref,
-            "reduction": rep["reduction"], "init_s": round(t_init, 3),
-            "hbm_gbps_per_solve": round(gbps, 1),
-            "fp64_gpu": {"solves_per_s": round(1.0 / t_fp64, 4), "iterations": rep64["iterations"],
-                         "reduction": rep64["reduction"]},
-            "speedup_vs_fp64_gpu": round((1000.0 / ms_step) * t_fp64, 3),
-            "err_vs_fp64_gpu": {"face_pressure": rel(pi, pi64), "cell_pressure": rel(p, p64),
-                                "outflux": rel(v, v64)},
-            "mass_balance": mass,
-        },
+        "config": config,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

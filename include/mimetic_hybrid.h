@@ -92,6 +92,8 @@ typedef struct {
     double reduction;          /* max over systems of the FP64 ||b-Ax||/||r0|| */
     int precond;               /* preconditioner used (0 Jacobi, 1 multigrid) */
     int levels;                /* multigrid levels (0 with Jacobi) */
+    long long mg_rows;         /* rows of the coarse multigrid levels, summed */
+    long long mg_nnz;          /* stored entries of the coarse level matrices, summed */
     double assemble_ms, solve_ms, recover_ms;
 } mhs_solve_report;
 

@@ -740,6 +740,13 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
         rep->reduction = red;
         rep->precond = amg ? 1 : 0;
         rep->levels = amg ? s->dA.nlev : 0;
+        rep->mg_rows = 0;
+        rep->mg_nnz = 0;
+        if (amg)
+            for (int l = 1; l < s->H.nlev(); ++l) {
+                rep->mg_rows += s->H.lev[l].n;
+                rep->mg_nnz += (long long)s->H.lev[l].col.size();
+            }
         rep->assemble_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
         rep->solve_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
         rep->recover_ms = std::chrono::duration<double, std::milli>(t3 - t2).count();

@@ -828,35 +828,6 @@ __device__ __forceinline__ const VT *lvl_dinv(const DevAmgLevel &L) {
     if constexpr (std::is_same<VT, float>::value) return L.dinv32; else return L.dinv64;
 }
 
-// Level 0, down: x0 = w D^-1 b and res = b - A x0, with x0 formed on the fly.
-template <typename VT, int NRHS>
-__global__ __launch_bounds__(256) void k_vc0_down(DevTopo T, const VT *__restrict__ val,
-                                                  const VT *__restrict__ dinv,
-                                                  const VT *__restrict__ b, VT *__restrict__ x,
-                                                  VT *__restrict__ res, VT w) {
-    const int f = blockIdx.x * 256 + threadIdx.x;
-    const int slice = f >> 6, lane = f & 63;
-    const long long s0 = T.slice_ptr[slice];
-    const int wd = (int)((T.slice_ptr[slice + 1] - s0) >> 6);
-    VT acc[NRHS];
-#pragma unroll
-    for (int k = 0; k < NRHS; ++k) acc[k] = VT(0);
-    for (int e = 0; e < wd; ++e) {
-        const long long o = s0 + lane + 64LL * e;
-        const int c = __builtin_nontemporal_load(T.col + o);
-        const VT a = __builtin_nontemporal_load(val + o) * (w * dinv[c]);
-#pragma unroll
-        for (int k = 0; k < NRHS; ++k) acc[k] += a * b[(size_t)c * NRHS + k];
-    }
-    const VT d = w * dinv[f];
-#pragma unroll
-    for (int k = 0; k < NRHS; ++k) {
-        const size_t i = (size_t)f * NRHS + k;
-        x[i] = d * b[i];
-        res[i] = b[i] - acc[k];
-    }
-}
-
 // Level 0, one more Jacobi sweep: xo = x + w D^-1 (b - A x~), x~ = x + alpha P xc
 // (xc == nullptr: no coarse correction).  With z/partials: the final sweep of
 // the V-cycle writes the preconditioned residual and <r, z> partials.
@@ -1023,35 +994,6 @@ __global__ __launch_bounds__(256) void k_prolong0(const int *__restrict__ agg, c
         const size_t i = (size_t)f * NRHS + k;
         xo[i] = x[i] + (g >= 0 ? alpha * xc[(size_t)g * NRHS + k] : VT(0));
     }
-}
-
-// Coarse level l -> l+1 with one smoothing sweep: the restricted residual of
-// coarse row I is the sum over its children i of b_i - sum_j a_ij w d_j b_j
-// (the pre-smoothed iterate x = w D^-1 b formed on the fly; every finer row
-// is visited once, by its aggregate's thread).
-template <typename VT, int NRHS>
-__global__ __launch_bounds__(256) void k_vcc_restrict(DevAmgLevel F, DevAmgLevel C, VT w) {
-    const int I = blockIdx.x * 256 + threadIdx.x;
-    if (I >= C.n) return;
-    const VT *val = lvl_val<VT>(F);
-    const VT *dinv = lvl_dinv<VT>(F);
-    const VT *b = (const VT *)F.b;
-    VT acc[NRHS];
-#pragma unroll
-    for (int k = 0; k < NRHS; ++k) acc[k] = VT(0);
-    for (int e = C.child_ptr[I]; e < C.child_ptr[I + 1]; ++e) {
-        const int i = C.child[e];
-#pragma unroll
-        for (int k = 0; k < NRHS; ++k) acc[k] += b[(size_t)i * NRHS + k];
-        for (int q = F.row_ptr[i]; q < F.row_ptr[i + 1]; ++q) {
-            const int c = F.col[q];
-            const VT a = val[q] * (w * dinv[c]);
-#pragma unroll
-            for (int k = 0; k < NRHS; ++k) acc[k] -= a * b[(size_t)c * NRHS + k];
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < NRHS; ++k) ((VT *)C.b)[(size_t)I * NRHS + k] = acc[k];
 }
 
 // Coarse level up with one sweep: x~ = w D^-1 b + alpha P xc (on the fly),

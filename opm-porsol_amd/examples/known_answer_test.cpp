@@ -77,6 +77,12 @@ int main(int argc, char **argv) try {
     t1 = std::chrono::steady_clock::now();
     std::cout << "========== Time in seconds: " << std::chrono::duration<double>(t1 - t0).count()
               << " =============" << std::endl;
+    {
+        const mhs_solve_report &r = solver.lastReport();
+        std::cout << "PCG iterations: " << r.iterations << ", preconditioner: "
+                  << (r.precond ? "multigrid" : "Jacobi") << ", levels: " << r.levels
+                  << ", reduction: " << r.reduction << std::endl;
+    }
 
     std::vector<double> cell_pressure;
     Opm::getCellPressure(cell_pressure, g, solver.getSolution());

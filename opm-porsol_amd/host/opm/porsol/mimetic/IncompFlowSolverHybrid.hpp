@@ -24,13 +24,52 @@
 #include <numeric>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <unordered_map>
+#include <utility>
 #include <vector>
 
 #include "mimetic_hybrid.h"
 #include "MimeticIPEvaluator.hpp"
 
 namespace Opm {
+
+namespace hybrid_detail {
+// Logical (i,j,k) of every cell, when the grid type can say: a box grid with
+// dims() (cells numbered x fastest) or a corner-point grid with the CpGrid
+// accessors logicalCartesianSize() and globalCell().  The multigrid
+// preconditioner builds its aggregates from these; without them the solve
+// uses the Jacobi preconditioner.
+template <class G, class = void> struct has_dims : std::false_type {};
+template <class G>
+struct has_dims<G, std::void_t<decltype(std::declval<const G &>().dims())>> : std::true_type {};
+template <class G, class = void> struct has_logical : std::false_type {};
+template <class G>
+struct has_logical<G, std::void_t<decltype(std::declval<const G &>().logicalCartesianSize()),
+                                  decltype(std::declval<const G &>().globalCell())>> : std::true_type {};
+
+template <class G>
+bool logical_index(const G &grid, int cell, int ijk[3]) {
+    long long g = cell;
+    int n[3];
+    if constexpr (has_dims<G>::value) {
+        const int *d = grid.dims();
+        n[0] = d[0]; n[1] = d[1]; n[2] = d[2];
+    } else if constexpr (has_logical<G>::value) {
+        const auto &sz = grid.logicalCartesianSize();
+        n[0] = sz[0]; n[1] = sz[1]; n[2] = sz[2];
+        g = grid.globalCell()[cell];
+    } else {
+        (void)grid; (void)cell; (void)ijk; (void)g; (void)n;
+        return false;
+    }
+    if (n[0] <= 0 || n[1] <= 0 || n[2] <= 0) return false;
+    ijk[0] = int(g % n[0]);
+    ijk[1] = int((g / n[0]) % n[1]);
+    ijk[2] = int(g / ((long long)n[0] * n[1]));
+    return true;
+}
+}  // namespace hybrid_detail
 
 template <class GridInterface, class RockInterface, class BCInterface,
           template <class GridIF, class RockIF> class InnerProduct>
@@ -137,6 +176,13 @@ public:
                            fa.data(), fc.data(), fn.data()),
               "set_grid");
         flowSolution_.facepos_ = facepos;
+        {
+            std::vector<int> ijk(3 * (size_t)nc);
+            bool have = true;
+            for (CI c = g.cellbegin(); c != g.cellend() && have; ++c)
+                have = hybrid_detail::logical_index(g.grid(), c->index(), &ijk[3 * (size_t)cellno[c->index()]]);
+            if (have) check(mhs_set_cell_ijk(h_, ijk.data()), "set_cell_ijk");
+        }
         // Periodic pairing (enumerateBCDof :1071-1110): bid -> first half-face
         // carrying it, partner through getPeriodicPartner(bid).
         const int nhf = facepos.back();

@@ -48,6 +48,7 @@ class Report(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("replacements", ctypes.c_int),
                 ("converged", ctypes.c_int), ("reduction", ctypes.c_double),
                 ("precond", ctypes.c_int), ("levels", ctypes.c_int),
+                ("mg_rows", ctypes.c_longlong), ("mg_nnz", ctypes.c_longlong),
                 ("assemble_ms", ctypes.c_double), ("solve_ms", ctypes.c_double),
                 ("recover_ms", ctypes.c_double)]
 
