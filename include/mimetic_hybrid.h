@@ -70,8 +70,9 @@ typedef struct {
                                   1..3: the AMG family, served by the same PCG */
     int verbosity;             /* linsolver_verbosity (:663) */
     int same_matrix;           /* (:665) accepted; the matrix is always rebuilt */
-    double prolongate_factor;  /* (:667) accepted for signature parity */
-    int smooth_steps;          /* (:668) accepted for signature parity */
+    double prolongate_factor;  /* (:667) multigrid coarse-correction factor (default 1.6) */
+    int smooth_steps;          /* (:668) multigrid smoothing sweeps before and after
+                                  the coarse correction */
     int precision;             /* enum mhs_precision */
     int nrhs;                  /* right-hand sides sharing the matrix, 1..4 */
     int chunk;                 /* PCG iterations per captured graph launch */
@@ -145,7 +146,7 @@ int mhs_init_system_structure(mhs_solver *s, const int *hf_bc_type,
 /* IncompFlowSolverHybrid::computeInnerProducts (:564-578) ->
  * MimeticIPEvaluator::buildStaticContrib (MimeticIPEvaluator.hpp:201-270).
  * perm[9*num_cells]: row-major 3x3 tensors (ReservoirPropertyCommon_impl.hpp:
- * 403-411); gravity[3]. */
+ * 403-411), used through their symmetric part (K + K')/2; gravity[3]. */
 int mhs_compute_inner_products(mhs_solver *s, const double *perm,
                                const double *gravity);
 /* The static inverse inner products after compute_inner_products, in the
@@ -204,6 +205,40 @@ int mhs_get_face_rows(mhs_solver *s, int *face_row);
  * numbering (explicit zeros kept).  Call with col == NULL to get *nnz. */
 int mhs_get_system(mhs_solver *s, int k, int *row_ptr, int *col, double *val,
                    double *rhs, long long *nnz);
+
+/* computeInnerProducts with perm[9*num_cells] in device memory (gravity[3]
+ * stays a host array); same result as mhs_compute_inner_products. */
+int mhs_compute_inner_products_device(mhs_solver *s, const double *perm,
+                                      const double *gravity);
+
+/* Copies of the last solution into caller-owned device buffers:
+ * cell_pressure[nrhs*nc], outflux[nrhs*nhf] (either may be NULL), ordered
+ * after the copy on the stream given to mhs_create. */
+int mhs_get_solution_device(mhs_solver *s, double *cell_pressure, double *outflux);
+
+/* Adjoint of the last solve.  No counterpart in the reference: this is the
+ * differentiated form of solve() (:657-703) for gradient-based calibration.
+ * dJ_dv[nrhs*nhf] is the gradient of a scalar objective J with respect to
+ * outflux(hf) of the last solve (before postProcessFluxes).  The call solves
+ * the adjoint system with the same face operator, preconditioner and
+ * precision as the last solve (opt gives tol, maxit, chunk, use_graph), then
+ * returns, each optional (NULL to skip):
+ *   grad_perm[9*nc]    dJ/dK_lm per cell (row-major, like perm); the inner
+ *                      products use the symmetric part of each tensor, so
+ *                      this gradient is symmetric;
+ *   grad_totmob[nc]    dJ/d(total mobility);
+ *   grad_rhomob[nc]    dJ/d(sum rho_i lambda_i);
+ *   grad_src[nrhs*nc]  dJ/dsrc per right-hand side.
+ * The first three are summed over right-hand sides.  Boundary values and
+ * geometry are held fixed.  Periodic boundaries are rejected.  The forward
+ * solution stays available through mhs_get_solution. */
+int mhs_adjoint(mhs_solver *s, const double *dJ_dv, const mhs_solve_options *opt,
+                double *grad_perm, double *grad_totmob, double *grad_rhomob,
+                double *grad_src, mhs_solve_report *report);
+/* Same with dJ_dv and every output in device memory. */
+int mhs_adjoint_device(mhs_solver *s, const double *dJ_dv, const mhs_solve_options *opt,
+                       double *grad_perm, double *grad_totmob, double *grad_rhomob,
+                       double *grad_src, mhs_solve_report *report);
 
 /* Profiling hook, no counterpart in the reference: average device time (ms)
  * of `reps` back-to-back launches of kernel `which` (0: the PCG SpMV with

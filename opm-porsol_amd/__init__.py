@@ -6,3 +6,11 @@ The directory name is not a Python identifier; load it with ``load_package``
 """
 from . import fields, grids, mhs  # noqa: F401
 from .mhs import HybridSolver, MhsError  # noqa: F401
+
+
+def __getattr__(name):
+    # torch is imported only when the autograd bridge is asked for
+    if name == "autograd":
+        import importlib
+        return importlib.import_module(__name__ + ".autograd")
+    raise AttributeError(name)

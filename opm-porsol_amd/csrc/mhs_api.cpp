@@ -62,6 +62,8 @@ struct mhs_solver {
 
     // static inner products
     double *binv = nullptr, *gflux = nullptr;
+    double grav[3] = {0.0, 0.0, 0.0};
+    double *perm = nullptr;             // K per cell (row-major 3x3), kept for the adjoint
 
     // multigrid hierarchy
     std::vector<int> ijk;
@@ -93,7 +95,13 @@ struct mhs_solver {
     size_t h_nsys = 0;
     std::vector<uint8_t> last_ftype;
     int last_nrhs = 0;
+    bool last_amg = false;
+    long long solve_count = 0;
     GraphCache graph;
+    // adjoint work arrays (allocated on the first adjoint call)
+    std::vector<DevBuf> adj_bufs;
+    double *adj_pi = nullptr, *adj_b = nullptr, *adj_w = nullptr, *adj_y = nullptr, *adj_gK = nullptr,
+           *adj_gtm = nullptr, *adj_grm = nullptr, *adj_gsrc = nullptr;
 
     ~mhs_solver();
     void free_topology();
@@ -137,7 +145,7 @@ T *dupload(mhs_solver *s, std::vector<DevBuf> &list, const std::vector<T> &v, hi
 void mhs_solver::free_topology() {
     for (auto &b : topo_bufs) (void)hipFree(b.p);
     topo_bufs.clear();
-    binv = gflux = nullptr;
+    binv = gflux = perm = nullptr;
     dT = DevTopo{};
     dG = DevGeom{};
     dA = DevAmg{};
@@ -150,6 +158,9 @@ void mhs_solver::free_solve_buffers() {
     graph = GraphCache{};
     for (auto &b : solve_bufs) (void)hipFree(b.p);
     solve_bufs.clear();
+    for (auto &b : adj_bufs) (void)hipFree(b.p);
+    adj_bufs.clear();
+    adj_pi = adj_b = adj_w = adj_y = adj_gK = adj_gtm = adj_grm = adj_gsrc = nullptr;
     cap_nrhs = 0;
     cap_mixed = false;
     if (h_sc) hipHostFree(h_sc);
@@ -354,26 +365,48 @@ int mhs_init_system_structure(mhs_solver *s, const int *hf_bc_type, const int *h
     return MHS_OK;
 }
 
-int mhs_compute_inner_products(mhs_solver *s, const double *perm, const double *gravity) {
+}  // extern "C"
+
+namespace {
+
+int inner_products_impl(mhs_solver *s, const double *perm, const double *gravity, bool device_perm) {
     if (!s) return MHS_ERR_INVALID;
     if (!s->structure_valid)
         return fail(s, MHS_ERR_STATE, "compute_inner_products: structure not initialised");
     if (!perm) return fail(s, MHS_ERR_INVALID, "compute_inner_products: perm is null");
     HIPCHECK(hipSetDevice(s->device));
-    double grav[3] = {0.0, 0.0, 0.0};
-    if (gravity) { grav[0] = gravity[0]; grav[1] = gravity[1]; grav[2] = gravity[2]; }
-    double *dperm = nullptr;
-    HIPCHECK(hipMalloc(&dperm, std::max<size_t>(1, 9 * (size_t)s->nc) * sizeof(double)));
-    hipError_t e = hipMemcpyAsync(dperm, perm, 9 * (size_t)s->nc * sizeof(double),
-                                  hipMemcpyHostToDevice, s->stream);
+    for (int d = 0; d < 3; ++d) s->grav[d] = gravity ? gravity[d] : 0.0;
+    const size_t n9 = 9 * (size_t)s->nc;
+    if (!s->perm) {
+        hipError_t he = hipSuccess;
+        s->perm = dalloc<double>(s, s->topo_bufs, n9, &he);
+        if (he != hipSuccess) return fail(s, MHS_ERR_ALLOC, std::string("device allocation: ") + hipGetErrorString(he));
+    }
+    if (device_perm && s->user_stream) {
+        HIPCHECK(hipEventRecord(s->ev_in, s->user_stream));
+        HIPCHECK(hipStreamWaitEvent(s->stream, s->ev_in, 0));
+    }
+    hipError_t e = hipMemcpyAsync(s->perm, perm, n9 * sizeof(double),
+                                  device_perm ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s->stream);
     if (e == hipSuccess)
-        e = launch_static_ip(s->stream, s->dT, s->dG, s->T.max_ncf, dperm, grav, s->binv, s->gflux);
+        e = launch_static_ip(s->stream, s->dT, s->dG, s->T.max_ncf, s->perm, s->grav, s->binv, s->gflux);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-    hipFree(dperm);
     if (e != hipSuccess) return fail(s, MHS_ERR_HIP, std::string("static inner products: ") + hipGetErrorString(e));
     s->ip_valid = true;
     s->solved = false;
     return MHS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mhs_compute_inner_products(mhs_solver *s, const double *perm, const double *gravity) {
+    return inner_products_impl(s, perm, gravity, false);
+}
+
+int mhs_compute_inner_products_device(mhs_solver *s, const double *perm, const double *gravity) {
+    return inner_products_impl(s, perm, gravity, true);
 }
 
 int mhs_assemble_static(mhs_solver *s, const double *perm, const double *gravity) {
@@ -563,6 +596,87 @@ int read_state(mhs_solver *s, size_t nsys, bool sc) {
     return MHS_OK;
 }
 
+struct LinearResult {
+    int iterations = 0, replacements = 0;
+    bool converged = false;
+    double reduction = 0.0;
+};
+
+// PCG on the assembled face system from the right-hand side in s->b and the
+// initial iterate in s->x64, to opt.tol relative FP64 residual (Jacobi or
+// multigrid preconditioner, FP64 or FP32 with FP64 residual replacement).
+int linear_solve(mhs_solver *s, const mhs_solve_options &opt, int nrhs, bool mixed, bool amg,
+                 LinearResult &out) {
+    const Topology &T = s->T;
+    const size_t nsys = (size_t)T.ninst * nrhs;
+    const int chunk = std::max(1, std::min(opt.chunk, 4096));
+    const int maxit = opt.maxit > 0 ? opt.maxit : std::max(1, T.nf_total);
+    const double tol2 = opt.tol * opt.tol;
+    s->S.mixed = mixed ? 1 : 0;
+    s->S.amg = amg ? 1 : 0;
+    s->S.rf2 = opt.replace_factor * opt.replace_factor;
+    HIPCHECK(hipMemsetAsync(s->S.status, 0, 4 * sizeof(int), s->stream));
+    HIPCHECK(hipMemsetAsync(s->S.iters, 0, nsys * sizeof(int), s->stream));
+    HIPCHECK(hipMemsetAsync(s->S.upd, 0, nsys * sizeof(int), s->stream));
+    if (!mixed) {
+        HIPCHECK(launch_pcg_start64(s->stream, s->dT, nrhs, s->val64, s->dinv64, s->b, s->x64, s->r64,
+                                    s->p64, tol2, s->R, s->S));
+    } else {
+        HIPCHECK(hipMemsetAsync(s->x32, 0, sizeof(float) * T.nfp * nrhs, s->stream));
+        HIPCHECK(launch_reliable(s->stream, s->dT, nrhs, s->val64, s->b, s->x64, s->dinv32, s->r32,
+                                 s->p32, 1, tol2, s->R, s->S));
+    }
+    if (amg) {
+        if (!mixed)
+            HIPCHECK(launch_amg_start<double>(s->stream, s->dT, s->dA, nrhs, s->val64, s->dinv64, s->r64,
+                                              s->z64, s->p64, s->R, s->S, nullptr, s->x64));
+        else
+            HIPCHECK(launch_amg_start<float>(s->stream, s->dT, s->dA, nrhs, s->val32, s->dinv32, s->r32,
+                                             s->z32, s->p32, s->R, s->S, nullptr, s->x32));
+    }
+    int rc = read_state(s, nsys, false);
+    if (rc) return rc;
+    while (s->h_status[0] > 0) {
+        int mx = 0;
+        for (size_t i = 0; i < nsys; ++i) mx = std::max(mx, s->h_iters[i]);
+        if (mx >= maxit) break;
+        const int n = std::min(chunk, maxit - mx);
+        if (!mixed)
+            rc = run_chunk<double, double>(s, nrhs, n, opt.use_graph && n == chunk, s->val64, s->dinv64,
+                                           s->x64, s->r64, s->p64, s->q64, s->z64, amg, MHS_FP64, tol2);
+        else
+            rc = run_chunk<float, float>(s, nrhs, n, opt.use_graph && n == chunk, s->val32, s->dinv32,
+                                         s->x32, s->r32, s->p32, s->q32, s->z32, amg, MHS_MIXED, tol2);
+        if (rc) return rc;
+        rc = read_state(s, nsys, opt.verbosity >= 2);
+        if (rc) return rc;
+        if (opt.verbosity >= 2)
+            for (size_t i = 0; i < std::min<size_t>(nsys, 4); ++i) {
+                const double *sc = s->h_sc + i * SC_N;
+                std::fprintf(stderr, "[mhs] sys %zu it %d act %d |r|/|r0| %.3e last FP64 %.3e\n", i,
+                             s->h_iters[i], s->h_act[i], std::sqrt(sc[SC_RR] / sc[SC_RR0]),
+                             std::sqrt(sc[SC_RRT] / sc[SC_RR0]));
+            }
+        if (s->h_status[1] > 0)
+            return fail(s, MHS_ERR_BREAKDOWN, "PCG breakdown: face operator not positive definite");
+    }
+    out.converged = s->h_status[0] == 0;
+    out.iterations = 0;
+    for (size_t i = 0; i < nsys; ++i) out.iterations = std::max(out.iterations, s->h_iters[i]);
+    if (mixed) HIPCHECK(launch_merge_all(s->stream, s->dT, nrhs, s->x64, s->x32));
+    HIPCHECK(launch_resnorm(s->stream, s->dT, nrhs, s->val64, s->b, s->x64, s->R, s->S));
+    rc = read_state(s, nsys, true);
+    if (rc) return rc;
+    out.replacements = s->h_status[2];
+    double red = 0.0;
+    for (size_t i = 0; i < nsys; ++i) {
+        const double *sc = s->h_sc + i * SC_N;
+        if (sc[SC_RR0] > 0) red = std::max(red, std::sqrt(sc[SC_RRT] / sc[SC_RR0]));
+    }
+    out.reduction = red;
+    return MHS_OK;
+}
+
 int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const double *src,
                const int *hf_bc_type, const double *bcval, const mhs_solve_options *opt_in,
                mhs_solve_report *rep, bool device_inputs) {
@@ -580,15 +694,12 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
         return fail(s, MHS_ERR_INVALID, "solve: unknown precision");
     if (!totmob || !rhomob || !src || !hf_bc_type || !bcval)
         return fail(s, MHS_ERR_INVALID, "solve: null input array");
-    const int chunk = std::max(1, std::min(opt.chunk, 4096));
     const int nrhs = opt.nrhs;
     const bool mixed = opt.precision == MHS_MIXED;
     HIPCHECK(hipSetDevice(s->device));
     const Topology &T = s->T;
     int rc = ensure_solve_buffers(s, nrhs, mixed);
     if (rc != MHS_OK) return rc;
-    const size_t nsys = (size_t)T.ninst * nrhs;
-    const int maxit = opt.maxit > 0 ? opt.maxit : std::max(1, T.nf_total);
 
     // Face types (FaceType, :378) and per-instance regularisation flags.
     std::vector<uint8_t> ft(T.nhf, FT_INTERNAL);
@@ -629,6 +740,12 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
         HIPCHECK(hipMemcpyAsync(s->in_src, src, sizeof(double) * T.nc * nrhs, kind, s->stream));
         HIPCHECK(hipMemcpyAsync(s->in_bcval, bcval, sizeof(double) * T.nhf * nrhs, kind, s->stream));
         d_tm = s->in_totmob; d_rm = s->in_rhomob; d_src = s->in_src; d_bv = s->in_bcval;
+    } else {
+        // the adjoint needs the mobilities after the caller's buffers may be gone
+        auto kind = hipMemcpyDeviceToDevice;
+        HIPCHECK(hipMemcpyAsync(s->in_totmob, totmob, sizeof(double) * T.nc, kind, s->stream));
+        HIPCHECK(hipMemcpyAsync(s->in_rhomob, rhomob, sizeof(double) * T.nc, kind, s->stream));
+        d_tm = s->in_totmob; d_rm = s->in_rhomob;
     }
     // --- assembly -----------------------------------------------------------
     HIPCHECK(launch_assemble(s->stream, s->dT, T.max_ncf, nrhs, s->binv, s->gflux, d_tm, d_rm, d_src,
@@ -658,68 +775,13 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
     }
 
     // --- linear solve ----------------------------------------------------------
-    const double tol2 = opt.tol * opt.tol;
-    int total_iters = 0;
-    s->S.mixed = mixed ? 1 : 0;
-    s->S.amg = amg ? 1 : 0;
-    s->S.rf2 = opt.replace_factor * opt.replace_factor;
-    HIPCHECK(hipMemsetAsync(s->S.status, 0, 4 * sizeof(int), s->stream));
-    HIPCHECK(hipMemsetAsync(s->S.iters, 0, nsys * sizeof(int), s->stream));
-    HIPCHECK(hipMemsetAsync(s->S.upd, 0, nsys * sizeof(int), s->stream));
-    if (!mixed) {
-        HIPCHECK(launch_pcg_start64(s->stream, s->dT, nrhs, s->val64, s->dinv64, s->b, s->x64, s->r64,
-                                    s->p64, tol2, s->R, s->S));
-    } else {
-        HIPCHECK(hipMemsetAsync(s->x32, 0, sizeof(float) * T.nfp * nrhs, s->stream));
-        HIPCHECK(launch_reliable(s->stream, s->dT, nrhs, s->val64, s->b, s->x64, s->dinv32, s->r32,
-                                 s->p32, 1, tol2, s->R, s->S));
-    }
-    if (amg) {
-        if (!mixed)
-            HIPCHECK(launch_amg_start<double>(s->stream, s->dT, s->dA, nrhs, s->val64, s->dinv64, s->r64,
-                                              s->z64, s->p64, s->R, s->S, nullptr, s->x64));
-        else
-            HIPCHECK(launch_amg_start<float>(s->stream, s->dT, s->dA, nrhs, s->val32, s->dinv32, s->r32,
-                                             s->z32, s->p32, s->R, s->S, nullptr, s->x32));
-    }
-    rc = read_state(s, nsys, false);
+    LinearResult lr;
+    rc = linear_solve(s, opt, nrhs, mixed, amg, lr);
     if (rc) return rc;
-    while (s->h_status[0] > 0) {
-        int mx = 0;
-        for (size_t i = 0; i < nsys; ++i) mx = std::max(mx, s->h_iters[i]);
-        if (mx >= maxit) break;
-        const int n = std::min(chunk, maxit - mx);
-        if (!mixed)
-            rc = run_chunk<double, double>(s, nrhs, n, opt.use_graph && n == chunk, s->val64, s->dinv64,
-                                           s->x64, s->r64, s->p64, s->q64, s->z64, amg, MHS_FP64, tol2);
-        else
-            rc = run_chunk<float, float>(s, nrhs, n, opt.use_graph && n == chunk, s->val32, s->dinv32,
-                                         s->x32, s->r32, s->p32, s->q32, s->z32, amg, MHS_MIXED, tol2);
-        if (rc) return rc;
-        rc = read_state(s, nsys, opt.verbosity >= 2);
-        if (rc) return rc;
-        if (opt.verbosity >= 2)
-            for (size_t i = 0; i < std::min<size_t>(nsys, 4); ++i) {
-                const double *sc = s->h_sc + i * SC_N;
-                std::fprintf(stderr, "[mhs] sys %zu it %d act %d |r|/|r0| %.3e last FP64 %.3e\n", i,
-                             s->h_iters[i], s->h_act[i], std::sqrt(sc[SC_RR] / sc[SC_RR0]),
-                             std::sqrt(sc[SC_RRT] / sc[SC_RR0]));
-            }
-        if (s->h_status[1] > 0)
-            return fail(s, MHS_ERR_BREAKDOWN, "PCG breakdown: face operator not positive definite");
-    }
-    const bool converged = s->h_status[0] == 0;
-    for (size_t i = 0; i < nsys; ++i) total_iters = std::max(total_iters, s->h_iters[i]);
-    if (mixed) HIPCHECK(launch_merge_all(s->stream, s->dT, nrhs, s->x64, s->x32));
-    HIPCHECK(launch_resnorm(s->stream, s->dT, nrhs, s->val64, s->b, s->x64, s->R, s->S));
-    rc = read_state(s, nsys, true);
-    if (rc) return rc;
-    const int replacements = s->h_status[2];
-    double red = 0.0;
-    for (size_t i = 0; i < nsys; ++i) {
-        const double *sc = s->h_sc + i * SC_N;
-        if (sc[SC_RR0] > 0) red = std::max(red, std::sqrt(sc[SC_RRT] / sc[SC_RR0]));
-    }
+    const int total_iters = lr.iterations, replacements = lr.replacements;
+    const bool converged = lr.converged;
+    const double red = lr.reduction;
+    s->last_amg = amg;
     auto t2 = std::chrono::steady_clock::now();
 
     // --- recovery -------------------------------------------------------------
@@ -733,6 +795,7 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
     auto t3 = std::chrono::steady_clock::now();
     s->solved = true;
     s->last_nrhs = nrhs;
+    ++s->solve_count;
     if (rep) {
         rep->iterations = total_iters;
         rep->replacements = replacements;
@@ -758,9 +821,123 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
     return MHS_OK;
 }
 
+int adjoint_impl(mhs_solver *s, const double *dJ_dv, const mhs_solve_options *opt_in, double *grad_perm,
+                 double *grad_totmob, double *grad_rhomob, double *grad_src, mhs_solve_report *rep,
+                 bool device_io) {
+    if (!s) return MHS_ERR_INVALID;
+    if (!s->solved) return fail(s, MHS_ERR_STATE, "adjoint: no solution (call solve)");
+    if (!dJ_dv) return fail(s, MHS_ERR_INVALID, "adjoint: dJ_dv is null");
+    if (s->T.has_periodic) return fail(s, MHS_ERR_INVALID, "adjoint: periodic boundaries are not supported");
+    if (!s->perm) return fail(s, MHS_ERR_STATE, "adjoint: no permeability (call compute_inner_products)");
+    mhs_solve_options opt;
+    mhs_default_options(&opt);
+    if (opt_in) opt = *opt_in;
+    if (!(opt.tol > 0.0)) return fail(s, MHS_ERR_INVALID, "adjoint: tol must be positive");
+    HIPCHECK(hipSetDevice(s->device));
+    const Topology &T = s->T;
+    const int nrhs = s->last_nrhs;
+    const bool mixed = s->cap_mixed, amg = s->last_amg;
+    const size_t nv = (size_t)T.nfp * nrhs, nh = (size_t)T.nhf * nrhs, nc = (size_t)T.nc;
+    if (s->adj_bufs.empty()) {
+        hipError_t he = hipSuccess;
+        auto &L = s->adj_bufs;
+        s->adj_pi = dalloc<double>(s, L, nv, &he);
+        if (he == hipSuccess) s->adj_b = dalloc<double>(s, L, nv, &he);
+        if (he == hipSuccess) s->adj_w = dalloc<double>(s, L, nh, &he);
+        if (he == hipSuccess) s->adj_y = dalloc<double>(s, L, nh, &he);
+        if (he == hipSuccess) s->adj_gK = dalloc<double>(s, L, 9 * nc, &he);
+        if (he == hipSuccess) s->adj_gtm = dalloc<double>(s, L, nc, &he);
+        if (he == hipSuccess) s->adj_grm = dalloc<double>(s, L, nc, &he);
+        if (he == hipSuccess) s->adj_gsrc = dalloc<double>(s, L, (size_t)nrhs * nc, &he);
+        if (he != hipSuccess) {
+            for (auto &b : L) (void)hipFree(b.p);
+            L.clear();
+            return fail(s, MHS_ERR_ALLOC, std::string("adjoint buffers: ") + hipGetErrorString(he));
+        }
+    }
+    if (device_io && s->user_stream) {
+        HIPCHECK(hipEventRecord(s->ev_in, s->user_stream));
+        HIPCHECK(hipStreamWaitEvent(s->stream, s->ev_in, 0));
+    }
+    const auto d2d = hipMemcpyDeviceToDevice;
+    HIPCHECK(hipMemcpyAsync(s->adj_w, dJ_dv, sizeof(double) * nh, device_io ? d2d : hipMemcpyHostToDevice,
+                            s->stream));
+    // keep the forward face pressures and right-hand side; the adjoint solve reuses x64 and b
+    HIPCHECK(hipMemcpyAsync(s->adj_pi, s->x64, sizeof(double) * nv, d2d, s->stream));
+    HIPCHECK(hipMemcpyAsync(s->adj_b, s->b, sizeof(double) * nv, d2d, s->stream));
+    HIPCHECK(launch_adjoint_rhs(s->stream, s->dT, T.max_ncf, nrhs, s->binv, s->in_totmob, s->C, s->ftype,
+                                s->adj_w, s->adj_y, s->b, s->x64));
+    LinearResult lr;
+    int rc = linear_solve(s, opt, nrhs, mixed, amg, lr);
+    if (rc == MHS_OK && lr.converged)
+        HIPCHECK(launch_adjoint_grad(s->stream, s->dT, s->dG, T.max_ncf, nrhs, s->binv, s->gflux, s->in_totmob,
+                                     s->in_rhomob, s->grav, s->C, s->press, s->adj_pi, s->x64, s->ftype,
+                                     s->adj_w, s->adj_gK, s->adj_gtm, s->adj_grm, s->adj_gsrc));
+    // restore the forward state whatever happened
+    HIPCHECK(hipMemcpyAsync(s->x64, s->adj_pi, sizeof(double) * nv, d2d, s->stream));
+    HIPCHECK(hipMemcpyAsync(s->b, s->adj_b, sizeof(double) * nv, d2d, s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    if (rc) return rc;
+    if (rep) {
+        *rep = mhs_solve_report{};
+        rep->iterations = lr.iterations;
+        rep->replacements = lr.replacements;
+        rep->converged = lr.converged ? 1 : 0;
+        rep->reduction = lr.reduction;
+        rep->precond = amg ? 1 : 0;
+        rep->levels = amg ? s->dA.nlev : 0;
+    }
+    if (!lr.converged)
+        return fail(s, MHS_ERR_NOT_CONVERGED,
+                    "Adjoint solve failed to converge in " + std::to_string(lr.iterations) +
+                        " iterations.\nResidual reduction achieved is " + std::to_string(lr.reduction) + "\n");
+    const auto out = device_io ? d2d : hipMemcpyDeviceToHost;
+    if (grad_perm) HIPCHECK(hipMemcpyAsync(grad_perm, s->adj_gK, sizeof(double) * 9 * nc, out, s->stream));
+    if (grad_totmob) HIPCHECK(hipMemcpyAsync(grad_totmob, s->adj_gtm, sizeof(double) * nc, out, s->stream));
+    if (grad_rhomob) HIPCHECK(hipMemcpyAsync(grad_rhomob, s->adj_grm, sizeof(double) * nc, out, s->stream));
+    if (grad_src)
+        HIPCHECK(hipMemcpyAsync(grad_src, s->adj_gsrc, sizeof(double) * nrhs * nc, out, s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    if (device_io && s->user_stream) {
+        HIPCHECK(hipEventRecord(s->ev_out, s->stream));
+        HIPCHECK(hipStreamWaitEvent(s->user_stream, s->ev_out, 0));
+    }
+    return MHS_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int mhs_adjoint(mhs_solver *s, const double *dJ_dv, const mhs_solve_options *opt, double *grad_perm,
+                double *grad_totmob, double *grad_rhomob, double *grad_src, mhs_solve_report *report) {
+    return adjoint_impl(s, dJ_dv, opt, grad_perm, grad_totmob, grad_rhomob, grad_src, report, false);
+}
+
+int mhs_adjoint_device(mhs_solver *s, const double *dJ_dv, const mhs_solve_options *opt, double *grad_perm,
+                       double *grad_totmob, double *grad_rhomob, double *grad_src,
+                       mhs_solve_report *report) {
+    return adjoint_impl(s, dJ_dv, opt, grad_perm, grad_totmob, grad_rhomob, grad_src, report, true);
+}
+
+int mhs_get_solution_device(mhs_solver *s, double *cell_pressure, double *outflux) {
+    if (!s) return MHS_ERR_INVALID;
+    if (!s->solved) return fail(s, MHS_ERR_STATE, "get_solution_device: no solution (call solve)");
+    HIPCHECK(hipSetDevice(s->device));
+    const Topology &T = s->T;
+    const size_t nrhs = s->last_nrhs;
+    const auto d2d = hipMemcpyDeviceToDevice;
+    if (cell_pressure)
+        HIPCHECK(hipMemcpyAsync(cell_pressure, s->press, sizeof(double) * T.nc * nrhs, d2d, s->stream));
+    if (outflux) HIPCHECK(hipMemcpyAsync(outflux, s->outflux, sizeof(double) * T.nhf * nrhs, d2d, s->stream));
+    if (s->user_stream) {
+        HIPCHECK(hipEventRecord(s->ev_out, s->stream));
+        HIPCHECK(hipStreamWaitEvent(s->user_stream, s->ev_out, 0));
+    } else {
+        HIPCHECK(hipStreamSynchronize(s->stream));
+    }
+    return MHS_OK;
+}
 
 int mhs_solve(mhs_solver *s, const double *totmob, const double *rhomob, const double *src,
               const int *hf_bc_type, const double *hf_bc_value, const mhs_solve_options *opt,

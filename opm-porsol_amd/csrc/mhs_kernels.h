@@ -140,6 +140,18 @@ hipError_t launch_amg_start(hipStream_t st, const DevTopo &T, const DevAmg &A, i
                             double *x64, VT *x);
 
 // Recovery and post-processing.
+// Adjoint of the recovered fluxes (see k_adj_cell / k_adj_grad): the adjoint
+// right-hand side into b (x zeroed), and the gradients after the adjoint solve.
+hipError_t launch_adjoint_rhs(hipStream_t st, const DevTopo &T, int maxn, int nrhs, const double *binv_pk,
+                              const double *totmob, const DevCellSys &C, const uint8_t *ftype,
+                              const double *w, double *y, double *b, double *x);
+hipError_t launch_adjoint_grad(hipStream_t st, const DevTopo &T, const DevGeom &G, int maxn, int nrhs,
+                               const double *binv_pk, const double *gflux, const double *totmob,
+                               const double *rhomob, const double *grav, const DevCellSys &C,
+                               const double *press, const double *pi, const double *mu,
+                               const uint8_t *ftype, const double *w, double *gK, double *gtm,
+                               double *grm, double *gsrc);
+
 hipError_t launch_recover(hipStream_t st, const DevTopo &T, int maxn, int nrhs,
                           const double *binv_pk, const double *gflux, const double *totmob,
                           const double *rhomob, const DevCellSys &C, const double *x64,

@@ -212,25 +212,14 @@ __device__ __forceinline__ void sell_row(const DevTopo &T, const MT *__restrict_
 // N rows area*normal, C rows area*(x_f - x_c)  (MimeticIPEvaluator.hpp:236-269).
 // Q comes from Gram-Schmidt applied twice (only QQ' is used, so any orthonormal
 // basis of range(C) gives the same B^-1).
+// Per-cell frame of the mimetic inner product: N (rows area * unit normal),
+// the face areas a and an orthonormal basis Q of range(C), C rows
+// area * (x_f - x_c); rows i >= n are zero.
 template <int MAXN>
-__global__ __launch_bounds__(64) void k_static_ip(DevTopo T, DevGeom G,
-                                                  const double *__restrict__ perm, double gx,
-                                                  double gy, double gz,
-                                                  double *__restrict__ binv,
-                                                  double *__restrict__ gflux) {
-    const int c = blockIdx.x * 64 + threadIdx.x;
-    if (c >= T.nc) return;
-    const int f0 = T.facepos[c], n = T.facepos[c + 1] - f0;
-    double K[9];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) K[i] = perm[9 * (size_t)c + i];
-    const double Kg0 = K[0] * gx + K[1] * gy + K[2] * gz;
-    const double Kg1 = K[3] * gx + K[4] * gy + K[5] * gz;
-    const double Kg2 = K[6] * gx + K[7] * gy + K[8] * gz;
-    const double vol = G.cvol[c];
+__device__ __forceinline__ void cell_frame(const DevGeom &G, int c, int f0, int n, double (&N)[MAXN][3],
+                                           double (&Q)[MAXN][3], double (&a)[MAXN]) {
     const double cc0 = G.ccen[3 * (size_t)c], cc1 = G.ccen[3 * (size_t)c + 1],
                  cc2 = G.ccen[3 * (size_t)c + 2];
-    double N[MAXN][3], Q[MAXN][3], a[MAXN];
 #pragma unroll
     for (int i = 0; i < MAXN; ++i) {
         if (i < n) {
@@ -243,7 +232,6 @@ __global__ __launch_bounds__(64) void k_static_ip(DevTopo T, DevGeom G,
             Q[i][0] = (G.fcen[3 * h] - cc0) * ar;
             Q[i][1] = (G.fcen[3 * h + 1] - cc1) * ar;
             Q[i][2] = (G.fcen[3 * h + 2] - cc2) * ar;
-            gflux[h] = N[i][0] * Kg0 + N[i][1] * Kg1 + N[i][2] * Kg2;
         } else {
             a[i] = 0.0;
             N[i][0] = N[i][1] = N[i][2] = 0.0;
@@ -270,6 +258,34 @@ __global__ __launch_bounds__(64) void k_static_ip(DevTopo T, DevGeom G,
 #pragma unroll
         for (int i = 0; i < MAXN; ++i) Q[i][j] *= inv;
     }
+}
+
+template <int MAXN>
+__global__ __launch_bounds__(64) void k_static_ip(DevTopo T, DevGeom G,
+                                                  const double *__restrict__ perm, double gx,
+                                                  double gy, double gz,
+                                                  double *__restrict__ binv,
+                                                  double *__restrict__ gflux) {
+    const int c = blockIdx.x * 64 + threadIdx.x;
+    if (c >= T.nc) return;
+    const int f0 = T.facepos[c], n = T.facepos[c + 1] - f0;
+    // K enters through its symmetric part (a permeability tensor is symmetric;
+    // this makes the result, and its gradient, well defined for any input).
+    double K[9];
+#pragma unroll
+    for (int l = 0; l < 3; ++l)
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+            K[3 * l + m] = 0.5 * (perm[9 * (size_t)c + 3 * l + m] + perm[9 * (size_t)c + 3 * m + l]);
+    const double Kg0 = K[0] * gx + K[1] * gy + K[2] * gz;
+    const double Kg1 = K[3] * gx + K[4] * gy + K[5] * gz;
+    const double Kg2 = K[6] * gx + K[7] * gy + K[8] * gz;
+    const double vol = G.cvol[c];
+    double N[MAXN][3], Q[MAXN][3], a[MAXN];
+    cell_frame<MAXN>(G, c, f0, n, N, Q, a);
+#pragma unroll
+    for (int i = 0; i < MAXN; ++i)
+        if (i < n) gflux[f0 + i] = N[i][0] * Kg0 + N[i][1] * Kg1 + N[i][2] * Kg2;
     double NK[MAXN][3];
 #pragma unroll
     for (int i = 0; i < MAXN; ++i)
@@ -1187,6 +1203,200 @@ __global__ __launch_bounds__(256) void k_post(DevTopo T, int nrhs, const double 
 }
 
 // ---------------------------------------------------------------------------
+// Adjoint of the recovered fluxes.  Cell c's flux at fixed face pressures is
+//   v_c = B (p 1 - pi_c) + rl gf_c = -S_c pi_c + F g/L + rl gf_c,
+//   B = lam Binv, F = B 1, L = 1'F, p = (g + F'pi_c)/L, S_c = B - F F'/L,
+// so for an objective J(v) with w = dJ/dv the adjoint system is
+//   S mu = -sum_c P_c' S_c w_c   (zero on Dirichlet rows, which fix pi)
+// and dJ/dtheta = sum_c u_c' dv_c/dtheta at fixed pi, u_c = w_c + P_c mu.
+
+// y = -S_c w_c per half-face.
+template <int MAXN, int NRHS>
+__global__ __launch_bounds__(64) void k_adj_cell(DevTopo T, const double *__restrict__ binv,
+                                                 const double *__restrict__ totmob, DevCellSys C,
+                                                 const double *__restrict__ w, double *__restrict__ y) {
+    const int c = blockIdx.x * 64 + threadIdx.x;
+    if (c >= T.nc) return;
+    const int f0 = T.facepos[c], n = T.facepos[c + 1] - f0;
+    const double lam = totmob[c], L = C.L[c];
+    const long long pk = T.pk_pos[c];
+    double F[MAXN];
+#pragma unroll
+    for (int i = 0; i < MAXN; ++i) F[i] = (i < n) ? C.F[f0 + i] : 0.0;
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) {
+        double wv[MAXN], fw = 0.0;
+#pragma unroll
+        for (int i = 0; i < MAXN; ++i) {
+            wv[i] = (i < n) ? w[(size_t)k * T.nhf + f0 + i] : 0.0;
+            fw += F[i] * wv[i];
+        }
+#pragma unroll
+        for (int i = 0; i < MAXN; ++i) {
+            if (i >= n) continue;
+            double s = 0.0;
+#pragma unroll
+            for (int j = 0; j < MAXN; ++j)
+                if (j < n) s += binv[pk + (i <= j ? pidx(i, j, n) : pidx(j, i, n))] * wv[j];
+            y[(size_t)k * T.nhf + f0 + i] = -(lam * s - F[i] * fw / L);
+        }
+    }
+}
+
+// Adjoint right-hand side per row: the sum of the row's half-face values, zero
+// on Dirichlet and padding rows; the initial iterate is zero.
+template <int NRHS>
+__global__ __launch_bounds__(256) void k_adj_rows(DevTopo T, const uint8_t *__restrict__ ftype,
+                                                  const double *__restrict__ y, double *__restrict__ b,
+                                                  double *__restrict__ x) {
+    const int f = blockIdx.x * 256 + threadIdx.x;
+    if (f >= T.nfp) return;
+    const int h0 = T.dof_hf[f];
+    double acc[NRHS];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) acc[k] = 0.0;
+    if (h0 >= 0 && ftype[h0] != FT_DIRICHLET) {
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int h = T.row_hf[2 * (size_t)f + t];
+            if (h < 0) continue;
+#pragma unroll
+            for (int k = 0; k < NRHS; ++k) acc[k] += y[(size_t)k * T.nhf + h];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) {
+        b[(size_t)f * NRHS + k] = acc[k];
+        x[(size_t)f * NRHS + k] = 0.0;
+    }
+}
+
+// Gradient per cell.  With Binv = (N K N' + t W)/|c|, t = 2 tr K,
+// W = A (I - Q Q') A and gf = N K g:
+//   dBinv/dK_lm = (N_l N_m' + 2 delta_lm W)/|c|,  d gf/dK_lm = N_l g_m,
+// and for any parameter dv = B' d + F p' + rl gf',  d = p 1 - pi,
+//   p' = (g' + 1'B' pi - p 1'B' 1)/L,  g' = -sum rl gf'.
+// Outputs (any may be null): gK[9c + 3l + m] = dJ/dK_lm with K entering
+// through sym(K) as in k_static_ip, gtm[c] = dJ/dlam, grm[c] = dJ/drl (summed over right-hand
+// sides), gsrc[k nc + c] = dJ/dsrc_c = u'F/L.
+template <int MAXN, int NRHS>
+__global__ __launch_bounds__(64) void k_adj_grad(DevTopo T, DevGeom G, const double *__restrict__ binv,
+                                                 const double *__restrict__ gfs,
+                                                 const double *__restrict__ totmob,
+                                                 const double *__restrict__ rhomob, double gx, double gy,
+                                                 double gz, DevCellSys C, const double *__restrict__ press,
+                                                 const double *__restrict__ pi, const double *__restrict__ mu,
+                                                 const uint8_t *__restrict__ ftype,
+                                                 const double *__restrict__ w, double *__restrict__ gK,
+                                                 double *__restrict__ gtm, double *__restrict__ grm,
+                                                 double *__restrict__ gsrc) {
+    const int c = blockIdx.x * 64 + threadIdx.x;
+    if (c >= T.nc) return;
+    const int f0 = T.facepos[c], n = T.facepos[c + 1] - f0;
+    const double lam = totmob[c], rl = rhomob[c], L = C.L[c], vol = G.cvol[c];
+    const long long pk = T.pk_pos[c];
+    const double g[3] = {gx, gy, gz};
+    double N[MAXN][3], Q[MAXN][3], a[MAXN];
+    cell_frame<MAXN>(G, c, f0, n, N, Q, a);
+    double F[MAXN], gf[MAXN];
+    int dof[MAXN];
+    bool dir[MAXN];
+    double e[3] = {0.0, 0.0, 0.0}, aq[3] = {0.0, 0.0, 0.0}, a2 = 0.0, sg = 0.0;
+#pragma unroll
+    for (int i = 0; i < MAXN; ++i) {
+        const bool in = i < n;
+        F[i] = in ? C.F[f0 + i] : 0.0;
+        gf[i] = in ? gfs[f0 + i] : 0.0;
+        dof[i] = in ? T.hf_dof[f0 + i] : 0;
+        dir[i] = in ? ftype[f0 + i] == FT_DIRICHLET : true;
+        sg += gf[i];
+        a2 += a[i] * a[i];
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            e[l] += N[i][l];
+            aq[l] += a[i] * Q[i][l];
+        }
+    }
+    const double oWo = a2 - (aq[0] * aq[0] + aq[1] * aq[1] + aq[2] * aq[2]);
+    const double rv = lam / vol;
+    double GK[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) GK[q] = 0.0;
+    double gt = 0.0, gr = 0.0;
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) {
+        const double p = press[(size_t)k * T.nc + c], gc = C.g[(size_t)k * T.nc + c];
+        double u[MAXN], pv[MAXN], d[MAXN];
+#pragma unroll
+        for (int i = 0; i < MAXN; ++i) {
+            if (i < n) {
+                const size_t r = (size_t)dof[i] * NRHS + k;
+                u[i] = w[(size_t)k * T.nhf + f0 + i] + (dir[i] ? 0.0 : mu[r]);
+                pv[i] = pi[r];
+                d[i] = p - pv[i];
+            } else {
+                u[i] = pv[i] = d[i] = 0.0;
+            }
+        }
+        double aU[3] = {0.0, 0.0, 0.0}, bD[3] = {0.0, 0.0, 0.0}, fP[3] = {0.0, 0.0, 0.0};
+        double uq[3] = {0.0, 0.0, 0.0}, dq[3] = {0.0, 0.0, 0.0}, pq[3] = {0.0, 0.0, 0.0};
+        double ua2d = 0.0, a2p = 0.0, uF = 0.0, ugf = 0.0;
+#pragma unroll
+        for (int i = 0; i < MAXN; ++i) {
+#pragma unroll
+            for (int l = 0; l < 3; ++l) {
+                aU[l] += u[i] * N[i][l];
+                bD[l] += N[i][l] * d[i];
+                fP[l] += N[i][l] * pv[i];
+                uq[l] += u[i] * a[i] * Q[i][l];
+                dq[l] += d[i] * a[i] * Q[i][l];
+                pq[l] += pv[i] * a[i] * Q[i][l];
+            }
+            ua2d += u[i] * a[i] * a[i] * d[i];
+            a2p += a[i] * a[i] * pv[i];
+            uF += u[i] * F[i];
+            ugf += u[i] * gf[i];
+        }
+        const double uWd = ua2d - (uq[0] * dq[0] + uq[1] * dq[1] + uq[2] * dq[2]);
+        const double oWp = a2p - (aq[0] * pq[0] + aq[1] * pq[1] + aq[2] * pq[2]);
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                const double del = (l == m) ? 2.0 : 0.0;
+                const double uBd = rv * (aU[l] * bD[m] + del * uWd);
+                const double oBp = rv * (e[l] * fP[m] + del * oWp);
+                const double oBo = rv * (e[l] * e[m] + del * oWo);
+                const double pp = (-rl * e[l] * g[m] + oBp - p * oBo) / L;
+                GK[3 * l + m] += uBd + uF * pp + rl * aU[l] * g[m];
+            }
+        // lam: B' = Binv, p' = -g/(lam L)
+        double uBd = 0.0;
+#pragma unroll
+        for (int i = 0; i < MAXN; ++i) {
+            if (i >= n) continue;
+            double s = 0.0;
+#pragma unroll
+            for (int j = 0; j < MAXN; ++j)
+                if (j < n) s += binv[pk + (i <= j ? pidx(i, j, n) : pidx(j, i, n))] * d[j];
+            uBd += u[i] * s;
+        }
+        gt += uBd - uF * gc / (lam * L);
+        // rl: gf' = gf, g' = -sum gf, p' = g'/L
+        gr += ugf - uF * sg / L;
+        if (gsrc) gsrc[(size_t)k * T.nc + c] = uF / L;
+    }
+    // the inner products use sym(K), so dJ/dK_lm = (G_lm + G_ml)/2
+    if (gK)
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+#pragma unroll
+            for (int m = 0; m < 3; ++m) gK[9 * (size_t)c + 3 * l + m] = 0.5 * (GK[3 * l + m] + GK[3 * m + l]);
+    if (gtm) gtm[c] = gt;
+    if (grm) grm[c] = gr;
+}
+
+// ---------------------------------------------------------------------------
 // Finishers of the per-instance reductions (run by k_finish on thread 0).
 template <int NRHS>
 __device__ __forceinline__ bool none_set(const int *flags, int s) {
@@ -1695,6 +1905,42 @@ hipError_t launch_post_process(hipStream_t st, const DevTopo &T, int nrhs, const
     if (tot == 0) return hipSuccess;
     k_post<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(T, nrhs, vin, vout, maxbits);
     return hipGetLastError();
+}
+
+hipError_t launch_adjoint_rhs(hipStream_t st, const DevTopo &T, int maxn, int nrhs, const double *binv_pk,
+                              const double *totmob, const DevCellSys &C, const uint8_t *ftype,
+                              const double *w, double *y, double *b, double *x) {
+    hipError_t e = hipSuccess;
+    hipError_t e2 = with_maxn(maxn, [&](auto mn) {
+        constexpr int MN = decltype(mn)::value;
+        e = with_nrhs(nrhs, [&](auto nr) {
+            constexpr int NR = decltype(nr)::value;
+            if (T.nc > 0) k_adj_cell<MN, NR><<<cell_blocks(T.nc), 64, 0, st>>>(T, binv_pk, totmob, C, w, y);
+            k_adj_rows<NR><<<(unsigned)((T.nfp + 255) / 256), 256, 0, st>>>(T, ftype, y, b, x);
+        });
+    });
+    return e != hipSuccess ? e : e2;
+}
+
+hipError_t launch_adjoint_grad(hipStream_t st, const DevTopo &T, const DevGeom &G, int maxn, int nrhs,
+                               const double *binv_pk, const double *gflux, const double *totmob,
+                               const double *rhomob, const double *grav, const DevCellSys &C,
+                               const double *press, const double *pi, const double *mu,
+                               const uint8_t *ftype, const double *w, double *gK, double *gtm,
+                               double *grm, double *gsrc) {
+    if (T.nc == 0) return hipSuccess;
+    const double gx = grav[0], gy = grav[1], gz = grav[2];
+    hipError_t e = hipSuccess;
+    hipError_t e2 = with_maxn(maxn, [&](auto mn) {
+        constexpr int MN = decltype(mn)::value;
+        e = with_nrhs(nrhs, [&](auto nr) {
+            constexpr int NR = decltype(nr)::value;
+            k_adj_grad<MN, NR><<<cell_blocks(T.nc), 64, 0, st>>>(T, G, binv_pk, gflux, totmob, rhomob, gx,
+                                                                 gy, gz, C, press, pi, mu, ftype, w, gK,
+                                                                 gtm, grm, gsrc);
+        });
+    });
+    return e != hipSuccess ? e : e2;
 }
 
 }  // namespace mhs

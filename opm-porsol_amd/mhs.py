@@ -30,7 +30,8 @@ EXPORTS = (
     "mhs_set_grid", "mhs_set_cell_ijk", "mhs_init_system_structure", "mhs_compute_inner_products",
     "mhs_assemble_static", "mhs_get_inner_products", "mhs_solve", "mhs_solve_device", "mhs_get_solution",
     "mhs_solution_device", "mhs_post_process_fluxes", "mhs_get_dims", "mhs_get_cell_faces",
-    "mhs_get_face_rows", "mhs_get_system", "mhs_time_kernel",
+    "mhs_get_face_rows", "mhs_get_system", "mhs_time_kernel", "mhs_compute_inner_products_device",
+    "mhs_get_solution_device", "mhs_adjoint", "mhs_adjoint_device",
 )
 
 
@@ -74,6 +75,13 @@ def lib():
     if _LIB is None:
         if not os.path.exists(LIBPATH):
             raise ImportError(f"{LIBPATH} is missing: build it with __graft_entry__.build()")
+        # torch ships its own HIP runtime under the same soname as the system
+        # one.  Loading torch first makes libmhs bind to that runtime, so the
+        # process holds one HIP runtime; two cannot both open the GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIBPATH)
         L.mhs_default_options.argtypes = [ctypes.POINTER(Options)]
         L.mhs_default_options.restype = None
@@ -102,6 +110,12 @@ def lib():
         L.mhs_get_system.argtypes = [_P, ctypes.c_int, _I, _I, _D, _D,
                                      ctypes.POINTER(ctypes.c_longlong)]
         L.mhs_time_kernel.argtypes = [_P, ctypes.c_int, ctypes.c_int, _D]
+        L.mhs_compute_inner_products_device.argtypes = [_P, _P, _D]
+        L.mhs_get_solution_device.argtypes = [_P, _P, _P]
+        L.mhs_adjoint.argtypes = [_P, _D, ctypes.POINTER(Options), _D, _D, _D, _D,
+                                  ctypes.POINTER(Report)]
+        L.mhs_adjoint_device.argtypes = [_P, _P, ctypes.POINTER(Options), _P, _P, _P, _P,
+                                         ctypes.POINTER(Report)]
         for name in EXPORTS:
             if name not in ("mhs_default_options", "mhs_last_error"):
                 getattr(L, name).restype = ctypes.c_int
@@ -135,9 +149,11 @@ class HybridSolver:
         if rc != MHS_OK:
             raise MhsError(rc, "mhs_create failed (no usable GPU?)")
         self._h = h
+        self.stream = stream
         self.nrhs = 1
         self.grid = None
         self.report = None
+        self.solve_count = 0
 
     def __del__(self):
         try:
@@ -195,6 +211,11 @@ class HybridSolver:
         self._check(self._L.mhs_compute_inner_products(self._h, pK, pg))
 
     assembleStatic = computeInnerProducts
+
+    def computeInnerProducts_device(self, perm_ptr, gravity=(0.0, 0.0, 0.0)):
+        """computeInnerProducts with perm[nc*9] in device memory (a data_ptr())."""
+        gr, pg = _d(gravity)
+        self._check(self._L.mhs_compute_inner_products_device(self._h, _P(perm_ptr), pg))
 
     def inner_products(self):
         """(binv, gflux): per-cell B^-1 column-major, concatenated; N K g per half-face."""
@@ -258,6 +279,7 @@ class HybridSolver:
                                ctypes.byref(rep))
         self.report = rep
         self.nrhs = nrhs
+        self.solve_count += 1
         if rc == MHS_ERR_NOT_CONVERGED and not raise_on_failure:
             return rep
         self._check(rc)
@@ -272,6 +294,7 @@ class HybridSolver:
                                       _P(bcval_ptr), ctypes.byref(o), ctypes.byref(rep))
         self.report = rep
         self.nrhs = nrhs
+        self.solve_count += 1
         self._check(rc)
         return rep
 
@@ -292,6 +315,47 @@ class HybridSolver:
         self._check(self._L.mhs_solution_device(self._h, ctypes.byref(a), ctypes.byref(b),
                                                 ctypes.byref(c)))
         return a.value, b.value, c.value
+
+    def get_solution_device(self, cell_pressure_ptr=None, outflux_ptr=None):
+        """Copy the last cell pressures / outfluxes into caller device buffers."""
+        self._check(self._L.mhs_get_solution_device(self._h, _P(cell_pressure_ptr or 0),
+                                                    _P(outflux_ptr or 0)))
+
+    # --- adjoint --------------------------------------------------------------------
+    def adjoint(self, dJ_dv, tol=1e-10, **kw):
+        """Gradients of J through the last solve, given dJ/dv per half-face
+        ([nrhs, nhf] or [nhf]).  Returns a dict with grad_perm [nc, 9],
+        grad_totmob [nc], grad_rhomob [nc] and grad_src ([nrhs, nc] or [nc])."""
+        d = self.dims()
+        w, pw = _d(np.asarray(dJ_dv).reshape(-1))
+        if w.shape[0] != self.nrhs * d.num_half_faces:
+            raise ValueError("dJ_dv must have nrhs * num_half_faces entries")
+        o = self.options(tol=tol, nrhs=self.nrhs, **kw)
+        gK = np.empty((d.num_cells, 9))
+        gt = np.empty(d.num_cells)
+        gr = np.empty(d.num_cells)
+        gs = np.empty((self.nrhs, d.num_cells))
+        rep = Report()
+        rc = self._L.mhs_adjoint(self._h, pw, ctypes.byref(o), gK.ctypes.data_as(_D),
+                                 gt.ctypes.data_as(_D), gr.ctypes.data_as(_D),
+                                 gs.ctypes.data_as(_D), ctypes.byref(rep))
+        self.adjoint_report = rep
+        self._check(rc)
+        return {"grad_perm": gK, "grad_totmob": gt, "grad_rhomob": gr,
+                "grad_src": gs[0] if self.nrhs == 1 else gs}
+
+    def adjoint_device(self, dJ_dv_ptr, grad_perm_ptr=None, grad_totmob_ptr=None,
+                       grad_rhomob_ptr=None, grad_src_ptr=None, tol=1e-10, **kw):
+        """adjoint() with device pointers for dJ/dv and every output."""
+        o = self.options(tol=tol, nrhs=self.nrhs, **kw)
+        rep = Report()
+        rc = self._L.mhs_adjoint_device(self._h, _P(dJ_dv_ptr), ctypes.byref(o),
+                                        _P(grad_perm_ptr or 0), _P(grad_totmob_ptr or 0),
+                                        _P(grad_rhomob_ptr or 0), _P(grad_src_ptr or 0),
+                                        ctypes.byref(rep))
+        self.adjoint_report = rep
+        self._check(rc)
+        return rep
 
     def postProcessFluxes(self) -> float:
         mm = ctypes.c_double(0.0)
