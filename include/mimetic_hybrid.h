@@ -216,6 +216,15 @@ int mhs_compute_inner_products_device(mhs_solver *s, const double *perm,
  * after the copy on the stream given to mhs_create. */
 int mhs_get_solution_device(mhs_solver *s, double *cell_pressure, double *outflux);
 
+/* Sums of the last outflux over groups of half-faces, in a fixed order on
+ * the device: sums[k*ngroups + g] = sum over e in [group_ptr[g],
+ * group_ptr[g+1]) of outflux(k, group_hf[e]).  No counterpart in the
+ * reference: the boundary-flux integrals of periodic upscaling (effective
+ * permeability) and of well or producer rates, without copying every flux to
+ * the host. */
+int mhs_sum_outflux(mhs_solver *s, int ngroups, const int *group_ptr,
+                    const int *group_hf, double *sums);
+
 /* Adjoint of the last solve.  No counterpart in the reference: this is the
  * differentiated form of solve() (:657-703) for gradient-based calibration.
  * dJ_dv[nrhs*nhf] is the gradient of a scalar objective J with respect to

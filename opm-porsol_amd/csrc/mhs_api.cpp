@@ -920,6 +920,42 @@ int mhs_adjoint_device(mhs_solver *s, const double *dJ_dv, const mhs_solve_optio
     return adjoint_impl(s, dJ_dv, opt, grad_perm, grad_totmob, grad_rhomob, grad_src, report, true);
 }
 
+int mhs_sum_outflux(mhs_solver *s, int ngroups, const int *group_ptr, const int *group_hf, double *sums) {
+    if (!s) return MHS_ERR_INVALID;
+    if (!s->solved) return fail(s, MHS_ERR_STATE, "sum_outflux: no solution (call solve)");
+    if (ngroups < 0 || (ngroups > 0 && (!group_ptr || !group_hf || !sums)))
+        return fail(s, MHS_ERR_INVALID, "sum_outflux: bad arguments");
+    if (ngroups == 0) return MHS_OK;
+    const Topology &T = s->T;
+    if (group_ptr[0] != 0) return fail(s, MHS_ERR_INVALID, "sum_outflux: group_ptr[0] must be 0");
+    for (int g = 0; g < ngroups; ++g)
+        if (group_ptr[g + 1] < group_ptr[g]) return fail(s, MHS_ERR_INVALID, "sum_outflux: group_ptr not monotone");
+    const size_t ne = (size_t)group_ptr[ngroups];
+    for (size_t e = 0; e < ne; ++e)
+        if (group_hf[e] < 0 || group_hf[e] >= T.nhf)
+            return fail(s, MHS_ERR_INVALID, "sum_outflux: half-face index out of range");
+    HIPCHECK(hipSetDevice(s->device));
+    const int nrhs = s->last_nrhs;
+    int *dptr = nullptr, *dhf = nullptr;
+    double *dout = nullptr;
+    hipError_t e = hipMalloc(&dptr, sizeof(int) * (ngroups + 1));
+    if (e == hipSuccess) e = hipMalloc(&dhf, sizeof(int) * std::max<size_t>(ne, 1));
+    if (e == hipSuccess) e = hipMalloc(&dout, sizeof(double) * (size_t)nrhs * ngroups);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dptr, group_ptr, sizeof(int) * (ngroups + 1), hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess && ne > 0)
+        e = hipMemcpyAsync(dhf, group_hf, sizeof(int) * ne, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = launch_group_sum(s->stream, nrhs, ngroups, T.nhf, dptr, dhf, s->outflux, dout);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(sums, dout, sizeof(double) * (size_t)nrhs * ngroups, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(dptr);
+    (void)hipFree(dhf);
+    (void)hipFree(dout);
+    if (e != hipSuccess) return fail(s, MHS_ERR_HIP, std::string("sum_outflux: ") + hipGetErrorString(e));
+    return MHS_OK;
+}
+
 int mhs_get_solution_device(mhs_solver *s, double *cell_pressure, double *outflux) {
     if (!s) return MHS_ERR_INVALID;
     if (!s->solved) return fail(s, MHS_ERR_STATE, "get_solution_device: no solution (call solve)");

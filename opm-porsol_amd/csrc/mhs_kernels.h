@@ -140,6 +140,10 @@ hipError_t launch_amg_start(hipStream_t st, const DevTopo &T, const DevAmg &A, i
                             double *x64, VT *x);
 
 // Recovery and post-processing.
+// out[k*ngroups + g] = sum of v[k*nhf + ghf[e]] over e in [gptr[g], gptr[g+1]).
+hipError_t launch_group_sum(hipStream_t st, int nrhs, int ngroups, int nhf, const int *gptr, const int *ghf,
+                            const double *v, double *out);
+
 // Adjoint of the recovered fluxes (see k_adj_cell / k_adj_grad): the adjoint
 // right-hand side into b (x zeroed), and the gradients after the adjoint solve.
 hipError_t launch_adjoint_rhs(hipStream_t st, const DevTopo &T, int maxn, int nrhs, const double *binv_pk,

@@ -1202,6 +1202,19 @@ __global__ __launch_bounds__(256) void k_post(DevTopo T, int nrhs, const double 
         atomicMax(maxbits, (unsigned long long)__double_as_longlong(d));
 }
 
+// Sums of outflux over groups of half-faces: one wavefront per (group, rhs),
+// strided accumulation then a fixed-order shuffle reduction (deterministic).
+__global__ __launch_bounds__(64) void k_group_sum(int ngroups, int nhf, const int *__restrict__ gptr,
+                                                  const int *__restrict__ ghf, const double *__restrict__ v,
+                                                  double *__restrict__ out) {
+    const int grp = blockIdx.x, k = blockIdx.y;
+    double s = 0.0;
+    for (int e = gptr[grp] + (int)threadIdx.x; e < gptr[grp + 1]; e += 64) s += v[(size_t)k * nhf + ghf[e]];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (threadIdx.x == 0) out[(size_t)k * ngroups + grp] = s;
+}
+
 // ---------------------------------------------------------------------------
 // Adjoint of the recovered fluxes.  Cell c's flux at fixed face pressures is
 //   v_c = B (p 1 - pi_c) + rl gf_c = -S_c pi_c + F g/L + rl gf_c,
@@ -1904,6 +1917,13 @@ hipError_t launch_post_process(hipStream_t st, const DevTopo &T, int nrhs, const
     const long long tot = (long long)nrhs * T.nhf;
     if (tot == 0) return hipSuccess;
     k_post<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(T, nrhs, vin, vout, maxbits);
+    return hipGetLastError();
+}
+
+hipError_t launch_group_sum(hipStream_t st, int nrhs, int ngroups, int nhf, const int *gptr, const int *ghf,
+                            const double *v, double *out) {
+    if (ngroups == 0) return hipSuccess;
+    k_group_sum<<<dim3((unsigned)ngroups, (unsigned)nrhs), 64, 0, st>>>(ngroups, nhf, gptr, ghf, v, out);
     return hipGetLastError();
 }
 

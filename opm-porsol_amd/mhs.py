@@ -31,7 +31,7 @@ EXPORTS = (
     "mhs_assemble_static", "mhs_get_inner_products", "mhs_solve", "mhs_solve_device", "mhs_get_solution",
     "mhs_solution_device", "mhs_post_process_fluxes", "mhs_get_dims", "mhs_get_cell_faces",
     "mhs_get_face_rows", "mhs_get_system", "mhs_time_kernel", "mhs_compute_inner_products_device",
-    "mhs_get_solution_device", "mhs_adjoint", "mhs_adjoint_device",
+    "mhs_get_solution_device", "mhs_adjoint", "mhs_adjoint_device", "mhs_sum_outflux",
 )
 
 
@@ -112,6 +112,7 @@ def lib():
         L.mhs_time_kernel.argtypes = [_P, ctypes.c_int, ctypes.c_int, _D]
         L.mhs_compute_inner_products_device.argtypes = [_P, _P, _D]
         L.mhs_get_solution_device.argtypes = [_P, _P, _P]
+        L.mhs_sum_outflux.argtypes = [_P, ctypes.c_int, _I, _I, _D]
         L.mhs_adjoint.argtypes = [_P, _D, ctypes.POINTER(Options), _D, _D, _D, _D,
                                   ctypes.POINTER(Report)]
         L.mhs_adjoint_device.argtypes = [_P, _P, ctypes.POINTER(Options), _P, _P, _P, _P,
@@ -320,6 +321,15 @@ class HybridSolver:
         """Copy the last cell pressures / outfluxes into caller device buffers."""
         self._check(self._L.mhs_get_solution_device(self._h, _P(cell_pressure_ptr or 0),
                                                     _P(outflux_ptr or 0)))
+
+    def sum_outflux(self, group_ptr, group_hf):
+        """Sums of the last outflux over half-face groups: [nrhs, ngroups]."""
+        gp, pgp = _i(group_ptr)
+        gh, pgh = _i(group_hf)
+        ng = gp.shape[0] - 1
+        out = np.empty((self.nrhs, ng))
+        self._check(self._L.mhs_sum_outflux(self._h, ng, pgp, pgh, out.ctypes.data_as(_D)))
+        return out
 
     # --- adjoint --------------------------------------------------------------------
     def adjoint(self, dJ_dv, tol=1e-10, **kw):
