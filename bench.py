@@ -119,9 +119,256 @@ def _cpu_baseline(pkg, g, perm, bt, bv, iters_needed, sample_iters):
     }
 
 
+
+def _spmv_roofline(s, nnz, nf, nrhs, tag):
+    """Roofline entry of the PCG SpMV of the last solve (HIP-event timing)."""
+    v = 4 if tag == "32" else 8
+    alg = nnz * (v + 4) + 2 * nf * v * nrhs
+    ms = s.time_kernel(0, reps=50)
+    ach = alg / (ms * 1e-3) / 1e9
+    return {"kernel": f"k_spmv_pq<{'float' if tag == '32' else 'double'}, nrhs={nrhs}> (face-matrix "
+                      f"SpMV with fused <p,Ap>)",
+            "bound": "hbm", "achieved": round(ach, 1), "peak": 8000.0, "unit": "GB/s",
+            "frac": round(ach / 8000.0, 4), "traffic": None,
+            "alg_bytes_per_launch": alg, "avg_launch_us": round(ms * 1e3, 2)}
+
+
+def bench_upscale(args, pkg, torch, dev):
+    """256 fully periodic cell problems, n^3 with n uniform on [16, 48], i.i.d.
+    lognormal isotropic K (sigma 1, mean ln 100 mD), three unit pressure drops
+    as three right-hand sides: one step = one batched solve (768 cell-problem
+    solves) plus the effective-permeability reduction."""
+    from opm_porsol_amd import upscaling as U
+    t0 = time.perf_counter()
+    sizes = U.random_sizes(256, 16, 48, seed=args.seed)
+    batch = U.periodic_batch(sizes)
+    perm = U.lognormal_isotropic(sizes, seed=args.seed, sigma=1.0)
+    t_gen = time.perf_counter() - t0
+    g = batch.grid
+    nc, nhf = g.num_cells, g.num_half_faces
+    s = pkg.HybridSolver(0)
+    t0 = time.perf_counter()
+    s.init(g, perm, (0.0, 0.0, 0.0), batch.bc_type, batch.partner, inst_cellpos=batch.inst_cellpos)
+    t_init = time.perf_counter() - t0
+    tm = torch.ones(nc, dtype=torch.float64, device=dev)
+    rm = torch.zeros(nc, dtype=torch.float64, device=dev)
+    src = torch.zeros(3 * nc, dtype=torch.float64, device=dev)
+    bvd = torch.from_numpy(U.pressure_drop_values(batch).reshape(-1)).to(dev)
+    torch.cuda.synchronize()
+
+    def step(precision):
+        rep = s.solve_device(tm.data_ptr(), rm.data_ptr(), src.data_ptr(), batch.bc_type, bvd.data_ptr(),
+                             nrhs=3, tol=TOL, precision=precision, chunk=args.chunk,
+                             use_graph=not args.no_graph)
+        return rep
+
+    cmp = None
+    if not args.no_compare:
+        step("fp64")
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        r64 = step("fp64").as_dict()
+        torch.cuda.synchronize()
+        cmp = (time.perf_counter() - t, r64,
+               U.effective_permeability(s.sum_outflux(batch.group_ptr, batch.group_hf), batch))
+    for _ in range(args.warmup):
+        step(args.precision)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(args.precision)
+        keff = U.effective_permeability(s.sum_outflux(batch.group_ptr, batch.group_hf), batch)
+    torch.cuda.synchronize()
+    ms_step = (time.perf_counter() - t0) / args.steps * 1e3
+    rep = s.report.as_dict()
+    d = s.dims()
+    nnz = int(s.system(0)[0][-1])
+    tag = "32" if args.precision == "mixed" else "64"
+    roof = _spmv_roofline(s, nnz, d.num_faces, 3, tag)
+    mD = 9.869233e-16
+    kd = np.stack([keff[:, i, i] for i in range(3)], 1) / mD
+    config = {
+        "workload": "256 fully periodic cell problems n^3, n~U[16,48], lognormal isotropic K "
+                    "(sigma 1, mean ln 100 mD), 3 unit pressure drops, multigrid-PCG to 1e-10",
+        "instances": batch.ninst, "cells": nc, "faces": d.num_faces, "half_faces": nhf, "nnz": nnz,
+        "precision": args.precision, "iterations": rep["iterations"], "levels": rep["levels"],
+        "fp64_replacements": rep["replacements"], "reduction": rep["reduction"],
+        "init_s": round(t_init, 2), "generate_s": round(t_gen, 2),
+        "keff_mD_diag_mean": [round(float(x), 3) for x in kd.mean(0)],
+        "keff_asymmetry_max": float(np.max(np.abs(keff - keff.transpose(0, 2, 1))) / np.max(np.abs(keff))),
+    }
+    if cmp is not None:
+        t64, r64, k64 = cmp
+        config["fp64_gpu"] = {"cell_solves_per_s": round(768.0 / t64, 2), "iterations": r64["iterations"],
+                              "speedup": round(t64 / (ms_step * 1e-3), 3)}
+        config["keff_err_vs_fp64_gpu"] = float(np.max(np.abs(keff - k64)) / np.max(np.abs(k64)))
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = _cpu_baseline_upscale(pkg, sizes, perm, batch)
+    print(json.dumps({
+        "metric": "periodic upscaling cell-problem solves/sec (256 instances 16^3..48^3 x 3 drops), "
+                  "rel. residual 1e-10",
+        "value": round(768.0 / (ms_step * 1e-3), 3), "unit": "cell-problem solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 PCG / f64 residual, assembly, recovery" if args.precision == "mixed" else "f64",
+        "data": "synthetic (seeded i.i.d. lognormal permeability)", "config": config,
+        "roofline": roof, "cpu_baseline": cpu}))
+
+
+def _cpu_baseline_upscale(pkg, sizes, perm, batch, sample=2):
+    """The oracle (FP64, Jacobi-PCG, one core) on the first `sample` cell
+    problems of the batch, all three drops each."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    t_tot, n_solved, cells = 0.0, 0, 0
+    for t in range(sample):
+        c0, c1 = batch.inst_cellpos[t], batch.inst_cellpos[t + 1]
+        gt = pkg.grids.cartesian(*sizes[t])
+        pp = pkg.grids.periodic_partners_cartesian(gt, (0, 1, 2))
+        bt = np.where(gt.nb < 0, 2, 0).astype(np.int32)
+        n = c1 - c0
+        for j in range(3):
+            bv = np.zeros(gt.num_half_faces)
+            bv[gt.side == 2 * j] = 1.0
+            bv[gt.side == 2 * j + 1] = -1.0
+            t0 = time.perf_counter()
+            oracle.solve(gt, perm[c0:c1], np.ones(n), np.zeros(n), np.zeros(n), bt, bv, partner=pp,
+                         tol=TOL, precond=1)
+            t_tot += time.perf_counter() - t0
+            n_solved += 1
+        cells += n
+    # scale by cells: the batch's mean problem is larger than the sample's
+    mean_cells = batch.grid.num_cells / batch.ninst
+    per_solve = t_tot / n_solved * (mean_cells / (cells / sample))
+    return {"value": 1.0 / per_solve, "unit": "cell-problem solves/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle/oracle.c FP64 Jacobi-PCG on 1 host core: cell problems "
+                       f"{[sizes[t][0] for t in range(sample)]}^3, 3 drops each, {t_tot:.2f} s; "
+                       f"scaled linearly in cells to the batch mean of {mean_cells:.0f} cells "
+                       f"(iteration growth with size not included, so this favours the CPU)")}
+
+
+def bench_adjoint(args, pkg, torch, dev):
+    """Gradients dJ/dln K of 32 realizations of a 64x64x32 grid (lognormal
+    isotropic K, correlation length 6 cells, sigma 1), J = sum over 16
+    producer faces per realization of (flux - target)^2, targets from a seeded
+    reference field; x pressure drop 1 bar.  One step = the batched forward
+    solve plus the adjoint solve and gradient kernels."""
+    dims, R = (64, 64, 32), 32
+    mD, bar = 9.869233e-16, 1.0e5
+    grids_, perms, refs = [], [], []
+    for r in range(R):
+        grids_.append(pkg.grids.cartesian(*dims))
+        lnk = np.log(100 * mD) + pkg.fields.gaussian_filtered(dims, corr_len=6.0, seed=args.seed, stream=r)
+        perms.append(pkg.fields.diag_perm(np.exp(lnk)))
+        lnr = np.log(100 * mD) + pkg.fields.gaussian_filtered(dims, corr_len=6.0, seed=args.seed + 1, stream=r)
+        refs.append(pkg.fields.diag_perm(np.exp(lnr)))
+    G, icp = pkg.grids.concat(grids_)
+    K, Kref = np.concatenate(perms), np.concatenate(refs)
+    nc, nhf = G.num_cells, G.num_half_faces
+    bt = np.zeros(nhf, np.int32)
+    bt[G.side == 0] = 1
+    bt[G.side == 1] = 1
+    bv = np.where(G.side == 0, bar, 0.0)
+    # producers: the 4x4 block of x+ faces at the centre of each realization's x+ side
+    cell = G.hf_cell()
+    loc = cell - icp[np.searchsorted(icp, cell, side="right") - 1]
+    jj, kk = (loc // dims[0]) % dims[1], loc // (dims[0] * dims[1])
+    prod = np.nonzero((G.side == 1) & (np.abs(jj - 31.5) < 2) & (np.abs(kk - 15.5) < 2))[0]
+    lam = 1.0 / 1.0e-3
+    s = pkg.HybridSolver(0)
+    t0 = time.perf_counter()
+    s.init(G, Kref, (0.0, 0.0, 0.0), bt, None, inst_cellpos=icp)
+    t_init = time.perf_counter() - t0
+    tm = np.full(nc, lam)
+    z = np.zeros(nc)
+    s.solve(tm, z, z, bt, bv, residual_tolerance=1e-12, precision="fp64")
+    target = s.getSolution()[2][prod]
+    s.computeInnerProducts(K)
+    tmd = torch.from_numpy(tm).to(dev)
+    zd = torch.zeros(nc, dtype=torch.float64, device=dev)
+    bvd = torch.from_numpy(bv).to(dev)
+    prod_d = torch.from_numpy(prod).to(dev)
+    tgt_d = torch.from_numpy(target).to(dev)
+    vd = torch.empty(nhf, dtype=torch.float64, device=dev)
+    wd = torch.zeros(nhf, dtype=torch.float64, device=dev)
+    gK = torch.empty((nc, 9), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    def step(precision):
+        s.solve_device(tmd.data_ptr(), zd.data_ptr(), zd.data_ptr(), bt, bvd.data_ptr(), nrhs=1, tol=TOL,
+                       precision=precision, chunk=args.chunk, use_graph=not args.no_graph)
+        s.get_solution_device(None, vd.data_ptr())
+        wd.zero_()
+        wd[prod_d] = 2.0 * (vd[prod_d] - tgt_d)
+        torch.cuda.synchronize()
+        s.adjoint_device(wd.data_ptr(), gK.data_ptr(), tol=TOL, chunk=args.chunk,
+                         use_graph=not args.no_graph)
+        return gK
+
+    Kd = torch.from_numpy(K).to(dev)
+    def glnk():
+        return (Kd[:, 0] * (gK[:, 0] + gK[:, 4] + gK[:, 8])).cpu().numpy()
+    cmp = None
+    if not args.no_compare:
+        step("fp64")
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        step("fp64")
+        torch.cuda.synchronize()
+        cmp = (time.perf_counter() - t, glnk())
+    for _ in range(args.warmup):
+        step(args.precision)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(args.precision)
+    torch.cuda.synchronize()
+    ms_step = (time.perf_counter() - t0) / args.steps * 1e3
+    g_mixed = glnk()
+    rep, arep = s.report.as_dict(), s.adjoint_report.as_dict()
+    d = s.dims()
+    nnz = int(s.system(0)[0][-1])
+    roof = _spmv_roofline(s, nnz, d.num_faces, 1, "32" if args.precision == "mixed" else "64")
+    config = {
+        "workload": "32 realizations of 64x64x32, lognormal isotropic K (l=6, sigma 1), J = sum over 16 "
+                    "x+ producer faces per realization of (flux - target)^2, dJ/dln K by the adjoint",
+        "realizations": R, "cells": nc, "producer_faces": int(prod.shape[0]), "precision": args.precision,
+        "forward_iterations": rep["iterations"], "adjoint_iterations": arep["iterations"],
+        "init_s": round(t_init, 2),
+    }
+    if cmp is not None:
+        t64, g64 = cmp
+        config["fp64_gpu"] = {"gradients_per_s": round(R / t64, 3), "speedup": round(t64 / (ms_step * 1e-3), 3)}
+        config["grad_err_vs_fp64_gpu"] = float(np.max(np.abs(g_mixed - g64)) / np.max(np.abs(g64)))
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        n0 = int(icp[1])
+        t0 = time.perf_counter()
+        oracle.solve(grids_[0], K[:n0], tm[:n0], z[:n0], z[:n0], bt[:G.facepos[n0]], bv[:G.facepos[n0]],
+                     tol=TOL, precond=1)
+        t1 = time.perf_counter() - t0
+        cpu = {"value": 1.0 / (2.0 * t1), "unit": "gradients/s", "cores": 1, "kind": "port",
+               "sample": f"oracle/oracle.c FP64 Jacobi-PCG forward solve of one 64x64x32 realization on "
+                         f"1 host core ({t1:.2f} s); a gradient is counted as two such solves "
+                         f"(forward + adjoint), the per-cell derivative work not included"}
+    print(json.dumps({
+        "metric": "adjoint gradients/sec (32 realizations of 64x64x32, dJ/dln K), rel. residual 1e-10",
+        "value": round(R / (ms_step * 1e-3), 3), "unit": "gradients/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 PCG / f64 residual, assembly, recovery, gradient" if args.precision == "mixed" else "f64",
+        "data": "synthetic (seeded Gaussian-filtered lognormal permeability)", "config": config,
+        "roofline": roof, "cpu_baseline": cpu}))
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="solve", choices=["solve", "upscale", "adjoint"],
+                    help="solve: the 128x128x64 headline; upscale: 256 periodic cell problems "
+                         "x 3 pressure drops; adjoint: gradients of 32 realizations of 64x64x32")
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
@@ -146,6 +393,10 @@ def main():
     pkg = __graft_entry__.load_package()
     import torch
     dev = torch.device("cuda:0")
+    if args.workload == "upscale":
+        return bench_upscale(args, pkg, torch, dev)
+    if args.workload == "adjoint":
+        return bench_adjoint(args, pkg, torch, dev)
     g, perm, bt, bv = _problem(pkg, args.seed)
     nc = g.num_cells
 
