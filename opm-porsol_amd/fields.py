@@ -97,3 +97,28 @@ def perm_c2(dims, seed: int = 2, corr_len: float = 8.0) -> np.ndarray:
     z = gaussian_filtered(dims, corr_len, seed)
     kx = np.exp(np.log(100 * MILLIDARCY) + 1.5 * z)
     return diag_perm(kx, kx, 0.1 * kx)
+
+
+def perm_c4(nc: int, seed: int = 4, max_dip_deg: float = 20.0, anisotropy: float = 0.1) -> np.ndarray:
+    """Faulted corner-point case: per cell K = R diag(k, k, anisotropy k) R',
+    ln k ~ N(ln 100 mD, 1.0) i.i.d., R a rotation by a dip uniform on
+    [0, max_dip_deg] about a horizontal axis of uniform azimuth (all seeded).
+    The principal values differ (kz/kx = anisotropy), so the rotated tensors
+    are full."""
+    k = lognormal_iid(nc, np.log(100 * MILLIDARCY), 1.0, seed)
+    dip = np.deg2rad(max_dip_deg) * uniform(seed, 1, nc)
+    az = 2.0 * np.pi * uniform(seed, 2, nc)
+    a = np.stack([np.cos(az), np.sin(az), np.zeros(nc)], axis=1)     # rotation axis
+    c, s = np.cos(dip), np.sin(dip)
+    # Rodrigues: R = c I + s [a]x + (1 - c) a a'
+    ax = np.zeros((nc, 3, 3))
+    ax[:, 0, 2], ax[:, 1, 2] = a[:, 1], -a[:, 0]
+    ax[:, 2, 0], ax[:, 2, 1] = -a[:, 1], a[:, 0]
+    R = (c[:, None, None] * np.eye(3) + s[:, None, None] * ax
+         + (1.0 - c)[:, None, None] * np.einsum("ni,nj->nij", a, a))
+    D = np.zeros((nc, 3, 3))
+    D[:, 0, 0] = D[:, 1, 1] = k
+    D[:, 2, 2] = anisotropy * k
+    K = np.einsum("nij,njk,nlk->nil", R, D, R)
+    K = 0.5 * (K + K.transpose(0, 2, 1))
+    return K.reshape(nc, 9)

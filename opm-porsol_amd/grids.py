@@ -228,3 +228,274 @@ def concat(grids) -> tuple[Grid, np.ndarray]:
              if all(g.ijk is not None for g in grids) else None),
     )
     return u, np.asarray(cell_off, dtype=np.int32)
+
+
+# --- corner-point grids with faults ---------------------------------------------
+_GAUSS = (0.5 - 0.5 / np.sqrt(3.0), 0.5 + 0.5 / np.sqrt(3.0))
+
+
+def _bilinear_weights(xi, eta):
+    """Weights of the corners (0,0), (1,0), (0,1), (1,1) at (xi, eta)."""
+    return np.array([(1 - xi) * (1 - eta), xi * (1 - eta), (1 - xi) * eta, xi * eta])
+
+
+def _overlap_integrals(aL1, aL2, aH1, aH2, bL1, bL2, bH1, bH2):
+    """For two bands {aL(s) <= z <= aH(s)} and {bL(s) <= z <= bH(s)} over
+    s in [0, 1], each bound linear (value 1 at s = 0, value 2 at s = 1), the
+    integrals over their intersection of 1, s and z: (I0, Is, Iz).  The
+    height h(s) = min(aH, bH) - max(aL, bL) is piecewise linear with kinks
+    only where two bounds cross, so Simpson's rule between the sorted
+    crossings is exact."""
+    L1 = np.stack([aL1, bL1, aH1, bH1])
+    L2 = np.stack([aL2, bL2, aH2, bH2])
+    n = aL1.shape[0]
+    pts = [np.zeros(n), np.ones(n)]
+    for p in range(4):
+        for q in range(p + 1, 4):
+            d1, d2 = L1[p] - L1[q], L2[p] - L2[q]
+            den = d1 - d2
+            with np.errstate(divide="ignore", invalid="ignore"):
+                s = np.where(den != 0.0, d1 / np.where(den != 0.0, den, 1.0), 0.0)
+            pts.append(np.where((s > 0.0) & (s < 1.0), s, 0.0))
+    B = np.sort(np.stack(pts, axis=1), axis=1)
+
+    def band(s):
+        lo = np.maximum(aL1 + s * (aL2 - aL1), bL1 + s * (bL2 - bL1))
+        hi = np.minimum(aH1 + s * (aH2 - aH1), bH1 + s * (bH2 - bH1))
+        return hi - lo, 0.5 * (hi * hi - lo * lo)
+
+    I0 = np.zeros(n); Is = np.zeros(n); Iz = np.zeros(n)
+    for t in range(B.shape[1] - 1):
+        a, b = B[:, t], B[:, t + 1]
+        m = 0.5 * (a + b)
+        w = (b - a) / 6.0
+        ha, qa = band(a)
+        hm, qm = band(m)
+        hb, qb = band(b)
+        pos = (hm > 0.0) & (b > a)
+        I0 += np.where(pos, w * (ha + 4 * hm + hb), 0.0)
+        Is += np.where(pos, w * (a * ha + 4 * m * hm + b * hb), 0.0)
+        Iz += np.where(pos, w * (qa + 4 * qm + qb), 0.0)
+    return I0, Is, Iz
+
+
+def corner_point(nx: int, ny: int, nz: int, dx: float = 1.0, dy: float = 1.0, dz: float = 1.0,
+                 perturb: float = 0.3, fault_frac: float = 0.05, fault_shift: float = 0.5,
+                 seed: int = 0) -> Grid:
+    """Faulted corner-point grid on vertical pillars (z is depth).
+
+    Every pillar node depth is k dz + U(-perturb, perturb) dz (seeded); a
+    fraction fault_frac of the columns is shifted down by fault_shift dz.
+    Between columns with different shifts, each cell side is split into the
+    overlaps with the neighbour column's cells (exact polygons on the shared
+    vertical plane) plus the parts that face no cell (no-flow boundary, id 7,
+    side 6), so faulted cells have more than 6 faces.  Top and bottom faces
+    are bilinear surfaces described by their exact vector area (area times
+    unit normal) and the mean of their corner depths; lateral faces are
+    planar and exact; cell volumes and centroids are exact (2x2 Gauss
+    quadrature of the bilinear thickness).
+
+    Local face order per cell: x- side, x+, y-, y+ (sub-faces by neighbour
+    layer, exposed parts first/last), top (z-), bottom (z+).  Boundary ids
+    1..6 by side of the box, 7 for fault-exposed parts.  This is the layout
+    Dune::CpGrid builds from a corner-point description, up to its face
+    order, which is its own (parity unpinned here).
+    """
+    from . import fields
+    P = (nx + 1) * (ny + 1)
+    u = fields.uniform(seed, 11, P * (nz + 1)).reshape(P, nz + 1)
+    Z = (np.arange(nz + 1)[None, :] + perturb * (2.0 * u - 1.0)) * dz
+    fu = fields.uniform(seed, 12, nx * ny).reshape(ny, nx)
+    off = np.where(fu < fault_frac, fault_shift * dz, 0.0)
+    J, I = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")      # [ny, nx]
+
+    def pid(i, j):
+        return i + (nx + 1) * j
+
+    # zc[j, i, c, l]: depth of corner c (0:(i,j) 1:(i+1,j) 2:(i,j+1) 3:(i+1,j+1)) at level l
+    zc = np.stack([Z[pid(I, J)], Z[pid(I + 1, J)], Z[pid(I, J + 1)], Z[pid(I + 1, J + 1)]], axis=2)
+    zc = zc + off[:, :, None, None]
+    zmin, zmax = zc.min() - 10.0 * dz, zc.max() + 10.0 * dz
+    nc = nx * ny * nz
+
+    def cidx(i, j, k):
+        return i + nx * (j + ny * k)
+
+    # --- cells: exact volume and centroid of the column slab --------------------
+    V = np.zeros((ny, nx, nz)); MX = np.zeros_like(V); MY = np.zeros_like(V); MZ = np.zeros_like(V)
+    for xi in _GAUSS:
+        for eta in _GAUSS:
+            wgt = _bilinear_weights(xi, eta)
+            zt = np.einsum("c,jick->jik", wgt, zc[:, :, :, :-1])
+            zb = np.einsum("c,jick->jik", wgt, zc[:, :, :, 1:])
+            h = 0.25 * dx * dy * (zb - zt)
+            V += h
+            MX += h * ((I[:, :, None] + xi) * dx)
+            MY += h * ((J[:, :, None] + eta) * dy)
+            MZ += 0.25 * dx * dy * 0.5 * (zb * zb - zt * zt)
+    # [ny, nx, nz] -> cell order (x fastest, layer slowest)
+    to_cells = lambda a: np.transpose(a, (2, 0, 1)).reshape(-1)
+    cvol = to_cells(V)
+    ccen = np.stack([to_cells(MX), to_cells(MY), to_cells(MZ)], axis=1) / cvol[:, None]
+    if np.any(cvol <= 0.0):
+        raise ValueError("corner_point: non-positive cell volume")
+
+    rec = {k: [] for k in ("cell", "key", "nb", "area", "cen", "nrm", "bid", "side")}
+
+    def add(cell, key, nb, area, cen, nrm, bid, side):
+        n = cell.shape[0]
+        rec["cell"].append(cell.astype(np.int64)); rec["key"].append(np.broadcast_to(key, n).astype(np.int64))
+        rec["nb"].append(np.broadcast_to(nb, n).astype(np.int64)); rec["area"].append(area)
+        rec["cen"].append(cen); rec["nrm"].append(np.broadcast_to(nrm, (n, 3)).astype(np.float64))
+        rec["bid"].append(np.broadcast_to(bid, n).astype(np.int64))
+        rec["side"].append(np.broadcast_to(side, n).astype(np.int64))
+
+    # --- top / bottom faces: bilinear surfaces ----------------------------------
+    for l in range(nz + 1):
+        z = zc[:, :, :, l]                                                 # [ny, nx, 4]
+        gx = dy * 0.5 * ((z[..., 1] - z[..., 0]) + (z[..., 3] - z[..., 2]))   # integral of dz/dx
+        gy = dx * 0.5 * ((z[..., 2] - z[..., 0]) + (z[..., 3] - z[..., 1]))
+        N = np.stack([-gx, -gy, np.full_like(gx, dx * dy)], axis=-1).reshape(-1, 3)   # +z (down)
+        A = np.linalg.norm(N, axis=1)
+        nrm = N / A[:, None]
+        cen = np.stack([((I + 0.5) * dx).reshape(-1), ((J + 0.5) * dy).reshape(-1),
+                        z.mean(axis=-1).reshape(-1)], axis=1)
+        ii, jj = I.reshape(-1), J.reshape(-1)
+        if l > 0:      # bottom of layer l-1
+            up = cidx(ii, jj, l - 1)
+            add(up, 5 * 1000, cidx(ii, jj, l) if l < nz else -1, A, cen, nrm,
+                0 if l < nz else 6, -1 if l < nz else 5)
+        if l < nz:     # top of layer l
+            lo = cidx(ii, jj, l)
+            add(lo, 4 * 1000, cidx(ii, jj, l - 1) if l > 0 else -1, A, cen, -nrm,
+                0 if l > 0 else 5, -1 if l > 0 else 4)
+
+    # --- lateral faces ---------------------------------------------------------
+    def lateral(iA, jA, iB, jB, cA, cB, direction):
+        """Columns A -> B (B may be absent: iB < 0) across the plane through
+        A's corners cA = (p1, p2); B's matching corners cB.  direction 0: x, 1: y."""
+        nrm = np.zeros(3); nrm[direction] = 1.0
+        sideA, sideB = 2 * direction + 1, 2 * direction
+        L = dy if direction == 0 else dx
+        zA1, zA2 = zc[jA, iA, cA[0]], zc[jA, iA, cA[1]]                     # [npair, nz+1]
+        if direction == 0:
+            px = (iA + 1) * dx * np.ones_like(iA, dtype=np.float64); p1y = jA * dy
+        else:
+            py = (jA + 1) * dy * np.ones_like(jA, dtype=np.float64); p1x = iA * dx
+
+        def point(sbar, zbar, sel):
+            if direction == 0:
+                return np.stack([px[sel], p1y[sel] + sbar * dy, zbar], axis=1)
+            return np.stack([p1x[sel] + sbar * dx, py[sel], zbar], axis=1)
+
+        if iB is None:   # domain boundary: whole side, outward normal along +direction
+            for k in range(nz):
+                I0, Is, Iz = _overlap_integrals(zA1[:, k], zA2[:, k], zA1[:, k + 1], zA2[:, k + 1],
+                                                zA1[:, k], zA2[:, k], zA1[:, k + 1], zA2[:, k + 1])
+                sel = np.arange(iA.shape[0])
+                add(cidx(iA, jA, k), sideA * 1000 + k + 1, -1, L * I0, point(Is / I0, Iz / I0, sel),
+                    nrm, sideA + 1, sideA)
+            return
+        zB1, zB2 = zc[jB, iB, cB[0]], zc[jB, iB, cB[1]]
+        same = off[jA, iA] == off[jB, iB]
+        # matching columns: 1:1 faces
+        sel = np.nonzero(same)[0]
+        for k in range(nz):
+            I0, Is, Iz = _overlap_integrals(zA1[sel, k], zA2[sel, k], zA1[sel, k + 1], zA2[sel, k + 1],
+                                            zA1[sel, k], zA2[sel, k], zA1[sel, k + 1], zA2[sel, k + 1])
+            cen = point(Is / I0, Iz / I0, sel)
+            a, b = cidx(iA[sel], jA[sel], k), cidx(iB[sel], jB[sel], k)
+            add(a, sideA * 1000 + k + 1, b, L * I0, cen, nrm, 0, -1)
+            add(b, sideB * 1000 + k + 1, a, L * I0, cen, -nrm, 0, -1)
+        # faulted pairs: every overlap of A's layers with B's layers and the
+        # virtual layers above (-1) and below (nz) each column
+        sel = np.nonzero(~same)[0]
+        if sel.shape[0] == 0:
+            return
+
+        def bounds(z1, z2, k):
+            if k == -1:
+                return np.full(z1.shape[0], zmin), np.full(z1.shape[0], zmin), z1[:, 0], z2[:, 0]
+            if k == nz:
+                return z1[:, nz], z2[:, nz], np.full(z1.shape[0], zmax), np.full(z1.shape[0], zmax)
+            return z1[:, k], z2[:, k], z1[:, k + 1], z2[:, k + 1]
+
+        a1, a2, b1, b2 = zA1[sel], zA2[sel], zB1[sel], zB2[sel]
+        for ka in range(-1, nz + 1):
+            for kb in range(-1, nz + 1):
+                if (ka < 0 or ka >= nz) and (kb < 0 or kb >= nz):
+                    continue
+                if 0 <= ka < nz and 0 <= kb < nz and abs(ka - kb) > 3:
+                    continue
+                aL1, aL2, aH1, aH2 = bounds(a1, a2, ka)
+                bL1, bL2, bH1, bH2 = bounds(b1, b2, kb)
+                I0, Is, Iz = _overlap_integrals(aL1, aL2, aH1, aH2, bL1, bL2, bH1, bH2)
+                keep = I0 > 1e-12 * dz
+                if not np.any(keep):
+                    continue
+                s2 = sel[keep]
+                cen = point(Is[keep] / I0[keep], Iz[keep] / I0[keep], s2)
+                area = L * I0[keep]
+                real_a, real_b = 0 <= ka < nz, 0 <= kb < nz
+                if real_a:
+                    a = cidx(iA[s2], jA[s2], ka)
+                    nbr = cidx(iB[s2], jB[s2], kb) if real_b else -1
+                    add(a, sideA * 1000 + kb + 1, nbr, area, cen, nrm, 0 if real_b else 7,
+                        -1 if real_b else 6)
+                if real_b:
+                    b = cidx(iB[s2], jB[s2], kb)
+                    nbr = cidx(iA[s2], jA[s2], ka) if real_a else -1
+                    add(b, sideB * 1000 + ka + 1, nbr, area, cen, -nrm, 0 if real_a else 7,
+                        -1 if real_a else 6)
+
+    ii, jj = I.reshape(-1), J.reshape(-1)
+    # x direction: A = (i, j), B = (i+1, j); plane through corners 1, 3 of A (0, 2 of B)
+    m = ii < nx - 1
+    lateral(ii[m], jj[m], ii[m] + 1, jj[m], (1, 3), (0, 2), 0)
+    # y direction: A = (i, j), B = (i, j+1); corners 2, 3 of A (0, 1 of B)
+    m = jj < ny - 1
+    lateral(ii[m], jj[m], ii[m], jj[m] + 1, (2, 3), (0, 1), 1)
+    # domain sides: x+ and y+ use the same routine with no neighbour; x- and y-
+    # are the mirrored planes (corners 0, 2 resp. 0, 1 with the normal reversed)
+    m = ii == nx - 1
+    lateral(ii[m], jj[m], None, None, (1, 3), None, 0)
+    m = jj == ny - 1
+    lateral(ii[m], jj[m], None, None, (2, 3), None, 1)
+    for direction, corners in ((0, (0, 2)), (1, (0, 1))):
+        m = (ii == 0) if direction == 0 else (jj == 0)
+        iA, jA = ii[m], jj[m]
+        z1, z2 = zc[jA, iA, corners[0]], zc[jA, iA, corners[1]]
+        nrm = np.zeros(3); nrm[direction] = -1.0
+        side = 2 * direction
+        for k in range(nz):
+            I0, Is, Iz = _overlap_integrals(z1[:, k], z2[:, k], z1[:, k + 1], z2[:, k + 1],
+                                            z1[:, k], z2[:, k], z1[:, k + 1], z2[:, k + 1])
+            sb, zb = Is / I0, Iz / I0
+            if direction == 0:
+                cen = np.stack([np.zeros_like(sb), (jA + sb) * dy, zb], axis=1)
+                L = dy
+            else:
+                cen = np.stack([(iA + sb) * dx, np.zeros_like(sb), zb], axis=1)
+                L = dx
+            add(cidx(iA, jA, k), side * 1000 + k + 1, -1, L * I0, cen, nrm, side + 1, side)
+
+    cell = np.concatenate(rec["cell"]); key = np.concatenate(rec["key"])
+    order = np.lexsort((key, cell))
+    cell = cell[order]
+    nhf_per = np.bincount(cell, minlength=nc)
+    facepos = np.concatenate([[0], np.cumsum(nhf_per)]).astype(np.int32)
+    cat = lambda name: np.concatenate(rec[name])[order]
+    return Grid(
+        facepos=facepos,
+        nb=cat("nb").astype(np.int32),
+        bid=cat("bid").astype(np.int32),
+        cell_volume=cvol,
+        cell_centroid=ccen,
+        face_area=cat("area"),
+        face_centroid=cat("cen"),
+        face_normal=cat("nrm"),
+        side=cat("side").astype(np.int8),
+        dims=None,
+        ijk=np.stack([np.tile(np.arange(nx), ny * nz), np.tile(np.repeat(np.arange(ny), nx), nz),
+                      np.repeat(np.arange(nz), nx * ny)], axis=1).astype(np.int32),
+    )
