@@ -600,6 +600,8 @@ def main():
         "reduction": rep["reduction"], "init_s": round(t_init, 3),
         "hbm_gbps_per_solve": round(gbps, 1), "alg_bytes_per_solve": int(bytes_solve),
         "mass_balance": mass,
+        "launch_shape": {k: rep[k] for k in ("spmv_threads", "spmv_xcd", "sweep_threads", "sweep_xcd",
+                                             "tail_rows")},
     }
     for key, (t, r, sol) in cmp.items():
         p64, pi64, v64 = sol

@@ -84,6 +84,17 @@ typedef struct {
                                   when logical cell indices were given); 0: Jacobi;
                                   1: aggregation multigrid V-cycle */
     double smoother_weight;    /* Jacobi weight of the multigrid smoother */
+    int autotune;              /* 1 (default): on the first solve of a structure (per
+                                  nrhs, precision and preconditioner) time every
+                                  candidate launch shape of the PCG SpMV and of the
+                                  multigrid smoothing sweep and keep the fastest;
+                                  0: fixed shapes (256 threads, XCD-remapped) */
+    int spmv_threads;          /* 0: from autotune; 256, 512 or 1024: force this many
+                                  threads per 1024-row block for the PCG SpMV */
+    int sweep_threads;         /* same for the multigrid smoothing sweep */
+    int tail_rows;             /* -1: from autotune (0 when autotune is 0); >= 0: coarse
+                                  levels whose largest instance has at most this many
+                                  rows run in one workgroup per instance */
 } mhs_solve_options;
 
 typedef struct {
@@ -96,6 +107,11 @@ typedef struct {
     long long mg_rows;         /* rows of the coarse multigrid levels, summed */
     long long mg_nnz;          /* stored entries of the coarse level matrices, summed */
     double assemble_ms, solve_ms, recover_ms;
+    int spmv_threads, spmv_xcd;   /* launch shape of the PCG SpMV (threads per 1024 rows,
+                                     XCD-contiguous row ranges) */
+    int sweep_threads, sweep_xcd; /* same for the multigrid smoothing sweep */
+    int tail_rows;                /* coarse-level size limit of the one-workgroup tail */
+    double tune_ms;               /* time spent autotuning in this call */
 } mhs_solve_report;
 
 typedef struct {

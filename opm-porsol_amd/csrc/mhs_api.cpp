@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <new>
 #include <string>
@@ -34,8 +35,16 @@ struct DevBuf {
 struct GraphCache {
     hipGraphExec_t exec = nullptr;
     int precision = -1, nrhs = 0, chunk = 0, amg = -1, nu = -1;
+    int shape[5] = {-1, -1, -1, -1, -1};
     double tol2 = -1.0, rf2 = -1.0, omega = -1.0, alpha = -1.0;
     const void *key = nullptr;
+};
+
+// Launch shapes of the row-block kernels picked by the autotuner, per
+// (right-hand sides, precision, preconditioner).
+struct Shape {
+    int spmv_tb = 256, spmv_xcd = 1, sweep_tb = 256, sweep_xcd = 1;
+    int tail_rows = 0;
 };
 
 }  // namespace
@@ -98,6 +107,7 @@ struct mhs_solver {
     bool last_amg = false;
     long long solve_count = 0;
     GraphCache graph;
+    std::map<int, Shape> shapes;        // autotuned launch shapes of this structure
     // adjoint work arrays (allocated on the first adjoint call)
     std::vector<DevBuf> adj_bufs;
     double *adj_pi = nullptr, *adj_b = nullptr, *adj_w = nullptr, *adj_y = nullptr, *adj_gK = nullptr,
@@ -147,6 +157,7 @@ void mhs_solver::free_topology() {
     topo_bufs.clear();
     binv = gflux = perm = nullptr;
     dT = DevTopo{};
+    shapes.clear();
     dG = DevGeom{};
     dA = DevAmg{};
     has_amg = false;
@@ -200,6 +211,10 @@ void mhs_default_options(mhs_solve_options *o) {
     o->use_graph = 1;
     o->precond = -1;
     o->smoother_weight = 0.7;
+    o->autotune = 1;
+    o->spmv_threads = 0;
+    o->sweep_threads = 0;
+    o->tail_rows = -1;
 }
 
 int mhs_create(mhs_solver **out, int device, void *stream) {
@@ -340,6 +355,9 @@ int mhs_init_system_structure(mhs_solver *s, const int *hf_bc_type, const int *h
             const AmgLevel &H = s->H.lev[l];
             DevAmgLevel &D = A.lev[l];
             D.n = H.n;
+            A.max_rows[l] = 0;
+            for (size_t t = 0; t + 1 < H.inst_ptr.size(); ++t)
+                A.max_rows[l] = std::max(A.max_rows[l], H.inst_ptr[t + 1] - H.inst_ptr[t]);
             D.nnz = H.row_ptr.empty() ? 0 : H.row_ptr.back();
             D.inst_ptr = dupload(s, L, H.inst_ptr, &he);
             if (he == hipSuccess) D.row_ptr = dupload(s, L, H.row_ptr, &he);
@@ -545,9 +563,11 @@ int run_chunk(mhs_solver *s, int nrhs, int chunk, bool use_graph, const MT *val,
               VT *x, VT *r, VT *p, VT *q, VT *z, bool amg, int precision, double tol2) {
     if (use_graph) {
         GraphCache &G = s->graph;
+        const int shape[5] = {s->dT.spmv_tb, s->dT.spmv_xcd, s->dT.sweep_tb, s->dT.sweep_xcd, s->dT.tail_rows};
         if (!(G.exec && G.precision == precision && G.nrhs == nrhs && G.chunk == chunk &&
               G.key == (const void *)val && G.tol2 == tol2 && G.rf2 == s->S.rf2 && G.amg == int(amg) &&
-              G.omega == s->dA.omega && G.alpha == s->dA.alpha && G.nu == s->dA.nu)) {
+              G.omega == s->dA.omega && G.alpha == s->dA.alpha && G.nu == s->dA.nu &&
+              std::equal(shape, shape + 5, G.shape))) {
             if (G.exec) (void)hipGraphExecDestroy(G.exec);
             G = GraphCache{};
             hipGraph_t graph = nullptr;
@@ -577,6 +597,7 @@ int run_chunk(mhs_solver *s, int nrhs, int chunk, bool use_graph, const MT *val,
             G.omega = s->dA.omega;
             G.alpha = s->dA.alpha;
             G.nu = s->dA.nu;
+            std::copy(shape, shape + 5, G.shape);
         }
         HIPCHECK(hipGraphLaunch(G.exec, s->stream));
     } else {
@@ -593,6 +614,93 @@ int read_state(mhs_solver *s, size_t nsys, bool sc) {
     if (sc)
         HIPCHECK(hipMemcpyAsync(s->h_sc, s->S.sc, nsys * SC_N * sizeof(double), hipMemcpyDeviceToHost, s->stream));
     HIPCHECK(hipStreamSynchronize(s->stream));
+    return MHS_OK;
+}
+
+// Autotuner: time every candidate launch shape of the PCG SpMV and of the
+// level-0 smoothing sweep on the assembled matrix (HIP events on the solver's
+// stream, 2 untimed + 10 timed launches each) and keep the fastest.  Done
+// once per structure and (nrhs, precision, preconditioner); the choice only
+// changes the order of floating-point partial sums.
+int autotune(mhs_solver *s, int nrhs, bool mixed, bool amg, double *tune_ms) {
+    const int key = nrhs * 4 + (mixed ? 2 : 0) + (amg ? 1 : 0);
+    auto it = s->shapes.find(key);
+    *tune_ms = 0.0;
+    if (it == s->shapes.end()) {
+        auto t0 = std::chrono::steady_clock::now();
+        const Topology &T = s->T;
+        const size_t nsys = (size_t)T.ninst * nrhs;
+        std::vector<int> ones(nsys, 1);  // every system active: no workgroup returns early
+        HIPCHECK(hipMemcpyAsync(s->S.act, ones.data(), nsys * sizeof(int), hipMemcpyHostToDevice, s->stream));
+        hipEvent_t e0, e1;
+        HIPCHECK(hipEventCreate(&e0));
+        HIPCHECK(hipEventCreate(&e1));
+        hipError_t err = hipSuccess;
+        auto time_it = [&](auto launch) -> float {
+            for (int w = 0; w < 2 && err == hipSuccess; ++w) err = launch();
+            if (err == hipSuccess) err = hipEventRecord(e0, s->stream);
+            for (int r = 0; r < 10 && err == hipSuccess; ++r) err = launch();
+            if (err == hipSuccess) err = hipEventRecord(e1, s->stream);
+            if (err == hipSuccess) err = hipEventSynchronize(e1);
+            float ms = 0.0f;
+            if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+            return err == hipSuccess ? ms / 10.0f : 1e30f;
+        };
+        Shape best;
+        float bs = 1e30f, bw = 1e30f;
+        for (int tb : {256, 512, 1024})
+            for (int xcd : {1, 0}) {
+                s->dT.spmv_tb = tb;
+                s->dT.spmv_xcd = xcd;
+                const float ms = time_it([&]() {
+                    return mixed ? launch_spmv<float, float>(s->stream, s->dT, nrhs, s->val32, s->p32, s->q32,
+                                                             s->R, s->S)
+                                 : launch_spmv<double, double>(s->stream, s->dT, nrhs, s->val64, s->p64,
+                                                               s->q64, s->R, s->S);
+                });
+                if (ms < bs) { bs = ms; best.spmv_tb = tb; best.spmv_xcd = xcd; }
+            }
+        if (amg)
+            for (int tb : {256, 512, 1024})
+                for (int xcd : {1, 0}) {
+                    s->dT.sweep_tb = tb;
+                    s->dT.sweep_xcd = xcd;
+                    const float ms = time_it([&]() {
+                        return mixed ? launch_sweep0<float>(s->stream, s->dT, nrhs, s->val32, s->dinv32, s->r32,
+                                                            (const float *)s->dA.x0, s->z32, s->dA.omega, s->R)
+                                     : launch_sweep0<double>(s->stream, s->dT, nrhs, s->val64, s->dinv64, s->r64,
+                                                             (const double *)s->dA.x0, s->z64, s->dA.omega,
+                                                             s->R);
+                    });
+                    if (ms < bw) { bw = ms; best.sweep_tb = tb; best.sweep_xcd = xcd; }
+                }
+        if (amg && s->dA.nu == 1) {
+            // levels below this many rows per instance run in the one-workgroup tail
+            s->dT.sweep_tb = best.sweep_tb;
+            s->dT.sweep_xcd = best.sweep_xcd;
+            float bt = 1e30f;
+            for (int rows : {0, 128, 512, 2048, 8192, 32768}) {
+                s->dT.tail_rows = rows;
+                const float ms = time_it([&]() {
+                    return mixed ? launch_vcycle<float>(s->stream, s->dT, s->dA, nrhs, s->val32, s->dinv32, s->r32,
+                                                        s->z32, s->R)
+                                 : launch_vcycle<double>(s->stream, s->dT, s->dA, nrhs, s->val64, s->dinv64,
+                                                         s->r64, s->z64, s->R);
+                });
+                if (ms < bt) { bt = ms; best.tail_rows = rows; }
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (err != hipSuccess) return fail(s, MHS_ERR_HIP, std::string("autotune: ") + hipGetErrorString(err));
+        it = s->shapes.emplace(key, best).first;
+        *tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    s->dT.spmv_tb = it->second.spmv_tb;
+    s->dT.spmv_xcd = it->second.spmv_xcd;
+    s->dT.sweep_tb = it->second.sweep_tb;
+    s->dT.sweep_xcd = it->second.sweep_xcd;
+    s->dT.tail_rows = it->second.tail_rows;
     return MHS_OK;
 }
 
@@ -774,6 +882,30 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
         HIPCHECK(launch_amg_setup(s->stream, s->dT, s->dA, s->val64, mixed));
     }
 
+    // --- launch shapes -------------------------------------------------------
+    double tune_ms = 0.0;
+    if (opt.autotune) {
+        rc = autotune(s, nrhs, mixed, amg, &tune_ms);
+        if (rc) return rc;
+    } else {
+        const Shape d;
+        s->dT.spmv_tb = d.spmv_tb; s->dT.spmv_xcd = d.spmv_xcd;
+        s->dT.sweep_tb = d.sweep_tb; s->dT.sweep_xcd = d.sweep_xcd;
+        s->dT.tail_rows = d.tail_rows;
+    }
+    auto valid_tb = [](int t) { return t == 256 || t == 512 || t == 1024; };
+    if (opt.spmv_threads) {
+        if (!valid_tb(opt.spmv_threads)) return fail(s, MHS_ERR_INVALID, "solve: spmv_threads must be 256, 512 or 1024");
+        s->dT.spmv_tb = opt.spmv_threads;
+        s->dT.spmv_xcd = 1;
+    }
+    if (opt.tail_rows >= 0) s->dT.tail_rows = opt.tail_rows;
+    if (opt.sweep_threads) {
+        if (!valid_tb(opt.sweep_threads)) return fail(s, MHS_ERR_INVALID, "solve: sweep_threads must be 256, 512 or 1024");
+        s->dT.sweep_tb = opt.sweep_threads;
+        s->dT.sweep_xcd = 1;
+    }
+
     // --- linear solve ----------------------------------------------------------
     LinearResult lr;
     rc = linear_solve(s, opt, nrhs, mixed, amg, lr);
@@ -813,6 +945,12 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
         rep->assemble_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
         rep->solve_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
         rep->recover_ms = std::chrono::duration<double, std::milli>(t3 - t2).count();
+        rep->spmv_threads = s->dT.spmv_tb;
+        rep->spmv_xcd = s->dT.spmv_xcd;
+        rep->sweep_threads = s->dT.sweep_tb;
+        rep->sweep_xcd = s->dT.sweep_xcd;
+        rep->tail_rows = s->dT.tail_rows;
+        rep->tune_ms = tune_ms;
     }
     if (!converged)
         return fail(s, MHS_ERR_NOT_CONVERGED,

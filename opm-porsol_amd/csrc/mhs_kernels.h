@@ -23,6 +23,13 @@ struct DevTopo {
     const uint8_t *slotmap;
     const long long *sq_pos, *pk_pos;
     const int *block_seg, *seg_blk, *inst_rowbase;
+    // launch shapes of the row-block kernels, chosen by the autotuner:
+    // workgroup size (256, 512 or 1024 threads per 1024 rows) and whether
+    // workgroups are remapped so each XCD gets a contiguous row range
+    int spmv_tb = 256, spmv_xcd = 1, sweep_tb = 256, sweep_xcd = 1;
+    // multigrid levels whose largest instance has at most tail_rows rows run
+    // in one workgroup-per-instance kernel (0: only the coarsest solve)
+    int tail_rows = 0;
 };
 
 // Per-solve cell data produced by the assembly kernel.
@@ -56,7 +63,7 @@ struct DevScal {
 };
 
 // Multigrid hierarchy on the device (level 0 is the SELL face matrix).
-constexpr int kMaxLevels = 24;
+constexpr int kMaxLevels = 16;   // DevAmg travels as a kernel argument (< 4 KB)
 struct DevAmgLevel {
     int n, nnz;
     const int *inst_ptr;            // [ninst+1] first row of each instance
@@ -77,7 +84,9 @@ struct DevAmg {
     float *cinv32;
     double omega, alpha;            // Jacobi weight, coarse-correction (prolongation) factor
     int nu;                         // smoothing sweeps before and after the coarse correction
+    int max_rows[kMaxLevels];       // rows of the largest instance on each level
 };
+static_assert(sizeof(DevAmg) < 4000, "DevAmg must fit the kernel-argument limit");
 
 hipError_t launch_static_ip(hipStream_t st, const DevTopo &T, const DevGeom &G, int maxn,
                             const double *perm, const double *grav, double *binv_pk,
@@ -124,6 +133,11 @@ hipError_t launch_resnorm(hipStream_t st, const DevTopo &T, int nrhs, const doub
 // and the preconditioned iteration pieces.
 hipError_t launch_amg_setup(hipStream_t st, const DevTopo &T, const DevAmg &A, const double *val64,
                             bool mixed);
+// One level-0 Jacobi sweep with the <b, xo> partials (the last multigrid
+// smoothing sweep alone; the autotuner times it).
+template <typename VT>
+hipError_t launch_sweep0(hipStream_t st, const DevTopo &T, int nrhs, const VT *val0, const VT *dinv0, const VT *b,
+                         const VT *x, VT *xo, double w, DevRed R);
 template <typename VT>
 hipError_t launch_vcycle(hipStream_t st, const DevTopo &T, const DevAmg &A, int nrhs, const VT *val0,
                          const VT *dinv0, const VT *r, VT *z, DevRed R);

@@ -60,7 +60,9 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-template <int NQ>
+// Sum over the NW wavefronts of a workgroup; thread 0 gets the total
+// (fixed order: deterministic for a given workgroup size).
+template <int NQ, int NW = 4>
 __device__ __forceinline__ void block_sum256(double (&v)[NQ], double *lds) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
@@ -71,8 +73,12 @@ __device__ __forceinline__ void block_sum256(double (&v)[NQ], double *lds) {
     __syncthreads();
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int q = 0; q < NQ; ++q)
-            v[q] = (lds[q] + lds[NQ + q]) + (lds[2 * NQ + q] + lds[3 * NQ + q]);
+        for (int q = 0; q < NQ; ++q) {
+            double acc = 0.0;
+#pragma unroll
+            for (int i = 0; i < NW; ++i) acc += lds[i * NQ + q];
+            v[q] = acc;
+        }
     }
     __syncthreads();
 }
@@ -82,10 +88,10 @@ __device__ __forceinline__ void block_sum256(double (&v)[NQ], double *lds) {
 // of each instance in index order.  Visibility across XCDs comes from the
 // kernel boundary; no per-workgroup fences (on CDNA4 an agent-scope fence
 // writes back and invalidates the whole L2 of the XCD).
-template <int NQ>
+template <int NQ, int NW = 4>
 __device__ __forceinline__ void block_partials(double (&v)[NQ], int b, const DevRed &R) {
-    __shared__ double lds[4 * NQ];
-    block_sum256<NQ>(v, lds);
+    __shared__ double lds[NW * NQ];
+    block_sum256<NQ, NW>(v, lds);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < NQ; ++q) R.partials[(size_t)q * R.nblk + b] = v[q];
@@ -93,12 +99,12 @@ __device__ __forceinline__ void block_partials(double (&v)[NQ], int b, const Dev
 }
 
 // Same, reducing only the first M values (the finisher reads no more).
-template <int NQ, int M>
+template <int NQ, int M, int NW = 4>
 __device__ __forceinline__ void block_partials_n(double (&v)[NQ], int b, const DevRed &R) {
     double w[M];
 #pragma unroll
     for (int q = 0; q < M; ++q) w[q] = v[q];
-    block_partials<M>(w, b, R);
+    block_partials<M, NW>(w, b, R);
 }
 
 // One workgroup per instance: tot[q] = sum of the instance's partials, then
@@ -491,11 +497,11 @@ __global__ __launch_bounds__(256) void k_fill(DevTopo T, DevCellSys C,
 // replacement is due (upd != 0) also get r = b - A64 x64 in the same pass:
 // the row's FP64 residual replaces the FP32 recursive one, and the partials
 // carry |r64|^2, r.M^-1 r and |r|^2 of the replaced residual.
-template <typename MT, typename VT, int NRHS, bool REL>
-__global__ __launch_bounds__(256) void k_spmv_pq(DevTopo T, const MT *__restrict__ val,
-                                                 const VT *__restrict__ p, VT *__restrict__ q,
-                                                 DevRed R, DevScal S, RelArgs L) {
-    const int b = xcd_block(gridDim.x);
+template <typename MT, typename VT, int NRHS, bool REL, int TB>
+__global__ __launch_bounds__(TB) void k_spmv_pq(DevTopo T, const MT *__restrict__ val,
+                                                const VT *__restrict__ p, VT *__restrict__ q,
+                                                DevRed R, DevScal S, RelArgs L) {
+    const int b = T.spmv_xcd ? xcd_block(gridDim.x) : (int)blockIdx.x;
     const int s = T.block_seg[b];
     int act[NRHS];
     if (!load_act<NRHS>(S.act, s, act)) return;
@@ -511,8 +517,8 @@ __global__ __launch_bounds__(256) void k_spmv_pq(DevTopo T, const MT *__restrict
 #pragma unroll
     for (int q = 0; q < NQ; ++q) part[q] = 0.0;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int f = b * kRowBlock + j * 256 + threadIdx.x;
+    for (int j = 0; j < kRowBlock / TB; ++j) {
+        const int f = b * kRowBlock + j * TB + threadIdx.x;
         VT acc[NRHS];
         sell_row<MT, VT, VT, NRHS>(T, val, p, f, acc);
 #pragma unroll
@@ -540,10 +546,10 @@ __global__ __launch_bounds__(256) void k_spmv_pq(DevTopo T, const MT *__restrict
         }
     }
     if constexpr (REL) {
-        if (rel) block_partials<NQ>(part, b, R);
-        else block_partials_n<NQ, NRHS>(part, b, R);
+        if (rel) block_partials<NQ, TB / 64>(part, b, R);
+        else block_partials_n<NQ, NRHS, TB / 64>(part, b, R);
     } else {
-        block_partials<NQ>(part, b, R);
+        block_partials<NQ, TB / 64>(part, b, R);
     }
 }
 
@@ -844,23 +850,123 @@ __device__ __forceinline__ const VT *lvl_dinv(const DevAmgLevel &L) {
     if constexpr (std::is_same<VT, float>::value) return L.dinv32; else return L.dinv64;
 }
 
+// Coarse-level rows with batched loads: the column indices of kChunk entries
+// are loaded together, then everything that depends on them, so a row costs a
+// few memory round trips instead of one dependent chain per entry (the small
+// levels are latency-bound).  Padding lanes use the row itself with a zero
+// coefficient, so every address stays valid.
+constexpr int kChunk = 8;
+
+// acc = sum_j a_ij (w d_j b_j): the residual of the pre-smoothed x = w D^-1 b.
+template <typename VT, int NRHS>
+__device__ __forceinline__ void row_down(const DevAmgLevel &F, const VT *__restrict__ val,
+                                         const VT *__restrict__ dinv, const VT *__restrict__ b, VT w, int i,
+                                         VT (&acc)[NRHS]) {
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) acc[k] = VT(0);
+    const int e1 = F.row_ptr[i + 1];
+    for (int base = F.row_ptr[i]; base < e1; base += kChunk) {
+        int c[kChunk];
+        VT a[kChunk];
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+            const bool in = base + u < e1;
+            c[u] = in ? F.col[base + u] : i;
+            a[u] = in ? val[base + u] : VT(0);
+        }
+        VT d[kChunk], bv[kChunk][NRHS];
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+            d[u] = dinv[c[u]];
+#pragma unroll
+            for (int k = 0; k < NRHS; ++k) bv[u][k] = b[(size_t)c[u] * NRHS + k];
+        }
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u)
+#pragma unroll
+            for (int k = 0; k < NRHS; ++k) acc[k] += a[u] * (w * d[u]) * bv[u][k];
+    }
+}
+
+// acc = sum_j a_ij x~_j with x~_j = w d_j b_j + alpha xc[agg_j] (prolongation on the fly).
+template <typename VT, int NRHS>
+__device__ __forceinline__ void row_up1(const DevAmgLevel &F, const VT *__restrict__ val,
+                                        const VT *__restrict__ dinv, const VT *__restrict__ b,
+                                        const int *__restrict__ agg, const VT *__restrict__ xc, VT w, VT al,
+                                        int i, VT (&acc)[NRHS]) {
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) acc[k] = VT(0);
+    const int e1 = F.row_ptr[i + 1];
+    for (int base = F.row_ptr[i]; base < e1; base += kChunk) {
+        int c[kChunk];
+        VT a[kChunk];
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+            const bool in = base + u < e1;
+            c[u] = in ? F.col[base + u] : i;
+            a[u] = in ? val[base + u] : VT(0);
+        }
+        VT d[kChunk], bv[kChunk][NRHS];
+        int g[kChunk];
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+            d[u] = dinv[c[u]];
+            g[u] = agg[c[u]];
+#pragma unroll
+            for (int k = 0; k < NRHS; ++k) bv[u][k] = b[(size_t)c[u] * NRHS + k];
+        }
+        VT xg[kChunk][NRHS];
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u)
+#pragma unroll
+            for (int k = 0; k < NRHS; ++k) xg[u][k] = xc[(size_t)g[u] * NRHS + k];
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u)
+#pragma unroll
+            for (int k = 0; k < NRHS; ++k) acc[k] += a[u] * (w * d[u] * bv[u][k] + al * xg[u][k]);
+    }
+}
+
+// acc = sum over the children of coarse row I of res.
+template <typename VT, int NRHS>
+__device__ __forceinline__ void row_restrict(const int *__restrict__ child_ptr, const int *__restrict__ child,
+                                             const VT *__restrict__ res, int I, VT (&acc)[NRHS]) {
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) acc[k] = VT(0);
+    const int e1 = child_ptr[I + 1];
+    for (int base = child_ptr[I]; base < e1; base += kChunk) {
+        int c[kChunk];
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) c[u] = base + u < e1 ? child[base + u] : -1;
+        VT r[kChunk][NRHS];
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u)
+#pragma unroll
+            for (int k = 0; k < NRHS; ++k) r[u][k] = c[u] >= 0 ? res[(size_t)c[u] * NRHS + k] : VT(0);
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u)
+#pragma unroll
+            for (int k = 0; k < NRHS; ++k) acc[k] += r[u][k];
+    }
+}
+
 // Level 0, one more Jacobi sweep: xo = x + w D^-1 (b - A x~), x~ = x + alpha P xc
 // (xc == nullptr: no coarse correction).  With z/partials: the final sweep of
 // the V-cycle writes the preconditioned residual and <r, z> partials.
-template <typename VT, int NRHS, bool LAST>
-__global__ __launch_bounds__(256) void k_vc0_up(DevTopo T, const VT *__restrict__ val,
-                                                const VT *__restrict__ dinv,
-                                                const int *__restrict__ agg,
-                                                const VT *__restrict__ b, const VT *__restrict__ x,
-                                                const VT *__restrict__ xc, VT *__restrict__ xo,
-                                                VT w, VT alpha, DevRed R) {
-    const int blk = xcd_block(gridDim.x);
+template <typename VT, int NRHS, bool LAST, int TB>
+__global__ __launch_bounds__(TB) void k_vc0_up(DevTopo T, const VT *__restrict__ val,
+                                               const VT *__restrict__ dinv,
+                                               const int *__restrict__ agg,
+                                               const VT *__restrict__ b, const VT *__restrict__ x,
+                                               const VT *__restrict__ xc, VT *__restrict__ xo,
+                                               VT w, VT alpha, DevRed R) {
+    const int blk = T.sweep_xcd ? xcd_block(gridDim.x) : (int)blockIdx.x;
     double part[NRHS];
 #pragma unroll
     for (int k = 0; k < NRHS; ++k) part[k] = 0.0;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int f = blk * kRowBlock + j * 256 + threadIdx.x;
+    for (int j = 0; j < kRowBlock / TB; ++j) {
+        const int f = blk * kRowBlock + j * TB + threadIdx.x;
         const int slice = f >> 6, lane = f & 63;
         const long long s0 = T.slice_ptr[slice];
         const int wd = (int)((T.slice_ptr[slice + 1] - s0) >> 6);
@@ -895,7 +1001,7 @@ __global__ __launch_bounds__(256) void k_vc0_up(DevTopo T, const VT *__restrict_
             if (LAST) part[k] += (double)b[i] * (double)out;
         }
     }
-    if (LAST) block_partials<NRHS>(part, blk, R);
+    if (LAST) block_partials<NRHS, TB / 64>(part, blk, R);
 }
 
 // Level 0: res = b - A x (after extra down sweeps).
@@ -916,17 +1022,9 @@ __global__ __launch_bounds__(256) void k_vcc_down(DevAmgLevel L, const VT *__res
                                                   VT *__restrict__ x, VT *__restrict__ res, VT w) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= L.n) return;
-    const VT *val = lvl_val<VT>(L);
     const VT *dinv = lvl_dinv<VT>(L);
     VT acc[NRHS];
-#pragma unroll
-    for (int k = 0; k < NRHS; ++k) acc[k] = VT(0);
-    for (int e = L.row_ptr[i]; e < L.row_ptr[i + 1]; ++e) {
-        const int c = L.col[e];
-        const VT a = val[e] * (w * dinv[c]);
-#pragma unroll
-        for (int k = 0; k < NRHS; ++k) acc[k] += a * b[(size_t)c * NRHS + k];
-    }
+    row_down<VT, NRHS>(L, lvl_val<VT>(L), dinv, b, w, i, acc);
     const VT d = w * dinv[i];
 #pragma unroll
     for (int k = 0; k < NRHS; ++k) {
@@ -1020,20 +1118,10 @@ __global__ __launch_bounds__(256) void k_vcc_up1(DevAmgLevel L, const int *__res
                                                  VT alpha) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= L.n) return;
-    const VT *val = lvl_val<VT>(L);
     const VT *dinv = lvl_dinv<VT>(L);
     const VT *b = (const VT *)L.b;
     VT acc[NRHS];
-#pragma unroll
-    for (int k = 0; k < NRHS; ++k) acc[k] = VT(0);
-    for (int e = L.row_ptr[i]; e < L.row_ptr[i + 1]; ++e) {
-        const int c = L.col[e];
-        const VT dc = w * dinv[c];
-        const int gc = agg[c];
-#pragma unroll
-        for (int k = 0; k < NRHS; ++k)
-            acc[k] += val[e] * (dc * b[(size_t)c * NRHS + k] + alpha * xc[(size_t)gc * NRHS + k]);
-    }
+    row_up1<VT, NRHS>(L, lvl_val<VT>(L), dinv, b, agg, xc, w, alpha, i, acc);
     const VT d = w * dinv[i];
     const int g = agg[i];
 #pragma unroll
@@ -1052,13 +1140,7 @@ __global__ __launch_bounds__(256) void k_restrict(int n, const int *__restrict__
     const int I = blockIdx.x * 256 + threadIdx.x;
     if (I >= n) return;
     VT acc[NRHS];
-#pragma unroll
-    for (int k = 0; k < NRHS; ++k) acc[k] = VT(0);
-    for (int e = child_ptr[I]; e < child_ptr[I + 1]; ++e) {
-        const int c = child[e];
-#pragma unroll
-        for (int k = 0; k < NRHS; ++k) acc[k] += res[(size_t)c * NRHS + k];
-    }
+    row_restrict<VT, NRHS>(child_ptr, child, res, I, acc);
 #pragma unroll
     for (int k = 0; k < NRHS; ++k) bc[(size_t)I * NRHS + k] = acc[k];
 }
@@ -1077,6 +1159,79 @@ __global__ __launch_bounds__(256) void k_coarse_solve(const int *__restrict__ in
         VT s = VT(0);
         for (int j = 0; j < m; ++j) s += A[i * m + j] * b[(size_t)(r0 + j) * NRHS + k];
         x[(size_t)(r0 + i) * NRHS + k] = s;
+    }
+}
+
+// The small levels lt..L-1 of one V(1,1) cycle (one sweep each way), one
+// workgroup per instance: the same arithmetic as k_vcc_down + k_restrict,
+// k_coarse_solve and k_vcc_up1, with workgroup barriers between the steps
+// instead of kernel launches.  Reads lev[lt].b, writes lev[l].x for l >= lt.
+template <typename VT, int NRHS>
+__global__ __launch_bounds__(1024) void k_vcycle_tail(DevAmg A, int lt, VT w, VT al) {
+    const int s = blockIdx.x;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int L = A.nlev;
+    for (int l = lt; l < L - 1; ++l) {
+        const DevAmgLevel &F = A.lev[l];
+        const DevAmgLevel &C = A.lev[l + 1];
+        const VT *val = lvl_val<VT>(F);
+        const VT *dinv = lvl_dinv<VT>(F);
+        const VT *b = (const VT *)F.b;
+        VT *res = (VT *)F.res;
+        const int r0 = F.inst_ptr[s], r1 = F.inst_ptr[s + 1];
+        for (int i = r0 + tid; i < r1; i += nt) {
+            VT acc[NRHS];
+            row_down<VT, NRHS>(F, val, dinv, b, w, i, acc);
+#pragma unroll
+            for (int k = 0; k < NRHS; ++k) res[(size_t)i * NRHS + k] = b[(size_t)i * NRHS + k] - acc[k];
+        }
+        __syncthreads();
+        VT *bc = (VT *)C.b;
+        for (int I = C.inst_ptr[s] + tid; I < C.inst_ptr[s + 1]; I += nt) {
+            VT acc[NRHS];
+            row_restrict<VT, NRHS>(C.child_ptr, C.child, res, I, acc);
+#pragma unroll
+            for (int k = 0; k < NRHS; ++k) bc[(size_t)I * NRHS + k] = acc[k];
+        }
+        __syncthreads();
+    }
+    {
+        const DevAmgLevel &C = A.lev[L - 1];
+        const int r0 = C.inst_ptr[s], m = C.inst_ptr[s + 1] - r0;
+        const VT *inv;
+        if constexpr (std::is_same<VT, float>::value) inv = A.cinv32 + A.coarse_off[s];
+        else inv = A.cinv64 + A.coarse_off[s];
+        const VT *b = (const VT *)C.b;
+        VT *x = (VT *)C.x;
+        for (int e = tid; e < m * NRHS; e += nt) {
+            const int i = e / NRHS, k = e % NRHS;
+            VT acc = VT(0);
+            for (int j = 0; j < m; ++j) acc += inv[i * m + j] * b[(size_t)(r0 + j) * NRHS + k];
+            x[(size_t)(r0 + i) * NRHS + k] = acc;
+        }
+        __syncthreads();
+    }
+    for (int l = L - 2; l >= lt; --l) {
+        const DevAmgLevel &F = A.lev[l];
+        const int *agg = A.lev[l + 1].agg;
+        const VT *xc = (const VT *)A.lev[l + 1].x;
+        const VT *val = lvl_val<VT>(F);
+        const VT *dinv = lvl_dinv<VT>(F);
+        const VT *b = (const VT *)F.b;
+        VT *xo = (VT *)F.x;
+        for (int i = F.inst_ptr[s] + tid; i < F.inst_ptr[s + 1]; i += nt) {
+            VT acc[NRHS];
+            row_up1<VT, NRHS>(F, val, dinv, b, agg, xc, w, al, i, acc);
+            const VT d = w * dinv[i];
+            const int g = agg[i];
+#pragma unroll
+            for (int k = 0; k < NRHS; ++k) {
+                const size_t q = (size_t)i * NRHS + k;
+                const VT xv = d * b[q] + al * xc[(size_t)g * NRHS + k];
+                xo[q] = xv + d * (b[q] - acc[k]);
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -1620,6 +1775,17 @@ static hipError_t with_maxn(int maxn, Fn fn) {
 
 static inline unsigned cell_blocks(int nc) { return (unsigned)((nc + 63) / 64); }
 
+// Workgroup size of the row-block kernels (each workgroup owns 1024 rows).
+template <class Fn>
+static hipError_t with_tb(int tb, Fn fn) {
+    switch (tb) {
+        case 512: fn(IC<512>{}); break;
+        case 1024: fn(IC<1024>{}); break;
+        default: fn(IC<256>{}); break;     // 256 and anything unset
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_static_ip(hipStream_t st, const DevTopo &T, const DevGeom &G, int maxn,
                             const double *perm, const double *grav, double *binv_pk,
                             double *gflux) {
@@ -1670,7 +1836,10 @@ hipError_t launch_pcg_iteration(hipStream_t st, const DevTopo &T, int nrhs, cons
         constexpr bool REL = std::is_same<VT, float>::value;
         RelArgs L{};
         if constexpr (REL) L = RelArgs{val64, x64, b, dinv, r};
-        k_spmv_pq<MT, VT, NR, REL><<<T.nblk, 256, 0, st>>>(T, val, p, q, R, S, L);
+        with_tb(T.spmv_tb, [&](auto tb) {
+            constexpr int TBv = decltype(tb)::value;
+            k_spmv_pq<MT, VT, NR, REL, TBv><<<T.nblk, TBv, 0, st>>>(T, val, p, q, R, S, L);
+        });
         k_finish<REL ? 4 * NR : NR, FinPQ<NR, REL>><<<T.ninst, 256, 0, st>>>(R, S, A);
         k_update_xr<VT, NR><<<T.nblk, 256, 0, st>>>(T, dinv, x, r, p, q, R, S);
         k_finish<2 * NR, FinXR<NR>><<<T.ninst, 256, 0, st>>>(R, S, A);
@@ -1695,7 +1864,10 @@ hipError_t launch_spmv(hipStream_t st, const DevTopo &T, int nrhs, const MT *val
                        VT *q, DevRed R, DevScal S) {
     return with_nrhs(nrhs, [&](auto nr) {
         constexpr int NR = decltype(nr)::value;
-        k_spmv_pq<MT, VT, NR, false><<<T.nblk, 256, 0, st>>>(T, val, p, q, R, S, RelArgs{});
+        with_tb(T.spmv_tb, [&](auto tb) {
+            constexpr int TBv = decltype(tb)::value;
+            k_spmv_pq<MT, VT, NR, false, TBv><<<T.nblk, TBv, 0, st>>>(T, val, p, q, R, S, RelArgs{});
+        });
     });
 }
 template hipError_t launch_spmv<double, double>(hipStream_t, const DevTopo &, int, const double *,
@@ -1758,6 +1930,18 @@ hipError_t launch_amg_setup(hipStream_t st, const DevTopo &T, const DevAmg &A, c
     return hipGetLastError();
 }
 
+// Level-0 Jacobi sweep xo = x + w D^-1 (b - A x) in the tuned workgroup shape;
+// LAST also forms the <b, xo> partials.
+template <typename VT, int NR, bool LAST>
+static void sweep0(hipStream_t st, const DevTopo &T, const VT *val0, const VT *dinv0, const VT *b, const VT *x,
+                   VT *xo, VT w, DevRed R) {
+    with_tb(T.sweep_tb, [&](auto tb) {
+        constexpr int TBv = decltype(tb)::value;
+        k_vc0_up<VT, NR, LAST, TBv><<<T.nblk, TBv, 0, st>>>(T, val0, dinv0, nullptr, b, x, nullptr, xo, w,
+                                                            VT(0), R);
+    });
+}
+
 template <typename VT, int NR>
 static void vcycle_impl(hipStream_t st, const DevTopo &T, const DevAmg &A, const VT *val0,
                         const VT *dinv0, const VT *r, VT *z, DevRed R, bool x0_ready) {
@@ -1769,8 +1953,7 @@ static void vcycle_impl(hipStream_t st, const DevTopo &T, const DevAmg &A, const
     // ---- level 0 down: x0 = w D^-1 r (+ nu-1 sweeps), res0 = r - A x0
     if (!x0_ready) k_scale0<VT, NR><<<rows_blocks, 256, 0, st>>>(dinv0, r, x0, w);
     for (int it = 1; it < nu; ++it) {
-        k_vc0_up<VT, NR, false><<<T.nblk, 256, 0, st>>>(T, val0, dinv0, nullptr, r, x0, nullptr, xt0, w,
-                                                        VT(0), R);
+        sweep0<VT, NR, false>(st, T, val0, dinv0, r, x0, xt0, w, R);
         std::swap(x0, xt0);
     }
     k_vc0_res<VT, NR><<<rows_blocks, 256, 0, st>>>(T, val0, r, x0, res0);
@@ -1783,56 +1966,88 @@ static void vcycle_impl(hipStream_t st, const DevTopo &T, const DevAmg &A, const
         xcur[l] = (VT *)A.lev[l].x;
         xalt[l] = (VT *)A.lev[l].xt;
     }
-    // ---- coarse levels down
-    for (int l = 1; l < L - 1; ++l) {
-        const DevAmgLevel &F = A.lev[l];
-        const DevAmgLevel &C = A.lev[l + 1];
-        const unsigned nb = (F.n + 255) / 256;
-        k_vcc_down<VT, NR><<<nb, 256, 0, st>>>(F, (const VT *)F.b, xcur[l], (VT *)F.res, w);
-        for (int it = 1; it < nu; ++it) {
-            k_vcc_up<VT, NR><<<nb, 256, 0, st>>>(F, nullptr, (const VT *)F.b, xcur[l], nullptr, xalt[l], w,
-                                                 VT(0));
-            std::swap(xcur[l], xalt[l]);
+    if (nu == 1) {
+        // first level handled by the workgroup-per-instance tail kernel
+        int lt = L - 1;
+        while (lt > 1 && A.max_rows[lt - 1] <= T.tail_rows) --lt;
+        for (int l = 1; l < lt; ++l) {
+            const DevAmgLevel &F = A.lev[l];
+            const DevAmgLevel &C = A.lev[l + 1];
+            k_vcc_down<VT, NR><<<(F.n + 255) / 256, 256, 0, st>>>(F, (const VT *)F.b, (VT *)F.x, (VT *)F.res, w);
+            k_restrict<VT, NR><<<(C.n + 255) / 256, 256, 0, st>>>(C.n, C.child_ptr, C.child, (const VT *)F.res,
+                                                                  (VT *)C.b);
         }
-        if (nu > 1) k_vcc_res<VT, NR><<<nb, 256, 0, st>>>(F, (const VT *)F.b, xcur[l], (VT *)F.res);
-        k_restrict<VT, NR><<<(C.n + 255) / 256, 256, 0, st>>>(C.n, C.child_ptr, C.child, (const VT *)F.res,
-                                                              (VT *)C.b);
-    }
-    {
-        const DevAmgLevel &C = A.lev[L - 1];
-        const VT *inv;
-        if constexpr (std::is_same<VT, float>::value) inv = A.cinv32; else inv = A.cinv64;
-        k_coarse_solve<VT, NR><<<A.ninst, 256, 0, st>>>(C.inst_ptr, A.coarse_off, inv, (const VT *)C.b,
-                                                        xcur[L - 1]);
-    }
-    // ---- coarse levels up
-    for (int l = L - 2; l >= 1; --l) {
-        const DevAmgLevel &F = A.lev[l];
-        const unsigned nb = (F.n + 255) / 256;
-        if (nu == 1) {
-            k_vcc_up1<VT, NR><<<nb, 256, 0, st>>>(F, A.lev[l + 1].agg, xcur[l + 1], xalt[l], w, al);
-            std::swap(xcur[l], xalt[l]);
-            continue;
+        k_vcycle_tail<VT, NR><<<A.ninst, 1024, 0, st>>>(A, lt, w, al);
+        for (int l = lt - 1; l >= 1; --l) {
+            const DevAmgLevel &F = A.lev[l];
+            k_vcc_up1<VT, NR><<<(F.n + 255) / 256, 256, 0, st>>>(F, A.lev[l + 1].agg, (const VT *)A.lev[l + 1].x,
+                                                                 (VT *)F.x, w, al);
         }
-        k_vcc_up<VT, NR><<<nb, 256, 0, st>>>(F, A.lev[l + 1].agg, (const VT *)F.b, xcur[l], xcur[l + 1],
-                                             xalt[l], w, al);
-        std::swap(xcur[l], xalt[l]);
-        for (int it = 1; it < nu; ++it) {
-            k_vcc_up<VT, NR><<<nb, 256, 0, st>>>(F, nullptr, (const VT *)F.b, xcur[l], nullptr, xalt[l], w,
-                                                 VT(0));
+        xcur[1] = (VT *)A.lev[1].x;
+    } else {
+        // ---- coarse levels down
+        for (int l = 1; l < L - 1; ++l) {
+            const DevAmgLevel &F = A.lev[l];
+            const DevAmgLevel &C = A.lev[l + 1];
+            const unsigned nb = (F.n + 255) / 256;
+            k_vcc_down<VT, NR><<<nb, 256, 0, st>>>(F, (const VT *)F.b, xcur[l], (VT *)F.res, w);
+            for (int it = 1; it < nu; ++it) {
+                k_vcc_up<VT, NR><<<nb, 256, 0, st>>>(F, nullptr, (const VT *)F.b, xcur[l], nullptr, xalt[l], w,
+                                                     VT(0));
+                std::swap(xcur[l], xalt[l]);
+            }
+            if (nu > 1) k_vcc_res<VT, NR><<<nb, 256, 0, st>>>(F, (const VT *)F.b, xcur[l], (VT *)F.res);
+            k_restrict<VT, NR><<<(C.n + 255) / 256, 256, 0, st>>>(C.n, C.child_ptr, C.child, (const VT *)F.res,
+                                                                  (VT *)C.b);
+        }
+        {
+            const DevAmgLevel &C = A.lev[L - 1];
+            const VT *inv;
+            if constexpr (std::is_same<VT, float>::value) inv = A.cinv32; else inv = A.cinv64;
+            k_coarse_solve<VT, NR><<<A.ninst, 256, 0, st>>>(C.inst_ptr, A.coarse_off, inv, (const VT *)C.b,
+                                                            xcur[L - 1]);
+        }
+        // ---- coarse levels up
+        for (int l = L - 2; l >= 1; --l) {
+            const DevAmgLevel &F = A.lev[l];
+            const unsigned nb = (F.n + 255) / 256;
+            if (nu == 1) {
+                k_vcc_up1<VT, NR><<<nb, 256, 0, st>>>(F, A.lev[l + 1].agg, xcur[l + 1], xalt[l], w, al);
+                std::swap(xcur[l], xalt[l]);
+                continue;
+            }
+            k_vcc_up<VT, NR><<<nb, 256, 0, st>>>(F, A.lev[l + 1].agg, (const VT *)F.b, xcur[l], xcur[l + 1],
+                                                 xalt[l], w, al);
             std::swap(xcur[l], xalt[l]);
+            for (int it = 1; it < nu; ++it) {
+                k_vcc_up<VT, NR><<<nb, 256, 0, st>>>(F, nullptr, (const VT *)F.b, xcur[l], nullptr, xalt[l], w,
+                                                     VT(0));
+                std::swap(xcur[l], xalt[l]);
+            }
         }
     }
     // ---- level 0 up: x~ = x0 + alpha P x1, then nu sweeps; the last writes z and <r, z>
     k_prolong0<VT, NR><<<rows_blocks, 256, 0, st>>>(A.lev[1].agg, x0, xcur[1], xt0, al);
     std::swap(x0, xt0);
     for (int it = 1; it < nu; ++it) {
-        k_vc0_up<VT, NR, false><<<T.nblk, 256, 0, st>>>(T, val0, dinv0, nullptr, r, x0, nullptr, xt0, w,
-                                                        VT(0), R);
+        sweep0<VT, NR, false>(st, T, val0, dinv0, r, x0, xt0, w, R);
         std::swap(x0, xt0);
     }
-    k_vc0_up<VT, NR, true><<<T.nblk, 256, 0, st>>>(T, val0, dinv0, nullptr, r, x0, nullptr, z, w, VT(0), R);
+    sweep0<VT, NR, true>(st, T, val0, dinv0, r, x0, z, w, R);
 }
+
+template <typename VT>
+hipError_t launch_sweep0(hipStream_t st, const DevTopo &T, int nrhs, const VT *val0, const VT *dinv0, const VT *b,
+                         const VT *x, VT *xo, double w, DevRed R) {
+    return with_nrhs(nrhs, [&](auto nr) {
+        constexpr int NR = decltype(nr)::value;
+        sweep0<VT, NR, true>(st, T, val0, dinv0, b, x, xo, (VT)w, R);
+    });
+}
+template hipError_t launch_sweep0<double>(hipStream_t, const DevTopo &, int, const double *, const double *,
+                                          const double *, const double *, double *, double, DevRed);
+template hipError_t launch_sweep0<float>(hipStream_t, const DevTopo &, int, const float *, const float *,
+                                         const float *, const float *, float *, double, DevRed);
 
 template <typename VT>
 hipError_t launch_vcycle(hipStream_t st, const DevTopo &T, const DevAmg &A, int nrhs, const VT *val0,
@@ -1858,7 +2073,10 @@ hipError_t launch_pcg_iteration_amg(hipStream_t st, const DevTopo &T, const DevA
         constexpr bool REL = std::is_same<VT, float>::value;
         RelArgs L{};
         if constexpr (REL) L = RelArgs{val64, x64, b, dinv, r};
-        k_spmv_pq<MT, VT, NR, REL><<<T.nblk, 256, 0, st>>>(T, val, p, q, R, S, L);
+        with_tb(T.spmv_tb, [&](auto tb) {
+            constexpr int TBv = decltype(tb)::value;
+            k_spmv_pq<MT, VT, NR, REL, TBv><<<T.nblk, TBv, 0, st>>>(T, val, p, q, R, S, L);
+        });
         k_finish<REL ? 4 * NR : NR, FinPQ<NR, REL>><<<T.ninst, 256, 0, st>>>(R, S, Af);
         k_update_xr<VT, NR><<<T.nblk, 256, 0, st>>>(T, dinv, x, r, p, q, R, S, (VT *)A.x0, (VT)A.omega);
         k_finish<2 * NR, FinXR<NR>><<<T.ninst, 256, 0, st>>>(R, S, Af);

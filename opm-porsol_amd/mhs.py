@@ -42,7 +42,9 @@ class Options(ctypes.Structure):
                 ("smooth_steps", ctypes.c_int), ("precision", ctypes.c_int),
                 ("nrhs", ctypes.c_int), ("chunk", ctypes.c_int),
                 ("replace_factor", ctypes.c_double), ("use_graph", ctypes.c_int),
-                ("precond", ctypes.c_int), ("smoother_weight", ctypes.c_double)]
+                ("precond", ctypes.c_int), ("smoother_weight", ctypes.c_double),
+                ("autotune", ctypes.c_int), ("spmv_threads", ctypes.c_int),
+                ("sweep_threads", ctypes.c_int), ("tail_rows", ctypes.c_int)]
 
 
 class Report(ctypes.Structure):
@@ -51,7 +53,10 @@ class Report(ctypes.Structure):
                 ("precond", ctypes.c_int), ("levels", ctypes.c_int),
                 ("mg_rows", ctypes.c_longlong), ("mg_nnz", ctypes.c_longlong),
                 ("assemble_ms", ctypes.c_double), ("solve_ms", ctypes.c_double),
-                ("recover_ms", ctypes.c_double)]
+                ("recover_ms", ctypes.c_double), ("spmv_threads", ctypes.c_int),
+                ("spmv_xcd", ctypes.c_int), ("sweep_threads", ctypes.c_int),
+                ("sweep_xcd", ctypes.c_int), ("tail_rows", ctypes.c_int),
+                ("tune_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -245,7 +250,8 @@ class HybridSolver:
     def options(self, tol=1e-8, linsolver_verbosity=0, linsolver_type=1, same_matrix=False,
                 linsolver_maxit=0, prolongate_factor=1.6, smooth_steps=1, precision=FP64,
                 nrhs=1, chunk=32, replace_factor=0.1, use_graph=True, precond=-1,
-                smoother_weight=None):
+                smoother_weight=None, autotune=True, spmv_threads=0, sweep_threads=0,
+                tail_rows=-1):
         o = Options()
         self._L.mhs_default_options(ctypes.byref(o))
         o.tol, o.maxit, o.linsolver_type, o.verbosity = tol, linsolver_maxit, linsolver_type, \
@@ -258,13 +264,17 @@ class HybridSolver:
         o.precond = {"auto": -1, "jacobi": 0, "amg": 1}.get(precond, precond)
         if smoother_weight is not None:
             o.smoother_weight = smoother_weight
+        o.autotune = int(autotune)
+        o.spmv_threads, o.sweep_threads = int(spmv_threads), int(sweep_threads)
+        o.tail_rows = int(tail_rows)
         return o
 
     def solve(self, totmob, rhomob, src, bc_type, bc_value, residual_tolerance=1e-8,
               linsolver_verbosity=0, linsolver_type=1, same_matrix=False, linsolver_maxit=0,
               prolongate_factor=1.6, smooth_steps=1, precision="fp64", chunk=32,
               replace_factor=0.1, use_graph=True, raise_on_failure=True, precond=-1,
-              smoother_weight=None):
+              smoother_weight=None, autotune=True, spmv_threads=0, sweep_threads=0,
+              tail_rows=-1):
         """IncompFlowSolverHybrid::solve (:657-703) with per-cell mobilities."""
         g = self.grid
         src = np.asarray(src, dtype=np.float64)
@@ -274,7 +284,8 @@ class HybridSolver:
         bt, pbt = _i(bc_type); bvv, pbv = _d(bv)
         o = self.options(residual_tolerance, linsolver_verbosity, linsolver_type, same_matrix,
                          linsolver_maxit, prolongate_factor, smooth_steps, precision, nrhs, chunk,
-                         replace_factor, use_graph, precond, smoother_weight)
+                         replace_factor, use_graph, precond, smoother_weight, autotune,
+                         spmv_threads, sweep_threads, tail_rows)
         rep = Report()
         rc = self._L.mhs_solve(self._h, ptm, prm, psr, pbt, pbv, ctypes.byref(o),
                                ctypes.byref(rep))
