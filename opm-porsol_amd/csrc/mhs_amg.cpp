@@ -52,12 +52,71 @@ void galerkin_pattern(AmgLevel &C, RowEntries row_entries) {
     }
 }
 
+void build_ap(ApPattern &P, const Topology &T, const std::vector<int> &agg) {
+    const int ns = T.nslices;
+    std::vector<int> cnt(T.nfp, 0);
+    std::vector<std::pair<int, long long>> buf;
+    auto row = [&](int f) {
+        buf.clear();
+        const long long base = T.slice_ptr[f / kSlice] + f % kSlice;
+        for (int e = 0; e < T.row_len[f]; ++e) {
+            const long long pos = base + (long long)e * kSlice;
+            const int J = agg[T.col[pos]];
+            if (J >= 0) buf.emplace_back(J, pos);
+        }
+        std::sort(buf.begin(), buf.end());
+    };
+    for (int f = 0; f < T.nfp; ++f) {
+        row(f);
+        int n = 0;
+        for (size_t k = 0; k < buf.size(); ++k) n += (k == 0 || buf[k].first != buf[k - 1].first);
+        cnt[f] = n;
+    }
+    P.slice_ptr.assign(ns + 1, 0);
+    for (int sl = 0; sl < ns; ++sl) {
+        int w = 0;
+        for (int l = 0; l < kSlice; ++l) w = std::max(w, cnt[sl * kSlice + l]);
+        P.slice_ptr[sl + 1] = P.slice_ptr[sl] + (long long)w * kSlice;
+    }
+    const long long slots = P.slice_ptr[ns];
+    P.col.assign(slots, 0);
+    std::vector<int> len(slots, 0);
+    // slot sizes, then the gather lists in slot order
+    for (int f = 0; f < T.nfp; ++f) {
+        row(f);
+        const long long base = P.slice_ptr[f / kSlice] + f % kSlice;
+        int e = -1;
+        for (size_t k = 0; k < buf.size(); ++k) {
+            if (k == 0 || buf[k].first != buf[k - 1].first) {
+                ++e;
+                P.col[base + (long long)e * kSlice] = buf[k].first;
+            }
+            len[base + (long long)e * kSlice]++;
+        }
+    }
+    P.src_ptr.assign(slots + 1, 0);
+    for (long long q = 0; q < slots; ++q) P.src_ptr[q + 1] = P.src_ptr[q] + len[q];
+    P.src.assign(P.src_ptr[slots], 0);
+    std::vector<int> fill(P.src_ptr.begin(), P.src_ptr.end() - 1);
+    for (int f = 0; f < T.nfp; ++f) {
+        row(f);
+        const long long base = P.slice_ptr[f / kSlice] + f % kSlice;
+        int e = -1;
+        for (size_t k = 0; k < buf.size(); ++k) {
+            if (k == 0 || buf[k].first != buf[k - 1].first) ++e;
+            P.src[fill[base + (long long)e * kSlice]++] = int(buf[k].second);
+        }
+    }
+}
+
 }  // namespace
 
 std::string build_amg(AmgHierarchy &H, const Topology &T, const std::vector<int> &ijk) {
     H = AmgHierarchy();
     const int ninst = T.ninst;
     if ((int)ijk.size() != 3 * T.nc) return "logical cell indices: need 3 per cell";
+    if (T.slice_ptr[T.nslices] > 0x7fffffffLL)
+        return "face matrix too large for the 32-bit multigrid gather lists";
     // instance dimensions from the logical indices
     std::vector<int> dims(3 * ninst, 0);
     for (int t = 0; t < ninst; ++t)
@@ -101,6 +160,7 @@ std::string build_amg(AmgHierarchy &H, const Topology &T, const std::vector<int>
                 emit(T.col[pos], int(pos));
             }
         });
+        build_ap(H.ap, T, L1.agg);
         H.lev.push_back(std::move(L1));
     }
     // ---- coarser levels: halve block indices until every instance is small

@@ -31,9 +31,22 @@ struct AmgLevel {
     std::vector<int> child;         // finer rows of each row
 };
 
+// A P with the level-1 aggregates: the fine-row by aggregate matrix of the
+// post-smoothing sweep, (A P)_{iJ} = sum of a_ij over the columns j in
+// aggregate J.  Stored in the SELL-64 layout of the face matrix (lane-major
+// slices; padding slots have column 0 and value 0), with, for every slot, the
+// list of face-matrix positions that sum into it.
+struct ApPattern {
+    std::vector<long long> slice_ptr;  // [nslices+1]
+    std::vector<int> col;              // level-1 row of each slot
+    std::vector<int> src_ptr;          // [slots+1]
+    std::vector<int> src;              // face-matrix SELL positions
+};
+
 struct AmgHierarchy {
     std::vector<AmgLevel> lev;      // lev[0] unused placeholder, lev[1..] coarse levels
     std::vector<int> coarse_off;    // [ninst+1] offsets of the dense coarsest inverses
+    ApPattern ap;
     int nlev() const { return int(lev.size()); }
 };
 

@@ -373,6 +373,13 @@ int mhs_init_system_structure(mhs_solver *s, const int *hf_bc_type, const int *h
             if (he == hipSuccess) D.dinv64 = dalloc<double>(s, L, (size_t)D.n, &he);
             if (he == hipSuccess) D.dinv32 = dalloc<float>(s, L, (size_t)D.n, &he);
         }
+        if (he == hipSuccess) A.ap_slice_ptr = dupload(s, L, s->H.ap.slice_ptr, &he);
+        if (he == hipSuccess) A.ap_col = dupload(s, L, s->H.ap.col, &he);
+        if (he == hipSuccess) A.ap_src_ptr = dupload(s, L, s->H.ap.src_ptr, &he);
+        if (he == hipSuccess) A.ap_src = dupload(s, L, s->H.ap.src, &he);
+        A.ap_slots = (long long)s->H.ap.col.size();
+        if (he == hipSuccess) A.ap64 = dalloc<double>(s, L, (size_t)A.ap_slots, &he);
+        if (he == hipSuccess) A.ap32 = dalloc<float>(s, L, (size_t)A.ap_slots, &he);
         if (he == hipSuccess) A.coarse_off = dupload(s, L, s->H.coarse_off, &he);
         if (he == hipSuccess) A.cinv64 = dalloc<double>(s, L, (size_t)s->H.coarse_off.back(), &he);
         if (he == hipSuccess) A.cinv32 = dalloc<float>(s, L, (size_t)s->H.coarse_off.back(), &he);
@@ -665,7 +672,13 @@ int autotune(mhs_solver *s, int nrhs, bool mixed, bool amg, double *tune_ms) {
                 for (int xcd : {1, 0}) {
                     s->dT.sweep_tb = tb;
                     s->dT.sweep_xcd = xcd;
+                    const bool fused = s->dA.nu == 1;   // the sweep the V-cycle actually runs
                     const float ms = time_it([&]() {
+                        if (fused)
+                            return mixed ? launch_post0<float>(s->stream, s->dT, s->dA, nrhs, s->dinv32, s->r32,
+                                                               s->z32, s->R)
+                                         : launch_post0<double>(s->stream, s->dT, s->dA, nrhs, s->dinv64, s->r64,
+                                                                s->z64, s->R);
                         return mixed ? launch_sweep0<float>(s->stream, s->dT, nrhs, s->val32, s->dinv32, s->r32,
                                                             (const float *)s->dA.x0, s->z32, s->dA.omega, s->R)
                                      : launch_sweep0<double>(s->stream, s->dT, nrhs, s->val64, s->dinv64, s->r64,

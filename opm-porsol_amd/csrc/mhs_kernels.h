@@ -85,6 +85,13 @@ struct DevAmg {
     double omega, alpha;            // Jacobi weight, coarse-correction (prolongation) factor
     int nu;                         // smoothing sweeps before and after the coarse correction
     int max_rows[kMaxLevels];       // rows of the largest instance on each level
+    // A P with the level-1 aggregates (SELL-64 like the face matrix), refilled
+    // every solve from the face matrix through the gather lists
+    const long long *ap_slice_ptr;
+    const int *ap_col, *ap_src_ptr, *ap_src;
+    long long ap_slots;
+    double *ap64;
+    float *ap32;
 };
 static_assert(sizeof(DevAmg) < 4000, "DevAmg must fit the kernel-argument limit");
 
@@ -138,6 +145,11 @@ hipError_t launch_amg_setup(hipStream_t st, const DevTopo &T, const DevAmg &A, c
 template <typename VT>
 hipError_t launch_sweep0(hipStream_t st, const DevTopo &T, int nrhs, const VT *val0, const VT *dinv0, const VT *b,
                          const VT *x, VT *xo, double w, DevRed R);
+// The fused post-smoothing sweep alone (z from x0, res0 and the level-1
+// correction of the last V-cycle; the autotuner times it).
+template <typename VT>
+hipError_t launch_post0(hipStream_t st, const DevTopo &T, const DevAmg &A, int nrhs, const VT *dinv0,
+                        const VT *r, VT *z, DevRed R);
 template <typename VT>
 hipError_t launch_vcycle(hipStream_t st, const DevTopo &T, const DevAmg &A, int nrhs, const VT *val0,
                          const VT *dinv0, const VT *r, VT *z, DevRed R);

@@ -1004,6 +1004,60 @@ __global__ __launch_bounds__(TB) void k_vc0_up(DevTopo T, const VT *__restrict__
     if (LAST) block_partials<NRHS, TB / 64>(part, blk, R);
 }
 
+// Post-smoothing sweep after the coarse correction without reading A: with
+// x~ = x0 + alpha P xc and A x0 = r - res0 (from the pre-smoothing pass),
+//   z = x~ + w D^-1 (r - A x~) = x~ + w D^-1 (res0 - alpha (A P) xc),
+// where A P (fine rows x level-1 aggregates, a few entries per row) is
+// refilled every solve.  Also the <r, z> partials of the preconditioned CG.
+template <typename VT, int NRHS, int TB>
+__global__ __launch_bounds__(TB) void k_vc0_post(DevTopo T, const long long *__restrict__ ap_ptr,
+                                                 const int *__restrict__ ap_col, const VT *__restrict__ ap,
+                                                 const int *__restrict__ agg, const VT *__restrict__ dinv,
+                                                 const VT *__restrict__ r, const VT *__restrict__ x0,
+                                                 const VT *__restrict__ res0, const VT *__restrict__ xc,
+                                                 VT *__restrict__ z, VT w, VT al, DevRed R) {
+    const int blk = T.sweep_xcd ? xcd_block(gridDim.x) : (int)blockIdx.x;
+    double part[NRHS];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) part[k] = 0.0;
+#pragma unroll
+    for (int j = 0; j < kRowBlock / TB; ++j) {
+        const int f = blk * kRowBlock + j * TB + threadIdx.x;
+        const int slice = f >> 6, lane = f & 63;
+        const long long s0 = ap_ptr[slice];
+        const int wd = (int)((ap_ptr[slice + 1] - s0) >> 6);
+        VT acc[NRHS];
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) acc[k] = VT(0);
+        for (int e0 = 0; e0 < wd; e0 += kChunk) {
+            int c[kChunk];
+            VT a[kChunk];
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u) {
+                const bool in = e0 + u < wd;
+                const long long o = s0 + lane + 64LL * (e0 + u);
+                c[u] = in ? __builtin_nontemporal_load(ap_col + o) : 0;
+                a[u] = in ? __builtin_nontemporal_load(ap + o) : VT(0);
+            }
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u)
+#pragma unroll
+                for (int k = 0; k < NRHS; ++k) acc[k] += a[u] * xc[(size_t)c[u] * NRHS + k];
+        }
+        const int g = agg[f];
+        const VT d = w * dinv[f];
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) {
+            const size_t i = (size_t)f * NRHS + k;
+            const VT xt = x0[i] + (g >= 0 ? al * xc[(size_t)g * NRHS + k] : VT(0));
+            const VT out = xt + d * (res0[i] - al * acc[k]);
+            z[i] = out;
+            part[k] += (double)r[i] * (double)out;
+        }
+    }
+    block_partials<NRHS, TB / 64>(part, blk, R);
+}
+
 // Level 0: res = b - A x (after extra down sweeps).
 template <typename VT, int NRHS>
 __global__ __launch_bounds__(256) void k_vc0_res(DevTopo T, const VT *__restrict__ val,
@@ -1924,11 +1978,41 @@ hipError_t launch_amg_setup(hipStream_t st, const DevTopo &T, const DevAmg &A, c
             k_coarse_diag<<<(L.n + 255) / 256, 256, 0, st>>>(L.n, L.diag, L.val64, L.dinv64,
                                                             mixed ? L.dinv32 : nullptr);
     }
+    if (A.ap_slots > 0)
+        k_galerkin<<<(unsigned)((A.ap_slots + 255) / 256), 256, 0, st>>>((int)A.ap_slots, A.ap_src_ptr, A.ap_src,
+                                                                        val64, A.ap64, mixed ? A.ap32 : nullptr);
     const DevAmgLevel &C = A.lev[A.nlev - 1];
     k_coarse_inverse<<<A.ninst, 256, 0, st>>>(C, C.inst_ptr, A.coarse_off, A.cinv64,
                                               mixed ? A.cinv32 : nullptr);
     return hipGetLastError();
 }
+
+// The fused post-smoothing sweep (k_vc0_post) in the tuned workgroup shape.
+template <typename VT, int NR>
+static void post0(hipStream_t st, const DevTopo &T, const DevAmg &A, const VT *dinv0, const VT *r, const VT *x0,
+                  const VT *res0, const VT *xc, VT *z, VT w, VT al, DevRed R) {
+    const VT *ap;
+    if constexpr (std::is_same<VT, float>::value) ap = A.ap32; else ap = A.ap64;
+    with_tb(T.sweep_tb, [&](auto tb) {
+        constexpr int TBv = decltype(tb)::value;
+        k_vc0_post<VT, NR, TBv><<<T.nblk, TBv, 0, st>>>(T, A.ap_slice_ptr, A.ap_col, ap, A.lev[1].agg, dinv0, r,
+                                                        x0, res0, xc, z, w, al, R);
+    });
+}
+
+template <typename VT>
+hipError_t launch_post0(hipStream_t st, const DevTopo &T, const DevAmg &A, int nrhs, const VT *dinv0,
+                        const VT *r, VT *z, DevRed R) {
+    return with_nrhs(nrhs, [&](auto nr) {
+        constexpr int NR = decltype(nr)::value;
+        post0<VT, NR>(st, T, A, dinv0, r, (const VT *)A.x0, (const VT *)A.res0, (const VT *)A.lev[1].x, z,
+                      (VT)A.omega, (VT)A.alpha, R);
+    });
+}
+template hipError_t launch_post0<double>(hipStream_t, const DevTopo &, const DevAmg &, int, const double *,
+                                         const double *, double *, DevRed);
+template hipError_t launch_post0<float>(hipStream_t, const DevTopo &, const DevAmg &, int, const float *,
+                                        const float *, float *, DevRed);
 
 // Level-0 Jacobi sweep xo = x + w D^-1 (b - A x) in the tuned workgroup shape;
 // LAST also forms the <b, xo> partials.
@@ -2027,6 +2111,10 @@ static void vcycle_impl(hipStream_t st, const DevTopo &T, const DevAmg &A, const
         }
     }
     // ---- level 0 up: x~ = x0 + alpha P x1, then nu sweeps; the last writes z and <r, z>
+    if (nu == 1) {
+        post0<VT, NR>(st, T, A, dinv0, r, x0, res0, xcur[1], z, w, al, R);
+        return;
+    }
     k_prolong0<VT, NR><<<rows_blocks, 256, 0, st>>>(A.lev[1].agg, x0, xcur[1], xt0, al);
     std::swap(x0, xt0);
     for (int it = 1; it < nu; ++it) {
