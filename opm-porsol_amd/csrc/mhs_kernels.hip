@@ -172,11 +172,13 @@ __device__ __forceinline__ bool load_act(const int *act, int s, int (&a)[NRHS]) 
     return any;
 }
 
-// One SELL-64 row times NRHS vectors.  The slice width is wave-uniform; rows
-// up to 16 stored entries load every index and value first (non-temporal: the
-// matrix is streamed once, the gathered vector should stay in L2), then issue
-// all gathers, so each lane has up to 32 independent loads in flight.
-template <typename MT, typename XT, typename AT, int NRHS>
+// One SELL-64 row times NRHS vectors.  The slice width is wave-uniform.  The
+// row is taken CH entries at a time: every index and value of a chunk is
+// loaded first (non-temporal: the matrix is streamed once, the gathered vector
+// should stay in L2), then all of the chunk's gathers are issued, so each lane
+// has up to 2 CH independent loads in flight.  CH trades registers for
+// memory-level parallelism.
+template <typename MT, typename XT, typename AT, int NRHS, int CH = 16>
 __device__ __forceinline__ void sell_row(const DevTopo &T, const MT *__restrict__ val,
                                          const XT *__restrict__ x, int f, AT (&acc)[NRHS]) {
     const int slice = f >> 6, lane = f & 63;
@@ -185,29 +187,22 @@ __device__ __forceinline__ void sell_row(const DevTopo &T, const MT *__restrict_
     const long long base = s0 + lane;
 #pragma unroll
     for (int k = 0; k < NRHS; ++k) acc[k] = AT(0);
-    if (w <= 16) {
-        int ci[16];
-        MT a[16];
+    for (int e0 = 0; e0 < w; e0 += CH) {
+        int ci[CH];
+        MT a[CH];
 #pragma unroll
-        for (int e = 0; e < 16; ++e) {
-            if (e < w) {
-                ci[e] = __builtin_nontemporal_load(T.col + base + 64LL * e);
-                a[e] = __builtin_nontemporal_load(val + base + 64LL * e);
+        for (int e = 0; e < CH; ++e) {
+            if (e0 + e < w) {
+                ci[e] = __builtin_nontemporal_load(T.col + base + 64LL * (e0 + e));
+                a[e] = __builtin_nontemporal_load(val + base + 64LL * (e0 + e));
             }
         }
 #pragma unroll
-        for (int e = 0; e < 16; ++e) {
-            if (e < w) {
+        for (int e = 0; e < CH; ++e) {
+            if (e0 + e < w) {
 #pragma unroll
                 for (int k = 0; k < NRHS; ++k) acc[k] += (AT)a[e] * (AT)x[(size_t)ci[e] * NRHS + k];
             }
-        }
-    } else {
-        for (int e = 0; e < w; ++e) {
-            const int c = __builtin_nontemporal_load(T.col + base + 64LL * e);
-            const AT a = (AT)__builtin_nontemporal_load(val + base + 64LL * e);
-#pragma unroll
-            for (int k = 0; k < NRHS; ++k) acc[k] += a * (AT)x[(size_t)c * NRHS + k];
         }
     }
 }
@@ -526,10 +521,16 @@ __global__ __launch_bounds__(TB) void k_spmv_pq(DevTopo T, const MT *__restrict_
             q[(size_t)f * NRHS + k] = acc[k];
             part[k] += (double)p[(size_t)f * NRHS + k] * (double)acc[k];
         }
-        if constexpr (REL) {
-            if (rel) {  // block-uniform: the flags are per instance
+    }
+    // The FP64 residual replacement in a loop of its own, so that its
+    // registers are not live during the FP32 product (rel is block-uniform:
+    // the flags are per instance).
+    if constexpr (REL) {
+        if (__builtin_expect(rel, 0)) {
+            for (int j = 0; j < kRowBlock / TB; ++j) {
+                const int f = b * kRowBlock + j * TB + threadIdx.x;
                 double a64[NRHS];
-                sell_row<double, double, double, NRHS>(T, L.val64, L.x64, f, a64);
+                sell_row<double, double, double, NRHS, 4>(T, L.val64, L.x64, f, a64);
                 const float d = L.dinv[f];
 #pragma unroll
                 for (int k = 0; k < NRHS; ++k) {
