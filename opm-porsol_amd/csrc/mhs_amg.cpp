@@ -2,6 +2,7 @@
 #include "mhs_amg.h"
 
 #include <algorithm>
+#include <array>
 #include <numeric>
 
 namespace mhs {
@@ -109,9 +110,105 @@ void build_ap(ApPattern &P, const Topology &T, const std::vector<int> &agg) {
     }
 }
 
+// Smoothed-aggregation transfer F -> C (C.agg set, F's CSR pattern known):
+// the patterns of P, P', T = A_F P and A_C = P' T, with the gather / pair
+// lists that let every solve fill their values on the device.
+void sa_transfer(AmgLevel &F, AmgLevel &C) {
+    const int n = F.n;
+    F.sa_out = true;
+    F.p_ptr.assign(n + 1, 0);
+    F.p_col.clear();
+    F.p_own.assign(n, -1);
+    F.p_src_ptr.assign(1, 0);
+    F.p_src.clear();
+    std::vector<std::pair<int, int>> buf;
+    for (int i = 0; i < n; ++i) {
+        buf.clear();
+        for (int k = F.row_ptr[i]; k < F.row_ptr[i + 1]; ++k) buf.emplace_back(C.agg[F.col[k]], k);
+        std::sort(buf.begin(), buf.end());
+        for (size_t k = 0; k < buf.size(); ++k) {
+            if (k == 0 || buf[k].first != buf[k - 1].first) {
+                if (k > 0) F.p_src_ptr.push_back(int(F.p_src.size()));
+                if (buf[k].first == C.agg[i]) F.p_own[i] = int(F.p_col.size());
+                F.p_col.push_back(buf[k].first);
+            }
+            F.p_src.push_back(buf[k].second);
+        }
+        if (!buf.empty()) F.p_src_ptr.push_back(int(F.p_src.size()));
+        F.p_ptr[i + 1] = int(F.p_col.size());
+    }
+    // P' (rows of C)
+    F.pt_ptr.assign(C.n + 1, 0);
+    for (int J : F.p_col) F.pt_ptr[J + 1]++;
+    for (int I = 0; I < C.n; ++I) F.pt_ptr[I + 1] += F.pt_ptr[I];
+    F.pt_col.assign(F.p_col.size(), 0);
+    F.pt_map.assign(F.p_col.size(), 0);
+    {
+        std::vector<int> fill(F.pt_ptr.begin(), F.pt_ptr.end() - 1);
+        for (int i = 0; i < n; ++i)
+            for (int q = F.p_ptr[i]; q < F.p_ptr[i + 1]; ++q) {
+                const int e = fill[F.p_col[q]]++;
+                F.pt_col[e] = i;
+                F.pt_map[e] = q;
+            }
+    }
+    // T = A P
+    std::vector<std::array<int, 3>> tb;
+    F.t_ptr.assign(n + 1, 0);
+    F.t_col.clear();
+    F.t_pair_ptr.assign(1, 0);
+    F.t_pa.clear();
+    F.t_pb.clear();
+    for (int i = 0; i < n; ++i) {
+        tb.clear();
+        for (int k = F.row_ptr[i]; k < F.row_ptr[i + 1]; ++k) {
+            const int j = F.col[k];
+            for (int q = F.p_ptr[j]; q < F.p_ptr[j + 1]; ++q) tb.push_back({F.p_col[q], k, q});
+        }
+        std::sort(tb.begin(), tb.end());
+        for (size_t k = 0; k < tb.size(); ++k) {
+            if (k == 0 || tb[k][0] != tb[k - 1][0]) {
+                if (k > 0) F.t_pair_ptr.push_back(int(F.t_pa.size()));
+                F.t_col.push_back(tb[k][0]);
+            }
+            F.t_pa.push_back(tb[k][1]);
+            F.t_pb.push_back(tb[k][2]);
+        }
+        if (!tb.empty()) F.t_pair_ptr.push_back(int(F.t_pa.size()));
+        F.t_ptr[i + 1] = int(F.t_col.size());
+    }
+    // A_C = P' T
+    C.sa_in = true;
+    C.row_ptr.assign(C.n + 1, 0);
+    C.col.clear();
+    C.src_ptr.assign(1, 0);
+    C.src.clear();
+    C.src2.clear();
+    C.diag.assign(C.n, -1);
+    for (int I = 0; I < C.n; ++I) {
+        tb.clear();
+        for (int e = F.pt_ptr[I]; e < F.pt_ptr[I + 1]; ++e) {
+            const int i = F.pt_col[e], q = F.pt_map[e];
+            for (int t = F.t_ptr[i]; t < F.t_ptr[i + 1]; ++t) tb.push_back({F.t_col[t], q, t});
+        }
+        std::sort(tb.begin(), tb.end());
+        for (size_t k = 0; k < tb.size(); ++k) {
+            if (k == 0 || tb[k][0] != tb[k - 1][0]) {
+                if (k > 0) C.src_ptr.push_back(int(C.src.size()));
+                if (tb[k][0] == I) C.diag[I] = int(C.col.size());
+                C.col.push_back(tb[k][0]);
+            }
+            C.src.push_back(tb[k][1]);
+            C.src2.push_back(tb[k][2]);
+        }
+        if (!tb.empty()) C.src_ptr.push_back(int(C.src.size()));
+        C.row_ptr[I + 1] = int(C.col.size());
+    }
+}
+
 }  // namespace
 
-std::string build_amg(AmgHierarchy &H, const Topology &T, const std::vector<int> &ijk) {
+std::string build_amg(AmgHierarchy &H, const Topology &T, const std::vector<int> &ijk, bool smoothed) {
     H = AmgHierarchy();
     const int ninst = T.ninst;
     if ((int)ijk.size() != 3 * T.nc) return "logical cell indices: need 3 per cell";
@@ -190,11 +287,18 @@ std::string build_amg(AmgHierarchy &H, const Topology &T, const std::vector<int>
                 C.agg[r] = C.inst_ptr[t] + i / 2 + cd[0] * (j / 2 + cd[1] * (k / 2));
             }
         }
-        const AmgLevel &Fr = H.lev.back();
+        AmgLevel &Fr = H.lev.back();
         children(C, F.n);
-        galerkin_pattern(C, [&](int r, auto emit) {
-            for (int k = Fr.row_ptr[r]; k < Fr.row_ptr[r + 1]; ++k) emit(Fr.col[k], k);
-        });
+        // smooth the transfer from level 1 only: most of the gain in
+        // iterations, while the deeper levels keep the sparse stencils of
+        // piecewise-constant aggregation (smoothing widens every stencil)
+        if (smoothed && H.lev.size() == 2) {
+            sa_transfer(Fr, C);
+        } else {
+            galerkin_pattern(C, [&](int r, auto emit) {
+                for (int k = Fr.row_ptr[r]; k < Fr.row_ptr[r + 1]; ++k) emit(Fr.col[k], k);
+            });
+        }
         H.lev.push_back(std::move(C));
         if (H.lev.size() > 40) return "multigrid hierarchy does not coarsen";
     }

@@ -29,6 +29,20 @@ struct AmgLevel {
     std::vector<int> agg;           // [n_finer] row of this level each finer row feeds (-1: none)
     std::vector<int> child_ptr;     // [n+1]
     std::vector<int> child;         // finer rows of each row
+    // Levels built by smoothed aggregation take their values as sums of
+    // products over pairs (P position, T position) of the finer level's
+    // transfer: src holds the P positions, src2 the T positions.
+    bool sa_in = false;
+    std::vector<int> src2;
+    // Smoothed-aggregation transfer to the next coarser level (when sa_out):
+    // P = (I - w_p D^-1 A) P_t with the tentative aggregates agg of the next
+    // level, T = A P, and P transposed for the restriction.
+    bool sa_out = false;
+    std::vector<int> p_ptr, p_col, p_own;      // P rows; p_own[i]: entry of row i in its own aggregate
+    std::vector<int> p_src_ptr, p_src;         // per P entry: positions of this level's values
+    std::vector<int> pt_ptr, pt_col, pt_map;   // rows of the coarser level: fine row, P position
+    std::vector<int> t_ptr, t_col;             // T = A P
+    std::vector<int> t_pair_ptr, t_pa, t_pb;   // per T entry: (value position, P position)
 };
 
 // A P with the level-1 aggregates: the fine-row by aggregate matrix of the
@@ -51,7 +65,11 @@ struct AmgHierarchy {
 };
 
 // Builds the hierarchy from logical cell indices ijk[3*nc] (instance-local,
-// 0-based).  Returns an empty string on success.
-std::string build_amg(AmgHierarchy &H, const Topology &T, const std::vector<int> &ijk);
+// 0-based).  The transfer from the face matrix to level 1 is the tentative
+// (piecewise-constant) one; with smoothed = true the transfer from level 1
+// to level 2 is smoothed, the others stay tentative.  Returns an empty
+// string on success.
+std::string build_amg(AmgHierarchy &H, const Topology &T, const std::vector<int> &ijk,
+                      bool smoothed = true);
 
 }  // namespace mhs

@@ -368,6 +368,30 @@ int mhs_init_system_structure(mhs_solver *s, const int *hf_bc_type, const int *h
             if (he == hipSuccess) D.agg = dupload(s, L, H.agg, &he);
             if (he == hipSuccess) D.child_ptr = dupload(s, L, H.child_ptr, &he);
             if (he == hipSuccess) D.child = dupload(s, L, H.child, &he);
+            D.sa_in = H.sa_in ? 1 : 0;
+            D.sa_out = H.sa_out ? 1 : 0;
+            if (H.sa_in && he == hipSuccess) D.src2 = dupload(s, L, H.src2, &he);
+            if (H.sa_out) {
+                D.p_nnz = int(H.p_col.size());
+                D.t_nnz = int(H.t_col.size());
+                if (he == hipSuccess) D.p_ptr = dupload(s, L, H.p_ptr, &he);
+                if (he == hipSuccess) D.p_col = dupload(s, L, H.p_col, &he);
+                if (he == hipSuccess) D.p_own = dupload(s, L, H.p_own, &he);
+                if (he == hipSuccess) D.p_src_ptr = dupload(s, L, H.p_src_ptr, &he);
+                if (he == hipSuccess) D.p_src = dupload(s, L, H.p_src, &he);
+                if (he == hipSuccess) D.pt_ptr = dupload(s, L, H.pt_ptr, &he);
+                if (he == hipSuccess) D.pt_col = dupload(s, L, H.pt_col, &he);
+                if (he == hipSuccess) D.pt_map = dupload(s, L, H.pt_map, &he);
+                if (he == hipSuccess) D.t_ptr = dupload(s, L, H.t_ptr, &he);
+                if (he == hipSuccess) D.t_col = dupload(s, L, H.t_col, &he);
+                if (he == hipSuccess) D.t_pair_ptr = dupload(s, L, H.t_pair_ptr, &he);
+                if (he == hipSuccess) D.t_pa = dupload(s, L, H.t_pa, &he);
+                if (he == hipSuccess) D.t_pb = dupload(s, L, H.t_pb, &he);
+                if (he == hipSuccess) D.p_val64 = dalloc<double>(s, L, (size_t)D.p_nnz, &he);
+                if (he == hipSuccess) D.p_val32 = dalloc<float>(s, L, (size_t)D.p_nnz, &he);
+                if (he == hipSuccess) D.t_val64 = dalloc<double>(s, L, (size_t)D.t_nnz, &he);
+                if (he == hipSuccess) D.t_val32 = dalloc<float>(s, L, (size_t)D.t_nnz, &he);
+            }
             if (he == hipSuccess) D.val64 = dalloc<double>(s, L, (size_t)D.nnz, &he);
             if (he == hipSuccess) D.val32 = dalloc<float>(s, L, (size_t)D.nnz, &he);
             if (he == hipSuccess) D.dinv64 = dalloc<double>(s, L, (size_t)D.n, &he);
@@ -381,6 +405,9 @@ int mhs_init_system_structure(mhs_solver *s, const int *hf_bc_type, const int *h
         if (he == hipSuccess) A.ap64 = dalloc<double>(s, L, (size_t)A.ap_slots, &he);
         if (he == hipSuccess) A.ap32 = dalloc<float>(s, L, (size_t)A.ap_slots, &he);
         if (he == hipSuccess) A.coarse_off = dupload(s, L, s->H.coarse_off, &he);
+        if (he == hipSuccess) A.dev = dalloc<DevAmg>(s, L, 1, &he);
+        A.alpha_sa = 1.0;
+        A.omega_p = 2.0 / 3.0;
         if (he == hipSuccess) A.cinv64 = dalloc<double>(s, L, (size_t)s->H.coarse_off.back(), &he);
         if (he == hipSuccess) A.cinv32 = dalloc<float>(s, L, (size_t)s->H.coarse_off.back(), &he);
         if (he != hipSuccess) return fail(s, MHS_ERR_ALLOC, std::string("multigrid allocation: ") + hipGetErrorString(he));
@@ -892,6 +919,9 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
         s->dA.omega = opt.smoother_weight;
         s->dA.alpha = opt.prolongate_factor;
         s->dA.nu = std::max(1, opt.smooth_steps);
+        // the kernels that walk the whole hierarchy read this copy (nothing of
+        // the previous solve is still running: it ended with a synchronise)
+        HIPCHECK(hipMemcpy(const_cast<DevAmg *>(s->dA.dev), &s->dA, sizeof(DevAmg), hipMemcpyHostToDevice));
         HIPCHECK(launch_amg_setup(s->stream, s->dT, s->dA, s->val64, mixed));
     }
 

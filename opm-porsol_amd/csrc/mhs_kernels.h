@@ -73,6 +73,16 @@ struct DevAmgLevel {
     double *val64, *dinv64;
     float *val32, *dinv32;
     void *b, *x, *res, *xt;         // work vectors (PCG precision, n * nrhs)
+    // smoothed aggregation (see AmgLevel): values of this level from pair
+    // sums (sa_in, src = P positions, src2 = T positions of the finer level),
+    // and the smoothed transfer to the next level (sa_out)
+    int sa_in, sa_out, p_nnz, t_nnz;
+    const int *src2;
+    const int *p_ptr, *p_col, *p_own, *p_src_ptr, *p_src;
+    const int *pt_ptr, *pt_col, *pt_map;
+    const int *t_ptr, *t_col, *t_pair_ptr, *t_pa, *t_pb;
+    double *p_val64, *t_val64;
+    float *p_val32, *t_val32;
 };
 struct DevAmg {
     int nlev;                       // levels including level 0
@@ -83,6 +93,9 @@ struct DevAmg {
     double *cinv64;
     float *cinv32;
     double omega, alpha;            // Jacobi weight, coarse-correction (prolongation) factor
+    double alpha_sa;                // coarse-correction factor across smoothed transfers
+    double omega_p;                 // prolongation-smoothing weight of those transfers
+    const DevAmg *dev;              // this structure's copy in device memory (refreshed by the host)
     int nu;                         // smoothing sweeps before and after the coarse correction
     int max_rows[kMaxLevels];       // rows of the largest instance on each level
     // A P with the level-1 aggregates (SELL-64 like the face matrix), refilled
@@ -93,7 +106,8 @@ struct DevAmg {
     double *ap64;
     float *ap32;
 };
-static_assert(sizeof(DevAmg) < 4000, "DevAmg must fit the kernel-argument limit");
+// DevAmg is larger than the kernel-argument limit: kernels that need the
+// whole hierarchy read a copy in device memory.
 
 hipError_t launch_static_ip(hipStream_t st, const DevTopo &T, const DevGeom &G, int maxn,
                             const double *perm, const double *grav, double *binv_pk,

@@ -803,6 +803,34 @@ __global__ __launch_bounds__(256) void k_coarse_diag(int n, const int *__restric
     if (dinv32) dinv32[i] = (float)d;
 }
 
+// Smoothed prolongation of a coarse transfer, row by row:
+// P_iJ = [J = agg_i] - w_p d_i sum_{j in row i, agg_j = J} a_ij.
+__global__ __launch_bounds__(256) void k_sa_p(DevAmgLevel F, double wp) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= F.n) return;
+    const double d = F.dinv64[i];
+    for (int q = F.p_ptr[i]; q < F.p_ptr[i + 1]; ++q) {
+        double s = 0.0;
+        for (int k = F.p_src_ptr[q]; k < F.p_src_ptr[q + 1]; ++k) s += F.val64[F.p_src[k]];
+        const double v = (q == F.p_own[i] ? 1.0 : 0.0) - wp * d * s;
+        F.p_val64[q] = v;
+        if (F.p_val32) F.p_val32[q] = (float)v;
+    }
+}
+
+// out[e] = sum over the pairs of entry e of a[pa] * b[pb] (T = A P, A_c = P' T).
+__global__ __launch_bounds__(256) void k_pair_sum(int n, const int *__restrict__ ptr,
+                                                  const int *__restrict__ pa, const int *__restrict__ pb,
+                                                  const double *__restrict__ a, const double *__restrict__ b,
+                                                  double *__restrict__ out64, float *__restrict__ out32) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    double s = 0.0;
+    for (int k = ptr[e]; k < ptr[e + 1]; ++k) s += a[pa[k]] * b[pb[k]];
+    out64[e] = s;
+    if (out32) out32[e] = (float)s;
+}
+
 // Dense inverse of each instance's coarsest operator (Gauss-Jordan, no
 // pivoting: the Galerkin operators are symmetric positive definite).
 __global__ __launch_bounds__(256) void k_coarse_inverse(DevAmgLevel L, const int *__restrict__ inst_ptr,
@@ -854,100 +882,218 @@ __device__ __forceinline__ const VT *lvl_dinv(const DevAmgLevel &L) {
 // Coarse-level rows with batched loads: the column indices of kChunk entries
 // are loaded together, then everything that depends on them, so a row costs a
 // few memory round trips instead of one dependent chain per entry (the small
-// levels are latency-bound).  Padding lanes use the row itself with a zero
-// coefficient, so every address stays valid.
+// levels are latency-bound).  Padding lanes either read the row's own entries
+// or issue no load at all: never a row of another instance, which the
+// one-workgroup-per-instance tail may not have written yet (0 * NaN != 0).
 constexpr int kChunk = 8;
 
+// Rows can be shared by a group of G lanes (G divides 64, so a group lies in
+// one wave): lane `sub` of the group takes entries e0 + sub, e0 + sub + G, ...
+// in chunks of KC, and group_sum adds the partial sums across the group, after
+// which every lane of the group holds the row's sum.  G = 1 is one lane per row.
+template <int G>
+constexpr int chunk_for() { return G == 1 ? kChunk : 4; }
+
+template <int G, typename VT, int NRHS>
+__device__ __forceinline__ void group_sum(VT (&acc)[NRHS]) {
+#pragma unroll
+    for (int m = G / 2; m >= 1; m >>= 1)
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) acc[k] += __shfl_xor(acc[k], m, G);
+}
+
 // acc = sum_j a_ij (w d_j b_j): the residual of the pre-smoothed x = w D^-1 b.
-template <typename VT, int NRHS>
+template <typename VT, int NRHS, int G = 1>
 __device__ __forceinline__ void row_down(const DevAmgLevel &F, const VT *__restrict__ val,
                                          const VT *__restrict__ dinv, const VT *__restrict__ b, VT w, int i,
-                                         VT (&acc)[NRHS]) {
+                                         VT (&acc)[NRHS], int sub = 0) {
+    constexpr int KC = chunk_for<G>();
 #pragma unroll
     for (int k = 0; k < NRHS; ++k) acc[k] = VT(0);
     const int e1 = F.row_ptr[i + 1];
-    for (int base = F.row_ptr[i]; base < e1; base += kChunk) {
-        int c[kChunk];
-        VT a[kChunk];
+    for (int base = F.row_ptr[i] + sub; base < e1; base += G * KC) {
+        int c[KC];
+        VT a[KC];
 #pragma unroll
-        for (int u = 0; u < kChunk; ++u) {
-            const bool in = base + u < e1;
-            c[u] = in ? F.col[base + u] : i;
-            a[u] = in ? val[base + u] : VT(0);
+        for (int u = 0; u < KC; ++u) {
+            const int e = base + G * u;
+            const bool in = e < e1;
+            c[u] = in ? F.col[e] : i;
+            a[u] = in ? val[e] : VT(0);
         }
-        VT d[kChunk], bv[kChunk][NRHS];
+        VT d[KC], bv[KC][NRHS];
 #pragma unroll
-        for (int u = 0; u < kChunk; ++u) {
+        for (int u = 0; u < KC; ++u) {
             d[u] = dinv[c[u]];
 #pragma unroll
             for (int k = 0; k < NRHS; ++k) bv[u][k] = b[(size_t)c[u] * NRHS + k];
         }
 #pragma unroll
-        for (int u = 0; u < kChunk; ++u)
+        for (int u = 0; u < KC; ++u)
 #pragma unroll
             for (int k = 0; k < NRHS; ++k) acc[k] += a[u] * (w * d[u]) * bv[u][k];
     }
+    group_sum<G>(acc);
 }
 
 // acc = sum_j a_ij x~_j with x~_j = w d_j b_j + alpha xc[agg_j] (prolongation on the fly).
-template <typename VT, int NRHS>
+template <typename VT, int NRHS, int G = 1>
 __device__ __forceinline__ void row_up1(const DevAmgLevel &F, const VT *__restrict__ val,
                                         const VT *__restrict__ dinv, const VT *__restrict__ b,
                                         const int *__restrict__ agg, const VT *__restrict__ xc, VT w, VT al,
-                                        int i, VT (&acc)[NRHS]) {
+                                        int i, VT (&acc)[NRHS], int sub = 0) {
+    constexpr int KC = chunk_for<G>();
 #pragma unroll
     for (int k = 0; k < NRHS; ++k) acc[k] = VT(0);
     const int e1 = F.row_ptr[i + 1];
-    for (int base = F.row_ptr[i]; base < e1; base += kChunk) {
-        int c[kChunk];
-        VT a[kChunk];
+    for (int base = F.row_ptr[i] + sub; base < e1; base += G * KC) {
+        int c[KC];
+        VT a[KC];
 #pragma unroll
-        for (int u = 0; u < kChunk; ++u) {
-            const bool in = base + u < e1;
-            c[u] = in ? F.col[base + u] : i;
-            a[u] = in ? val[base + u] : VT(0);
+        for (int u = 0; u < KC; ++u) {
+            const int e = base + G * u;
+            const bool in = e < e1;
+            c[u] = in ? F.col[e] : i;
+            a[u] = in ? val[e] : VT(0);
         }
-        VT d[kChunk], bv[kChunk][NRHS];
-        int g[kChunk];
+        VT d[KC], bv[KC][NRHS];
+        int g[KC];
 #pragma unroll
-        for (int u = 0; u < kChunk; ++u) {
+        for (int u = 0; u < KC; ++u) {
             d[u] = dinv[c[u]];
             g[u] = agg[c[u]];
 #pragma unroll
             for (int k = 0; k < NRHS; ++k) bv[u][k] = b[(size_t)c[u] * NRHS + k];
         }
-        VT xg[kChunk][NRHS];
+        VT xg[KC][NRHS];
 #pragma unroll
-        for (int u = 0; u < kChunk; ++u)
+        for (int u = 0; u < KC; ++u)
 #pragma unroll
             for (int k = 0; k < NRHS; ++k) xg[u][k] = xc[(size_t)g[u] * NRHS + k];
 #pragma unroll
-        for (int u = 0; u < kChunk; ++u)
+        for (int u = 0; u < KC; ++u)
 #pragma unroll
             for (int k = 0; k < NRHS; ++k) acc[k] += a[u] * (w * d[u] * bv[u][k] + al * xg[u][k]);
     }
+    group_sum<G>(acc);
 }
 
 // acc = sum over the children of coarse row I of res.
-template <typename VT, int NRHS>
+template <typename VT, int NRHS, int G = 1>
 __device__ __forceinline__ void row_restrict(const int *__restrict__ child_ptr, const int *__restrict__ child,
-                                             const VT *__restrict__ res, int I, VT (&acc)[NRHS]) {
+                                             const VT *__restrict__ res, int I, VT (&acc)[NRHS], int sub = 0) {
+    constexpr int KC = chunk_for<G>();
 #pragma unroll
     for (int k = 0; k < NRHS; ++k) acc[k] = VT(0);
     const int e1 = child_ptr[I + 1];
-    for (int base = child_ptr[I]; base < e1; base += kChunk) {
-        int c[kChunk];
+    for (int base = child_ptr[I] + sub; base < e1; base += G * KC) {
+        int c[KC];
 #pragma unroll
-        for (int u = 0; u < kChunk; ++u) c[u] = base + u < e1 ? child[base + u] : -1;
-        VT r[kChunk][NRHS];
+        for (int u = 0; u < KC; ++u) c[u] = base + G * u < e1 ? child[base + G * u] : -1;
+        VT r[KC][NRHS];
 #pragma unroll
-        for (int u = 0; u < kChunk; ++u)
+        for (int u = 0; u < KC; ++u)
 #pragma unroll
             for (int k = 0; k < NRHS; ++k) r[u][k] = c[u] >= 0 ? res[(size_t)c[u] * NRHS + k] : VT(0);
 #pragma unroll
-        for (int u = 0; u < kChunk; ++u)
+        for (int u = 0; u < KC; ++u)
 #pragma unroll
             for (int k = 0; k < NRHS; ++k) acc[k] += r[u][k];
+    }
+    group_sum<G>(acc);
+}
+
+template <typename VT>
+__device__ __forceinline__ const VT *lvl_p(const DevAmgLevel &L) {
+    if constexpr (std::is_same<VT, float>::value) return L.p_val32; else return L.p_val64;
+}
+template <typename VT>
+__device__ __forceinline__ const VT *lvl_t(const DevAmgLevel &L) {
+    if constexpr (std::is_same<VT, float>::value) return L.t_val32; else return L.t_val64;
+}
+
+// acc = (P' res)_I across a smoothed transfer (P' row I: fine rows and P positions).
+template <typename VT, int NRHS, int G = 1>
+__device__ __forceinline__ void row_restrict_sa(const DevAmgLevel &F, const VT *__restrict__ res, int I,
+                                                VT (&acc)[NRHS], int sub = 0) {
+    constexpr int KC = chunk_for<G>();
+    const VT *pval = lvl_p<VT>(F);
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) acc[k] = VT(0);
+    const int e1 = F.pt_ptr[I + 1];
+    for (int base = F.pt_ptr[I] + sub; base < e1; base += G * KC) {
+        int c[KC];
+        VT pv[KC];
+#pragma unroll
+        for (int u = 0; u < KC; ++u) {
+            const int e = base + G * u;
+            const bool in = e < e1;
+            c[u] = in ? F.pt_col[e] : 0;
+            pv[u] = in ? pval[F.pt_map[e]] : VT(0);
+        }
+        VT r[KC][NRHS];
+#pragma unroll
+        for (int u = 0; u < KC; ++u)
+#pragma unroll
+            for (int k = 0; k < NRHS; ++k) r[u][k] = base + G * u < e1 ? res[(size_t)c[u] * NRHS + k] : VT(0);
+#pragma unroll
+        for (int u = 0; u < KC; ++u)
+#pragma unroll
+            for (int k = 0; k < NRHS; ++k) acc[k] += pv[u] * r[u][k];
+    }
+    group_sum<G>(acc);
+}
+
+// acc = sum over the CSR row [e0, e1) of val[e] * x[col[e]] (batched loads).
+template <typename VT, int NRHS, int G = 1>
+__device__ __forceinline__ void row_gather(const int *__restrict__ col, const VT *__restrict__ val, int e0,
+                                           int e1, const VT *__restrict__ x, VT (&acc)[NRHS], int sub = 0) {
+    constexpr int KC = chunk_for<G>();
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) acc[k] = VT(0);
+    for (int base = e0 + sub; base < e1; base += G * KC) {
+        int c[KC];
+        VT a[KC];
+#pragma unroll
+        for (int u = 0; u < KC; ++u) {
+            const int e = base + G * u;
+            const bool in = e < e1;
+            c[u] = in ? col[e] : 0;
+            a[u] = in ? val[e] : VT(0);
+        }
+        VT xv[KC][NRHS];
+#pragma unroll
+        for (int u = 0; u < KC; ++u)
+#pragma unroll
+            for (int k = 0; k < NRHS; ++k) xv[u][k] = base + G * u < e1 ? x[(size_t)c[u] * NRHS + k] : VT(0);
+#pragma unroll
+        for (int u = 0; u < KC; ++u)
+#pragma unroll
+            for (int k = 0; k < NRHS; ++k) acc[k] += a[u] * xv[u][k];
+    }
+    group_sum<G>(acc);
+}
+
+// Up-sweep across a smoothed transfer without a second pass over A: with
+// x~ = w D^-1 b + al P xc and A (w D^-1 b) = b - res (the down sweep),
+//   x_i = x~_i + w d_i (res_i - al (T xc)_i),  T = A P.
+template <typename VT, int NRHS, int G = 1>
+__device__ __forceinline__ void up_sa_row(const DevAmgLevel &F, const VT *__restrict__ xc, VT w, VT al, int i,
+                                          int sub = 0) {
+    const VT *dinv = lvl_dinv<VT>(F);
+    const VT *b = (const VT *)F.b;
+    const VT *res = (const VT *)F.res;
+    VT *xo = (VT *)F.x;
+    VT px[NRHS], tx[NRHS];
+    row_gather<VT, NRHS, G>(F.p_col, lvl_p<VT>(F), F.p_ptr[i], F.p_ptr[i + 1], xc, px, sub);
+    row_gather<VT, NRHS, G>(F.t_col, lvl_t<VT>(F), F.t_ptr[i], F.t_ptr[i + 1], xc, tx, sub);
+    if (sub != 0) return;
+    const VT d = w * dinv[i];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) {
+        const size_t q = (size_t)i * NRHS + k;
+        const VT xt = d * b[q] + al * px[k];
+        xo[q] = xt + d * (res[q] - al * tx[k]);
     }
 }
 
@@ -1043,7 +1189,8 @@ __global__ __launch_bounds__(TB) void k_vc0_post(DevTopo T, const long long *__r
 #pragma unroll
             for (int u = 0; u < kChunk; ++u)
 #pragma unroll
-                for (int k = 0; k < NRHS; ++k) acc[k] += a[u] * xc[(size_t)c[u] * NRHS + k];
+                for (int k = 0; k < NRHS; ++k)
+                    acc[k] += e0 + u < wd ? a[u] * xc[(size_t)c[u] * NRHS + k] : VT(0);
         }
         const int g = agg[f];
         const VT d = w * dinv[f];
@@ -1071,15 +1218,17 @@ __global__ __launch_bounds__(256) void k_vc0_res(DevTopo T, const VT *__restrict
     for (int k = 0; k < NRHS; ++k) res[(size_t)f * NRHS + k] = b[(size_t)f * NRHS + k] - acc[k];
 }
 
-// Coarse levels (CSR).
-template <typename VT, int NRHS>
+// Coarse levels (CSR), G lanes per row.
+template <typename VT, int NRHS, int G = 1>
 __global__ __launch_bounds__(256) void k_vcc_down(DevAmgLevel L, const VT *__restrict__ b,
                                                   VT *__restrict__ x, VT *__restrict__ res, VT w) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int i = t / G, sub = t % G;
     if (i >= L.n) return;
     const VT *dinv = lvl_dinv<VT>(L);
     VT acc[NRHS];
-    row_down<VT, NRHS>(L, lvl_val<VT>(L), dinv, b, w, i, acc);
+    row_down<VT, NRHS, G>(L, lvl_val<VT>(L), dinv, b, w, i, acc, sub);
+    if (sub != 0) return;
     const VT d = w * dinv[i];
 #pragma unroll
     for (int k = 0; k < NRHS; ++k) {
@@ -1167,16 +1316,18 @@ __global__ __launch_bounds__(256) void k_prolong0(const int *__restrict__ agg, c
 
 // Coarse level up with one sweep: x~ = w D^-1 b + alpha P xc (on the fly),
 // xo = x~ + w D^-1 (b - A x~).
-template <typename VT, int NRHS>
+template <typename VT, int NRHS, int G = 1>
 __global__ __launch_bounds__(256) void k_vcc_up1(DevAmgLevel L, const int *__restrict__ agg,
                                                  const VT *__restrict__ xc, VT *__restrict__ xo, VT w,
                                                  VT alpha) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int i = t / G, sub = t % G;
     if (i >= L.n) return;
     const VT *dinv = lvl_dinv<VT>(L);
     const VT *b = (const VT *)L.b;
     VT acc[NRHS];
-    row_up1<VT, NRHS>(L, lvl_val<VT>(L), dinv, b, agg, xc, w, alpha, i, acc);
+    row_up1<VT, NRHS, G>(L, lvl_val<VT>(L), dinv, b, agg, xc, w, alpha, i, acc, sub);
+    if (sub != 0) return;
     const VT d = w * dinv[i];
     const int g = agg[i];
 #pragma unroll
@@ -1187,15 +1338,50 @@ __global__ __launch_bounds__(256) void k_vcc_up1(DevAmgLevel L, const int *__res
     }
 }
 
-// Restriction b_c = P' res: sum over the coarse row's children.
+// x~ = x + al P xc across a smoothed transfer (the path with several sweeps).
 template <typename VT, int NRHS>
+__global__ __launch_bounds__(256) void k_prolong_sa(DevAmgLevel F, const VT *__restrict__ x,
+                                                    const VT *__restrict__ xc, VT al, VT *__restrict__ xo) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= F.n) return;
+    VT px[NRHS];
+    row_gather<VT, NRHS>(F.p_col, lvl_p<VT>(F), F.p_ptr[i], F.p_ptr[i + 1], xc, px);
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) xo[(size_t)i * NRHS + k] = x[(size_t)i * NRHS + k] + al * px[k];
+}
+
+// Restriction across a smoothed transfer: b_c = P' res.
+template <typename VT, int NRHS, int G = 1>
+__global__ __launch_bounds__(256) void k_restrict_sa(DevAmgLevel F, int nc, VT *__restrict__ bc) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int I = t / G, sub = t % G;
+    if (I >= nc) return;
+    VT acc[NRHS];
+    row_restrict_sa<VT, NRHS, G>(F, (const VT *)F.res, I, acc, sub);
+    if (sub != 0) return;
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) bc[(size_t)I * NRHS + k] = acc[k];
+}
+
+template <typename VT, int NRHS, int G = 1>
+__global__ __launch_bounds__(256) void k_vcc_up_sa(DevAmgLevel F, const VT *__restrict__ xc, VT w, VT al) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int i = t / G;
+    if (i >= F.n) return;
+    up_sa_row<VT, NRHS, G>(F, xc, w, al, i, t % G);
+}
+
+// Restriction b_c = P' res: sum over the coarse row's children.
+template <typename VT, int NRHS, int G = 1>
 __global__ __launch_bounds__(256) void k_restrict(int n, const int *__restrict__ child_ptr,
                                                   const int *__restrict__ child,
                                                   const VT *__restrict__ res, VT *__restrict__ bc) {
-    const int I = blockIdx.x * 256 + threadIdx.x;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int I = t / G, sub = t % G;
     if (I >= n) return;
     VT acc[NRHS];
-    row_restrict<VT, NRHS>(child_ptr, child, res, I, acc);
+    row_restrict<VT, NRHS, G>(child_ptr, child, res, I, acc, sub);
+    if (sub != 0) return;
 #pragma unroll
     for (int k = 0; k < NRHS; ++k) bc[(size_t)I * NRHS + k] = acc[k];
 }
@@ -1222,7 +1408,9 @@ __global__ __launch_bounds__(256) void k_coarse_solve(const int *__restrict__ in
 // k_coarse_solve and k_vcc_up1, with workgroup barriers between the steps
 // instead of kernel launches.  Reads lev[lt].b, writes lev[l].x for l >= lt.
 template <typename VT, int NRHS>
-__global__ __launch_bounds__(1024) void k_vcycle_tail(DevAmg A, int lt, VT w, VT al) {
+__global__ __launch_bounds__(1024) void k_vcycle_tail(const DevAmg *__restrict__ Ap, int lt, VT w, VT al,
+                                                      VT als) {
+    const DevAmg &A = *Ap;
     const int s = blockIdx.x;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int L = A.nlev;
@@ -1244,7 +1432,8 @@ __global__ __launch_bounds__(1024) void k_vcycle_tail(DevAmg A, int lt, VT w, VT
         VT *bc = (VT *)C.b;
         for (int I = C.inst_ptr[s] + tid; I < C.inst_ptr[s + 1]; I += nt) {
             VT acc[NRHS];
-            row_restrict<VT, NRHS>(C.child_ptr, C.child, res, I, acc);
+            if (F.sa_out) row_restrict_sa<VT, NRHS>(F, res, I, acc);
+            else row_restrict<VT, NRHS>(C.child_ptr, C.child, res, I, acc);
 #pragma unroll
             for (int k = 0; k < NRHS; ++k) bc[(size_t)I * NRHS + k] = acc[k];
         }
@@ -1270,6 +1459,11 @@ __global__ __launch_bounds__(1024) void k_vcycle_tail(DevAmg A, int lt, VT w, VT
         const DevAmgLevel &F = A.lev[l];
         const int *agg = A.lev[l + 1].agg;
         const VT *xc = (const VT *)A.lev[l + 1].x;
+        if (F.sa_out) {
+            for (int i = F.inst_ptr[s] + tid; i < F.inst_ptr[s + 1]; i += nt) up_sa_row<VT, NRHS>(F, xc, w, als, i);
+            __syncthreads();
+            continue;
+        }
         const VT *val = lvl_val<VT>(F);
         const VT *dinv = lvl_dinv<VT>(F);
         const VT *b = (const VT *)F.b;
@@ -1841,6 +2035,26 @@ static hipError_t with_tb(int tb, Fn fn) {
     return hipGetLastError();
 }
 
+// Lanes per row of a coarse-level row loop over `nnz` entries in `n` rows:
+// up to about eight entries per lane keep one lane per row; longer rows (the
+// levels behind a smoothed transfer) spread over a group of lanes, which also
+// gives the small levels enough workgroups to cover the chip.
+static inline int lanes_per_row(long long nnz, int n) {
+    if (n <= 0) return 1;
+    const long long avg = (nnz + n - 1) / n;
+    return avg > 32 ? 16 : avg > 16 ? 8 : avg > 8 ? 4 : 1;
+}
+template <class Fn>
+static void with_lanes(int g, Fn fn) {
+    switch (g) {
+        case 4: fn(IC<4>{}); break;
+        case 8: fn(IC<8>{}); break;
+        case 16: fn(IC<16>{}); break;
+        default: fn(IC<1>{}); break;
+    }
+}
+static inline unsigned row_blocks(int n, int g) { return (unsigned)(((long long)n * g + 255) / 256); }
+
 hipError_t launch_static_ip(hipStream_t st, const DevTopo &T, const DevGeom &G, int maxn,
                             const double *perm, const double *grav, double *binv_pk,
                             double *gflux) {
@@ -1971,13 +2185,28 @@ hipError_t launch_amg_setup(hipStream_t st, const DevTopo &T, const DevAmg &A, c
                             bool mixed) {
     for (int l = 1; l < A.nlev; ++l) {
         const DevAmgLevel &L = A.lev[l];
-        const double *fine = (l == 1) ? val64 : A.lev[l - 1].val64;
-        if (L.nnz > 0)
-            k_galerkin<<<(L.nnz + 255) / 256, 256, 0, st>>>(L.nnz, L.src_ptr, L.src, fine, L.val64,
-                                                           mixed ? L.val32 : nullptr);
+        const DevAmgLevel &F = A.lev[l - 1];
+        if (L.nnz > 0) {
+            if (L.sa_in)
+                k_pair_sum<<<(L.nnz + 255) / 256, 256, 0, st>>>(L.nnz, L.src_ptr, L.src, L.src2, F.p_val64,
+                                                               F.t_val64, L.val64, mixed ? L.val32 : nullptr);
+            else
+                k_galerkin<<<(L.nnz + 255) / 256, 256, 0, st>>>(L.nnz, L.src_ptr, L.src,
+                                                               (l == 1) ? val64 : F.val64, L.val64,
+                                                               mixed ? L.val32 : nullptr);
+        }
         if (L.n > 0)
             k_coarse_diag<<<(L.n + 255) / 256, 256, 0, st>>>(L.n, L.diag, L.val64, L.dinv64,
                                                             mixed ? L.dinv32 : nullptr);
+        if (L.sa_out && L.n > 0) {
+            DevAmgLevel Lw = L;
+            if (!mixed) Lw.p_val32 = nullptr;
+            k_sa_p<<<(L.n + 255) / 256, 256, 0, st>>>(Lw, A.omega_p);
+            if (L.t_nnz > 0)
+                k_pair_sum<<<(L.t_nnz + 255) / 256, 256, 0, st>>>(L.t_nnz, L.t_pair_ptr, L.t_pa, L.t_pb, L.val64,
+                                                                 L.p_val64, L.t_val64,
+                                                                 mixed ? L.t_val32 : nullptr);
+        }
     }
     if (A.ap_slots > 0)
         k_galerkin<<<(unsigned)((A.ap_slots + 255) / 256), 256, 0, st>>>((int)A.ap_slots, A.ap_src_ptr, A.ap_src,
@@ -2044,7 +2273,11 @@ static void vcycle_impl(hipStream_t st, const DevTopo &T, const DevAmg &A, const
     k_vc0_res<VT, NR><<<rows_blocks, 256, 0, st>>>(T, val0, r, x0, res0);
     {
         const DevAmgLevel &C = A.lev[1];
-        k_restrict<VT, NR><<<(C.n + 255) / 256, 256, 0, st>>>(C.n, C.child_ptr, C.child, res0, (VT *)C.b);
+        with_lanes(lanes_per_row(T.nfp, C.n), [&](auto g) {
+            constexpr int Gv = decltype(g)::value;
+            k_restrict<VT, NR, Gv><<<row_blocks(C.n, Gv), 256, 0, st>>>(C.n, C.child_ptr, C.child, res0,
+                                                                       (VT *)C.b);
+        });
     }
     VT *xcur[kMaxLevels], *xalt[kMaxLevels];
     for (int l = 1; l < L; ++l) {
@@ -2055,18 +2288,44 @@ static void vcycle_impl(hipStream_t st, const DevTopo &T, const DevAmg &A, const
         // first level handled by the workgroup-per-instance tail kernel
         int lt = L - 1;
         while (lt > 1 && A.max_rows[lt - 1] <= T.tail_rows) --lt;
+        const VT als = (VT)A.alpha_sa;
         for (int l = 1; l < lt; ++l) {
             const DevAmgLevel &F = A.lev[l];
             const DevAmgLevel &C = A.lev[l + 1];
-            k_vcc_down<VT, NR><<<(F.n + 255) / 256, 256, 0, st>>>(F, (const VT *)F.b, (VT *)F.x, (VT *)F.res, w);
-            k_restrict<VT, NR><<<(C.n + 255) / 256, 256, 0, st>>>(C.n, C.child_ptr, C.child, (const VT *)F.res,
-                                                                  (VT *)C.b);
+            with_lanes(lanes_per_row(F.nnz, F.n), [&](auto g) {
+                constexpr int Gv = decltype(g)::value;
+                k_vcc_down<VT, NR, Gv><<<row_blocks(F.n, Gv), 256, 0, st>>>(F, (const VT *)F.b, (VT *)F.x,
+                                                                          (VT *)F.res, w);
+            });
+            if (F.sa_out) {
+                with_lanes(lanes_per_row(F.p_nnz, C.n), [&](auto g) {
+                    constexpr int Gv = decltype(g)::value;
+                    k_restrict_sa<VT, NR, Gv><<<row_blocks(C.n, Gv), 256, 0, st>>>(F, C.n, (VT *)C.b);
+                });
+            } else {
+                with_lanes(lanes_per_row(F.n, C.n), [&](auto g) {
+                    constexpr int Gv = decltype(g)::value;
+                    k_restrict<VT, NR, Gv><<<row_blocks(C.n, Gv), 256, 0, st>>>(C.n, C.child_ptr, C.child,
+                                                                               (const VT *)F.res, (VT *)C.b);
+                });
+            }
         }
-        k_vcycle_tail<VT, NR><<<A.ninst, 1024, 0, st>>>(A, lt, w, al);
+        k_vcycle_tail<VT, NR><<<A.ninst, 1024, 0, st>>>(A.dev, lt, w, al, als);
         for (int l = lt - 1; l >= 1; --l) {
             const DevAmgLevel &F = A.lev[l];
-            k_vcc_up1<VT, NR><<<(F.n + 255) / 256, 256, 0, st>>>(F, A.lev[l + 1].agg, (const VT *)A.lev[l + 1].x,
-                                                                 (VT *)F.x, w, al);
+            const VT *xc = (const VT *)A.lev[l + 1].x;
+            if (F.sa_out) {
+                with_lanes(lanes_per_row(std::max(F.p_nnz, F.t_nnz), F.n), [&](auto g) {
+                    constexpr int Gv = decltype(g)::value;
+                    k_vcc_up_sa<VT, NR, Gv><<<row_blocks(F.n, Gv), 256, 0, st>>>(F, xc, w, als);
+                });
+            } else {
+                with_lanes(lanes_per_row(F.nnz, F.n), [&](auto g) {
+                    constexpr int Gv = decltype(g)::value;
+                    k_vcc_up1<VT, NR, Gv><<<row_blocks(F.n, Gv), 256, 0, st>>>(F, A.lev[l + 1].agg, xc, (VT *)F.x,
+                                                                             w, al);
+                });
+            }
         }
         xcur[1] = (VT *)A.lev[1].x;
     } else {
@@ -2082,8 +2341,11 @@ static void vcycle_impl(hipStream_t st, const DevTopo &T, const DevAmg &A, const
                 std::swap(xcur[l], xalt[l]);
             }
             if (nu > 1) k_vcc_res<VT, NR><<<nb, 256, 0, st>>>(F, (const VT *)F.b, xcur[l], (VT *)F.res);
-            k_restrict<VT, NR><<<(C.n + 255) / 256, 256, 0, st>>>(C.n, C.child_ptr, C.child, (const VT *)F.res,
-                                                                  (VT *)C.b);
+            if (F.sa_out)
+                k_restrict_sa<VT, NR><<<(C.n + 255) / 256, 256, 0, st>>>(F, C.n, (VT *)C.b);
+            else
+                k_restrict<VT, NR><<<(C.n + 255) / 256, 256, 0, st>>>(C.n, C.child_ptr, C.child,
+                                                                      (const VT *)F.res, (VT *)C.b);
         }
         {
             const DevAmgLevel &C = A.lev[L - 1];
@@ -2101,8 +2363,15 @@ static void vcycle_impl(hipStream_t st, const DevTopo &T, const DevAmg &A, const
                 std::swap(xcur[l], xalt[l]);
                 continue;
             }
-            k_vcc_up<VT, NR><<<nb, 256, 0, st>>>(F, A.lev[l + 1].agg, (const VT *)F.b, xcur[l], xcur[l + 1],
-                                                 xalt[l], w, al);
+            if (F.sa_out) {
+                k_prolong_sa<VT, NR><<<nb, 256, 0, st>>>(F, xcur[l], xcur[l + 1], (VT)A.alpha_sa, xalt[l]);
+                std::swap(xcur[l], xalt[l]);
+                k_vcc_up<VT, NR><<<nb, 256, 0, st>>>(F, nullptr, (const VT *)F.b, xcur[l], nullptr, xalt[l], w,
+                                                     VT(0));
+            } else {
+                k_vcc_up<VT, NR><<<nb, 256, 0, st>>>(F, A.lev[l + 1].agg, (const VT *)F.b, xcur[l],
+                                                     xcur[l + 1], xalt[l], w, al);
+            }
             std::swap(xcur[l], xalt[l]);
             for (int it = 1; it < nu; ++it) {
                 k_vcc_up<VT, NR><<<nb, 256, 0, st>>>(F, nullptr, (const VT *)F.b, xcur[l], nullptr, xalt[l], w,
