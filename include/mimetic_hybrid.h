@@ -78,12 +78,16 @@ typedef struct {
     int chunk;                 /* PCG iterations per captured graph launch */
     double replace_factor;     /* mixed: replace the FP32 recursive residual by
                                   the FP64 true residual once it has dropped by
-                                  this factor since the last replacement */
+                                  this factor since the last replacement
+                                  (default 0.01) */
     int use_graph;             /* capture PCG chunks in a hipGraph */
     int precond;               /* -1: from linsolver_type (0 -> Jacobi, 1..3 -> multigrid
                                   when logical cell indices were given); 0: Jacobi;
                                   1: aggregation multigrid V-cycle */
-    double smoother_weight;    /* Jacobi weight of the multigrid smoother */
+    double smoother_weight;    /* > 0: Jacobi weight of the multigrid smoother on every
+                                  level; 0 (default): 4 / (3 rho(D^-1 A)) per level,
+                                  rho from Lanczos steps on the first multigrid solve
+                                  of each hierarchy */
     int autotune;              /* 1 (default): on the first solve of a structure (per
                                   nrhs, precision and preconditioner) time every
                                   candidate launch shape of the PCG SpMV and of the
@@ -112,6 +116,9 @@ typedef struct {
     int sweep_threads, sweep_xcd; /* same for the multigrid smoothing sweep */
     int tail_rows;                /* coarse-level size limit of the one-workgroup tail */
     double tune_ms;               /* time spent autotuning in this call */
+    double smoother_weights[8];   /* multigrid Jacobi weight of levels 0..7 (0 beyond) */
+    int coarsening[8][3];         /* logical block factors of the transfers from levels
+                                     0..7 (1 or 2 along i, j, k; 0 beyond) */
 } mhs_solve_report;
 
 typedef struct {

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cmath>
 #include <numeric>
 
 namespace mhs {
@@ -206,32 +207,82 @@ void sa_transfer(AmgLevel &F, AmgLevel &C) {
     }
 }
 
+// Block factors for one transfer: 2 along the coarsenable directions whose
+// strength is at least theta times the strongest coarsenable one.
+std::array<int, 3> choose_blocks(const Strength &s, double theta, const std::vector<int> &cur) {
+    const int ninst = int(cur.size() / 3);
+    bool can[3] = {false, false, false};
+    for (int t = 0; t < ninst; ++t)
+        for (int d = 0; d < 3; ++d) can[d] = can[d] || cur[3 * t + d] > 1;
+    double smax = 0.0;
+    for (int d = 0; d < 3; ++d)
+        if (can[d]) smax = std::max(smax, s[d]);
+    std::array<int, 3> b{1, 1, 1};
+    for (int d = 0; d < 3; ++d)
+        if (can[d] && (!(smax > 0.0) || s[d] >= theta * smax)) b[d] = 2;
+    return b;
+}
+
 }  // namespace
 
-std::string build_amg(AmgHierarchy &H, const Topology &T, const std::vector<int> &ijk, bool smoothed) {
-    H = AmgHierarchy();
+std::string instance_dims(const Topology &T, const std::vector<int> &ijk, std::vector<int> &dims) {
     const int ninst = T.ninst;
     if ((int)ijk.size() != 3 * T.nc) return "logical cell indices: need 3 per cell";
-    if (T.slice_ptr[T.nslices] > 0x7fffffffLL)
-        return "face matrix too large for the 32-bit multigrid gather lists";
-    // instance dimensions from the logical indices
-    std::vector<int> dims(3 * ninst, 0);
+    dims.assign(3 * ninst, 0);
     for (int t = 0; t < ninst; ++t)
         for (int c = T.inst_cellpos[t]; c < T.inst_cellpos[t + 1]; ++c)
             for (int d = 0; d < 3; ++d) {
                 if (ijk[3 * c + d] < 0) return "logical cell indices must be non-negative";
                 dims[3 * t + d] = std::max(dims[3 * t + d], ijk[3 * c + d] + 1);
             }
+    return "";
+}
+
+std::vector<std::array<int, 3>> coarsening_plan(const Strength &strength, const std::vector<int> &dims) {
+    const int ninst = int(dims.size() / 3);
+    Strength s = strength;
+    for (double &v : s)
+        if (!(v > 0.0) || !std::isfinite(v)) v = 0.0;
+    std::vector<int> cur = dims;
+    std::vector<std::array<int, 3>> plan;
+    for (;;) {
+        if (!plan.empty()) {
+            long long mx = 0;
+            for (int t = 0; t < ninst; ++t)
+                mx = std::max(mx, (long long)cur[3 * t] * cur[3 * t + 1] * cur[3 * t + 2]);
+            if (mx <= kMaxCoarsest || plan.size() >= 40) break;
+        }
+        const std::array<int, 3> b = choose_blocks(s, plan.empty() ? 0.5 : 0.25, cur);
+        plan.push_back(b);
+        for (int t = 0; t < ninst; ++t)
+            for (int d = 0; d < 3; ++d) cur[3 * t + d] = (cur[3 * t + d] + b[d] - 1) / b[d];
+        const Strength s0 = s;
+        for (int d = 0; d < 3; ++d) s[d] = s0[d] * b[(d + 1) % 3] * b[(d + 2) % 3] / b[d];
+    }
+    return plan;
+}
+
+std::string build_amg(AmgHierarchy &H, const Topology &T, const std::vector<int> &ijk, const Strength &strength,
+                      bool smoothed) {
+    H = AmgHierarchy();
+    const int ninst = T.ninst;
+    if (T.slice_ptr[T.nslices] > 0x7fffffffLL)
+        return "face matrix too large for the 32-bit multigrid gather lists";
+    std::vector<int> dims;
+    std::string ed = instance_dims(T, ijk, dims);
+    if (!ed.empty()) return ed;
+    H.blocks = coarsening_plan(strength, dims);
     H.lev.emplace_back();  // level 0 is the SELL face matrix
-    // ---- level 1: faces -> 2x2x2 blocks of their owner cells
+    // ---- level 1: faces -> blocks of their owner cells
     {
+        const std::array<int, 3> &b0 = H.blocks[0];
         AmgLevel L1;
         L1.inst_ptr.assign(ninst + 1, 0);
         L1.dims.resize(3 * ninst);
         for (int t = 0; t < ninst; ++t) {
             long long m = 1;
             for (int d = 0; d < 3; ++d) {
-                L1.dims[3 * t + d] = (dims[3 * t + d] + 1) / 2;
+                L1.dims[3 * t + d] = (dims[3 * t + d] + b0[d] - 1) / b0[d];
                 m *= L1.dims[3 * t + d];
             }
             L1.inst_ptr[t + 1] = L1.inst_ptr[t] + int(m);
@@ -246,7 +297,7 @@ std::string build_amg(AmgHierarchy &H, const Topology &T, const std::vector<int>
             if (h < 0) continue;
             const int c = T.hf_cell[h], t = inst_of_cell[c];
             const int *bd = &L1.dims[3 * t];
-            const int I = ijk[3 * c] / 2, J = ijk[3 * c + 1] / 2, K = ijk[3 * c + 2] / 2;
+            const int I = ijk[3 * c] / b0[0], J = ijk[3 * c + 1] / b0[1], K = ijk[3 * c + 2] / b0[2];
             L1.agg[f] = L1.inst_ptr[t] + I + bd[0] * (J + bd[1] * K);
         }
         children(L1, T.nfp);
@@ -260,19 +311,17 @@ std::string build_amg(AmgHierarchy &H, const Topology &T, const std::vector<int>
         build_ap(H.ap, T, L1.agg);
         H.lev.push_back(std::move(L1));
     }
-    // ---- coarser levels: halve block indices until every instance is small
-    for (;;) {
+    // ---- coarser levels: group the blocks again until every instance is small
+    for (size_t l = 1; l < H.blocks.size(); ++l) {
         const AmgLevel &F = H.lev.back();
-        int mx = 0;
-        for (int t = 0; t < ninst; ++t) mx = std::max(mx, F.inst_ptr[t + 1] - F.inst_ptr[t]);
-        if (mx <= kMaxCoarsest) break;
+        const std::array<int, 3> &bl = H.blocks[l];
         AmgLevel C;
         C.inst_ptr.assign(ninst + 1, 0);
         C.dims.resize(3 * ninst);
         for (int t = 0; t < ninst; ++t) {
             long long m = 1;
             for (int d = 0; d < 3; ++d) {
-                C.dims[3 * t + d] = (F.dims[3 * t + d] + 1) / 2;
+                C.dims[3 * t + d] = (F.dims[3 * t + d] + bl[d] - 1) / bl[d];
                 m *= C.dims[3 * t + d];
             }
             C.inst_ptr[t + 1] = C.inst_ptr[t] + int(m);
@@ -284,7 +333,7 @@ std::string build_amg(AmgHierarchy &H, const Topology &T, const std::vector<int>
             for (int r = F.inst_ptr[t]; r < F.inst_ptr[t + 1]; ++r) {
                 const int loc = r - F.inst_ptr[t];
                 const int i = loc % fd[0], j = (loc / fd[0]) % fd[1], k = loc / (fd[0] * fd[1]);
-                C.agg[r] = C.inst_ptr[t] + i / 2 + cd[0] * (j / 2 + cd[1] * (k / 2));
+                C.agg[r] = C.inst_ptr[t] + i / bl[0] + cd[0] * (j / bl[1] + cd[1] * (k / bl[2]));
             }
         }
         AmgLevel &Fr = H.lev.back();
@@ -300,9 +349,10 @@ std::string build_amg(AmgHierarchy &H, const Topology &T, const std::vector<int>
             });
         }
         H.lev.push_back(std::move(C));
-        if (H.lev.size() > 40) return "multigrid hierarchy does not coarsen";
     }
     const AmgLevel &Lc = H.lev.back();
+    for (int t = 0; t < ninst; ++t)
+        if (Lc.inst_ptr[t + 1] - Lc.inst_ptr[t] > kMaxCoarsest) return "multigrid hierarchy does not coarsen";
     H.coarse_off.assign(ninst + 1, 0);
     for (int t = 0; t < ninst; ++t) {
         const int m = Lc.inst_ptr[t + 1] - Lc.inst_ptr[t];

@@ -1,14 +1,17 @@
 // Aggregation multigrid hierarchy for the face system (host-side set-up).
 //
 // Aggregates are geometric: a face belongs to its owner cell (the cell of its
-// first half-face), and cells are grouped in 2x2x2 blocks of their logical
-// (i, j, k) indices; every coarser level halves the block indices again.  The
-// coarse operators are algebraic, A_{l+1} = P' A_l P with the piecewise-
-// constant P of the aggregates.  Their sparsity and, for every coarse entry,
+// first half-face), and cells are grouped in blocks of their logical (i, j, k)
+// indices, 2 or 1 cells along each direction depending on the coupling
+// strength along it (semi-coarsening of anisotropic problems); every coarser
+// level groups the blocks again.  The coarse operators are algebraic,
+// A_{l+1} = P' A_l P with the piecewise-constant P of the aggregates (smoothed
+// on one transfer).  Their sparsity and, for every coarse entry,
 // the list of finer entries that sum into it are built here once per
 // structure, so that each solve forms the Galerkin products with a pure gather.
 #pragma once
 
+#include <array>
 #include <string>
 #include <vector>
 
@@ -59,17 +62,36 @@ struct ApPattern {
 
 struct AmgHierarchy {
     std::vector<AmgLevel> lev;      // lev[0] unused placeholder, lev[1..] coarse levels
+    std::vector<std::array<int, 3>> blocks;  // logical block factors of each transfer (l -> l+1)
     std::vector<int> coarse_off;    // [ninst+1] offsets of the dense coarsest inverses
     ApPattern ap;
     int nlev() const { return int(lev.size()); }
 };
 
+// Coupling strength along the three logical directions, e.g. the mean
+// two-point transmissibility of the faces between logical neighbours.  The
+// aggregates of every transfer are blocks of the finer level's logical grid,
+// 2 or 1 cells along each direction: a direction is coarsened when its
+// strength is at least half the strongest (a quarter below level 1), and the
+// strengths then scale like those of the coarse blocks (cross-section over
+// length).  Directions of extent 1 in every instance are never coarsened.
+using Strength = std::array<double, 3>;
+
+// Block factors of each transfer for these strengths and per-instance
+// logical dimensions dims[3 * ninst], until every instance has at most
+// kMaxCoarsest rows.
+std::vector<std::array<int, 3>> coarsening_plan(const Strength &strength, const std::vector<int> &dims);
+
+// Logical dimensions of every instance from ijk[3*nc] (instance-local,
+// 0-based).  Returns an empty string on success.
+std::string instance_dims(const Topology &T, const std::vector<int> &ijk, std::vector<int> &dims);
+
 // Builds the hierarchy from logical cell indices ijk[3*nc] (instance-local,
-// 0-based).  The transfer from the face matrix to level 1 is the tentative
-// (piecewise-constant) one; with smoothed = true the transfer from level 1
-// to level 2 is smoothed, the others stay tentative.  Returns an empty
-// string on success.
+// 0-based) with the blocks of coarsening_plan.  The transfer from the face
+// matrix to level 1 is the tentative (piecewise-constant) one; with smoothed
+// = true the transfer from level 1 to level 2 is smoothed, the others stay
+// tentative.  Returns an empty string on success.
 std::string build_amg(AmgHierarchy &H, const Topology &T, const std::vector<int> &ijk,
-                      bool smoothed = true);
+                      const Strength &strength = {1.0, 1.0, 1.0}, bool smoothed = true);
 
 }  // namespace mhs

@@ -36,7 +36,8 @@ struct GraphCache {
     hipGraphExec_t exec = nullptr;
     int precision = -1, nrhs = 0, chunk = 0, amg = -1, nu = -1;
     int shape[5] = {-1, -1, -1, -1, -1};
-    double tol2 = -1.0, rf2 = -1.0, omega = -1.0, alpha = -1.0;
+    double tol2 = -1.0, rf2 = -1.0, alpha = -1.0;
+    double omega[kMaxLevels] = {};
     const void *key = nullptr;
 };
 
@@ -79,6 +80,9 @@ struct mhs_solver {
     bool has_amg = false;
     AmgHierarchy H;
     DevAmg dA{};
+    std::vector<DevBuf> amg_bufs;       // structure of the hierarchy on the device
+    bool amg_planned = false;           // aggregate shapes chosen from the permeability
+    std::vector<double> rho;            // largest eigenvalue of D^-1 A per level (empty: not estimated)
     double *z64 = nullptr;
     float *z32 = nullptr;
 
@@ -116,6 +120,7 @@ struct mhs_solver {
     ~mhs_solver();
     void free_topology();
     void free_solve_buffers();
+    void free_amg();
 };
 
 namespace {
@@ -152,16 +157,25 @@ T *dupload(mhs_solver *s, std::vector<DevBuf> &list, const std::vector<T> &v, hi
 
 }  // namespace
 
+void mhs_solver::free_amg() {
+    for (auto &b : amg_bufs) (void)hipFree(b.p);
+    amg_bufs.clear();
+    amg_planned = false;
+    dA = DevAmg{};
+    has_amg = false;
+    H = AmgHierarchy();
+    rho.clear();
+    shapes.clear();
+}
+
 void mhs_solver::free_topology() {
+    free_amg();
     for (auto &b : topo_bufs) (void)hipFree(b.p);
     topo_bufs.clear();
     binv = gflux = perm = nullptr;
     dT = DevTopo{};
     shapes.clear();
     dG = DevGeom{};
-    dA = DevAmg{};
-    has_amg = false;
-    H = AmgHierarchy();
 }
 
 void mhs_solver::free_solve_buffers() {
@@ -207,10 +221,10 @@ void mhs_default_options(mhs_solve_options *o) {
     o->precision = MHS_FP64;
     o->nrhs = 1;
     o->chunk = 32;
-    o->replace_factor = 0.1;
+    o->replace_factor = 0.01;
     o->use_graph = 1;
     o->precond = -1;
-    o->smoother_weight = 0.7;
+    o->smoother_weight = 0.0;     // per level from the spectral radius
     o->autotune = 1;
     o->spmv_threads = 0;
     o->sweep_threads = 0;
@@ -307,47 +321,21 @@ int mhs_set_cell_ijk(mhs_solver *s, const int *ijk) {
     return MHS_OK;
 }
 
-int mhs_init_system_structure(mhs_solver *s, const int *hf_bc_type, const int *hf_periodic_partner) {
-    if (!s) return MHS_ERR_INVALID;
-    if (!s->grid_set) return fail(s, MHS_ERR_STATE, "init_system_structure: no grid (call set_grid)");
-    HIPCHECK(hipSetDevice(s->device));
-    HIPCHECK(hipStreamSynchronize(s->stream));
-    s->free_solve_buffers();
-    s->free_topology();
-    s->structure_valid = s->ip_valid = s->solved = false;
-    std::string e = build_topology(s->T, s->ninst, s->inst_cellpos.data(), s->nc, s->facepos.data(),
-                                   s->nb.data(), hf_bc_type, hf_periodic_partner);
-    if (!e.empty()) return fail(s, MHS_ERR_INVALID, "init_system_structure: " + e);
-    Topology &T = s->T;
+}  // extern "C"
+
+namespace {
+
+// (Re)builds the multigrid hierarchy for the logical cell indices and the
+// coupling strengths and uploads its structure (device buffers in amg_bufs).
+int build_amg_device(mhs_solver *s, const Strength &strength) {
+    s->free_amg();
+    const Topology &T = s->T;
     hipError_t he = hipSuccess;
-    auto &L = s->topo_bufs;
-    DevTopo &d = s->dT;
-    d.nc = T.nc; d.nhf = T.nhf; d.nfp = T.nfp; d.ninst = T.ninst; d.nslices = T.nslices;
-    d.nblk = T.nfp / kRowBlock;
-#define UP(field, vec)                                   \
-    do {                                                 \
-        d.field = dupload(s, L, T.vec, &he);             \
-        if (he != hipSuccess) return fail(s, MHS_ERR_HIP, "upload " #vec); \
-    } while (0)
-    UP(facepos, facepos); UP(hf_dof, hf_dof); UP(hf_cell, hf_cell); UP(hf_col, hf_col);
-    UP(hf_ppart, hf_ppart); UP(hf_twin, hf_twin); UP(hf_redirect, hf_redirect);
-    UP(row_hf, row_hf); UP(dof_hf, dof_hf); UP(ptn, ptn); UP(dslot, dslot); UP(pslot, pslot);
-    UP(slice_ptr, slice_ptr); UP(col, col); UP(slotmap, slotmap); UP(sq_pos, sq_pos);
-    UP(pk_pos, pk_pos); UP(block_seg, block_seg); UP(seg_blk, seg_blk); UP(inst_rowbase, inst_rowbase);
-#undef UP
-    DevGeom &g = s->dG;
-    g.cvol = dupload(s, L, s->cvol, &he);
-    if (he == hipSuccess) g.ccen = dupload(s, L, s->ccen, &he);
-    if (he == hipSuccess) g.farea = dupload(s, L, s->farea, &he);
-    if (he == hipSuccess) g.fcen = dupload(s, L, s->fcen, &he);
-    if (he == hipSuccess) g.fnrm = dupload(s, L, s->fnrm, &he);
-    if (he == hipSuccess) s->binv = dalloc<double>(s, L, (size_t)T.pk_pos[T.nc], &he);
-    if (he == hipSuccess) s->gflux = dalloc<double>(s, L, (size_t)T.nhf, &he);
-    if (he != hipSuccess) return fail(s, MHS_ERR_ALLOC, std::string("device allocation: ") + hipGetErrorString(he));
-    if (!s->ijk.empty()) {
-        std::string ea = build_amg(s->H, T, s->ijk);
-        if (!ea.empty()) return fail(s, MHS_ERR_INVALID, "init_system_structure: " + ea);
-        if (s->H.nlev() > kMaxLevels) return fail(s, MHS_ERR_INVALID, "too many multigrid levels");
+    auto &L = s->amg_bufs;
+    {
+        std::string ea = build_amg(s->H, T, s->ijk, strength);
+        if (!ea.empty()) return fail(s, MHS_ERR_INVALID, "multigrid: " + ea);
+        if (s->H.nlev() > kMaxLevels) return fail(s, MHS_ERR_INVALID, "multigrid: too many levels");
         DevAmg &A = s->dA;
         A.nlev = s->H.nlev();
         A.ninst = T.ninst;
@@ -359,6 +347,7 @@ int mhs_init_system_structure(mhs_solver *s, const int *hf_bc_type, const int *h
             for (size_t t = 0; t + 1 < H.inst_ptr.size(); ++t)
                 A.max_rows[l] = std::max(A.max_rows[l], H.inst_ptr[t + 1] - H.inst_ptr[t]);
             D.nnz = H.row_ptr.empty() ? 0 : H.row_ptr.back();
+            D.nsrc = H.src_ptr.empty() ? 0 : H.src_ptr.back();
             D.inst_ptr = dupload(s, L, H.inst_ptr, &he);
             if (he == hipSuccess) D.row_ptr = dupload(s, L, H.row_ptr, &he);
             if (he == hipSuccess) D.col = dupload(s, L, H.col, &he);
@@ -374,6 +363,7 @@ int mhs_init_system_structure(mhs_solver *s, const int *hf_bc_type, const int *h
             if (H.sa_out) {
                 D.p_nnz = int(H.p_col.size());
                 D.t_nnz = int(H.t_col.size());
+                D.t_npair = H.t_pair_ptr.empty() ? 0 : H.t_pair_ptr.back();
                 if (he == hipSuccess) D.p_ptr = dupload(s, L, H.p_ptr, &he);
                 if (he == hipSuccess) D.p_col = dupload(s, L, H.p_col, &he);
                 if (he == hipSuccess) D.p_own = dupload(s, L, H.p_own, &he);
@@ -402,6 +392,7 @@ int mhs_init_system_structure(mhs_solver *s, const int *hf_bc_type, const int *h
         if (he == hipSuccess) A.ap_src_ptr = dupload(s, L, s->H.ap.src_ptr, &he);
         if (he == hipSuccess) A.ap_src = dupload(s, L, s->H.ap.src, &he);
         A.ap_slots = (long long)s->H.ap.col.size();
+        A.ap_nsrc = s->H.ap.src_ptr.empty() ? 0 : s->H.ap.src_ptr.back();
         if (he == hipSuccess) A.ap64 = dalloc<double>(s, L, (size_t)A.ap_slots, &he);
         if (he == hipSuccess) A.ap32 = dalloc<float>(s, L, (size_t)A.ap_slots, &he);
         if (he == hipSuccess) A.coarse_off = dupload(s, L, s->H.coarse_off, &he);
@@ -412,6 +403,89 @@ int mhs_init_system_structure(mhs_solver *s, const int *hf_bc_type, const int *h
         if (he == hipSuccess) A.cinv32 = dalloc<float>(s, L, (size_t)s->H.coarse_off.back(), &he);
         if (he != hipSuccess) return fail(s, MHS_ERR_ALLOC, std::string("multigrid allocation: ") + hipGetErrorString(he));
         s->has_amg = true;
+    }
+    return MHS_OK;
+}
+
+// Coupling strength along the logical directions: the mean two-point
+// half-transmissibility |n.K (x_f - x_c)| A_f / |x_f - x_c|^2 over the
+// half-faces between logical neighbours, grouped by the first of i, j, k in
+// which the two cells differ (lateral for faces across a fault).
+Strength coupling_strength(const mhs_solver *s, const double *perm) {
+    double sum[3] = {0.0, 0.0, 0.0};
+    long long cnt[3] = {0, 0, 0};
+    for (int c = 0; c < s->nc; ++c)
+        for (int h = s->facepos[c]; h < s->facepos[c + 1]; ++h) {
+            const int c2 = s->nb[h];
+            if (c2 < 0) continue;
+            int d = -1;
+            for (int e = 0; e < 3 && d < 0; ++e)
+                if (s->ijk[3 * (size_t)c + e] != s->ijk[3 * (size_t)c2 + e]) d = e;
+            if (d < 0) continue;
+            const double *K = perm + 9 * (size_t)c;
+            const double *xf = &s->fcen[3 * (size_t)h], *xc = &s->ccen[3 * (size_t)c], *n = &s->fnrm[3 * (size_t)h];
+            const double dx[3] = {xf[0] - xc[0], xf[1] - xc[1], xf[2] - xc[2]};
+            const double nn = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            const double dd = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
+            if (!(nn > 0.0) || !(dd > 0.0)) continue;
+            double nkd = 0.0;
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) nkd += n[a] * K[3 * a + b] * dx[b];
+            sum[d] += std::fabs(nkd) / nn * s->farea[h] / dd;
+            ++cnt[d];
+        }
+    Strength st{};
+    for (int d = 0; d < 3; ++d) st[d] = cnt[d] ? sum[d] / (double)cnt[d] : 0.0;
+    return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mhs_init_system_structure(mhs_solver *s, const int *hf_bc_type, const int *hf_periodic_partner) {
+    if (!s) return MHS_ERR_INVALID;
+    if (!s->grid_set) return fail(s, MHS_ERR_STATE, "init_system_structure: no grid (call set_grid)");
+    HIPCHECK(hipSetDevice(s->device));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    s->free_solve_buffers();
+    s->free_topology();
+    s->structure_valid = s->ip_valid = s->solved = false;
+    std::string e = build_topology(s->T, s->ninst, s->inst_cellpos.data(), s->nc, s->facepos.data(),
+                                   s->nb.data(), hf_bc_type, hf_periodic_partner);
+    if (!e.empty()) return fail(s, MHS_ERR_INVALID, "init_system_structure: " + e);
+    Topology &T = s->T;
+    hipError_t he = hipSuccess;
+    auto &L = s->topo_bufs;
+    DevTopo &d = s->dT;
+    d.nc = T.nc; d.nhf = T.nhf; d.nfp = T.nfp; d.ninst = T.ninst; d.nslices = T.nslices;
+    d.nblk = T.nfp / kRowBlock;
+    d.max_w = 0;
+    for (int sl = 0; sl < T.nslices; ++sl)
+        d.max_w = std::max(d.max_w, (int)((T.slice_ptr[sl + 1] - T.slice_ptr[sl]) / kSlice));
+#define UP(field, vec)                                   \
+    do {                                                 \
+        d.field = dupload(s, L, T.vec, &he);             \
+        if (he != hipSuccess) return fail(s, MHS_ERR_HIP, "upload " #vec); \
+    } while (0)
+    UP(facepos, facepos); UP(hf_dof, hf_dof); UP(hf_cell, hf_cell); UP(hf_col, hf_col);
+    UP(hf_ppart, hf_ppart); UP(hf_twin, hf_twin); UP(hf_redirect, hf_redirect);
+    UP(row_hf, row_hf); UP(dof_hf, dof_hf); UP(ptn, ptn); UP(dslot, dslot); UP(pslot, pslot);
+    UP(slice_ptr, slice_ptr); UP(col, col); UP(slotmap, slotmap); UP(sq_pos, sq_pos);
+    UP(pk_pos, pk_pos); UP(block_seg, block_seg); UP(seg_blk, seg_blk); UP(inst_rowbase, inst_rowbase);
+#undef UP
+    DevGeom &g = s->dG;
+    g.cvol = dupload(s, L, s->cvol, &he);
+    if (he == hipSuccess) g.ccen = dupload(s, L, s->ccen, &he);
+    if (he == hipSuccess) g.farea = dupload(s, L, s->farea, &he);
+    if (he == hipSuccess) g.fcen = dupload(s, L, s->fcen, &he);
+    if (he == hipSuccess) g.fnrm = dupload(s, L, s->fnrm, &he);
+    if (he == hipSuccess) s->binv = dalloc<double>(s, L, (size_t)T.pk_pos[T.nc], &he);
+    if (he == hipSuccess) s->gflux = dalloc<double>(s, L, (size_t)T.nhf, &he);
+    if (he != hipSuccess) return fail(s, MHS_ERR_ALLOC, std::string("device allocation: ") + hipGetErrorString(he));
+    if (!s->ijk.empty()) {
+        const int rc = build_amg_device(s, Strength{1.0, 1.0, 1.0});
+        if (rc) return rc;
     }
     s->structure_valid = true;
     return MHS_OK;
@@ -444,6 +518,28 @@ int inner_products_impl(mhs_solver *s, const double *perm, const double *gravity
         e = launch_static_ip(s->stream, s->dT, s->dG, s->T.max_ncf, s->perm, s->grav, s->binv, s->gflux);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) return fail(s, MHS_ERR_HIP, std::string("static inner products: ") + hipGetErrorString(e));
+    if (s->has_amg && !s->amg_planned) {
+        // semi-coarsening, decided on the first permeability after the
+        // structure: rebuild the hierarchy when the permeability and the
+        // geometry call for other aggregate shapes than the current ones
+        std::vector<double> hperm;
+        const double *kp = perm;
+        if (device_perm) {
+            hperm.resize(n9);
+            HIPCHECK(hipMemcpy(hperm.data(), s->perm, n9 * sizeof(double), hipMemcpyDeviceToHost));
+            kp = hperm.data();
+        }
+        const Strength st = coupling_strength(s, kp);
+        std::vector<int> dims;
+        const std::string ed = instance_dims(s->T, s->ijk, dims);
+        if (!ed.empty()) return fail(s, MHS_ERR_INVALID, "multigrid: " + ed);
+        if (coarsening_plan(st, dims) != s->H.blocks) {
+            s->free_solve_buffers();
+            const int rc = build_amg_device(s, st);
+            if (rc) return rc;
+        }
+        s->amg_planned = true;
+    }
     s->ip_valid = true;
     s->solved = false;
     return MHS_OK;
@@ -600,7 +696,8 @@ int run_chunk(mhs_solver *s, int nrhs, int chunk, bool use_graph, const MT *val,
         const int shape[5] = {s->dT.spmv_tb, s->dT.spmv_xcd, s->dT.sweep_tb, s->dT.sweep_xcd, s->dT.tail_rows};
         if (!(G.exec && G.precision == precision && G.nrhs == nrhs && G.chunk == chunk &&
               G.key == (const void *)val && G.tol2 == tol2 && G.rf2 == s->S.rf2 && G.amg == int(amg) &&
-              G.omega == s->dA.omega && G.alpha == s->dA.alpha && G.nu == s->dA.nu &&
+              std::equal(G.omega, G.omega + kMaxLevels, s->dA.omega_l) && G.alpha == s->dA.alpha &&
+              G.nu == s->dA.nu &&
               std::equal(shape, shape + 5, G.shape))) {
             if (G.exec) (void)hipGraphExecDestroy(G.exec);
             G = GraphCache{};
@@ -628,7 +725,7 @@ int run_chunk(mhs_solver *s, int nrhs, int chunk, bool use_graph, const MT *val,
             G.tol2 = tol2;
             G.rf2 = s->S.rf2;
             G.amg = int(amg);
-            G.omega = s->dA.omega;
+            std::copy(s->dA.omega_l, s->dA.omega_l + kMaxLevels, G.omega);
             G.alpha = s->dA.alpha;
             G.nu = s->dA.nu;
             std::copy(shape, shape + 5, G.shape);
@@ -648,6 +745,96 @@ int read_state(mhs_solver *s, size_t nsys, bool sc) {
     if (sc)
         HIPCHECK(hipMemcpyAsync(s->h_sc, s->S.sc, nsys * SC_N * sizeof(double), hipMemcpyDeviceToHost, s->stream));
     HIPCHECK(hipStreamSynchronize(s->stream));
+    return MHS_OK;
+}
+
+// Largest eigenvalue of the symmetric tridiagonal matrix with diagonal a and
+// off-diagonal b (b[i] couples i and i+1): bisection on Sturm counts.
+double tridiag_max_eig(const std::vector<double> &a, const std::vector<double> &b) {
+    const int k = (int)a.size();
+    double lo = 1e300, hi = -1e300;
+    for (int i = 0; i < k; ++i) {
+        const double r = (i > 0 ? std::fabs(b[i - 1]) : 0.0) + (i + 1 < k ? std::fabs(b[i]) : 0.0);
+        lo = std::min(lo, a[i] - r);
+        hi = std::max(hi, a[i] + r);
+    }
+    auto below = [&](double x) {  // eigenvalues < x
+        int c = 0;
+        double q = 1.0;
+        for (int i = 0; i < k; ++i) {
+            q = a[i] - x - (i > 0 ? b[i - 1] * b[i - 1] / q : 0.0);
+            if (q == 0.0) q = -1e-300;
+            if (q < 0.0) ++c;
+        }
+        return c;
+    };
+    for (int it = 0; it < 200 && hi - lo > 1e-12 * std::max(1.0, std::fabs(hi)); ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (below(mid) == k) hi = mid; else lo = mid;
+    }
+    return hi;
+}
+
+// rho(D^-1 A) of every smoothed level (0 .. nlev-2): the largest Ritz value
+// of 12 Lanczos steps on D^-1/2 A D^-1/2 (FP64, from a fixed pseudo-random
+// start), which approaches rho from below.  Done once per hierarchy.
+int estimate_rho(mhs_solver *s) {
+    constexpr int kSteps = 12;
+    const DevAmg &A = s->dA;
+    const int nl = A.nlev - 1;
+    size_t nmax = (size_t)s->T.nfp;
+    for (int l = 1; l < nl; ++l) nmax = std::max(nmax, (size_t)A.lev[l].n);
+    std::vector<DevBuf> tmp;
+    struct Release {
+        std::vector<DevBuf> &t;
+        ~Release() { for (auto &b : t) (void)hipFree(b.p); }
+    } release{tmp};
+    hipError_t he = hipSuccess;
+    double *w = dalloc<double>(s, tmp, nmax, &he);
+    double *V[2] = {nullptr, nullptr}, *u = nullptr, *y = nullptr, *part = nullptr;
+    if (he == hipSuccess) V[0] = dalloc<double>(s, tmp, nmax, &he);
+    if (he == hipSuccess) V[1] = dalloc<double>(s, tmp, nmax, &he);
+    if (he == hipSuccess) u = dalloc<double>(s, tmp, nmax, &he);
+    if (he == hipSuccess) y = dalloc<double>(s, tmp, nmax, &he);
+    if (he == hipSuccess) part = dalloc<double>(s, tmp, nmax / 256 + 1, &he);
+    if (he != hipSuccess) return fail(s, MHS_ERR_ALLOC, std::string("smoother weights: ") + hipGetErrorString(he));
+    std::vector<double> hp(nmax / 256 + 1);
+    std::vector<double> rho(nl, 0.0);
+    for (int l = 0; l < nl; ++l) {
+        const int n = l == 0 ? s->T.nfp : A.lev[l].n;
+        const size_t nb = ((size_t)n + 255) / 256;
+        if (n == 0) continue;
+        auto total = [&](double *out) -> int {
+            HIPCHECK(hipMemcpyAsync(hp.data(), part, nb * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+            HIPCHECK(hipStreamSynchronize(s->stream));
+            double t = 0.0;
+            for (size_t i = 0; i < nb; ++i) t += hp[i];
+            *out = t;
+            return MHS_OK;
+        };
+        double ww = 0.0;
+        HIPCHECK(launch_lanczos_init(s->stream, n, w, part));
+        if (int rc = total(&ww)) return rc;
+        double beta = std::sqrt(ww);
+        std::vector<double> al, be;
+        for (int j = 0; j < kSteps && beta > 0.0; ++j) {
+            HIPCHECK(launch_lanczos_step(s->stream, s->dT, A, l, s->val64, s->dinv64, w, 1.0 / beta, V[j & 1], u,
+                                         y, part));
+            double a = 0.0;
+            if (int rc = total(&a)) return rc;
+            al.push_back(a);
+            HIPCHECK(launch_lanczos_update(s->stream, n, y, V[j & 1], j ? V[(j + 1) & 1] : nullptr, a, beta, w,
+                                           part));
+            if (int rc = total(&ww)) return rc;
+            const double bn = std::sqrt(ww);
+            if (!(bn > 1e-12 * std::fabs(a))) break;   // invariant subspace found
+            if (j + 1 < kSteps) be.push_back(bn);
+            beta = bn;
+        }
+        be.resize(al.empty() ? 0 : al.size() - 1);
+        rho[l] = al.empty() ? 0.0 : tridiag_max_eig(al, be);
+    }
+    s->rho = rho;
     return MHS_OK;
 }
 
@@ -707,9 +894,9 @@ int autotune(mhs_solver *s, int nrhs, bool mixed, bool amg, double *tune_ms) {
                                          : launch_post0<double>(s->stream, s->dT, s->dA, nrhs, s->dinv64, s->r64,
                                                                 s->z64, s->R);
                         return mixed ? launch_sweep0<float>(s->stream, s->dT, nrhs, s->val32, s->dinv32, s->r32,
-                                                            (const float *)s->dA.x0, s->z32, s->dA.omega, s->R)
+                                                            (const float *)s->dA.x0, s->z32, s->dA.omega_l[0], s->R)
                                      : launch_sweep0<double>(s->stream, s->dT, nrhs, s->val64, s->dinv64, s->r64,
-                                                             (const double *)s->dA.x0, s->z64, s->dA.omega,
+                                                             (const double *)s->dA.x0, s->z64, s->dA.omega_l[0],
                                                              s->R);
                     });
                     if (ms < bw) { bw = ms; best.sweep_tb = tb; best.sweep_xcd = xcd; }
@@ -916,13 +1103,27 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
         return fail(s, MHS_ERR_INVALID, "solve: unknown preconditioner");
     }
     if (amg) {
-        s->dA.omega = opt.smoother_weight;
         s->dA.alpha = opt.prolongate_factor;
         s->dA.nu = std::max(1, opt.smooth_steps);
-        // the kernels that walk the whole hierarchy read this copy (nothing of
-        // the previous solve is still running: it ended with a synchronise)
-        HIPCHECK(hipMemcpy(const_cast<DevAmg *>(s->dA.dev), &s->dA, sizeof(DevAmg), hipMemcpyHostToDevice));
         HIPCHECK(launch_amg_setup(s->stream, s->dT, s->dA, s->val64, mixed));
+        // Jacobi weights: the caller's, or 4 / (3 rho(D^-1 A)) per level with
+        // rho estimated on the first multigrid solve of this hierarchy
+        if (opt.smoother_weight > 0.0) {
+            for (int l = 0; l < kMaxLevels; ++l) s->dA.omega_l[l] = opt.smoother_weight;
+        } else {
+            if (s->rho.empty()) {
+                rc = estimate_rho(s);
+                if (rc) return rc;
+            }
+            for (int l = 0; l < kMaxLevels; ++l) {
+                const double r = l < (int)s->rho.size() ? s->rho[l] : 0.0;
+                s->dA.omega_l[l] = r > 0.0 ? std::min(1.0, std::max(0.2, 4.0 / (3.0 * r))) : 0.7;
+            }
+        }
+        // the kernels that walk the whole hierarchy read this copy (nothing of
+        // the previous solve is still running: it ended with a synchronise;
+        // the set-up kernels above do not read it)
+        HIPCHECK(hipMemcpy(const_cast<DevAmg *>(s->dA.dev), &s->dA, sizeof(DevAmg), hipMemcpyHostToDevice));
     }
 
     // --- launch shapes -------------------------------------------------------
@@ -990,6 +1191,11 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
         rep->recover_ms = std::chrono::duration<double, std::milli>(t3 - t2).count();
         rep->spmv_threads = s->dT.spmv_tb;
         rep->spmv_xcd = s->dT.spmv_xcd;
+        for (int l = 0; l < 8; ++l) {
+            rep->smoother_weights[l] = (amg && l < s->dA.nlev - 1) ? s->dA.omega_l[l] : 0.0;
+            for (int d = 0; d < 3; ++d)
+                rep->coarsening[l][d] = (amg && l < (int)s->H.blocks.size()) ? s->H.blocks[l][d] : 0;
+        }
         rep->sweep_threads = s->dT.sweep_tb;
         rep->sweep_xcd = s->dT.sweep_xcd;
         rep->tail_rows = s->dT.tail_rows;

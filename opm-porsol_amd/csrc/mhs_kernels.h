@@ -14,6 +14,7 @@ struct DevGeom {
 // Topology views (see mhs_structure.h for the meaning of each array).
 struct DevTopo {
     int nc, nhf, nfp, ninst, nslices, nblk;
+    int max_w;                      // widest SELL slice (stored entries per row)
     const int *facepos, *hf_dof, *hf_cell, *hf_col, *hf_ppart, *hf_twin;
     const uint8_t *hf_redirect;
     const int *row_hf, *dof_hf, *ptn;
@@ -63,9 +64,10 @@ struct DevScal {
 };
 
 // Multigrid hierarchy on the device (level 0 is the SELL face matrix).
-constexpr int kMaxLevels = 16;   // DevAmg travels as a kernel argument (< 4 KB)
+constexpr int kMaxLevels = 16;
 struct DevAmgLevel {
     int n, nnz;
+    long long nsrc;                 // entries of the src (and src2) lists
     const int *inst_ptr;            // [ninst+1] first row of each instance
     const int *row_ptr, *col, *diag, *src_ptr, *src;
     const int *agg;                 // rows of the finer level -> rows of this level
@@ -77,6 +79,7 @@ struct DevAmgLevel {
     // sums (sa_in, src = P positions, src2 = T positions of the finer level),
     // and the smoothed transfer to the next level (sa_out)
     int sa_in, sa_out, p_nnz, t_nnz;
+    long long t_npair;              // entries of the T pair lists
     const int *src2;
     const int *p_ptr, *p_col, *p_own, *p_src_ptr, *p_src;
     const int *pt_ptr, *pt_col, *pt_map;
@@ -92,7 +95,8 @@ struct DevAmg {
     const int *coarse_off;          // [ninst+1] offsets of the dense coarsest inverses
     double *cinv64;
     float *cinv32;
-    double omega, alpha;            // Jacobi weight, coarse-correction (prolongation) factor
+    double omega_l[kMaxLevels];     // Jacobi weight of each level's smoother
+    double alpha;                   // coarse-correction (prolongation) factor
     double alpha_sa;                // coarse-correction factor across smoothed transfers
     double omega_p;                 // prolongation-smoothing weight of those transfers
     const DevAmg *dev;              // this structure's copy in device memory (refreshed by the host)
@@ -102,7 +106,7 @@ struct DevAmg {
     // every solve from the face matrix through the gather lists
     const long long *ap_slice_ptr;
     const int *ap_col, *ap_src_ptr, *ap_src;
-    long long ap_slots;
+    long long ap_slots, ap_nsrc;
     double *ap64;
     float *ap32;
 };
@@ -154,6 +158,16 @@ hipError_t launch_resnorm(hipStream_t st, const DevTopo &T, int nrhs, const doub
 // and the preconditioned iteration pieces.
 hipError_t launch_amg_setup(hipStream_t st, const DevTopo &T, const DevAmg &A, const double *val64,
                             bool mixed);
+// Lanczos on D^-1/2 A_l D^-1/2 (FP64) for the Jacobi weight of level l:
+// start vector w with |w|^2 partials; one step v = scale w, y = D^-1/2 A
+// D^-1/2 v with <v, y> partials; w = y - a v - b vp with |w|^2 partials.
+// Partials: one per 256 rows (level 0: T.nfp rows, else A.lev[l].n).
+hipError_t launch_lanczos_init(hipStream_t st, int n, double *w, double *part);
+hipError_t launch_lanczos_step(hipStream_t st, const DevTopo &T, const DevAmg &A, int level, const double *val0,
+                               const double *dinv0, const double *w, double scale, double *v, double *u,
+                               double *y, double *part);
+hipError_t launch_lanczos_update(hipStream_t st, int n, const double *y, const double *v, const double *vp,
+                                 double a, double b, double *w, double *part);
 // One level-0 Jacobi sweep with the <b, xo> partials (the last multigrid
 // smoothing sweep alone; the autotuner times it).
 template <typename VT>

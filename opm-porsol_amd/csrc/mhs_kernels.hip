@@ -161,6 +161,21 @@ __device__ __forceinline__ void vstore(VT *__restrict__ p, const VT (&v)[N]) {
     }
 }
 
+// The costly preconditioner kernels inside the PCG loop skip finished
+// systems: act[s * NRHS + k] says whether system (instance s, rhs k) still
+// iterates.  A null pointer (a standalone V-cycle) runs everything.
+struct Gate {
+    const int *act = nullptr;
+};
+template <int NRHS>
+__device__ __forceinline__ bool gate_off(const Gate &g, int s) {   // instance s has finished
+    if (!g.act) return false;
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) any |= g.act[s * NRHS + k] != 0;
+    return !any;
+}
+
 template <int NRHS>
 __device__ __forceinline__ bool load_act(const int *act, int s, int (&a)[NRHS]) {
     bool any = false;
@@ -400,36 +415,42 @@ __global__ __launch_bounds__(64) void k_assemble(DevTopo T, const double *__rest
 
 // ---------------------------------------------------------------------------
 // Face-row gather: fills SELL values, right-hand sides and the Jacobi diagonal.
-template <int NRHS>
-__global__ __launch_bounds__(256) void k_fill(DevTopo T, DevCellSys C,
-                                              const uint8_t *__restrict__ ftype,
-                                              const double *__restrict__ bcval,
-                                              const int *__restrict__ inst_reg, int dguess,
-                                              double *__restrict__ val64,
-                                              float *__restrict__ val32,
-                                              double *__restrict__ b,
-                                              double *__restrict__ dinv64,
-                                              float *__restrict__ dinv32,
-                                              double *__restrict__ x0) {
-    const int f = blockIdx.x * 256 + threadIdx.x;
+// With ROW_LDS each thread sums its row in its own column of LDS (max_w
+// doubles, stride TB) and writes every stored entry once; otherwise the row
+// is summed in place in val64 (rows too wide for LDS).
+template <int NRHS, int TB, bool ROW_LDS>
+__global__ __launch_bounds__(TB) void k_fill(DevTopo T, DevCellSys C,
+                                             const uint8_t *__restrict__ ftype,
+                                             const double *__restrict__ bcval,
+                                             const int *__restrict__ inst_reg, int dguess,
+                                             double *__restrict__ val64,
+                                             float *__restrict__ val32,
+                                             double *__restrict__ b,
+                                             double *__restrict__ dinv64,
+                                             float *__restrict__ dinv32,
+                                             double *__restrict__ x0) {
+    extern __shared__ double rowbuf[];
+    const int f = blockIdx.x * TB + threadIdx.x;
     if (f >= T.nfp) return;
     const int slice = f >> 6, lane = f & 63;
     const long long base = T.slice_ptr[slice] + lane;
     const int w = (int)((T.slice_ptr[slice + 1] - T.slice_ptr[slice]) >> 6);
-    for (int e = 0; e < w; ++e) val64[base + 64LL * e] = 0.0;
+    double *row = ROW_LDS ? rowbuf + threadIdx.x : val64 + base;
+    constexpr long long rs = ROW_LDS ? TB : 64;    // stride between a row's entries
+    for (int e = 0; e < w; ++e) row[rs * e] = 0.0;
     const int h0 = T.dof_hf[f];
-    const long long dpos = base + 64LL * T.dslot[f];
+    const long long dpos = rs * T.dslot[f];
     double bk[NRHS], xk[NRHS];
 #pragma unroll
     for (int k = 0; k < NRHS; ++k) bk[k] = xk[k] = 0.0;
     double diag;
     if (h0 < 0) {  // padding row: identity
-        val64[dpos] = 1.0;
+        row[dpos] = 1.0;
         diag = 1.0;
     } else if (ftype[h0] == FT_DIRICHLET) {
         const int c = T.hf_cell[h0], f0 = T.facepos[c], n = T.facepos[c + 1] - f0, r = h0 - f0;
         const double a = C.schur[T.pk_pos[c] + pidx(r, r, n)];
-        val64[dpos] = a;
+        row[dpos] = a;
         diag = a;
 #pragma unroll
         for (int k = 0; k < NRHS; ++k) {
@@ -447,7 +468,7 @@ __global__ __launch_bounds__(256) void k_fill(DevTopo T, DevCellSys C,
             for (int j = 0; j < n; ++j) {
                 if (ftype[f0 + j] == FT_DIRICHLET) continue;
                 const double v = C.schur[pk + (r <= j ? pidx(r, j, n) : pidx(j, r, n))];
-                val64[base + 64LL * sm[j]] += v;
+                row[rs * sm[j]] += v;
             }
 #pragma unroll
             for (int k = 0; k < NRHS; ++k) bk[k] += C.rr[(size_t)k * T.nhf + h];
@@ -459,15 +480,15 @@ __global__ __launch_bounds__(256) void k_fill(DevTopo T, DevCellSys C,
             const int c1 = T.hf_cell[hp], f01 = T.facepos[c1], n1 = T.facepos[c1 + 1] - f01;
             const double a0 = C.schur[T.pk_pos[c0] + pidx(h0 - f00, h0 - f00, n0)];
             const double a1 = C.schur[T.pk_pos[c1] + pidx(hp - f01, hp - f01, n1)];
-            val64[dpos] += a0 + a1;
-            val64[base + 64LL * T.pslot[f]] -= a0 + a1;
+            row[dpos] += a0 + a1;
+            row[rs * T.pslot[f]] -= a0 + a1;
 #pragma unroll
             for (int k = 0; k < NRHS; ++k)
                 bk[k] += a0 * bcval[(size_t)k * T.nhf + h0] - a1 * bcval[(size_t)k * T.nhf + hp];
         }
         const int s = T.block_seg[f / kRowBlock];
-        if (inst_reg[s] && f == T.inst_rowbase[s]) val64[dpos] *= 2.0;  // S[0][0] *= 2 (:1398-1400)
-        diag = val64[dpos];
+        if (inst_reg[s] && f == T.inst_rowbase[s]) row[dpos] *= 2.0;  // S[0][0] *= 2 (:1398-1400)
+        diag = row[dpos];
     }
 #pragma unroll
     for (int k = 0; k < NRHS; ++k) {
@@ -476,10 +497,12 @@ __global__ __launch_bounds__(256) void k_fill(DevTopo T, DevCellSys C,
     }
     const double di = 1.0 / diag;
     if (dinv64) dinv64[f] = di;
-    if (val32) {
-        for (int e = 0; e < w; ++e) val32[base + 64LL * e] = (float)val64[base + 64LL * e];
-        dinv32[f] = (float)di;
+    for (int e = 0; e < w; ++e) {
+        const double v = row[rs * e];
+        if (ROW_LDS) val64[base + 64LL * e] = v;
+        if (val32) val32[base + 64LL * e] = (float)v;
     }
+    if (val32) dinv32[f] = (float)di;
 }
 
 // ---------------------------------------------------------------------------
@@ -777,19 +800,56 @@ __global__ __launch_bounds__(256) void k_resnorm(DevTopo T, const double *__rest
 // per instance.  With symmetric smoothing the V-cycle is a fixed symmetric
 // operator, as PCG requires.
 
+// Coarse-level rows with batched loads: the column indices of kChunk entries
+// are loaded together, then everything that depends on them, so a row costs a
+// few memory round trips instead of one dependent chain per entry (the small
+// levels are latency-bound).  Padding lanes either read the row's own entries
+// or issue no load at all: never a row of another instance, which the
+// one-workgroup-per-instance tail may not have written yet (0 * NaN != 0).
+constexpr int kChunk = 8;
+
+// Rows can be shared by a group of G lanes (G divides 64, so a group lies in
+// one wave): lane `sub` of the group takes entries e0 + sub, e0 + sub + G, ...
+// in chunks of KC, and group_sum adds the partial sums across the group, after
+// which every lane of the group holds the row's sum.  G = 1 is one lane per row.
+template <int G>
+constexpr int chunk_for() { return G == 1 ? kChunk : 4; }
+
+template <int G, typename VT, int NRHS>
+__device__ __forceinline__ void group_sum(VT (&acc)[NRHS]) {
+#pragma unroll
+    for (int m = G / 2; m >= 1; m >>= 1)
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) acc[k] += __shfl_xor(acc[k], m, G);
+}
+
 // Galerkin product A_c = P' A P with piecewise-constant P: every coarse entry
-// is the sum of the finer entries listed for it (no atomics).
+// is the sum of the finer entries listed for it (no atomics),
+// out[e] = sum of fine[src[k]] over the gather list of entry e, G lanes per
+// entry (coalesced list reads, several gathers in flight per lane).
+template <int G>
 __global__ __launch_bounds__(256) void k_galerkin(int nnz, const int *__restrict__ src_ptr,
                                                   const int *__restrict__ src,
                                                   const double *__restrict__ fine,
                                                   double *__restrict__ out64,
                                                   float *__restrict__ out32) {
-    const int e = blockIdx.x * 256 + threadIdx.x;
+    constexpr int KC = chunk_for<G>();
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int e = t / G, sub = t % G;
     if (e >= nnz) return;
-    double s = 0.0;
-    for (int k = src_ptr[e]; k < src_ptr[e + 1]; ++k) s += fine[src[k]];
-    out64[e] = s;
-    if (out32) out32[e] = (float)s;
+    double s[1] = {0.0};
+    const int k1 = src_ptr[e + 1];
+    for (int k = src_ptr[e] + sub; k < k1; k += G * KC) {
+        int q[KC];
+#pragma unroll
+        for (int u = 0; u < KC; ++u) q[u] = k + G * u < k1 ? src[k + G * u] : -1;
+#pragma unroll
+        for (int u = 0; u < KC; ++u) s[0] += q[u] >= 0 ? fine[q[u]] : 0.0;
+    }
+    group_sum<G>(s);
+    if (sub != 0) return;
+    out64[e] = s[0];
+    if (out32) out32[e] = (float)s[0];
 }
 
 __global__ __launch_bounds__(256) void k_coarse_diag(int n, const int *__restrict__ diag,
@@ -818,17 +878,34 @@ __global__ __launch_bounds__(256) void k_sa_p(DevAmgLevel F, double wp) {
     }
 }
 
-// out[e] = sum over the pairs of entry e of a[pa] * b[pb] (T = A P, A_c = P' T).
+// out[e] = sum over the pairs of entry e of a[pa] * b[pb] (T = A P, A_c = P' T),
+// G lanes per entry.
+template <int G>
 __global__ __launch_bounds__(256) void k_pair_sum(int n, const int *__restrict__ ptr,
                                                   const int *__restrict__ pa, const int *__restrict__ pb,
                                                   const double *__restrict__ a, const double *__restrict__ b,
                                                   double *__restrict__ out64, float *__restrict__ out32) {
-    const int e = blockIdx.x * 256 + threadIdx.x;
+    constexpr int KC = chunk_for<G>();
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int e = t / G, sub = t % G;
     if (e >= n) return;
-    double s = 0.0;
-    for (int k = ptr[e]; k < ptr[e + 1]; ++k) s += a[pa[k]] * b[pb[k]];
-    out64[e] = s;
-    if (out32) out32[e] = (float)s;
+    double s[1] = {0.0};
+    const int k1 = ptr[e + 1];
+    for (int k = ptr[e] + sub; k < k1; k += G * KC) {
+        int ia[KC], ib[KC];
+#pragma unroll
+        for (int u = 0; u < KC; ++u) {
+            const bool in = k + G * u < k1;
+            ia[u] = in ? pa[k + G * u] : -1;
+            ib[u] = in ? pb[k + G * u] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < KC; ++u) s[0] += ia[u] >= 0 ? a[ia[u]] * b[ib[u]] : 0.0;
+    }
+    group_sum<G>(s);
+    if (sub != 0) return;
+    out64[e] = s[0];
+    if (out32) out32[e] = (float)s[0];
 }
 
 // Dense inverse of each instance's coarsest operator (Gauss-Jordan, no
@@ -877,29 +954,6 @@ __device__ __forceinline__ const VT *lvl_val(const DevAmgLevel &L) {
 template <typename VT>
 __device__ __forceinline__ const VT *lvl_dinv(const DevAmgLevel &L) {
     if constexpr (std::is_same<VT, float>::value) return L.dinv32; else return L.dinv64;
-}
-
-// Coarse-level rows with batched loads: the column indices of kChunk entries
-// are loaded together, then everything that depends on them, so a row costs a
-// few memory round trips instead of one dependent chain per entry (the small
-// levels are latency-bound).  Padding lanes either read the row's own entries
-// or issue no load at all: never a row of another instance, which the
-// one-workgroup-per-instance tail may not have written yet (0 * NaN != 0).
-constexpr int kChunk = 8;
-
-// Rows can be shared by a group of G lanes (G divides 64, so a group lies in
-// one wave): lane `sub` of the group takes entries e0 + sub, e0 + sub + G, ...
-// in chunks of KC, and group_sum adds the partial sums across the group, after
-// which every lane of the group holds the row's sum.  G = 1 is one lane per row.
-template <int G>
-constexpr int chunk_for() { return G == 1 ? kChunk : 4; }
-
-template <int G, typename VT, int NRHS>
-__device__ __forceinline__ void group_sum(VT (&acc)[NRHS]) {
-#pragma unroll
-    for (int m = G / 2; m >= 1; m >>= 1)
-#pragma unroll
-        for (int k = 0; k < NRHS; ++k) acc[k] += __shfl_xor(acc[k], m, G);
 }
 
 // acc = sum_j a_ij (w d_j b_j): the residual of the pre-smoothed x = w D^-1 b.
@@ -1157,13 +1211,14 @@ __global__ __launch_bounds__(TB) void k_vc0_up(DevTopo T, const VT *__restrict__
 // where A P (fine rows x level-1 aggregates, a few entries per row) is
 // refilled every solve.  Also the <r, z> partials of the preconditioned CG.
 template <typename VT, int NRHS, int TB>
-__global__ __launch_bounds__(TB) void k_vc0_post(DevTopo T, const long long *__restrict__ ap_ptr,
+__global__ __launch_bounds__(TB) void k_vc0_post(Gate gt, DevTopo T, const long long *__restrict__ ap_ptr,
                                                  const int *__restrict__ ap_col, const VT *__restrict__ ap,
                                                  const int *__restrict__ agg, const VT *__restrict__ dinv,
                                                  const VT *__restrict__ r, const VT *__restrict__ x0,
                                                  const VT *__restrict__ res0, const VT *__restrict__ xc,
                                                  VT *__restrict__ z, VT w, VT al, DevRed R) {
     const int blk = T.sweep_xcd ? xcd_block(gridDim.x) : (int)blockIdx.x;
+    if (gate_off<NRHS>(gt, T.block_seg[blk])) return;
     double part[NRHS];
 #pragma unroll
     for (int k = 0; k < NRHS; ++k) part[k] = 0.0;
@@ -1208,10 +1263,11 @@ __global__ __launch_bounds__(TB) void k_vc0_post(DevTopo T, const long long *__r
 
 // Level 0: res = b - A x (after extra down sweeps).
 template <typename VT, int NRHS>
-__global__ __launch_bounds__(256) void k_vc0_res(DevTopo T, const VT *__restrict__ val,
+__global__ __launch_bounds__(256) void k_vc0_res(Gate gt, DevTopo T, const VT *__restrict__ val,
                                                  const VT *__restrict__ b, const VT *__restrict__ x,
                                                  VT *__restrict__ res) {
     const int f = blockIdx.x * 256 + threadIdx.x;
+    if (gate_off<NRHS>(gt, T.block_seg[f / kRowBlock])) return;
     VT acc[NRHS];
     sell_row<VT, VT, VT, NRHS>(T, val, x, f, acc);
 #pragma unroll
@@ -1292,9 +1348,11 @@ __global__ __launch_bounds__(256) void k_vcc_up(DevAmgLevel L, const int *__rest
 
 // x0 = w D^-1 r (level-0 pre-smoothing when r does not come from k_update_xr)
 template <typename VT, int NRHS>
-__global__ __launch_bounds__(256) void k_scale0(const VT *__restrict__ dinv, const VT *__restrict__ r,
+__global__ __launch_bounds__(256) void k_scale0(Gate gt, const int *__restrict__ block_seg,
+                                                const VT *__restrict__ dinv, const VT *__restrict__ r,
                                                 VT *__restrict__ x0, VT w) {
     const int f = blockIdx.x * 256 + threadIdx.x;
+    if (gate_off<NRHS>(gt, block_seg[f / kRowBlock])) return;
     const VT d = w * dinv[f];
 #pragma unroll
     for (int k = 0; k < NRHS; ++k) x0[(size_t)f * NRHS + k] = d * r[(size_t)f * NRHS + k];
@@ -1364,7 +1422,8 @@ __global__ __launch_bounds__(256) void k_restrict_sa(DevAmgLevel F, int nc, VT *
 }
 
 template <typename VT, int NRHS, int G = 1>
-__global__ __launch_bounds__(256) void k_vcc_up_sa(DevAmgLevel F, const VT *__restrict__ xc, VT w, VT al) {
+__global__ __launch_bounds__(256) void k_vcc_up_sa(DevAmgLevel F, const VT *__restrict__ xc, VT w,
+                                                   VT al) {
     const int t = blockIdx.x * 256 + threadIdx.x;
     const int i = t / G;
     if (i >= F.n) return;
@@ -1408,15 +1467,17 @@ __global__ __launch_bounds__(256) void k_coarse_solve(const int *__restrict__ in
 // k_coarse_solve and k_vcc_up1, with workgroup barriers between the steps
 // instead of kernel launches.  Reads lev[lt].b, writes lev[l].x for l >= lt.
 template <typename VT, int NRHS>
-__global__ __launch_bounds__(1024) void k_vcycle_tail(const DevAmg *__restrict__ Ap, int lt, VT w, VT al,
+__global__ __launch_bounds__(1024) void k_vcycle_tail(Gate gt, const DevAmg *__restrict__ Ap, int lt, VT al,
                                                       VT als) {
     const DevAmg &A = *Ap;
     const int s = blockIdx.x;
+    if (gate_off<NRHS>(gt, s)) return;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int L = A.nlev;
     for (int l = lt; l < L - 1; ++l) {
         const DevAmgLevel &F = A.lev[l];
         const DevAmgLevel &C = A.lev[l + 1];
+        const VT w = (VT)A.omega_l[l];
         const VT *val = lvl_val<VT>(F);
         const VT *dinv = lvl_dinv<VT>(F);
         const VT *b = (const VT *)F.b;
@@ -1457,6 +1518,7 @@ __global__ __launch_bounds__(1024) void k_vcycle_tail(const DevAmg *__restrict__
     }
     for (int l = L - 2; l >= lt; --l) {
         const DevAmgLevel &F = A.lev[l];
+        const VT w = (VT)A.omega_l[l];
         const int *agg = A.lev[l + 1].agg;
         const VT *xc = (const VT *)A.lev[l + 1].x;
         if (F.sa_out) {
@@ -1993,6 +2055,90 @@ struct FinRes {
     }
 };
 
+// ---------------------------------------------------------------------------
+// Lanczos steps on M = D^-1/2 A D^-1/2 of one multigrid level (FP64), for the
+// largest eigenvalue of D^-1 A that sets the level's Jacobi weight.  Each
+// kernel writes one partial sum per 256-row workgroup, summed on the host.
+
+__device__ __forceinline__ void block_sum256(double v, double *out) {
+    __shared__ double red[4];
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) out[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// w = a fixed pseudo-random start vector in [-1, 1); partials of |w|^2
+__global__ __launch_bounds__(256) void k_lz_init(int n, double *__restrict__ w, double *__restrict__ part) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double t = 0.0;
+    if (i < n) {
+        unsigned h = (unsigned)i * 2654435761u;
+        h ^= h >> 16;
+        h *= 2246822519u;
+        h ^= h >> 13;
+        const double v = (double)h * (2.0 / 4294967296.0) - 1.0;
+        w[i] = v;
+        t = v * v;
+    }
+    block_sum256(t, part);
+}
+
+// v = scale w, u = D^-1/2 v
+__global__ __launch_bounds__(256) void k_lz_prep(int n, const double *__restrict__ dinv,
+                                                 const double *__restrict__ w, double scale,
+                                                 double *__restrict__ v, double *__restrict__ u) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double vi = w[i] * scale;
+    v[i] = vi;
+    u[i] = sqrt(dinv[i]) * vi;
+}
+
+// y = D^-1/2 A u on the face matrix; partials of <v, y>
+__global__ __launch_bounds__(256) void k_lz_sell(DevTopo T, const double *__restrict__ val,
+                                                 const double *__restrict__ dinv, const double *__restrict__ u,
+                                                 const double *__restrict__ v, double *__restrict__ y,
+                                                 double *__restrict__ part) {
+    const int f = blockIdx.x * 256 + threadIdx.x;   // T.nfp is a multiple of 256
+    double acc[1];
+    sell_row<double, double, double, 1>(T, val, u, f, acc);
+    const double yi = sqrt(dinv[f]) * acc[0];
+    y[f] = yi;
+    block_sum256(v[f] * yi, part);
+}
+
+// y = D^-1/2 A u on a coarse (CSR) level; partials of <v, y>
+__global__ __launch_bounds__(256) void k_lz_csr(DevAmgLevel L, const double *__restrict__ u,
+                                                const double *__restrict__ v, double *__restrict__ y,
+                                                double *__restrict__ part) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double t = 0.0;
+    if (i < L.n) {
+        double acc = 0.0;
+        for (int e = L.row_ptr[i]; e < L.row_ptr[i + 1]; ++e) acc += L.val64[e] * u[L.col[e]];
+        const double yi = sqrt(L.dinv64[i]) * acc;
+        y[i] = yi;
+        t = v[i] * yi;
+    }
+    block_sum256(t, part);
+}
+
+// w = y - a v - b vp (vp null: first step); partials of |w|^2
+__global__ __launch_bounds__(256) void k_lz_update(int n, const double *__restrict__ y, const double *__restrict__ v,
+                                                   const double *__restrict__ vp, double a, double b,
+                                                   double *__restrict__ w, double *__restrict__ part) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double t = 0.0;
+    if (i < n) {
+        const double wi = y[i] - a * v[i] - (vp ? b * vp[i] : 0.0);
+        w[i] = wi;
+        t = wi * wi;
+    }
+    block_sum256(t, part);
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -2087,10 +2233,20 @@ hipError_t launch_fill(hipStream_t st, const DevTopo &T, int nrhs, const DevCell
                        const uint8_t *ftype, const double *bcval, const int *inst_reg,
                        int dguess, double *val64, float *val32, double *b, double *dinv64,
                        float *dinv32, double *x0) {
+    // rows of up to 24 entries: 256 threads and 48 KB of LDS; up to 128: 64
+    // threads and 64 KB; wider rows are summed in place in val64
+    const size_t w = (size_t)std::max(T.max_w, 1);
     return with_nrhs(nrhs, [&](auto nr) {
         constexpr int NR = decltype(nr)::value;
-        k_fill<NR><<<T.nfp / 256, 256, 0, st>>>(T, C, ftype, bcval, inst_reg, dguess, val64, val32, b,
-                                           dinv64, dinv32, x0);
+        if (w <= 24)
+            k_fill<NR, 256, true><<<T.nfp / 256, 256, w * 256 * sizeof(double), st>>>(
+                T, C, ftype, bcval, inst_reg, dguess, val64, val32, b, dinv64, dinv32, x0);
+        else if (w <= 128)
+            k_fill<NR, 64, true><<<T.nfp / 64, 64, w * 64 * sizeof(double), st>>>(
+                T, C, ftype, bcval, inst_reg, dguess, val64, val32, b, dinv64, dinv32, x0);
+        else
+            k_fill<NR, 256, false><<<T.nfp / 256, 256, 0, st>>>(T, C, ftype, bcval, inst_reg, dguess, val64,
+                                                                 val32, b, dinv64, dinv32, x0);
     });
 }
 
@@ -2187,13 +2343,15 @@ hipError_t launch_amg_setup(hipStream_t st, const DevTopo &T, const DevAmg &A, c
         const DevAmgLevel &L = A.lev[l];
         const DevAmgLevel &F = A.lev[l - 1];
         if (L.nnz > 0) {
-            if (L.sa_in)
-                k_pair_sum<<<(L.nnz + 255) / 256, 256, 0, st>>>(L.nnz, L.src_ptr, L.src, L.src2, F.p_val64,
-                                                               F.t_val64, L.val64, mixed ? L.val32 : nullptr);
-            else
-                k_galerkin<<<(L.nnz + 255) / 256, 256, 0, st>>>(L.nnz, L.src_ptr, L.src,
-                                                               (l == 1) ? val64 : F.val64, L.val64,
-                                                               mixed ? L.val32 : nullptr);
+            with_lanes(lanes_per_row(L.nsrc, L.nnz), [&](auto g) {
+                constexpr int Gv = decltype(g)::value;
+                if (L.sa_in)
+                    k_pair_sum<Gv><<<row_blocks(L.nnz, Gv), 256, 0, st>>>(
+                        L.nnz, L.src_ptr, L.src, L.src2, F.p_val64, F.t_val64, L.val64, mixed ? L.val32 : nullptr);
+                else
+                    k_galerkin<Gv><<<row_blocks(L.nnz, Gv), 256, 0, st>>>(
+                        L.nnz, L.src_ptr, L.src, (l == 1) ? val64 : F.val64, L.val64, mixed ? L.val32 : nullptr);
+            });
         }
         if (L.n > 0)
             k_coarse_diag<<<(L.n + 255) / 256, 256, 0, st>>>(L.n, L.diag, L.val64, L.dinv64,
@@ -2203,29 +2361,62 @@ hipError_t launch_amg_setup(hipStream_t st, const DevTopo &T, const DevAmg &A, c
             if (!mixed) Lw.p_val32 = nullptr;
             k_sa_p<<<(L.n + 255) / 256, 256, 0, st>>>(Lw, A.omega_p);
             if (L.t_nnz > 0)
-                k_pair_sum<<<(L.t_nnz + 255) / 256, 256, 0, st>>>(L.t_nnz, L.t_pair_ptr, L.t_pa, L.t_pb, L.val64,
-                                                                 L.p_val64, L.t_val64,
-                                                                 mixed ? L.t_val32 : nullptr);
+                with_lanes(lanes_per_row(L.t_npair, L.t_nnz), [&](auto g) {
+                    constexpr int Gv = decltype(g)::value;
+                    k_pair_sum<Gv><<<row_blocks(L.t_nnz, Gv), 256, 0, st>>>(
+                        L.t_nnz, L.t_pair_ptr, L.t_pa, L.t_pb, L.val64, L.p_val64, L.t_val64,
+                        mixed ? L.t_val32 : nullptr);
+                });
         }
     }
     if (A.ap_slots > 0)
-        k_galerkin<<<(unsigned)((A.ap_slots + 255) / 256), 256, 0, st>>>((int)A.ap_slots, A.ap_src_ptr, A.ap_src,
-                                                                        val64, A.ap64, mixed ? A.ap32 : nullptr);
+        with_lanes(lanes_per_row(A.ap_nsrc, (int)A.ap_slots), [&](auto g) {
+            constexpr int Gv = decltype(g)::value;
+            k_galerkin<Gv><<<row_blocks((int)A.ap_slots, Gv), 256, 0, st>>>(
+                (int)A.ap_slots, A.ap_src_ptr, A.ap_src, val64, A.ap64, mixed ? A.ap32 : nullptr);
+        });
     const DevAmgLevel &C = A.lev[A.nlev - 1];
     k_coarse_inverse<<<A.ninst, 256, 0, st>>>(C, C.inst_ptr, A.coarse_off, A.cinv64,
                                               mixed ? A.cinv32 : nullptr);
     return hipGetLastError();
 }
 
+static inline unsigned lz_blocks(int n) { return (unsigned)((n + 255) / 256); }
+
+hipError_t launch_lanczos_init(hipStream_t st, int n, double *w, double *part) {
+    k_lz_init<<<lz_blocks(n), 256, 0, st>>>(n, w, part);
+    return hipGetLastError();
+}
+
+hipError_t launch_lanczos_step(hipStream_t st, const DevTopo &T, const DevAmg &A, int level, const double *val0,
+                               const double *dinv0, const double *w, double scale, double *v, double *u,
+                               double *y, double *part) {
+    const int n = level == 0 ? T.nfp : A.lev[level].n;
+    const double *dinv = level == 0 ? dinv0 : A.lev[level].dinv64;
+    k_lz_prep<<<lz_blocks(n), 256, 0, st>>>(n, dinv, w, scale, v, u);
+    if (level == 0)
+        k_lz_sell<<<lz_blocks(n), 256, 0, st>>>(T, val0, dinv, u, v, y, part);
+    else
+        k_lz_csr<<<lz_blocks(n), 256, 0, st>>>(A.lev[level], u, v, y, part);
+    return hipGetLastError();
+}
+
+hipError_t launch_lanczos_update(hipStream_t st, int n, const double *y, const double *v, const double *vp,
+                                 double a, double b, double *w, double *part) {
+    k_lz_update<<<lz_blocks(n), 256, 0, st>>>(n, y, v, vp, a, b, w, part);
+    return hipGetLastError();
+}
+
 // The fused post-smoothing sweep (k_vc0_post) in the tuned workgroup shape.
 template <typename VT, int NR>
-static void post0(hipStream_t st, const DevTopo &T, const DevAmg &A, const VT *dinv0, const VT *r, const VT *x0,
+static void post0(hipStream_t st, const Gate &gt, const DevTopo &T, const DevAmg &A, const VT *dinv0, const VT *r,
+                  const VT *x0,
                   const VT *res0, const VT *xc, VT *z, VT w, VT al, DevRed R) {
     const VT *ap;
     if constexpr (std::is_same<VT, float>::value) ap = A.ap32; else ap = A.ap64;
     with_tb(T.sweep_tb, [&](auto tb) {
         constexpr int TBv = decltype(tb)::value;
-        k_vc0_post<VT, NR, TBv><<<T.nblk, TBv, 0, st>>>(T, A.ap_slice_ptr, A.ap_col, ap, A.lev[1].agg, dinv0, r,
+        k_vc0_post<VT, NR, TBv><<<T.nblk, TBv, 0, st>>>(gt, T, A.ap_slice_ptr, A.ap_col, ap, A.lev[1].agg, dinv0, r,
                                                         x0, res0, xc, z, w, al, R);
     });
 }
@@ -2235,8 +2426,8 @@ hipError_t launch_post0(hipStream_t st, const DevTopo &T, const DevAmg &A, int n
                         const VT *r, VT *z, DevRed R) {
     return with_nrhs(nrhs, [&](auto nr) {
         constexpr int NR = decltype(nr)::value;
-        post0<VT, NR>(st, T, A, dinv0, r, (const VT *)A.x0, (const VT *)A.res0, (const VT *)A.lev[1].x, z,
-                      (VT)A.omega, (VT)A.alpha, R);
+        post0<VT, NR>(st, Gate{}, T, A, dinv0, r, (const VT *)A.x0, (const VT *)A.res0, (const VT *)A.lev[1].x, z,
+                      (VT)A.omega_l[0], (VT)A.alpha, R);
     });
 }
 template hipError_t launch_post0<double>(hipStream_t, const DevTopo &, const DevAmg &, int, const double *,
@@ -2256,21 +2447,26 @@ static void sweep0(hipStream_t st, const DevTopo &T, const VT *val0, const VT *d
     });
 }
 
+// gt: inside the PCG loop the V-cycle skips the level-0 blocks and the tail
+// workgroups of finished instances.  The other coarse-level kernels are too
+// short for a gate to pay for its extra memory round trip: they run on, on
+// values nothing reads.
 template <typename VT, int NR>
-static void vcycle_impl(hipStream_t st, const DevTopo &T, const DevAmg &A, const VT *val0,
+static void vcycle_impl(hipStream_t st, const Gate &gt, const DevTopo &T, const DevAmg &A, const VT *val0,
                         const VT *dinv0, const VT *r, VT *z, DevRed R, bool x0_ready) {
-    const VT w = (VT)A.omega, al = (VT)A.alpha;
+    const VT w = (VT)A.omega_l[0], al = (VT)A.alpha;   // w: level 0
+    auto wl = [&](int l) { return (VT)A.omega_l[l]; };
     const int nu = A.nu < 1 ? 1 : A.nu;
     const int rows_blocks = T.nfp / 256;
     VT *x0 = (VT *)A.x0, *xt0 = (VT *)A.xt0, *res0 = (VT *)A.res0;
     const int L = A.nlev;
     // ---- level 0 down: x0 = w D^-1 r (+ nu-1 sweeps), res0 = r - A x0
-    if (!x0_ready) k_scale0<VT, NR><<<rows_blocks, 256, 0, st>>>(dinv0, r, x0, w);
+    if (!x0_ready) k_scale0<VT, NR><<<rows_blocks, 256, 0, st>>>(gt, T.block_seg, dinv0, r, x0, w);
     for (int it = 1; it < nu; ++it) {
         sweep0<VT, NR, false>(st, T, val0, dinv0, r, x0, xt0, w, R);
         std::swap(x0, xt0);
     }
-    k_vc0_res<VT, NR><<<rows_blocks, 256, 0, st>>>(T, val0, r, x0, res0);
+    k_vc0_res<VT, NR><<<rows_blocks, 256, 0, st>>>(gt, T, val0, r, x0, res0);
     {
         const DevAmgLevel &C = A.lev[1];
         with_lanes(lanes_per_row(T.nfp, C.n), [&](auto g) {
@@ -2295,7 +2491,7 @@ static void vcycle_impl(hipStream_t st, const DevTopo &T, const DevAmg &A, const
             with_lanes(lanes_per_row(F.nnz, F.n), [&](auto g) {
                 constexpr int Gv = decltype(g)::value;
                 k_vcc_down<VT, NR, Gv><<<row_blocks(F.n, Gv), 256, 0, st>>>(F, (const VT *)F.b, (VT *)F.x,
-                                                                          (VT *)F.res, w);
+                                                                          (VT *)F.res, wl(l));
             });
             if (F.sa_out) {
                 with_lanes(lanes_per_row(F.p_nnz, C.n), [&](auto g) {
@@ -2310,20 +2506,20 @@ static void vcycle_impl(hipStream_t st, const DevTopo &T, const DevAmg &A, const
                 });
             }
         }
-        k_vcycle_tail<VT, NR><<<A.ninst, 1024, 0, st>>>(A.dev, lt, w, al, als);
+        k_vcycle_tail<VT, NR><<<A.ninst, 1024, 0, st>>>(gt, A.dev, lt, al, als);
         for (int l = lt - 1; l >= 1; --l) {
             const DevAmgLevel &F = A.lev[l];
             const VT *xc = (const VT *)A.lev[l + 1].x;
             if (F.sa_out) {
                 with_lanes(lanes_per_row(std::max(F.p_nnz, F.t_nnz), F.n), [&](auto g) {
                     constexpr int Gv = decltype(g)::value;
-                    k_vcc_up_sa<VT, NR, Gv><<<row_blocks(F.n, Gv), 256, 0, st>>>(F, xc, w, als);
+                    k_vcc_up_sa<VT, NR, Gv><<<row_blocks(F.n, Gv), 256, 0, st>>>(F, xc, wl(l), als);
                 });
             } else {
                 with_lanes(lanes_per_row(F.nnz, F.n), [&](auto g) {
                     constexpr int Gv = decltype(g)::value;
                     k_vcc_up1<VT, NR, Gv><<<row_blocks(F.n, Gv), 256, 0, st>>>(F, A.lev[l + 1].agg, xc, (VT *)F.x,
-                                                                             w, al);
+                                                                             wl(l), al);
                 });
             }
         }
@@ -2334,9 +2530,9 @@ static void vcycle_impl(hipStream_t st, const DevTopo &T, const DevAmg &A, const
             const DevAmgLevel &F = A.lev[l];
             const DevAmgLevel &C = A.lev[l + 1];
             const unsigned nb = (F.n + 255) / 256;
-            k_vcc_down<VT, NR><<<nb, 256, 0, st>>>(F, (const VT *)F.b, xcur[l], (VT *)F.res, w);
+            k_vcc_down<VT, NR><<<nb, 256, 0, st>>>(F, (const VT *)F.b, xcur[l], (VT *)F.res, wl(l));
             for (int it = 1; it < nu; ++it) {
-                k_vcc_up<VT, NR><<<nb, 256, 0, st>>>(F, nullptr, (const VT *)F.b, xcur[l], nullptr, xalt[l], w,
+                k_vcc_up<VT, NR><<<nb, 256, 0, st>>>(F, nullptr, (const VT *)F.b, xcur[l], nullptr, xalt[l], wl(l),
                                                      VT(0));
                 std::swap(xcur[l], xalt[l]);
             }
@@ -2359,22 +2555,22 @@ static void vcycle_impl(hipStream_t st, const DevTopo &T, const DevAmg &A, const
             const DevAmgLevel &F = A.lev[l];
             const unsigned nb = (F.n + 255) / 256;
             if (nu == 1) {
-                k_vcc_up1<VT, NR><<<nb, 256, 0, st>>>(F, A.lev[l + 1].agg, xcur[l + 1], xalt[l], w, al);
+                k_vcc_up1<VT, NR><<<nb, 256, 0, st>>>(F, A.lev[l + 1].agg, xcur[l + 1], xalt[l], wl(l), al);
                 std::swap(xcur[l], xalt[l]);
                 continue;
             }
             if (F.sa_out) {
                 k_prolong_sa<VT, NR><<<nb, 256, 0, st>>>(F, xcur[l], xcur[l + 1], (VT)A.alpha_sa, xalt[l]);
                 std::swap(xcur[l], xalt[l]);
-                k_vcc_up<VT, NR><<<nb, 256, 0, st>>>(F, nullptr, (const VT *)F.b, xcur[l], nullptr, xalt[l], w,
+                k_vcc_up<VT, NR><<<nb, 256, 0, st>>>(F, nullptr, (const VT *)F.b, xcur[l], nullptr, xalt[l], wl(l),
                                                      VT(0));
             } else {
                 k_vcc_up<VT, NR><<<nb, 256, 0, st>>>(F, A.lev[l + 1].agg, (const VT *)F.b, xcur[l],
-                                                     xcur[l + 1], xalt[l], w, al);
+                                                     xcur[l + 1], xalt[l], wl(l), al);
             }
             std::swap(xcur[l], xalt[l]);
             for (int it = 1; it < nu; ++it) {
-                k_vcc_up<VT, NR><<<nb, 256, 0, st>>>(F, nullptr, (const VT *)F.b, xcur[l], nullptr, xalt[l], w,
+                k_vcc_up<VT, NR><<<nb, 256, 0, st>>>(F, nullptr, (const VT *)F.b, xcur[l], nullptr, xalt[l], wl(l),
                                                      VT(0));
                 std::swap(xcur[l], xalt[l]);
             }
@@ -2382,7 +2578,7 @@ static void vcycle_impl(hipStream_t st, const DevTopo &T, const DevAmg &A, const
     }
     // ---- level 0 up: x~ = x0 + alpha P x1, then nu sweeps; the last writes z and <r, z>
     if (nu == 1) {
-        post0<VT, NR>(st, T, A, dinv0, r, x0, res0, xcur[1], z, w, al, R);
+        post0<VT, NR>(st, gt, T, A, dinv0, r, x0, res0, xcur[1], z, w, al, R);
         return;
     }
     k_prolong0<VT, NR><<<rows_blocks, 256, 0, st>>>(A.lev[1].agg, x0, xcur[1], xt0, al);
@@ -2412,7 +2608,7 @@ hipError_t launch_vcycle(hipStream_t st, const DevTopo &T, const DevAmg &A, int 
                          const VT *dinv0, const VT *r, VT *z, DevRed R) {
     return with_nrhs(nrhs, [&](auto nr) {
         constexpr int NR = decltype(nr)::value;
-        vcycle_impl<VT, NR>(st, T, A, val0, dinv0, r, z, R, false);
+        vcycle_impl<VT, NR>(st, Gate{}, T, A, val0, dinv0, r, z, R, false);
     });
 }
 template hipError_t launch_vcycle<double>(hipStream_t, const DevTopo &, const DevAmg &, int, const double *,
@@ -2436,9 +2632,9 @@ hipError_t launch_pcg_iteration_amg(hipStream_t st, const DevTopo &T, const DevA
             k_spmv_pq<MT, VT, NR, REL, TBv><<<T.nblk, TBv, 0, st>>>(T, val, p, q, R, S, L);
         });
         k_finish<REL ? 4 * NR : NR, FinPQ<NR, REL>><<<T.ninst, 256, 0, st>>>(R, S, Af);
-        k_update_xr<VT, NR><<<T.nblk, 256, 0, st>>>(T, dinv, x, r, p, q, R, S, (VT *)A.x0, (VT)A.omega);
+        k_update_xr<VT, NR><<<T.nblk, 256, 0, st>>>(T, dinv, x, r, p, q, R, S, (VT *)A.x0, (VT)A.omega_l[0]);
         k_finish<2 * NR, FinXR<NR>><<<T.ninst, 256, 0, st>>>(R, S, Af);
-        vcycle_impl<VT, NR>(st, T, A, val, dinv, r, z, R, true);
+        vcycle_impl<VT, NR>(st, Gate{S.act}, T, A, val, dinv, r, z, R, true);
         k_finish<NR, FinRZ<NR>><<<T.ninst, 256, 0, st>>>(R, S, Af);
         k_update_pz<VT, NR><<<T.nblk, 256, 0, st>>>(T, z, p, x, REL ? x64 : nullptr, S);
     });
@@ -2459,7 +2655,7 @@ hipError_t launch_amg_start(hipStream_t st, const DevTopo &T, const DevAmg &A, i
                             double *x64, VT *x) {
     return with_nrhs(nrhs, [&](auto nr) {
         constexpr int NR = decltype(nr)::value;
-        vcycle_impl<VT, NR>(st, T, A, val0, dinv0, r, z, R, false);
+        vcycle_impl<VT, NR>(st, Gate{}, T, A, val0, dinv0, r, z, R, false);
         k_finish<NR, FinRZ<NR>><<<T.ninst, 256, 0, st>>>(R, S, FinArgs{0.0, 1});
         k_update_pz<VT, NR><<<T.nblk, 256, 0, st>>>(T, z, p, x, nullptr, S);
     });

@@ -56,10 +56,19 @@ class Report(ctypes.Structure):
                 ("recover_ms", ctypes.c_double), ("spmv_threads", ctypes.c_int),
                 ("spmv_xcd", ctypes.c_int), ("sweep_threads", ctypes.c_int),
                 ("sweep_xcd", ctypes.c_int), ("tail_rows", ctypes.c_int),
-                ("tune_ms", ctypes.c_double)]
+                ("tune_ms", ctypes.c_double), ("smoother_weights", ctypes.c_double * 8),
+                ("coarsening", (ctypes.c_int * 3) * 8)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        out = {}
+        for k, _ in self._fields_:
+            v = getattr(self, k)
+            if k == "smoother_weights":
+                v = [x for x in v if x > 0.0]
+            elif k == "coarsening":
+                v = [tuple(b) for b in v if b[0] > 0]
+            out[k] = v
+        return out
 
 
 class Dims(ctypes.Structure):
@@ -249,7 +258,7 @@ class HybridSolver:
     # --- solve ----------------------------------------------------------------------
     def options(self, tol=1e-8, linsolver_verbosity=0, linsolver_type=1, same_matrix=False,
                 linsolver_maxit=0, prolongate_factor=1.6, smooth_steps=1, precision=FP64,
-                nrhs=1, chunk=32, replace_factor=0.1, use_graph=True, precond=-1,
+                nrhs=1, chunk=32, replace_factor=0.01, use_graph=True, precond=-1,
                 smoother_weight=None, autotune=True, spmv_threads=0, sweep_threads=0,
                 tail_rows=-1):
         o = Options()
@@ -272,7 +281,7 @@ class HybridSolver:
     def solve(self, totmob, rhomob, src, bc_type, bc_value, residual_tolerance=1e-8,
               linsolver_verbosity=0, linsolver_type=1, same_matrix=False, linsolver_maxit=0,
               prolongate_factor=1.6, smooth_steps=1, precision="fp64", chunk=32,
-              replace_factor=0.1, use_graph=True, raise_on_failure=True, precond=-1,
+              replace_factor=0.01, use_graph=True, raise_on_failure=True, precond=-1,
               smoother_weight=None, autotune=True, spmv_threads=0, sweep_threads=0,
               tail_rows=-1):
         """IncompFlowSolverHybrid::solve (:657-703) with per-cell mobilities."""
