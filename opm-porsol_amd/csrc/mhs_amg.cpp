@@ -312,6 +312,7 @@ std::string build_amg(AmgHierarchy &H, const Topology &T, const std::vector<int>
         H.lev.push_back(std::move(L1));
     }
     // ---- coarser levels: group the blocks again until every instance is small
+    bool sa_done = false;
     for (size_t l = 1; l < H.blocks.size(); ++l) {
         const AmgLevel &F = H.lev.back();
         const std::array<int, 3> &bl = H.blocks[l];
@@ -338,10 +339,13 @@ std::string build_amg(AmgHierarchy &H, const Topology &T, const std::vector<int>
         }
         AmgLevel &Fr = H.lev.back();
         children(C, F.n);
-        // smooth the transfer from level 1 only: most of the gain in
-        // iterations, while the deeper levels keep the sparse stencils of
-        // piecewise-constant aggregation (smoothing widens every stencil)
-        if (smoothed && H.lev.size() == 2) {
+        // smooth one transfer: the first from a level with at most an eighth
+        // as many rows as there are faces (cheap next to the face level),
+        // most of the gain in iterations, while the other levels keep the
+        // sparse stencils of piecewise-constant aggregation (smoothing widens
+        // every stencil)
+        if (smoothed && !sa_done && 8LL * Fr.n <= (long long)T.nf_total) {
+            sa_done = true;
             sa_transfer(Fr, C);
         } else {
             galerkin_pattern(C, [&](int r, auto emit) {

@@ -89,8 +89,9 @@ std::string instance_dims(const Topology &T, const std::vector<int> &ijk, std::v
 // Builds the hierarchy from logical cell indices ijk[3*nc] (instance-local,
 // 0-based) with the blocks of coarsening_plan.  The transfer from the face
 // matrix to level 1 is the tentative (piecewise-constant) one; with smoothed
-// = true the transfer from level 1 to level 2 is smoothed, the others stay
-// tentative.  Returns an empty string on success.
+// = true one coarse transfer is smoothed (the first from a level with at most
+// an eighth as many rows as faces), the others stay tentative.  Returns an
+// empty string on success.
 std::string build_amg(AmgHierarchy &H, const Topology &T, const std::vector<int> &ijk,
                       const Strength &strength = {1.0, 1.0, 1.0}, bool smoothed = true);
 

@@ -66,3 +66,54 @@ def test_multigrid_beats_jacobi(pkg, batch):
         its[pc, sw] = s.report.iterations
     assert its["amg", 1] * 4 < its["jacobi", 1], its
     assert its["amg", 2] <= its["amg", 1], its
+
+
+def _aniso_perm(nc, kx, kz):
+    K = np.zeros((nc, 9))
+    K[:, 0] = K[:, 4] = kx
+    K[:, 8] = kz
+    return K
+
+
+@pytest.mark.parametrize("spacing,kz,first", [
+    ((1.0, 1.0, 1.0), 1.0, (2, 2, 2)),     # isotropic: full coarsening
+    ((1.0, 1.0, 1.0), 0.1, (2, 2, 1)),     # weak vertical coupling: coarsen in x and y first
+    ((10.0, 10.0, 1.0), 1.0, (1, 1, 2)),   # thin layers: vertical coupling dominates
+])
+def test_semi_coarsening_follows_coupling_strength(pkg, orc, spacing, kz, first):
+    """Aggregate shapes from K and the geometry (mean two-point half-
+    transmissibility along i, j, k), and per-level Jacobi weights 4/(3 rho)
+    with rho from Lanczos steps; the solution still matches the oracle."""
+    g = pkg.grids.cartesian(24, 24, 16, *spacing)
+    nc = g.num_cells
+    perm = _aniso_perm(nc, 1e-12, kz * 1e-12)
+    bt, bv = cases.xdrop_bc(g)
+    tm, rm = cases.unit_fluid(nc)
+    s = pkg.HybridSolver(0)
+    s.init(g, perm, (0, 0, 0), bt, None)
+    s.solve(tm, rm, np.zeros(nc), bt, bv, residual_tolerance=1e-11, precision="mixed")
+    r = s.report.as_dict()
+    assert r["precond"] == 1
+    assert r["coarsening"][0] == first, r["coarsening"]
+    # Cartesian face matrices: rho(D^-1 A) is about 2 to 2.5 on the face level,
+    # below 2 on the coarse levels
+    w = r["smoother_weights"]
+    assert len(w) == r["levels"] - 1
+    assert 0.45 < w[0] < 0.75, w
+    assert all(0.2 <= x <= 1.0 for x in w), w
+    ref = orc.solve(g, perm, tm, rm, np.zeros(nc), bt, bv, tol=1e-14)
+    assert cases.rel_err(ref.outflux, s.getSolution()[2]) < 1e-8
+
+
+def test_forced_smoother_weight(pkg):
+    g, perm, tm, rm, src, bt, bv = cases.case_c1()
+    s = pkg.HybridSolver(0)
+    s.init(g, perm, (0, 0, 0), bt, None)
+    s.solve(tm, rm, src, bt, bv, residual_tolerance=1e-10, smoother_weight=0.6)
+    w = s.report.as_dict()["smoother_weights"]
+    assert w and all(x == 0.6 for x in w), w
+    it_forced = s.report.iterations
+    s.solve(tm, rm, src, bt, bv, residual_tolerance=1e-10)
+    w2 = s.report.as_dict()["smoother_weights"]
+    assert w2 != w and s.report.converged
+    assert it_forced > 0
