@@ -110,7 +110,7 @@ struct mhs_solver {
     int last_nrhs = 0;
     bool last_amg = false;
     long long solve_count = 0;
-    GraphCache graph;
+    std::map<int, GraphCache> graphs;   // captured PCG chunks by iteration count
     std::map<int, Shape> shapes;        // autotuned launch shapes of this structure
     // adjoint work arrays (allocated on the first adjoint call)
     std::vector<DevBuf> adj_bufs;
@@ -179,8 +179,9 @@ void mhs_solver::free_topology() {
 }
 
 void mhs_solver::free_solve_buffers() {
-    if (graph.exec) (void)hipGraphExecDestroy(graph.exec);
-    graph = GraphCache{};
+    for (auto &g : graphs)
+        if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
+    graphs.clear();
     for (auto &b : solve_bufs) (void)hipFree(b.p);
     solve_bufs.clear();
     for (auto &b : adj_bufs) (void)hipFree(b.p);
@@ -692,7 +693,7 @@ template <typename MT, typename VT>
 int run_chunk(mhs_solver *s, int nrhs, int chunk, bool use_graph, const MT *val, const VT *dinv,
               VT *x, VT *r, VT *p, VT *q, VT *z, bool amg, int precision, double tol2) {
     if (use_graph) {
-        GraphCache &G = s->graph;
+        GraphCache &G = s->graphs[chunk];
         const int shape[5] = {s->dT.spmv_tb, s->dT.spmv_xcd, s->dT.sweep_tb, s->dT.sweep_xcd, s->dT.tail_rows};
         if (!(G.exec && G.precision == precision && G.nrhs == nrhs && G.chunk == chunk &&
               G.key == (const void *)val && G.tol2 == tol2 && G.rf2 == s->S.rf2 && G.amg == int(amg) &&
@@ -940,6 +941,28 @@ struct LinearResult {
 // PCG on the assembled face system from the right-hand side in s->b and the
 // initial iterate in s->x64, to opt.tol relative FP64 residual (Jacobi or
 // multigrid preconditioner, FP64 or FP32 with FP64 residual replacement).
+// PCG iterations the slowest active system still needs, from its mean
+// residual reduction per iteration so far (the mixed iteration needs one more
+// for the FP64 replacement that confirms convergence); 0 while unknown.
+int remaining_iterations(const mhs_solver *s, size_t nsys, double tol, bool mixed) {
+    int need = 0;
+    for (size_t i = 0; i < nsys; ++i) {
+        if (!s->h_act[i]) continue;
+        const double *sc = s->h_sc + i * SC_N;
+        const int it = s->h_iters[i];
+        if (it < 8 || !(sc[SC_RR0] > 0.0)) return 0;
+        const double q = std::sqrt(sc[SC_RR] / sc[SC_RR0]);
+        int r = 1;
+        if (q > tol) {
+            const double rate = std::pow(q, 1.0 / it);
+            if (!(rate < 1.0)) return 0;
+            r = (int)std::ceil(std::log(tol / q) / std::log(rate));
+        }
+        need = std::max(need, r + (mixed ? 1 : 0));
+    }
+    return need;
+}
+
 int linear_solve(mhs_solver *s, const mhs_solve_options &opt, int nrhs, bool mixed, bool amg,
                  LinearResult &out) {
     const Topology &T = s->T;
@@ -969,21 +992,31 @@ int linear_solve(mhs_solver *s, const mhs_solve_options &opt, int nrhs, bool mix
             HIPCHECK(launch_amg_start<float>(s->stream, s->dT, s->dA, nrhs, s->val32, s->dinv32, s->r32,
                                              s->z32, s->p32, s->R, s->S, nullptr, s->x32));
     }
-    int rc = read_state(s, nsys, false);
+    int rc = read_state(s, nsys, true);
     if (rc) return rc;
     while (s->h_status[0] > 0) {
         int mx = 0;
         for (size_t i = 0; i < nsys; ++i) mx = std::max(mx, s->h_iters[i]);
         if (mx >= maxit) break;
-        const int n = std::min(chunk, maxit - mx);
+        // after the first chunk, launch about as many iterations as the slowest
+        // system still needs (rounded down to a power of two: no iterations
+        // after convergence, a few more state reads at the end)
+        int n = std::min(chunk, maxit - mx);
+        const int need = remaining_iterations(s, nsys, opt.tol, mixed);
+        if (need > 0) {
+            int p2 = 1;
+            while (2 * p2 <= need) p2 *= 2;
+            n = std::min(n, p2);
+        }
+        const bool graph = opt.use_graph && (n == chunk || (n & (n - 1)) == 0);
         if (!mixed)
-            rc = run_chunk<double, double>(s, nrhs, n, opt.use_graph && n == chunk, s->val64, s->dinv64,
-                                           s->x64, s->r64, s->p64, s->q64, s->z64, amg, MHS_FP64, tol2);
+            rc = run_chunk<double, double>(s, nrhs, n, graph, s->val64, s->dinv64, s->x64, s->r64, s->p64,
+                                           s->q64, s->z64, amg, MHS_FP64, tol2);
         else
-            rc = run_chunk<float, float>(s, nrhs, n, opt.use_graph && n == chunk, s->val32, s->dinv32,
-                                         s->x32, s->r32, s->p32, s->q32, s->z32, amg, MHS_MIXED, tol2);
+            rc = run_chunk<float, float>(s, nrhs, n, graph, s->val32, s->dinv32, s->x32, s->r32, s->p32,
+                                         s->q32, s->z32, amg, MHS_MIXED, tol2);
         if (rc) return rc;
-        rc = read_state(s, nsys, opt.verbosity >= 2);
+        rc = read_state(s, nsys, true);
         if (rc) return rc;
         if (opt.verbosity >= 2)
             for (size_t i = 0; i < std::min<size_t>(nsys, 4); ++i) {

@@ -515,7 +515,7 @@ __global__ __launch_bounds__(TB) void k_fill(DevTopo T, DevCellSys C,
 // replacement is due (upd != 0) also get r = b - A64 x64 in the same pass:
 // the row's FP64 residual replaces the FP32 recursive one, and the partials
 // carry |r64|^2, r.M^-1 r and |r|^2 of the replaced residual.
-template <typename MT, typename VT, int NRHS, bool REL, int TB>
+template <typename MT, typename VT, int NRHS, bool REL, int TB, int CH>
 __global__ __launch_bounds__(TB) void k_spmv_pq(DevTopo T, const MT *__restrict__ val,
                                                 const VT *__restrict__ p, VT *__restrict__ q,
                                                 DevRed R, DevScal S, RelArgs L) {
@@ -538,7 +538,7 @@ __global__ __launch_bounds__(TB) void k_spmv_pq(DevTopo T, const MT *__restrict_
     for (int j = 0; j < kRowBlock / TB; ++j) {
         const int f = b * kRowBlock + j * TB + threadIdx.x;
         VT acc[NRHS];
-        sell_row<MT, VT, VT, NRHS>(T, val, p, f, acc);
+        sell_row<MT, VT, VT, NRHS, CH>(T, val, p, f, acc);
 #pragma unroll
         for (int k = 0; k < NRHS; ++k) {
             q[(size_t)f * NRHS + k] = acc[k];
@@ -1262,14 +1262,14 @@ __global__ __launch_bounds__(TB) void k_vc0_post(Gate gt, DevTopo T, const long 
 }
 
 // Level 0: res = b - A x (after extra down sweeps).
-template <typename VT, int NRHS>
+template <typename VT, int NRHS, int CH>
 __global__ __launch_bounds__(256) void k_vc0_res(Gate gt, DevTopo T, const VT *__restrict__ val,
                                                  const VT *__restrict__ b, const VT *__restrict__ x,
                                                  VT *__restrict__ res) {
     const int f = blockIdx.x * 256 + threadIdx.x;
     if (gate_off<NRHS>(gt, T.block_seg[f / kRowBlock])) return;
     VT acc[NRHS];
-    sell_row<VT, VT, VT, NRHS>(T, val, x, f, acc);
+    sell_row<VT, VT, VT, NRHS, CH>(T, val, x, f, acc);
 #pragma unroll
     for (int k = 0; k < NRHS; ++k) res[(size_t)f * NRHS + k] = b[(size_t)f * NRHS + k] - acc[k];
 }
@@ -2201,6 +2201,15 @@ static void with_lanes(int g, Fn fn) {
 }
 static inline unsigned row_blocks(int n, int g) { return (unsigned)(((long long)n * g + 255) / 256); }
 
+// Entries per chunk of the streaming face-matrix rows: one chunk of 12 when
+// every slice fits (hexahedral grids: 11 entries), else chunks of 16 (fewer
+// registers than 16 for the common width, more loads in flight for wide rows).
+template <class Fn>
+static void with_chunk(int max_w, Fn fn) {
+    if (max_w <= 12) fn(IC<12>{});
+    else fn(IC<16>{});
+}
+
 hipError_t launch_static_ip(hipStream_t st, const DevTopo &T, const DevGeom &G, int maxn,
                             const double *perm, const double *grav, double *binv_pk,
                             double *gflux) {
@@ -2263,7 +2272,9 @@ hipError_t launch_pcg_iteration(hipStream_t st, const DevTopo &T, int nrhs, cons
         if constexpr (REL) L = RelArgs{val64, x64, b, dinv, r};
         with_tb(T.spmv_tb, [&](auto tb) {
             constexpr int TBv = decltype(tb)::value;
-            k_spmv_pq<MT, VT, NR, REL, TBv><<<T.nblk, TBv, 0, st>>>(T, val, p, q, R, S, L);
+            with_chunk(T.max_w, [&](auto ch) {
+                k_spmv_pq<MT, VT, NR, REL, TBv, decltype(ch)::value><<<T.nblk, TBv, 0, st>>>(T, val, p, q, R, S, L);
+            });
         });
         k_finish<REL ? 4 * NR : NR, FinPQ<NR, REL>><<<T.ninst, 256, 0, st>>>(R, S, A);
         k_update_xr<VT, NR><<<T.nblk, 256, 0, st>>>(T, dinv, x, r, p, q, R, S);
@@ -2291,7 +2302,9 @@ hipError_t launch_spmv(hipStream_t st, const DevTopo &T, int nrhs, const MT *val
         constexpr int NR = decltype(nr)::value;
         with_tb(T.spmv_tb, [&](auto tb) {
             constexpr int TBv = decltype(tb)::value;
-            k_spmv_pq<MT, VT, NR, false, TBv><<<T.nblk, TBv, 0, st>>>(T, val, p, q, R, S, RelArgs{});
+            with_chunk(T.max_w, [&](auto ch) {
+                k_spmv_pq<MT, VT, NR, false, TBv, decltype(ch)::value><<<T.nblk, TBv, 0, st>>>(T, val, p, q, R, S, RelArgs{});
+            });
         });
     });
 }
@@ -2466,7 +2479,9 @@ static void vcycle_impl(hipStream_t st, const Gate &gt, const DevTopo &T, const 
         sweep0<VT, NR, false>(st, T, val0, dinv0, r, x0, xt0, w, R);
         std::swap(x0, xt0);
     }
-    k_vc0_res<VT, NR><<<rows_blocks, 256, 0, st>>>(gt, T, val0, r, x0, res0);
+    with_chunk(T.max_w, [&](auto ch) {
+        k_vc0_res<VT, NR, decltype(ch)::value><<<rows_blocks, 256, 0, st>>>(gt, T, val0, r, x0, res0);
+    });
     {
         const DevAmgLevel &C = A.lev[1];
         with_lanes(lanes_per_row(T.nfp, C.n), [&](auto g) {
@@ -2629,7 +2644,9 @@ hipError_t launch_pcg_iteration_amg(hipStream_t st, const DevTopo &T, const DevA
         if constexpr (REL) L = RelArgs{val64, x64, b, dinv, r};
         with_tb(T.spmv_tb, [&](auto tb) {
             constexpr int TBv = decltype(tb)::value;
-            k_spmv_pq<MT, VT, NR, REL, TBv><<<T.nblk, TBv, 0, st>>>(T, val, p, q, R, S, L);
+            with_chunk(T.max_w, [&](auto ch) {
+                k_spmv_pq<MT, VT, NR, REL, TBv, decltype(ch)::value><<<T.nblk, TBv, 0, st>>>(T, val, p, q, R, S, L);
+            });
         });
         k_finish<REL ? 4 * NR : NR, FinPQ<NR, REL>><<<T.ninst, 256, 0, st>>>(R, S, Af);
         k_update_xr<VT, NR><<<T.nblk, 256, 0, st>>>(T, dinv, x, r, p, q, R, S, (VT *)A.x0, (VT)A.omega_l[0]);
