@@ -178,6 +178,18 @@ int mhs_compute_inner_products(mhs_solver *s, const double *perm,
  * the static gravity fluxes N*K*g per half-face (gflux_, :554).  Either
  * pointer may be NULL. */
 int mhs_get_inner_products(mhs_solver *s, double *binv, double *gflux);
+/* MimeticIPEvaluator::gravityTerm (MimeticIPEvaluator.hpp:477-523) for every
+ * cell at once: gterm[h] = omega[c] * (x_f - x_c) . g for each half-face h of
+ * cell c, from the geometry given to mhs_set_grid; omega[nc] is the
+ * mobility-weighted density (sum_i rho_i lambda_i / lambda_t in the
+ * reference's fluid overload, :509-522).  Computed on the device; host arrays
+ * in and out. */
+int mhs_gravity_term(mhs_solver *s, const double *gravity, const double *omega, double *gterm);
+/* topologyIsSane (IncompFlowSolverHybrid.hpp:108-130, asserted in
+ * initSystemStructure :540): *sane = 1 when every cell's internal neighbours
+ * are distinct, i.e. no two cells share more than one face, else 0.  Host
+ * check on the arrays of mhs_set_grid; needs no handle and no GPU. */
+int mhs_topology_is_sane(int num_cells, const int *cell_facepos, const int *neighbour, int *sane);
 /* Alias: the "assembleStatic" step named in commented-out reference calls
  * (SimulatorBase.hpp:211) is computeInnerProducts. */
 int mhs_assemble_static(mhs_solver *s, const double *perm, const double *gravity);

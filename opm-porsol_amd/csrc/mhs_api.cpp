@@ -32,6 +32,14 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+// Device buffers of one call, freed when it returns.
+struct TempBufs {
+    std::vector<DevBuf> v;
+    ~TempBufs() {
+        for (auto &b : v) (void)hipFree(b.p);
+    }
+};
+
 struct GraphCache {
     hipGraphExec_t exec = nullptr;
     int precision = -1, nrhs = 0, chunk = 0, amg = -1, nu = -1;
@@ -580,6 +588,41 @@ int mhs_get_inner_products(mhs_solver *s, double *binv, double *gflux) {
     }
     if (gflux)
         HIPCHECK(hipMemcpy(gflux, s->gflux, sizeof(double) * T.nhf, hipMemcpyDeviceToHost));
+    return MHS_OK;
+}
+
+int mhs_gravity_term(mhs_solver *s, const double *gravity, const double *omega, double *gterm) {
+    if (!s) return MHS_ERR_INVALID;
+    if (!s->structure_valid) return fail(s, MHS_ERR_STATE, "gravity_term: structure not initialised");
+    if (!gravity || !omega || !gterm) return fail(s, MHS_ERR_INVALID, "gravity_term: null argument");
+    HIPCHECK(hipSetDevice(s->device));
+    const Topology &T = s->T;
+    TempBufs tmp;
+    hipError_t he = hipSuccess;
+    double *dom = dalloc<double>(s, tmp.v, (size_t)T.nc, &he);
+    double *dg = he == hipSuccess ? dalloc<double>(s, tmp.v, (size_t)T.nhf, &he) : nullptr;
+    if (he != hipSuccess) return fail(s, MHS_ERR_ALLOC, std::string("gravity_term: ") + hipGetErrorString(he));
+    HIPCHECK(hipMemcpyAsync(dom, omega, sizeof(double) * T.nc, hipMemcpyHostToDevice, s->stream));
+    HIPCHECK(launch_gravity_term(s->stream, s->dT, s->dG, gravity, dom, dg));
+    HIPCHECK(hipMemcpyAsync(gterm, dg, sizeof(double) * T.nhf, hipMemcpyDeviceToHost, s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    return MHS_OK;
+}
+
+int mhs_topology_is_sane(int num_cells, const int *cell_facepos, const int *neighbour, int *sane) {
+    if (!sane || num_cells < 0 || (num_cells > 0 && (!cell_facepos || !neighbour))) return MHS_ERR_INVALID;
+    // each cell's internal neighbours, sorted, must be distinct
+    int ok = 1;
+    std::vector<int> nb;
+    for (int c = 0; c < num_cells && ok; ++c) {
+        if (cell_facepos[c + 1] < cell_facepos[c]) return MHS_ERR_INVALID;
+        nb.clear();
+        for (int h = cell_facepos[c]; h < cell_facepos[c + 1]; ++h)
+            if (neighbour[h] >= 0) nb.push_back(neighbour[h]);
+        std::sort(nb.begin(), nb.end());
+        ok = std::adjacent_find(nb.begin(), nb.end()) == nb.end();
+    }
+    *sane = ok;
     return MHS_OK;
 }
 

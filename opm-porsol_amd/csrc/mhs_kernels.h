@@ -117,6 +117,10 @@ hipError_t launch_static_ip(hipStream_t st, const DevTopo &T, const DevGeom &G, 
                             const double *perm, const double *grav, double *binv_pk,
                             double *gflux);
 
+// gterm[h] = omega[cell(h)] * (x_f - x_c) . grav (MimeticIPEvaluator::gravityTerm).
+hipError_t launch_gravity_term(hipStream_t st, const DevTopo &T, const DevGeom &G, const double *grav,
+                               const double *omega, double *gterm);
+
 hipError_t launch_assemble(hipStream_t st, const DevTopo &T, int maxn, int nrhs,
                            const double *binv_pk, const double *gflux, const double *totmob,
                            const double *rhomob, const double *src, const uint8_t *ftype,

@@ -2139,6 +2139,19 @@ __global__ __launch_bounds__(256) void k_lz_update(int n, const double *__restri
     block_sum256(t, part);
 }
 
+// gterm[h] = omega[c] * (x_f - x_c) . g, one half-face per thread
+// (MimeticIPEvaluator::gravityTerm, MimeticIPEvaluator.hpp:477-523).
+__global__ __launch_bounds__(256) void k_gravity_term(DevTopo T, DevGeom G, double gx, double gy, double gz,
+                                                      const double *__restrict__ omega, double *__restrict__ gterm) {
+    const int h = blockIdx.x * 256 + threadIdx.x;
+    if (h >= T.nhf) return;
+    const int c = T.hf_cell[h];
+    const double dx = G.fcen[3 * (size_t)h] - G.ccen[3 * (size_t)c];
+    const double dy = G.fcen[3 * (size_t)h + 1] - G.ccen[3 * (size_t)c + 1];
+    const double dz = G.fcen[3 * (size_t)h + 2] - G.ccen[3 * (size_t)c + 2];
+    gterm[h] = omega[c] * (dx * gx + dy * gy + dz * gz);
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -2208,6 +2221,13 @@ template <class Fn>
 static void with_chunk(int max_w, Fn fn) {
     if (max_w <= 12) fn(IC<12>{});
     else fn(IC<16>{});
+}
+
+hipError_t launch_gravity_term(hipStream_t st, const DevTopo &T, const DevGeom &G, const double *grav,
+                               const double *omega, double *gterm) {
+    if (T.nhf == 0) return hipSuccess;
+    k_gravity_term<<<(unsigned)((T.nhf + 255) / 256), 256, 0, st>>>(T, G, grav[0], grav[1], grav[2], omega, gterm);
+    return hipGetLastError();
 }
 
 hipError_t launch_static_ip(hipStream_t st, const DevTopo &T, const DevGeom &G, int maxn,

@@ -59,6 +59,7 @@ int main(int argc, char **argv) try {
     FBC flow_bc(&u);
     Vec gravity(0.0);
 
+    std::cout << "topology is sane: " << Opm::topologyIsSane(g) << std::endl;
     std::cout << "========== Init pressure solver =============" << std::endl;
     auto t0 = std::chrono::steady_clock::now();
     solver.init(g, res_prop, gravity, flow_bc);

@@ -71,6 +71,25 @@ bool logical_index(const G &grid, int cell, int ijk[3]) {
 }
 }  // namespace hybrid_detail
 
+/// topologyIsSane (IncompFlowSolverHybrid.hpp:108-130): true when every pair
+/// of connected cells shares a single face.  The neighbour lists go to the
+/// C-ABI's host check mhs_topology_is_sane.
+template <class GI>
+bool topologyIsSane(const GI &g) {
+    typedef typename GI::CellIterator CI;
+    typedef typename CI::FaceIterator FI;
+    std::vector<int> facepos(1, 0), nb;
+    for (CI c = g.cellbegin(); c != g.cellend(); ++c) {
+        for (FI f = c->facebegin(); f != c->faceend(); ++f)
+            nb.push_back(f->boundary() ? -1 : f->neighbourCellIndex());
+        facepos.push_back((int)nb.size());
+    }
+    int sane = 0;
+    if (mhs_topology_is_sane((int)facepos.size() - 1, facepos.data(), nb.data(), &sane) != MHS_OK)
+        return false;
+    return sane != 0;
+}
+
 template <class GridInterface, class RockInterface, class BCInterface,
           template <class GridIF, class RockIF> class InnerProduct>
 class IncompFlowSolverHybrid {
@@ -171,6 +190,12 @@ public:
                 for (int d = 0; d < 3; ++d) { fc.push_back(x[d]); fn.push_back(nn[d]); }
             }
             facepos.push_back(facepos.back() + n);
+        }
+        {   // as the reference asserts (:540)
+            int sane = 0;
+            (void)mhs_topology_is_sane(nc, facepos.data(), nb.data(), &sane);
+            assert(sane);
+            (void)sane;
         }
         check(mhs_set_grid(h_, 1, nullptr, nc, facepos.data(), nb.data(), cvol.data(), ccen.data(),
                            fa.data(), fc.data(), fn.data()),

@@ -32,6 +32,7 @@ EXPORTS = (
     "mhs_solution_device", "mhs_post_process_fluxes", "mhs_get_dims", "mhs_get_cell_faces",
     "mhs_get_face_rows", "mhs_get_system", "mhs_time_kernel", "mhs_compute_inner_products_device",
     "mhs_get_solution_device", "mhs_adjoint", "mhs_adjoint_device", "mhs_sum_outflux",
+    "mhs_gravity_term", "mhs_topology_is_sane",
 )
 
 
@@ -127,6 +128,8 @@ def lib():
         L.mhs_compute_inner_products_device.argtypes = [_P, _P, _D]
         L.mhs_get_solution_device.argtypes = [_P, _P, _P]
         L.mhs_sum_outflux.argtypes = [_P, ctypes.c_int, _I, _I, _D]
+        L.mhs_gravity_term.argtypes = [_P, _D, _D, _D]
+        L.mhs_topology_is_sane.argtypes = [ctypes.c_int, _I, _I, ctypes.POINTER(ctypes.c_int)]
         L.mhs_adjoint.argtypes = [_P, _D, ctypes.POINTER(Options), _D, _D, _D, _D,
                                   ctypes.POINTER(Report)]
         L.mhs_adjoint_device.argtypes = [_P, _P, ctypes.POINTER(Options), _P, _P, _P, _P,
@@ -152,6 +155,18 @@ def _i(a):
 def _d(a):
     a = np.ascontiguousarray(a, dtype=np.float64)
     return a, a.ctypes.data_as(_D)
+
+
+def topology_is_sane(g):
+    """topologyIsSane (IncompFlowSolverHybrid.hpp:108-130): no two cells share
+    more than one face.  Host check through the C-ABI, no GPU needed."""
+    fp, pfp = _i(g.facepos)
+    nb, pnb = _i(g.nb)
+    ok = ctypes.c_int(-1)
+    rc = lib().mhs_topology_is_sane(int(g.num_cells), pfp, pnb, ctypes.byref(ok))
+    if rc != 0:
+        raise MhsError(rc, "topology_is_sane: invalid arguments")
+    return bool(ok.value)
 
 
 class HybridSolver:
@@ -231,6 +246,16 @@ class HybridSolver:
         """computeInnerProducts with perm[nc*9] in device memory (a data_ptr())."""
         gr, pg = _d(gravity)
         self._check(self._L.mhs_compute_inner_products_device(self._h, _P(perm_ptr), pg))
+
+    def gravityTerm(self, gravity, omega):
+        """MimeticIPEvaluator::gravityTerm (MimeticIPEvaluator.hpp:477-523) for every cell:
+        omega[c] (x_f - x_c) . g per half-face, computed on the device."""
+        g = self.grid
+        om, pom = _d(np.broadcast_to(np.asarray(omega, dtype=np.float64), (g.num_cells,)))
+        gr, pg = _d(gravity)
+        out = np.empty(g.num_half_faces)
+        self._check(self._L.mhs_gravity_term(self._h, pg, pom, out.ctypes.data_as(_D)))
+        return out
 
     def inner_products(self):
         """(binv, gflux): per-cell B^-1 column-major, concatenated; N K g per half-face."""

@@ -217,3 +217,29 @@ def solve(g, perm, totmob, rhomob, src, bctype, bcval, partner=None, grav=(0.0, 
     r.iterations, r.reduction, r.rc = its, red, rc
     r.t_init, r.t_assemble, r.t_cg, r.t_recover = t1 - t0, t2 - t1, t3 - t2, t4 - t3
     return r
+
+
+def topology_is_sane(num_cells, facepos, nb):
+    """topologyIsSane (IncompFlowSolverHybrid.hpp:108-130): true when, for
+    every cell, its internal neighbours (non-boundary faces) are pairwise
+    distinct, i.e. no two cells share more than one face.  Plain loops."""
+    for c in range(num_cells):
+        seen = set()
+        for h in range(int(facepos[c]), int(facepos[c + 1])):
+            n = int(nb[h])
+            if n < 0:
+                continue
+            if n in seen:
+                return False
+            seen.add(n)
+    return True
+
+
+def gravity_term(g, grav, omega):
+    """MimeticIPEvaluator::gravityTerm (MimeticIPEvaluator.hpp:477-523) for
+    every cell: gterm[h] = omega[c] * (x_f - x_c) . g over the half-faces h of
+    cell c (the face centroid minus the cell centroid, dotted with gravity)."""
+    cell = g.hf_cell()
+    d = np.asarray(g.face_centroid, dtype=np.float64).reshape(-1, 3) - \
+        np.asarray(g.cell_centroid, dtype=np.float64).reshape(-1, 3)[cell]
+    return np.asarray(omega, dtype=np.float64)[cell] * (d @ np.asarray(grav, dtype=np.float64))

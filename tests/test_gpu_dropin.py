@@ -48,6 +48,7 @@ def test_known_answer_matches_reference_numbers(n, precision):
     # the box grid hands its logical indices over, so the default solver type
     # runs the multigrid-preconditioned CG, like the reference's AMG-CG
     assert "preconditioner: multigrid" in out, out
+    assert "topology is sane: 1" in out, out
     ref = np.array(KNOWN[n])
     # the two reference runs (YaspGrid / CpGrid) differ by up to 3.7e-5 in Linf
     assert np.all(np.abs(got - ref) / ref < np.array([5e-5, 5e-5, 1e-4])), (got, ref)

@@ -241,3 +241,21 @@ def test_full_size_properties(pkg):
     assert _mass_balance(g, v, src) < 1e-11
     assert rel_err(pi, pi64) < 1e-6
     assert rel_err(v, v64) < 1e-6
+
+
+@pytest.mark.parametrize("kind", ["cartesian", "corner_point"])
+def test_gravity_term(pkg, orc, kind):
+    """MimeticIPEvaluator::gravityTerm for every cell on the device against the
+    oracle (rel. 1e-14)."""
+    if kind == "cartesian":
+        g = pkg.grids.cartesian(7, 5, 4, 2.0, 3.0, 0.5)
+    else:
+        g = pkg.grids.corner_point(9, 7, 5, 10.0, 10.0, 2.0, seed=4)
+    rng = np.random.default_rng(11)
+    om = rng.uniform(0.1, 3.0, g.num_cells)
+    grav = np.array([0.4, -0.7, 9.81])
+    s = pkg.HybridSolver(0)
+    s.init(g, pkg.fields.perm_c1(g.num_cells, 2), (0, 0, 0))
+    got = s.gravityTerm(grav, om)
+    want = orc.gravity_term(g, grav, om)
+    assert np.max(np.abs(got - want)) <= 1e-14 * np.max(np.abs(want))
