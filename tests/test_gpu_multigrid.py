@@ -117,3 +117,61 @@ def test_forced_smoother_weight(pkg):
     w2 = s.report.as_dict()["smoother_weights"]
     assert w2 != w and s.report.converged
     assert it_forced > 0
+
+
+@pytest.mark.parametrize("dims", [(40, 30, 1), (1, 1, 60), (1, 24, 20)])
+def test_flat_and_thin_grids(pkg, orc, dims):
+    """Directions of extent 1 are never coarsened; the hierarchy still reaches
+    a small coarsest level and the solution matches the oracle."""
+    g = pkg.grids.cartesian(*dims)
+    nc = g.num_cells
+    perm = pkg.fields.perm_c1(nc, 4)
+    bt, bv = cases.xdrop_bc(g) if dims[0] > 1 else _zdrop(g)
+    tm, rm = cases.unit_fluid(nc)
+    s = pkg.HybridSolver(0)
+    s.init(g, perm, (0, 0, 0), bt, None)
+    s.solve(tm, rm, np.zeros(nc), bt, bv, residual_tolerance=1e-11, precision="mixed")
+    r = s.report.as_dict()
+    assert r["precond"] == 1
+    for blk in r["coarsening"]:
+        for d in range(3):
+            if dims[d] == 1:
+                assert blk[d] == 1, r["coarsening"]
+    ref = orc.solve(g, perm, tm, rm, np.zeros(nc), bt, bv, tol=1e-14)
+    assert cases.rel_err(ref.outflux, s.getSolution()[2]) < 1e-8
+
+
+def _zdrop(g):
+    """Dirichlet 1 on z- and 0 on z+, no-flow elsewhere."""
+    bt = np.zeros(g.num_half_faces, np.int32)
+    bv = np.zeros(g.num_half_faces)
+    side = np.asarray(g.side)
+    bt[(side == 4) | (side == 5)] = cases.DIR
+    bv[side == 4] = 1.0
+    return bt, bv
+
+
+def test_batch_of_differently_anisotropic_instances(pkg, orc):
+    """One coarsening plan for the batch (strengths averaged over all
+    instances); every instance still converges to the oracle's solution."""
+    gs = [pkg.grids.cartesian(20, 20, 8), pkg.grids.cartesian(12, 12, 12, 10.0, 10.0, 1.0)]
+    G, icp = pkg.grids.concat(gs)
+    nc = G.num_cells
+    perm = np.zeros((nc, 9))
+    n0 = gs[0].num_cells
+    perm[:n0, 0] = perm[:n0, 4] = 1e-12
+    perm[:n0, 8] = 1e-14                     # weak vertical coupling
+    perm[n0:, [0, 4, 8]] = 1e-12              # isotropic K on thin layers
+    bt, bv = cases.xdrop_bc(G)
+    tm, rm = cases.unit_fluid(nc)
+    s = pkg.HybridSolver(0)
+    s.init(G, perm, (0, 0, 0), bt, None, inst_cellpos=icp)
+    s.solve(tm, rm, np.zeros(nc), bt, bv, residual_tolerance=1e-11, precision="mixed")
+    assert s.report.precond == 1 and s.report.converged
+    v = s.getSolution()[2]
+    for t in range(2):
+        c0, c1 = icp[t], icp[t + 1]
+        h0, h1 = G.facepos[c0], G.facepos[c1]
+        ref = orc.solve(gs[t], perm[c0:c1], tm[c0:c1], rm[c0:c1], np.zeros(c1 - c0), bt[h0:h1], bv[h0:h1],
+                        tol=1e-14)
+        assert cases.rel_err(ref.outflux, v[h0:h1]) < 1e-8
