@@ -909,45 +909,42 @@ __global__ __launch_bounds__(256) void k_pair_sum(int n, const int *__restrict__
 }
 
 // Dense inverse of each instance's coarsest operator (Gauss-Jordan, no
-// pivoting: the Galerkin operators are symmetric positive definite).  One
-// workgroup of kMaxCoarsest threads per instance, thread i owning row i of
-// the matrix in LDS: each elimination step is one barrier for the pivot row
-// and one for the update, with no index arithmetic beyond the row.
-__global__ __launch_bounds__(kMaxCoarsest) void k_coarse_inverse(DevAmgLevel L, const int *__restrict__ inst_ptr,
-                                                                 const int *__restrict__ off,
-                                                                 double *__restrict__ inv64,
-                                                                 float *__restrict__ inv32) {
+// pivoting: the Galerkin operators are symmetric positive definite).
+__global__ __launch_bounds__(256) void k_coarse_inverse(DevAmgLevel L, const int *__restrict__ inst_ptr,
+                                                        const int *__restrict__ off,
+                                                        double *__restrict__ inv64,
+                                                        float *__restrict__ inv32) {
     constexpr int M = kMaxCoarsest;
     __shared__ double A[M][M + 1];
-    __shared__ double prow[M];
-    const int t = blockIdx.x, i = threadIdx.x;
+    __shared__ double colk[M];
+    const int t = blockIdx.x;
     const int r0 = inst_ptr[t], m = inst_ptr[t + 1] - r0;
-    for (int j = 0; j < M; ++j) A[i][j] = 0.0;
-    if (i < m)
-        for (int k = L.row_ptr[r0 + i]; k < L.row_ptr[r0 + i + 1]; ++k) A[i][L.col[k] - r0] = L.val64[k];
+    for (int e = threadIdx.x; e < M * M; e += 256) A[e / M][e % M] = 0.0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < m; i += 256)
+        for (int k = L.row_ptr[r0 + i]; k < L.row_ptr[r0 + i + 1]; ++k)
+            A[i][L.col[k] - r0] = L.val64[k];
     __syncthreads();
     for (int k = 0; k < m; ++k) {
-        // pivot row k scaled, with the identity column folded in: row k of
-        // [A | I] after the step, stored where column k of A was
         const double piv = A[k][k];
-        prow[i] = (i == k ? 1.0 : A[k][i]) / piv;
         __syncthreads();
-        if (i < m) {
-            if (i == k) {
-                for (int j = 0; j < m; ++j) A[i][j] = prow[j];
-            } else {
-                const double f = A[i][k];
-                for (int j = 0; j < m; ++j) A[i][j] = (j == k ? 0.0 : A[i][j]) - f * prow[j];
-            }
+        for (int j = threadIdx.x; j < m; j += 256) A[k][j] = (j == k ? 1.0 : A[k][j]) / piv;
+        __syncthreads();
+        for (int i = threadIdx.x; i < m; i += 256) colk[i] = A[i][k];
+        __syncthreads();
+        for (int e = threadIdx.x; e < m * m; e += 256) {
+            const int i = e / m, j = e % m;
+            if (i == k) continue;
+            const double base = (j == k) ? 0.0 : A[i][j];
+            A[i][j] = base - colk[i] * A[k][j];
         }
         __syncthreads();
     }
-    if (i < m)
-        for (int j = 0; j < m; ++j) {
-            const double v = A[i][j];
-            inv64[off[t] + i * m + j] = v;
-            if (inv32) inv32[off[t] + i * m + j] = (float)v;
-        }
+    for (int e = threadIdx.x; e < m * m; e += 256) {
+        const double v = A[e / m][e % m];
+        inv64[off[t] + e] = v;
+        if (inv32) inv32[off[t] + e] = (float)v;
+    }
 }
 
 template <typename VT>
@@ -2418,7 +2415,7 @@ hipError_t launch_amg_setup(hipStream_t st, const DevTopo &T, const DevAmg &A, c
                 (int)A.ap_slots, A.ap_src_ptr, A.ap_src, val64, A.ap64, mixed ? A.ap32 : nullptr);
         });
     const DevAmgLevel &C = A.lev[A.nlev - 1];
-    k_coarse_inverse<<<A.ninst, kMaxCoarsest, 0, st>>>(C, C.inst_ptr, A.coarse_off, A.cinv64,
+    k_coarse_inverse<<<A.ninst, 256, 0, st>>>(C, C.inst_ptr, A.coarse_off, A.cinv64,
                                               mixed ? A.cinv32 : nullptr);
     return hipGetLastError();
 }
