@@ -175,3 +175,31 @@ def test_batch_of_differently_anisotropic_instances(pkg, orc):
         ref = orc.solve(gs[t], perm[c0:c1], tm[c0:c1], rm[c0:c1], np.zeros(c1 - c0), bt[h0:h1], bv[h0:h1],
                         tol=1e-14)
         assert cases.rel_err(ref.outflux, v[h0:h1]) < 1e-8
+
+
+def test_face_level_weight_from_spectral_radius(pkg, orc):
+    """The face-level Jacobi weight is 4/(3 rho(D^-1 A)) with rho from the
+    device Lanczos steps: against rho of the oracle's assembled face matrix
+    (scipy, symmetric form D^-1/2 A D^-1/2).  Lanczos approaches rho from
+    below, so the weight may only be a little larger."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    g = pkg.grids.cartesian(16, 12, 10, 1.0, 1.0, 0.5)
+    nc = g.num_cells
+    perm = pkg.fields.perm_c1(nc, 6)
+    bt, bv = cases.xdrop_bc(g)
+    tm, rm = cases.unit_fluid(nc)
+    s = pkg.HybridSolver(0)
+    s.init(g, perm, (0, 0, 0), bt, None)
+    s.solve(tm, rm, np.zeros(nc), bt, bv, residual_tolerance=1e-10, precision="mixed")
+    w0 = s.report.as_dict()["smoother_weights"][0]
+    hf, nint, nf, _ = orc.enumerate_dof(g)
+    pp, _ = orc.periodic_dof(g, hf, nf, bt, np.full(g.num_half_faces, -1, np.int32))
+    binv, pos, gfl = orc.static_ip(g, perm)
+    S = orc.assemble(g, hf, nf, pp, binv, gfl, tm, rm, np.zeros(nc), bt, bv)
+    A = sp.csr_matrix((S.val, S.col, S.row_ptr), shape=(nf, nf))
+    d = 1.0 / np.sqrt(A.diagonal())
+    M = sp.diags(d) @ A @ sp.diags(d)
+    rho = spl.eigsh(M, k=1, which="LA", return_eigenvectors=False)[0]
+    w_exact = 4.0 / (3.0 * rho)
+    assert w_exact * 0.999 <= w0 <= w_exact * 1.05, (w0, w_exact, rho)
