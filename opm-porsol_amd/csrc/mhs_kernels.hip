@@ -430,7 +430,9 @@ __global__ __launch_bounds__(TB) void k_fill(DevTopo T, DevCellSys C,
                                              float *__restrict__ dinv32,
                                              double *__restrict__ x0) {
     extern __shared__ double rowbuf[];
-    const int f = blockIdx.x * TB + threadIdx.x;
+    // contiguous row ranges per XCD: the cells a row range touches (up to a
+    // layer of cells away) stay in that XCD's L2 for the rows that reuse them
+    const int f = xcd_block(gridDim.x) * TB + threadIdx.x;
     if (f >= T.nfp) return;
     const int slice = f >> 6, lane = f & 63;
     const long long base = T.slice_ptr[slice] + lane;
@@ -465,10 +467,22 @@ __global__ __launch_bounds__(TB) void k_fill(DevTopo T, DevCellSys C,
             const int c = T.hf_cell[h], f0 = T.facepos[c], n = T.facepos[c + 1] - f0, r = h - f0;
             const long long pk = T.pk_pos[c];
             const uint8_t *sm = T.slotmap + T.sq_pos[c] + (long long)r * n;
-            for (int j = 0; j < n; ++j) {
-                if (ftype[f0 + j] == FT_DIRICHLET) continue;
-                const double v = C.schur[pk + (r <= j ? pidx(r, j, n) : pidx(j, r, n))];
-                row[rs * sm[j]] += v;
+            // the cell's row r of S_c, 8 faces at a time: all loads of a batch
+            // are issued before the LDS sums
+            for (int j0 = 0; j0 < n; j0 += 8) {
+                double v[8];
+                int slot[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int j = j0 + u;
+                    const bool in = j < n;
+                    const bool dir = in && ftype[f0 + j] == FT_DIRICHLET;
+                    slot[u] = in ? (int)sm[j] : 0;
+                    v[u] = (in && !dir) ? C.schur[pk + (r <= j ? pidx(r, j, n) : pidx(j, r, n))] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (j0 + u < n) row[rs * slot[u]] += v[u];
             }
 #pragma unroll
             for (int k = 0; k < NRHS; ++k) bk[k] += C.rr[(size_t)k * T.nhf + h];
