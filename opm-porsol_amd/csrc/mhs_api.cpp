@@ -959,7 +959,7 @@ int autotune(mhs_solver *s, int nrhs, bool mixed, bool amg, double *tune_ms) {
                                                          s->r64, s->z64, s->R);
                 });
                 if (ms < bt) { bt = ms; best.tail_rows = rows; }
-                if (ms > 3.0f * bt) break;   // larger tails only take longer
+                if (ms > 1.5f * bt) break;   // past the best, larger tails only take longer
             }
         }
         (void)hipEventDestroy(e0);
