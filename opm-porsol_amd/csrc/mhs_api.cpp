@@ -460,8 +460,15 @@ int mhs_init_system_structure(mhs_solver *s, const int *hf_bc_type, const int *h
     s->free_solve_buffers();
     s->free_topology();
     s->structure_valid = s->ip_valid = s->solved = false;
+    // face family of every half-face: the dominant axis of its normal
+    std::vector<int> family(s->nhf, 0);
+    for (int h = 0; h < s->nhf; ++h) {
+        const double *n = &s->fnrm[3 * (size_t)h];
+        const double ax = std::fabs(n[0]), ay = std::fabs(n[1]), az = std::fabs(n[2]);
+        family[h] = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+    }
     std::string e = build_topology(s->T, s->ninst, s->inst_cellpos.data(), s->nc, s->facepos.data(),
-                                   s->nb.data(), hf_bc_type, hf_periodic_partner);
+                                   s->nb.data(), hf_bc_type, hf_periodic_partner, family.data());
     if (!e.empty()) return fail(s, MHS_ERR_INVALID, "init_system_structure: " + e);
     Topology &T = s->T;
     hipError_t he = hipSuccess;
@@ -482,6 +489,7 @@ int mhs_init_system_structure(mhs_solver *s, const int *hf_bc_type, const int *h
     UP(row_hf, row_hf); UP(dof_hf, dof_hf); UP(ptn, ptn); UP(dslot, dslot); UP(pslot, pslot);
     UP(slice_ptr, slice_ptr); UP(col, col); UP(slotmap, slotmap); UP(sq_pos, sq_pos);
     UP(pk_pos, pk_pos); UP(block_seg, block_seg); UP(seg_blk, seg_blk); UP(inst_rowbase, inst_rowbase);
+    UP(inst_regrow, inst_regrow);
 #undef UP
     DevGeom &g = s->dG;
     g.cvol = dupload(s, L, s->cvol, &he);
@@ -1522,7 +1530,7 @@ int mhs_get_cell_faces(mhs_solver *s, int *hf_face, int *inst_facepos) {
         for (int c = 0; c < T.nc; ++c) {
             while (c >= T.inst_cellpos[t + 1]) ++t;
             for (int h = T.facepos[c]; h < T.facepos[c + 1]; ++h)
-                hf_face[h] = T.hf_dof[h] - T.inst_rowbase[t] + T.inst_facebase[t];
+                hf_face[h] = T.hf_face[h];
         }
     }
     if (inst_facepos)

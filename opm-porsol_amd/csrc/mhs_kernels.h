@@ -24,6 +24,7 @@ struct DevTopo {
     const uint8_t *slotmap;
     const long long *sq_pos, *pk_pos;
     const int *block_seg, *seg_blk, *inst_rowbase;
+    const int *inst_regrow;         // row of each instance's dof 0 (regularised when no Dirichlet face)
     // launch shapes of the row-block kernels, chosen by the autotuner:
     // workgroup size (256, 512 or 1024 threads per 1024 rows) and whether
     // workgroups are remapped so each XCD gets a contiguous row range

@@ -501,7 +501,7 @@ __global__ __launch_bounds__(TB) void k_fill(DevTopo T, DevCellSys C,
                 bk[k] += a0 * bcval[(size_t)k * T.nhf + h0] - a1 * bcval[(size_t)k * T.nhf + hp];
         }
         const int s = T.block_seg[f / kRowBlock];
-        if (inst_reg[s] && f == T.inst_rowbase[s]) row[dpos] *= 2.0;  // S[0][0] *= 2 (:1398-1400)
+        if (inst_reg[s] && f == T.inst_regrow[s]) row[dpos] *= 2.0;  // S[0][0] *= 2 (:1398-1400)
         diag = row[dpos];
     }
 #pragma unroll

@@ -49,7 +49,7 @@ std::string enumerate_instance(int c0, int c1, const int *facepos, const int *nb
 
 std::string build_topology(Topology &T, int ninst, const int *inst_cellpos, int nc,
                            const int *facepos, const int *nb, const int *hf_bc_type,
-                           const int *hf_partner) {
+                           const int *hf_partner, const int *hf_family) {
     T = Topology();
     T.nc = nc;
     T.ninst = ninst;
@@ -92,12 +92,41 @@ std::string build_topology(Topology &T, int ninst, const int *inst_cellpos, int 
     }
     T.nfp = (int)rowbase;
     const int nfp = T.nfp;
+    // --- row order: dof -> row within each instance --------------------------
+    std::vector<int> row_of_dof(T.nf_total);   // indexed by instance face base + dof
+    T.inst_regrow.assign(ninst, 0);
+    {
+        std::vector<int> fam, len, order;
+        for (int t = 0; t < ninst; ++t) {
+            const int nf = T.inst_nf[t], fb = T.inst_facebase[t];
+            fam.assign(nf, 3);
+            len.assign(nf, 0);
+            for (int c = T.inst_cellpos[t]; c < T.inst_cellpos[t + 1]; ++c)
+                for (int h = facepos[c]; h < facepos[c + 1]; ++h) {
+                    const int d = hf_local[h];
+                    if (hf_family) fam[d] = std::min(fam[d], std::max(0, std::min(2, hf_family[h])));
+                    len[d] += facepos[c + 1] - facepos[c];   // row length up to one
+                }
+            order.resize(nf);
+            for (int d = 0; d < nf; ++d) order[d] = d;
+            std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return fam[a] < fam[b]; });
+            for (int w0 = 0; w0 < nf; w0 += kRowBlock) {
+                const int w1 = std::min(nf, w0 + kRowBlock);
+                std::stable_sort(order.begin() + w0, order.begin() + w1,
+                                 [&](int a, int b) { return len[a] < len[b]; });
+            }
+            for (int r = 0; r < nf; ++r) row_of_dof[fb + order[r]] = r;
+            if (nf > 0) T.inst_regrow[t] = T.inst_rowbase[t] + row_of_dof[fb];
+        }
+    }
     T.hf_dof.resize(nhf);
+    T.hf_face.resize(nhf);
     T.hf_cell.resize(nhf);
     for (int t = 0; t < ninst; ++t)
         for (int c = T.inst_cellpos[t]; c < T.inst_cellpos[t + 1]; ++c)
             for (int h = facepos[c]; h < facepos[c + 1]; ++h) {
-                T.hf_dof[h] = T.inst_rowbase[t] + hf_local[h];
+                T.hf_face[h] = T.inst_facebase[t] + hf_local[h];
+                T.hf_dof[h] = T.inst_rowbase[t] + row_of_dof[T.hf_face[h]];
                 T.hf_cell[h] = c;
             }
     T.face_row.resize(T.nf_total);
@@ -110,7 +139,7 @@ std::string build_topology(Topology &T, int ninst, const int *inst_cellpos, int 
     }
     for (int t = 0; t < ninst; ++t)
         for (int f = 0; f < T.inst_nf[t]; ++f)
-            T.face_row[T.inst_facebase[t] + f] = T.inst_rowbase[t] + f;
+            T.face_row[T.inst_facebase[t] + f] = T.inst_rowbase[t] + row_of_dof[T.inst_facebase[t] + f];
 
     // --- periodic pairing (enumerateBCDof, :1071-1110) ---------------------
     T.hf_twin.assign(nhf, -1);
@@ -148,7 +177,8 @@ std::string build_topology(Topology &T, int ninst, const int *inst_cellpos, int 
     T.hf_redirect.assign(nhf, 0);
     for (int h = 0; h < nhf; ++h) {
         int d = T.hf_dof[h];
-        if (T.hf_periodic[h] && T.hf_dof[T.hf_ppart[h]] < d) {
+        // the min-dof rule compares reference dof numbers (same instance)
+        if (T.hf_periodic[h] && T.hf_face[T.hf_ppart[h]] < T.hf_face[h]) {
             T.hf_redirect[h] = 1;            // addCellContrib :1842-1877 (min-dof rule)
             T.hf_col[h] = T.hf_dof[T.hf_ppart[h]];
         } else {

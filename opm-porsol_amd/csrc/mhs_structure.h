@@ -25,7 +25,9 @@ struct Topology {
     int nfp = 0;                          // padded rows (sum over instances)
     int nf_total = 0, nint_total = 0;     // unpadded faces / internal faces
     std::vector<int> inst_nf, inst_nint, inst_rowbase, inst_facebase;
+    std::vector<int> inst_regrow;         // row of each instance's dof 0 (the regularised entry)
     std::vector<int> hf_dof;              // padded global row of each half-face
+    std::vector<int> hf_face;             // its face (instance face base + reference dof number)
     std::vector<int> hf_cell;
     std::vector<int> hf_twin;             // neighbour half-face or periodic partner half-face
     std::vector<int> hf_ppart;            // periodic partner half-face (structure time)
@@ -51,10 +53,17 @@ struct Topology {
     bool has_periodic = false;
 };
 
-// Builds the topology.  Returns an empty string on success, else an error.
+// Builds the topology.  Rows are the faces of each instance in the order:
+// face family (hf_family, e.g. the dominant axis of the face normal; null:
+// one family), then, within windows of kRowBlock rows, estimated row length,
+// then reference dof number.  A wavefront then holds faces of one family
+// (its gathers point the same way) and SELL slices hold rows of similar
+// length (little padding); everything defined by the reference numbering
+// (periodic redirection, the regularised dof 0, face outputs) goes through
+// the dof numbers.  Returns an empty string on success, else an error.
 std::string build_topology(Topology &T, int ninst, const int *inst_cellpos, int nc,
                            const int *facepos, const int *nb, const int *hf_bc_type,
-                           const int *hf_partner);
+                           const int *hf_partner, const int *hf_family = nullptr);
 
 // Number the dofs of one instance exactly as enumerateGridDof does
 // (IncompFlowSolverHybrid.hpp:956-1067).  Cells [c0,c1); writes local dof ids.
