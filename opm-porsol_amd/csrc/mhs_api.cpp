@@ -647,7 +647,9 @@ int ensure_solve_buffers(mhs_solver *s, int nrhs, bool mixed) {
     const size_t nfp = T.nfp, nv = nfp * nrhs, nnz = (size_t)T.slice_ptr[T.nslices];
     const size_t nsys = (size_t)T.ninst * nrhs;
     auto ok = [&]() { return e == hipSuccess; };
-    s->C.schur = dalloc<double>(s, L, (size_t)T.pk_pos[T.nc], &e);
+    // component-major: packed entry q of cell c at q * nc + c (q below the
+    // largest cell's packed size), so lanes on consecutive cells coalesce
+    s->C.schur = dalloc<double>(s, L, (size_t)T.max_ncf * (T.max_ncf + 1) / 2 * T.nc, &e);
     if (ok()) s->C.rr = dalloc<double>(s, L, (size_t)nrhs * T.nhf, &e);
     if (ok()) s->C.F = dalloc<double>(s, L, (size_t)T.nhf, &e);
     if (ok()) s->C.L = dalloc<double>(s, L, (size_t)T.nc, &e);

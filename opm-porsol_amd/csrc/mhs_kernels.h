@@ -36,7 +36,7 @@ struct DevTopo {
 
 // Per-solve cell data produced by the assembly kernel.
 struct DevCellSys {
-    double *schur;     // packed local Schur complements (FP64)
+    double *schur;     // packed local Schur complements (FP64), component-major: entry q of cell c at q*nc + c
     double *rr;        // [nrhs][nhf] row right-hand-side contributions
     double *F;         // [nhf]
     double *L;         // [nc]

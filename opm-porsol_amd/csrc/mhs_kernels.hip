@@ -374,7 +374,7 @@ __global__ __launch_bounds__(64) void k_assemble(DevTopo T, const double *__rest
         for (int j = i; j < MAXN; ++j) {
             double v = -rL * F[i] * F[j] + S[pidx(i, j, MAXN)];
             S[pidx(i, j, MAXN)] = v;
-            if (j < n) C.schur[pk + pidx(i, j, n)] = v;
+            if (j < n) C.schur[(size_t)pidx(i, j, n) * T.nc + c] = v;
         }
     uint8_t ft[MAXN], rd[MAXN];
 #pragma unroll
@@ -451,7 +451,7 @@ __global__ __launch_bounds__(TB) void k_fill(DevTopo T, DevCellSys C,
         diag = 1.0;
     } else if (ftype[h0] == FT_DIRICHLET) {
         const int c = T.hf_cell[h0], f0 = T.facepos[c], n = T.facepos[c + 1] - f0, r = h0 - f0;
-        const double a = C.schur[T.pk_pos[c] + pidx(r, r, n)];
+        const double a = C.schur[(size_t)pidx(r, r, n) * T.nc + c];
         row[dpos] = a;
         diag = a;
 #pragma unroll
@@ -465,7 +465,6 @@ __global__ __launch_bounds__(TB) void k_fill(DevTopo T, DevCellSys C,
             const int h = T.row_hf[2 * (size_t)f + t];
             if (h < 0) continue;
             const int c = T.hf_cell[h], f0 = T.facepos[c], n = T.facepos[c + 1] - f0, r = h - f0;
-            const long long pk = T.pk_pos[c];
             const uint8_t *sm = T.slotmap + T.sq_pos[c] + (long long)r * n;
             // the cell's row r of S_c, 8 faces at a time: all loads of a batch
             // are issued before the LDS sums
@@ -478,7 +477,7 @@ __global__ __launch_bounds__(TB) void k_fill(DevTopo T, DevCellSys C,
                     const bool in = j < n;
                     const bool dir = in && ftype[f0 + j] == FT_DIRICHLET;
                     slot[u] = in ? (int)sm[j] : 0;
-                    v[u] = (in && !dir) ? C.schur[pk + (r <= j ? pidx(r, j, n) : pidx(j, r, n))] : 0.0;
+                    v[u] = (in && !dir) ? C.schur[(size_t)(r <= j ? pidx(r, j, n) : pidx(j, r, n)) * T.nc + c] : 0.0;
                 }
 #pragma unroll
                 for (int u = 0; u < 8; ++u)
@@ -492,8 +491,8 @@ __global__ __launch_bounds__(TB) void k_fill(DevTopo T, DevCellSys C,
             const int hp = T.dof_hf[fp];
             const int c0 = T.hf_cell[h0], f00 = T.facepos[c0], n0 = T.facepos[c0 + 1] - f00;
             const int c1 = T.hf_cell[hp], f01 = T.facepos[c1], n1 = T.facepos[c1 + 1] - f01;
-            const double a0 = C.schur[T.pk_pos[c0] + pidx(h0 - f00, h0 - f00, n0)];
-            const double a1 = C.schur[T.pk_pos[c1] + pidx(hp - f01, hp - f01, n1)];
+            const double a0 = C.schur[(size_t)pidx(h0 - f00, h0 - f00, n0) * T.nc + c0];
+            const double a1 = C.schur[(size_t)pidx(hp - f01, hp - f01, n1) * T.nc + c1];
             row[dpos] += a0 + a1;
             row[rs * T.pslot[f]] -= a0 + a1;
 #pragma unroll
