@@ -488,7 +488,7 @@ int mhs_init_system_structure(mhs_solver *s, const int *hf_bc_type, const int *h
     UP(hf_ppart, hf_ppart); UP(hf_twin, hf_twin); UP(hf_redirect, hf_redirect);
     UP(row_hf, row_hf); UP(dof_hf, dof_hf); UP(ptn, ptn); UP(dslot, dslot); UP(pslot, pslot);
     UP(slice_ptr, slice_ptr); UP(col, col); UP(slotmap, slotmap); UP(sq_pos, sq_pos);
-    UP(pk_pos, pk_pos); UP(block_seg, block_seg); UP(seg_blk, seg_blk); UP(inst_rowbase, inst_rowbase);
+    UP(block_seg, block_seg); UP(seg_blk, seg_blk); UP(inst_rowbase, inst_rowbase);
     UP(inst_regrow, inst_regrow);
 #undef UP
     DevGeom &g = s->dG;
@@ -497,7 +497,8 @@ int mhs_init_system_structure(mhs_solver *s, const int *hf_bc_type, const int *h
     if (he == hipSuccess) g.farea = dupload(s, L, s->farea, &he);
     if (he == hipSuccess) g.fcen = dupload(s, L, s->fcen, &he);
     if (he == hipSuccess) g.fnrm = dupload(s, L, s->fnrm, &he);
-    if (he == hipSuccess) s->binv = dalloc<double>(s, L, (size_t)T.pk_pos[T.nc], &he);
+    // static B^-1 component-major: packed entry q of cell c at q * nc + c
+    if (he == hipSuccess) s->binv = dalloc<double>(s, L, (size_t)T.max_ncf * (T.max_ncf + 1) / 2 * T.nc, &he);
     if (he == hipSuccess) s->gflux = dalloc<double>(s, L, (size_t)T.nhf, &he);
     if (he != hipSuccess) return fail(s, MHS_ERR_ALLOC, std::string("device allocation: ") + hipGetErrorString(he));
     if (!s->ijk.empty()) {
@@ -584,14 +585,14 @@ int mhs_get_inner_products(mhs_solver *s, double *binv, double *gflux) {
     HIPCHECK(hipSetDevice(s->device));
     const Topology &T = s->T;
     if (binv) {
-        std::vector<double> pk(T.pk_pos[T.nc]);
+        std::vector<double> pk((size_t)T.max_ncf * (T.max_ncf + 1) / 2 * T.nc);
         HIPCHECK(hipMemcpy(pk.data(), s->binv, sizeof(double) * pk.size(), hipMemcpyDeviceToHost));
         for (int c = 0; c < T.nc; ++c) {
             const int n = T.facepos[c + 1] - T.facepos[c];
             double *out = binv + T.sq_pos[c];
             for (int i = 0; i < n; ++i)
                 for (int j = 0; j < n; ++j)
-                    out[i + j * n] = pk[T.pk_pos[c] + packed_index(std::min(i, j), std::max(i, j), n)];
+                    out[i + j * n] = pk[(size_t)packed_index(std::min(i, j), std::max(i, j), n) * T.nc + c];
         }
     }
     if (gflux)

@@ -22,7 +22,7 @@ struct DevTopo {
     const long long *slice_ptr;
     const int *col;
     const uint8_t *slotmap;
-    const long long *sq_pos, *pk_pos;
+    const long long *sq_pos;
     const int *block_seg, *seg_blk, *inst_rowbase;
     const int *inst_regrow;         // row of each instance's dof 0 (regularised when no Dirichlet face)
     // launch shapes of the row-block kernels, chosen by the autotuner:

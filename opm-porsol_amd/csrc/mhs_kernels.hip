@@ -310,7 +310,7 @@ __global__ __launch_bounds__(64) void k_static_ip(DevTopo T, DevGeom G,
             NK[i][l] = N[i][0] * K[l] + N[i][1] * K[3 + l] + N[i][2] * K[6 + l];
     const double t = 6.0 * (K[0] + K[4] + K[8]) / 3.0;
     const double rv = 1.0 / vol, tv = t / vol;
-    double *out = binv + T.pk_pos[c];
+    double *out = binv + c;                       // component-major: entry q at q * nc
 #pragma unroll
     for (int i = 0; i < MAXN; ++i) {
 #pragma unroll
@@ -319,7 +319,7 @@ __global__ __launch_bounds__(64) void k_static_ip(DevTopo T, DevGeom G,
                 const double s = NK[i][0] * N[j][0] + NK[i][1] * N[j][1] + NK[i][2] * N[j][2];
                 const double qq = Q[i][0] * Q[j][0] + Q[i][1] * Q[j][1] + Q[i][2] * Q[j][2];
                 const double B = a[i] * ((i == j ? 1.0 : 0.0) - qq) * a[j];
-                out[pidx(i, j, n)] = rv * s + tv * B;
+                out[(size_t)pidx(i, j, n) * T.nc] = rv * s + tv * B;
             }
         }
     }
@@ -346,13 +346,12 @@ __global__ __launch_bounds__(64) void k_assemble(DevTopo T, const double *__rest
     if (c >= T.nc) return;
     const int f0 = T.facepos[c], n = T.facepos[c + 1] - f0;
     const double lam = totmob[c], rl = rhomob[c];
-    const long long pk = T.pk_pos[c];
     double S[NP];
 #pragma unroll
     for (int i = 0; i < MAXN; ++i)
 #pragma unroll
         for (int j = i; j < MAXN; ++j)
-            S[pidx(i, j, MAXN)] = (j < n) ? lam * binv[pk + pidx(i, j, n)] : 0.0;
+            S[pidx(i, j, MAXN)] = (j < n) ? lam * binv[(size_t)pidx(i, j, n) * T.nc + c] : 0.0;
     double F[MAXN], gfl[MAXN];
     double L = 0.0, gsum = 0.0;
 #pragma unroll
@@ -1626,12 +1625,11 @@ __global__ __launch_bounds__(64) void k_recover(DevTopo T, const double *__restr
     if (c >= T.nc) return;
     const int f0 = T.facepos[c], n = T.facepos[c + 1] - f0;
     const double lam = totmob[c], rl = rhomob[c];
-    const long long pk = T.pk_pos[c];
     double B[NP];
 #pragma unroll
     for (int i = 0; i < MAXN; ++i)
 #pragma unroll
-        for (int j = i; j < MAXN; ++j) B[pidx(i, j, MAXN)] = (j < n) ? binv[pk + pidx(i, j, n)] : 0.0;
+        for (int j = i; j < MAXN; ++j) B[pidx(i, j, MAXN)] = (j < n) ? binv[(size_t)pidx(i, j, n) * T.nc + c] : 0.0;
     double F[MAXN];
     int dof[MAXN];
 #pragma unroll
@@ -1717,7 +1715,6 @@ __global__ __launch_bounds__(64) void k_adj_cell(DevTopo T, const double *__rest
     if (c >= T.nc) return;
     const int f0 = T.facepos[c], n = T.facepos[c + 1] - f0;
     const double lam = totmob[c], L = C.L[c];
-    const long long pk = T.pk_pos[c];
     double F[MAXN];
 #pragma unroll
     for (int i = 0; i < MAXN; ++i) F[i] = (i < n) ? C.F[f0 + i] : 0.0;
@@ -1735,7 +1732,7 @@ __global__ __launch_bounds__(64) void k_adj_cell(DevTopo T, const double *__rest
             double s = 0.0;
 #pragma unroll
             for (int j = 0; j < MAXN; ++j)
-                if (j < n) s += binv[pk + (i <= j ? pidx(i, j, n) : pidx(j, i, n))] * wv[j];
+                if (j < n) s += binv[(size_t)(i <= j ? pidx(i, j, n) : pidx(j, i, n)) * T.nc + c] * wv[j];
             y[(size_t)k * T.nhf + f0 + i] = -(lam * s - F[i] * fw / L);
         }
     }
@@ -1792,7 +1789,6 @@ __global__ __launch_bounds__(64) void k_adj_grad(DevTopo T, DevGeom G, const dou
     if (c >= T.nc) return;
     const int f0 = T.facepos[c], n = T.facepos[c + 1] - f0;
     const double lam = totmob[c], rl = rhomob[c], L = C.L[c], vol = G.cvol[c];
-    const long long pk = T.pk_pos[c];
     const double g[3] = {gx, gy, gz};
     double N[MAXN][3], Q[MAXN][3], a[MAXN];
     cell_frame<MAXN>(G, c, f0, n, N, Q, a);
@@ -1876,7 +1872,7 @@ __global__ __launch_bounds__(64) void k_adj_grad(DevTopo T, DevGeom G, const dou
             double s = 0.0;
 #pragma unroll
             for (int j = 0; j < MAXN; ++j)
-                if (j < n) s += binv[pk + (i <= j ? pidx(i, j, n) : pidx(j, i, n))] * d[j];
+                if (j < n) s += binv[(size_t)(i <= j ? pidx(i, j, n) : pidx(j, i, n)) * T.nc + c] * d[j];
             uBd += u[i] * s;
         }
         gt += uBd - uF * gc / (lam * L);
