@@ -476,6 +476,7 @@ int mhs_init_system_structure(mhs_solver *s, const int *hf_bc_type, const int *h
     DevTopo &d = s->dT;
     d.nc = T.nc; d.nhf = T.nhf; d.nfp = T.nfp; d.ninst = T.ninst; d.nslices = T.nslices;
     d.nblk = T.nfp / kRowBlock;
+    d.max_ncf = T.max_ncf;
     d.max_w = 0;
     for (int sl = 0; sl < T.nslices; ++sl)
         d.max_w = std::max(d.max_w, (int)((T.slice_ptr[sl + 1] - T.slice_ptr[sl]) / kSlice));

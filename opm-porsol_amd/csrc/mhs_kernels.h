@@ -15,6 +15,7 @@ struct DevGeom {
 struct DevTopo {
     int nc, nhf, nfp, ninst, nslices, nblk;
     int max_w;                      // widest SELL slice (stored entries per row)
+    int max_ncf;                    // most faces of a cell (stride of the slot map)
     const int *facepos, *hf_dof, *hf_cell, *hf_col, *hf_ppart, *hf_twin;
     const uint8_t *hf_redirect;
     const int *row_hf, *dof_hf, *ptn;

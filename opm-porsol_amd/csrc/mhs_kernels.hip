@@ -464,7 +464,7 @@ __global__ __launch_bounds__(TB) void k_fill(DevTopo T, DevCellSys C,
             const int h = T.row_hf[2 * (size_t)f + t];
             if (h < 0) continue;
             const int c = T.hf_cell[h], f0 = T.facepos[c], n = T.facepos[c + 1] - f0, r = h - f0;
-            const uint8_t *sm = T.slotmap + T.sq_pos[c] + (long long)r * n;
+            const uint8_t *sm = T.slotmap + (size_t)r * T.max_ncf * T.nc + c;   // entry j at sm[j * nc]
             // the cell's row r of S_c, 8 faces at a time: all loads of a batch
             // are issued before the LDS sums
             for (int j0 = 0; j0 < n; j0 += 8) {
@@ -475,7 +475,7 @@ __global__ __launch_bounds__(TB) void k_fill(DevTopo T, DevCellSys C,
                     const int j = j0 + u;
                     const bool in = j < n;
                     const bool dir = in && ftype[f0 + j] == FT_DIRICHLET;
-                    slot[u] = in ? (int)sm[j] : 0;
+                    slot[u] = in ? (int)sm[(size_t)j * T.nc] : 0;
                     v[u] = (in && !dir) ? C.schur[(size_t)(r <= j ? pidx(r, j, n) : pidx(j, r, n)) * T.nc + c] : 0.0;
                 }
 #pragma unroll

@@ -272,7 +272,9 @@ std::string build_topology(Topology &T, int ninst, const int *inst_cellpos, int 
         T.sq_pos[c + 1] = T.sq_pos[c] + n * n;
         T.pk_pos[c + 1] = T.pk_pos[c] + n * (n + 1) / 2;
     }
-    T.slotmap.assign(T.sq_pos[nc], 255);
+    // component-major: entry (r, j) of cell c at (r * max_ncf + j) * nc + c
+    const long long mf = T.max_ncf;
+    T.slotmap.assign((size_t)(mf * mf) * nc, 255);
     for (int c = 0; c < nc; ++c) {
         int f0 = facepos[c], n = facepos[c + 1] - f0;
         for (int r = 0; r < n; ++r) {
@@ -283,7 +285,7 @@ std::string build_topology(Topology &T, int ninst, const int *inst_cellpos, int 
                 int cj = T.hf_col[f0 + j];
                 const int *p = std::lower_bound(rc, rc + len, cj);
                 if (p == rc + len || *p != cj) return "slot lookup failed";
-                T.slotmap[T.sq_pos[c] + (long long)r * n + j] = (uint8_t)(p - rc);
+                T.slotmap[(size_t)(r * mf + j) * nc + c] = (uint8_t)(p - rc);
             }
         }
     }

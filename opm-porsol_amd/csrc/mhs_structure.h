@@ -49,7 +49,7 @@ struct Topology {
     // per-cell maps
     std::vector<long long> sq_pos;        // sum n^2 prefix
     std::vector<long long> pk_pos;        // sum n(n+1)/2 prefix
-    std::vector<uint8_t> slotmap;         // [sq_pos.back()] slot of (cell,r,j) in row(cell,r)
+    std::vector<uint8_t> slotmap;         // slot of (cell c, r, j) in row(c, r), at (r * max_ncf + j) * nc + c
     bool has_periodic = false;
 };
 
