@@ -944,18 +944,21 @@ __global__ __launch_bounds__(256) void k_coarse_inverse(DevAmgLevel L, const int
         __syncthreads();
         for (int i = threadIdx.x; i < m; i += 256) colk[i] = A[i][k];
         __syncthreads();
-        for (int e = threadIdx.x; e < m * m; e += 256) {
-            const int i = e / m, j = e % m;
-            if (i == k) continue;
+        // (i, j) from the compile-time width M: shifts, no integer division
+        for (int e = threadIdx.x; e < M * M; e += 256) {
+            const int i = e / M, j = e % M;
+            if (i >= m || j >= m || i == k) continue;
             const double base = (j == k) ? 0.0 : A[i][j];
             A[i][j] = base - colk[i] * A[k][j];
         }
         __syncthreads();
     }
-    for (int e = threadIdx.x; e < m * m; e += 256) {
-        const double v = A[e / m][e % m];
-        inv64[off[t] + e] = v;
-        if (inv32) inv32[off[t] + e] = (float)v;
+    for (int e = threadIdx.x; e < M * M; e += 256) {
+        const int i = e / M, j = e % M;
+        if (i >= m || j >= m) continue;
+        const double v = A[i][j];
+        inv64[off[t] + i * m + j] = v;
+        if (inv32) inv32[off[t] + i * m + j] = (float)v;
     }
 }
 
