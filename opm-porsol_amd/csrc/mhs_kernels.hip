@@ -2005,6 +2005,22 @@ struct FinRZ {
     }
 };
 
+// Multigrid iteration: the update_xr step (convergence / replacement request,
+// iteration count) and beta = <r, z> / <r, z>_old in one launch after the
+// V-cycle; update_xr's partials at q 0..2 NRHS-1, the V-cycle's <r, z> at
+// 2 NRHS + k.  Deciding after the V-cycle only lets a system that converged in
+// this iteration (FP64 recurrence) run one more V-cycle, whose result nothing
+// reads.
+template <int NRHS>
+struct FinXRZ {
+    __device__ static bool skip(int s, const DevScal &S, const FinArgs &) { return none_set<NRHS>(S.act, s); }
+    __device__ static int nq(int, const DevScal &) { return 3 * NRHS; }
+    __device__ static void run(int s, const double *tot, DevScal &S, const FinArgs &A) {
+        FinXR<NRHS>::run(s, tot, S, A);
+        FinRZ<NRHS>::run(s, tot + 2 * NRHS, S, A);
+    }
+};
+
 // after k_start64: |r0|, thresholds, activation (dune CGSolver: def0 < 1e-30 -> done)
 template <int NRHS>
 struct FinStart {
@@ -2691,9 +2707,11 @@ hipError_t launch_pcg_iteration_amg(hipStream_t st, const DevTopo &T, const DevA
         });
         k_finish<REL ? 4 * NR : NR, FinPQ<NR, REL>><<<T.ninst, 256, 0, st>>>(R, S, Af);
         k_update_xr<VT, NR><<<T.nblk, 256, 0, st>>>(T, dinv, x, r, p, q, R, S, (VT *)A.x0, (VT)A.omega_l[0]);
-        k_finish<2 * NR, FinXR<NR>><<<T.ninst, 256, 0, st>>>(R, S, Af);
-        vcycle_impl<VT, NR>(st, Gate{S.act}, T, A, val, dinv, r, z, R, true);
-        k_finish<NR, FinRZ<NR>><<<T.ninst, 256, 0, st>>>(R, S, Af);
+        // the V-cycle's <r, z> partials after update_xr's: one finisher for both
+        DevRed Rz = R;
+        Rz.partials += (size_t)2 * NR * R.nblk;
+        vcycle_impl<VT, NR>(st, Gate{S.act}, T, A, val, dinv, r, z, Rz, true);
+        k_finish<3 * NR, FinXRZ<NR>><<<T.ninst, 256, 0, st>>>(R, S, Af);
         k_update_pz<VT, NR><<<T.nblk, 256, 0, st>>>(T, z, p, x, REL ? x64 : nullptr, S);
     });
 }
