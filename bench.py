@@ -12,7 +12,7 @@ The timed solve is the default solver type (multigrid-preconditioned CG, as the
 reference's default AMG-CG) in mixed precision.  Beside it, once each: the same
 solve in FP64, and the plain method in FP64 (Jacobi-PCG, no multigrid).
 
-Prints one JSON line.  --gpus N > 1 is rejected: this build targets one GPU.
+Prints one JSON line.  --gpus N > 1 is rejected: the solver runs on one GPU.
 """
 from __future__ import annotations
 
@@ -491,8 +491,9 @@ def main():
                     help="also run one device copy of this size (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
-              "(DESIGN.md describes how the solve would shard).", file=sys.stderr)
+        print("bench.py: the solver runs on one GPU; for several GPUs, run one process per GPU on "
+              "separate instances (the batched workloads) -- there is no multi-GPU solve.",
+              file=sys.stderr)
         sys.exit(2)
 
     import __graft_entry__
