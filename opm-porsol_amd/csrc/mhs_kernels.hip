@@ -222,6 +222,45 @@ __device__ __forceinline__ void sell_row(const DevTopo &T, const MT *__restrict_
     }
 }
 
+// The same row once, for both products of a residual-replacement iteration:
+// acc = A32 p with A32 = (float) A64 (exactly the stored FP32 values) and
+// a64 = A64 x64, reading the indices and the FP64 values a single time.
+template <typename VT, int NRHS, int CH>
+__device__ __forceinline__ void sell_row_dual(const DevTopo &T, const double *__restrict__ val64,
+                                              const VT *__restrict__ p, const double *__restrict__ x64, int f,
+                                              VT (&acc)[NRHS], double (&a64)[NRHS]) {
+    const int slice = f >> 6, lane = f & 63;
+    const long long s0 = T.slice_ptr[slice];
+    const int w = (int)((T.slice_ptr[slice + 1] - s0) >> 6);
+    const long long base = s0 + lane;
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) {
+        acc[k] = VT(0);
+        a64[k] = 0.0;
+    }
+    for (int e0 = 0; e0 < w; e0 += CH) {
+        int ci[CH];
+        double a[CH];
+#pragma unroll
+        for (int e = 0; e < CH; ++e) {
+            if (e0 + e < w) {
+                ci[e] = __builtin_nontemporal_load(T.col + base + 64LL * (e0 + e));
+                a[e] = __builtin_nontemporal_load(val64 + base + 64LL * (e0 + e));
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < CH; ++e) {
+            if (e0 + e < w) {
+#pragma unroll
+                for (int k = 0; k < NRHS; ++k) {
+                    acc[k] += (VT)a[e] * p[(size_t)ci[e] * NRHS + k];
+                    a64[k] += a[e] * x64[(size_t)ci[e] * NRHS + k];
+                }
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Static inverse inner products (once per computeInnerProducts).
 // B^-1 = (N K N' + t A (I - Q Q') A) / |c|,  t = 6 tr(K)/3, Q = orth(C),
@@ -546,26 +585,35 @@ __global__ __launch_bounds__(TB) void k_spmv_pq(DevTopo T, const MT *__restrict_
     double part[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) part[q] = 0.0;
+    // A residual-replacement iteration (rel is block-uniform: the flags are
+    // per instance) forms the FP32 product and the FP64 residual from one
+    // pass over the indices and the FP64 values; other iterations stream the
+    // FP32 values.
+    if (!(REL && rel)) {
 #pragma unroll
-    for (int j = 0; j < kRowBlock / TB; ++j) {
-        const int f = b * kRowBlock + j * TB + threadIdx.x;
-        VT acc[NRHS];
-        sell_row<MT, VT, VT, NRHS, CH>(T, val, p, f, acc);
+        for (int j = 0; j < kRowBlock / TB; ++j) {
+            const int f = b * kRowBlock + j * TB + threadIdx.x;
+            VT acc[NRHS];
+            sell_row<MT, VT, VT, NRHS, CH>(T, val, p, f, acc);
 #pragma unroll
-        for (int k = 0; k < NRHS; ++k) {
-            q[(size_t)f * NRHS + k] = acc[k];
-            part[k] += (double)p[(size_t)f * NRHS + k] * (double)acc[k];
+            for (int k = 0; k < NRHS; ++k) {
+                q[(size_t)f * NRHS + k] = acc[k];
+                part[k] += (double)p[(size_t)f * NRHS + k] * (double)acc[k];
+            }
         }
     }
-    // The FP64 residual replacement in a loop of its own, so that its
-    // registers are not live during the FP32 product (rel is block-uniform:
-    // the flags are per instance).
     if constexpr (REL) {
         if (__builtin_expect(rel, 0)) {
             for (int j = 0; j < kRowBlock / TB; ++j) {
                 const int f = b * kRowBlock + j * TB + threadIdx.x;
+                VT acc[NRHS];
                 double a64[NRHS];
-                sell_row<double, double, double, NRHS, 4>(T, L.val64, L.x64, f, a64);
+                sell_row_dual<VT, NRHS, 4>(T, L.val64, p, L.x64, f, acc, a64);
+#pragma unroll
+                for (int k = 0; k < NRHS; ++k) {
+                    q[(size_t)f * NRHS + k] = acc[k];
+                    part[k] += (double)p[(size_t)f * NRHS + k] * (double)acc[k];
+                }
                 const float d = L.dinv[f];
 #pragma unroll
                 for (int k = 0; k < NRHS; ++k) {
