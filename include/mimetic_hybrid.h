@@ -51,7 +51,9 @@ enum mhs_status {
     MHS_ERR_NOT_CONVERGED = 3,  /* linear solver did not converge (:1416-1419) */
     MHS_ERR_HIP = 4,            /* HIP runtime error */
     MHS_ERR_ALLOC = 5,          /* device or host allocation failed */
-    MHS_ERR_BREAKDOWN = 6       /* CG breakdown: operator not positive definite */
+    MHS_ERR_BREAKDOWN = 6,      /* CG breakdown: operator not positive definite */
+    MHS_ERR_RANGE = 7           /* transport: saturation out of range after every retry
+                                   (EulerUpstream_impl.hpp:198-205, 343-345) */
 };
 
 /* FlowBC types (opm/porsol/common/BoundaryConditions.hpp:57, 121-171).

@@ -4,8 +4,9 @@ The directory name is not a Python identifier; load it with ``load_package``
 (in __graft_entry__.py, tests/conftest.py and bench.py), which registers it as
 ``opm_porsol_amd``.
 """
-from . import fields, grids, mhs  # noqa: F401
+from . import fields, grids, mhs, transport  # noqa: F401
 from .mhs import HybridSolver, MhsError  # noqa: F401
+from .transport import EulerUpstream  # noqa: F401
 
 
 def __getattr__(name):

@@ -6,9 +6,11 @@
 // per-system state between chunks to decide convergence and, in mixed
 // precision, when to replace the FP32 recursive residual by the FP64 true one.
 #include "../../include/mimetic_hybrid.h"
+#include "../../include/euler_upstream.h"
 #include "mhs_amg.h"
 #include "mhs_kernels.h"
 #include "mhs_structure.h"
+#include "mhs_transport.h"
 
 #include <hip/hip_runtime.h>
 
@@ -1615,6 +1617,404 @@ int mhs_get_system(mhs_solver *s, int k, int *row_ptr, int *col, double *val, do
         }
         if (row_ptr) row_ptr[f + 1] = (int)m;
         if (rhs) rhs[f] = bb[(size_t)r * nrhs + k];
+    }
+    return MHS_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Explicit upstream transport (include/euler_upstream.h)
+// ===========================================================================
+
+struct mhs_transport {
+    mhs_solver *flow = nullptr;
+    std::string err;
+    bool ready = false;
+    const int *topo_key = nullptr;      // the solver's device hf_cell when init ran
+    std::vector<DevBuf> bufs;
+    DevTransport D{};
+    double *F = nullptr, *G = nullptr, *C = nullptr, *pv = nullptr, *sat_bc = nullptr;
+    double *s_tmp = nullptr, *s_init = nullptr, *s_host = nullptr, *inj = nullptr, *res = nullptr;
+    unsigned long long *mins = nullptr;
+    int *flags = nullptr;               // [0] saturation out of range, [1] zero viscous CFL time
+    double fac[3] = {0.0, 0.0, 0.0};
+    ~mhs_transport() {
+        for (auto &b : bufs) (void)hipFree(b.p);
+    }
+};
+
+namespace {
+
+int tfail(mhs_transport *t, int code, const std::string &msg) {
+    if (t) t->err = msg;
+    return code;
+}
+
+#define TCHECK(expr)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return tfail(t, MHS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class X>
+X *talloc(mhs_transport *t, size_t n, hipError_t *e) {
+    void *p = nullptr;
+    *e = hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(X));
+    if (*e != hipSuccess) return nullptr;
+    t->bufs.push_back({p, std::max<size_t>(n, 1) * sizeof(X)});
+    return static_cast<X *>(p);
+}
+
+// computeSingleRockCflFactors(-1, 0, 0) (ReservoirPropertyCapillary_impl.hpp:
+// 215-250): 1 / the largest difference quotient of f_w and of
+// lambda_w lambda_o / lambda_t over 257 saturations; no capillary pressure
+// derivative without rock tables.
+void cfl_factors(double mu_w, double mu_o, double fac[3]) {
+    const int N = 257;
+    const double delta = 1.0 / double(N - 1);
+    auto ff = [&](double s, double &f1, double &fg) {
+        const double l1 = s * s / mu_w, l2 = (1.0 - s) * (1.0 - s) / mu_o;
+        f1 = l1 / (l1 + l2);
+        fg = l1 * l2 / (l1 + l2);
+    };
+    double last1, lastg;
+    ff(0.0, last1, lastg);
+    double max_der1 = -1e100, max_derg = -1e100, max_ffg = lastg;
+    for (int i = 1; i < N; ++i) {
+        const double s = double(i) * delta;
+        double f1, fg;
+        ff(s, f1, fg);
+        max_der1 = std::max(max_der1, std::fabs(f1 - last1) / delta);
+        max_derg = std::max(max_derg, std::fabs(fg - lastg) / delta);
+        max_ffg = std::max(max_ffg, fg);
+        last1 = f1;
+        lastg = fg;
+    }
+    fac[0] = 1.0 / max_der1;
+    fac[1] = 1.0 / max_derg;
+    fac[2] = max_ffg * 0.0;
+}
+
+mhs_transport_options default_transport_options() {
+    mhs_transport_options o;
+    o.courant_number = 0.5;
+    o.method_viscous = o.method_gravity = o.method_capillary = 1;
+    o.use_cfl_viscous = o.use_cfl_gravity = o.use_cfl_capillary = 1;
+    o.minimum_small_steps = 1;
+    o.maximum_small_steps = 10000;
+    o.check_sat = 1;
+    o.clamp_sat = 0;
+    return o;
+}
+
+int transport_ready(mhs_transport *t) {
+    if (!t) return MHS_ERR_INVALID;
+    if (!t->ready) return tfail(t, MHS_ERR_STATE, "transport: call mhs_transport_init first");
+    mhs_solver *s = t->flow;
+    if (!s->structure_valid || s->dT.hf_cell != t->topo_key)
+        return tfail(t, MHS_ERR_STATE, "transport: the solver's structure changed; call mhs_transport_init again");
+    if (!s->perm) return tfail(t, MHS_ERR_STATE, "transport: no permeability (call compute_inner_products)");
+    return MHS_OK;
+}
+
+// Per-face coefficients from the last outflux and the CFL time
+// (EulerUpstream::computeCflTime, EulerUpstream_impl.hpp:265-330).
+int transport_prepare(mhs_transport *t, const double *grav, const mhs_transport_options &o, double *cfl_dt) {
+    mhs_solver *s = t->flow;
+    if (!s->solved) return tfail(t, MHS_ERR_STATE, "transport: the solver has no solution (call solve)");
+    t->D.method_viscous = o.method_viscous != 0;
+    t->D.method_gravity = o.method_gravity != 0;
+    t->D.method_capillary = o.method_capillary != 0;
+    TCHECK(launch_tr_faces(s->stream, t->D, s->dT, s->dG, s->perm, s->outflux, grav));
+    const int use_v = o.method_viscous && o.use_cfl_viscous;
+    const int use_g = o.method_gravity && o.use_cfl_gravity;
+    const int use_c = o.method_capillary && o.use_cfl_capillary;
+    double init[2] = {1e100, 1e100};
+    int zero[2] = {0, 0};
+    TCHECK(hipMemcpyAsync(t->mins, init, sizeof(init), hipMemcpyHostToDevice, s->stream));
+    TCHECK(hipMemcpyAsync(t->flags, zero, sizeof(zero), hipMemcpyHostToDevice, s->stream));
+    TCHECK(launch_tr_cfl(s->stream, t->D, s->dT, s->dG, s->perm, s->outflux, grav, t->fac[0], t->fac[1], use_v,
+                         use_g, t->mins, t->flags));
+    double mins[2];
+    int fl[2];
+    TCHECK(hipMemcpyAsync(mins, t->mins, sizeof(mins), hipMemcpyDeviceToHost, s->stream));
+    TCHECK(hipMemcpyAsync(fl, t->flags, sizeof(fl), hipMemcpyDeviceToHost, s->stream));
+    TCHECK(hipStreamSynchronize(s->stream));
+    if (fl[1]) return tfail(t, MHS_ERR_INVALID, "Cfl computation gave dt = 0.0");
+    // findCFLtimeCapillary (CflCalculator.hpp:143-170) divides by the
+    // capillary factor, which is 0 for this fluid: every local time is +inf
+    // (or NaN) and the time stays 1e100.
+    double dt_c = 1e100;
+    (void)use_c;
+    *cfl_dt = std::min(std::min(mins[0], mins[1]), dt_c) * o.courant_number;
+    return MHS_OK;
+}
+
+// transportSolve (:153-215) on device saturation `sat`.
+int transport_run(mhs_transport *t, double *sat, double time, const double *grav, const double *inj,
+                  const mhs_transport_options &o, mhs_transport_report *rep) {
+    mhs_solver *s = t->flow;
+    const auto t0 = std::chrono::steady_clock::now();
+    double cfl_dt = 0.0;
+    int rc = transport_prepare(t, grav, o, &cfl_dt);
+    if (rc != MHS_OK) return rc;
+    int nsteps;
+    if (cfl_dt > time) {
+        nsteps = o.minimum_small_steps;
+    } else {
+        const double steps = std::min<double>(std::ceil(time / cfl_dt), 2147483647.0);
+        nsteps = std::min(std::max(int(steps), o.minimum_small_steps), o.maximum_small_steps);
+    }
+    const size_t nc = (size_t)s->nc;
+    TCHECK(hipMemcpyAsync(t->s_init, sat, nc * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+    int repeats = 0;
+    const int max_repeats = 10;
+    for (;;) {
+        const double dt = time / nsteps;
+        int zero = 0, bad = 0;
+        TCHECK(hipMemcpyAsync(t->flags, &zero, sizeof(int), hipMemcpyHostToDevice, s->stream));
+        double *a = sat, *b = t->s_tmp;
+        for (int q = 0; q < nsteps; ++q) {
+            TCHECK(launch_tr_step(s->stream, t->D, s->dT, a, b, inj, dt, o.check_sat, o.clamp_sat, t->flags));
+            std::swap(a, b);
+        }
+        if (a != sat) TCHECK(hipMemcpyAsync(sat, a, nc * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+        TCHECK(hipMemcpyAsync(&bad, t->flags, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+        TCHECK(hipStreamSynchronize(s->stream));
+        if (!bad) {
+            if (rep) {
+                rep->small_steps = nsteps;
+                rep->repeats = repeats;
+                rep->cfl_dt = cfl_dt;
+                rep->dt = dt;
+            }
+            break;
+        }
+        if (++repeats > max_repeats) {
+            TCHECK(hipMemcpyAsync(sat, t->s_init, nc * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+            TCHECK(hipStreamSynchronize(s->stream));
+            return tfail(t, MHS_ERR_RANGE, "Saturation out of range in EulerUpstream (after " +
+                                               std::to_string(max_repeats) + " retries with more steps)");
+        }
+        nsteps *= 2;
+        TCHECK(hipMemcpyAsync(sat, t->s_init, nc * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+    }
+    if (rep) {
+        rep->cfl_factor = t->fac[0];
+        rep->cfl_factor_gravity = t->fac[1];
+        rep->cfl_factor_capillary = t->fac[2];
+        rep->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return MHS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void mhs_transport_default_options(mhs_transport_options *opt) {
+    if (opt) *opt = default_transport_options();
+}
+
+int mhs_transport_create(mhs_transport **out, mhs_solver *flow) {
+    if (!out || !flow) return MHS_ERR_INVALID;
+    *out = nullptr;
+    mhs_transport *t = new (std::nothrow) mhs_transport();
+    if (!t) return MHS_ERR_ALLOC;
+    t->flow = flow;
+    *out = t;
+    return MHS_OK;
+}
+
+int mhs_transport_destroy(mhs_transport *t) {
+    if (!t) return MHS_OK;
+    (void)hipSetDevice(t->flow->device);
+    delete t;
+    return MHS_OK;
+}
+
+const char *mhs_transport_last_error(const mhs_transport *t) { return t ? t->err.c_str() : "null transport"; }
+
+int mhs_transport_init(mhs_transport *t, const double *porosity, const double *sat_bc, const mhs_fluid *fluid) {
+    if (!t) return MHS_ERR_INVALID;
+    mhs_solver *s = t->flow;
+    if (!porosity || !fluid) return tfail(t, MHS_ERR_INVALID, "transport_init: porosity and fluid required");
+    if (!s->structure_valid)
+        return tfail(t, MHS_ERR_STATE, "transport_init: call mhs_init_system_structure on the solver first");
+    if (!(fluid->viscosity[0] > 0.0) || !(fluid->viscosity[1] > 0.0))
+        return tfail(t, MHS_ERR_INVALID, "transport_init: viscosities must be positive");
+    TCHECK(hipSetDevice(s->device));
+    const Topology &T = s->T;
+    const int nc = T.nc, nhf = T.nhf;
+    // EulerUpstreamResidual::initFinal (:415-445) keeps the periodic partner
+    // of every periodic boundary face; the owner of a face is its
+    // lower-numbered cell (UpdateForCell skips the face from the other side).
+    std::vector<int> other(nhf), owner(nhf), nbf(nhf);
+    std::vector<uint8_t> kind(nhf);
+    for (int h = 0; h < nhf; ++h) {
+        if (T.nb[h] >= 0) {
+            other[h] = T.nb[h];
+            kind[h] = 1;
+        } else if (T.hf_ppart[h] >= 0) {
+            other[h] = T.hf_cell[T.hf_ppart[h]];
+            kind[h] = 2;
+        } else {
+            other[h] = -1;
+            kind[h] = 0;
+        }
+        if (other[h] == T.hf_cell[h])
+            return tfail(t, MHS_ERR_INVALID, "transport_init: a face links a cell to itself (half-face " +
+                                                 std::to_string(h) + ")");
+    }
+    for (int h = 0; h < nhf; ++h) {
+        const bool own = other[h] < 0 || T.hf_cell[h] < other[h];
+        const int o = own ? h : T.hf_twin[h];
+        if (o < 0 || o >= nhf) return tfail(t, MHS_ERR_INVALID, "transport_init: face without its twin half-face");
+        owner[h] = o;
+        nbf[h] = kind[o] == 2 ? T.hf_ppart[o] : o;
+    }
+    std::vector<double> pv(nc), sbc(nhf, 1.0);
+    for (int c = 0; c < nc; ++c) pv[c] = s->cvol[c] * porosity[c];
+    if (sat_bc)
+        for (int h = 0; h < nhf; ++h) sbc[h] = sat_bc[h];
+    for (auto &b : t->bufs) (void)hipFree(b.p);
+    t->bufs.clear();
+    t->ready = false;
+    hipError_t e = hipSuccess;
+    int *d_other = talloc<int>(t, nhf, &e);
+    int *d_owner = e == hipSuccess ? talloc<int>(t, nhf, &e) : nullptr;
+    int *d_nbf = e == hipSuccess ? talloc<int>(t, nhf, &e) : nullptr;
+    uint8_t *d_kind = e == hipSuccess ? talloc<uint8_t>(t, nhf, &e) : nullptr;
+    if (e == hipSuccess) t->sat_bc = talloc<double>(t, nhf, &e);
+    if (e == hipSuccess) t->pv = talloc<double>(t, nc, &e);
+    if (e == hipSuccess) t->F = talloc<double>(t, nhf, &e);
+    if (e == hipSuccess) t->G = talloc<double>(t, nhf, &e);
+    if (e == hipSuccess) t->C = talloc<double>(t, nhf, &e);
+    if (e == hipSuccess) t->s_tmp = talloc<double>(t, nc, &e);
+    if (e == hipSuccess) t->s_init = talloc<double>(t, nc, &e);
+    if (e == hipSuccess) t->s_host = talloc<double>(t, nc, &e);
+    if (e == hipSuccess) t->inj = talloc<double>(t, nc, &e);
+    if (e == hipSuccess) t->res = talloc<double>(t, nc, &e);
+    if (e == hipSuccess) t->mins = talloc<unsigned long long>(t, 2, &e);
+    if (e == hipSuccess) t->flags = talloc<int>(t, 2, &e);
+    if (e != hipSuccess) return tfail(t, MHS_ERR_ALLOC, std::string("transport_init: ") + hipGetErrorString(e));
+    TCHECK(hipMemcpy(d_other, other.data(), nhf * sizeof(int), hipMemcpyHostToDevice));
+    TCHECK(hipMemcpy(d_owner, owner.data(), nhf * sizeof(int), hipMemcpyHostToDevice));
+    TCHECK(hipMemcpy(d_nbf, nbf.data(), nhf * sizeof(int), hipMemcpyHostToDevice));
+    TCHECK(hipMemcpy(d_kind, kind.data(), nhf * sizeof(uint8_t), hipMemcpyHostToDevice));
+    TCHECK(hipMemcpy(t->sat_bc, sbc.data(), nhf * sizeof(double), hipMemcpyHostToDevice));
+    TCHECK(hipMemcpy(t->pv, pv.data(), nc * sizeof(double), hipMemcpyHostToDevice));
+    DevTransport &D = t->D;
+    D = DevTransport{};
+    D.nc = nc;
+    D.nhf = nhf;
+    D.other = d_other;
+    D.kind = d_kind;
+    D.owner_hf = d_owner;
+    D.owner_nbf = d_nbf;
+    D.sat_bc = t->sat_bc;
+    D.pv = t->pv;
+    D.F = t->F;
+    D.G = t->G;
+    D.C = t->C;
+    D.visc_w = fluid->viscosity[0];
+    D.visc_o = fluid->viscosity[1];
+    D.rho_w = fluid->density[0];
+    D.rho_o = fluid->density[1];
+    D.drho = fluid->density[0] - fluid->density[1];
+    cfl_factors(D.visc_w, D.visc_o, t->fac);
+    t->topo_key = s->dT.hf_cell;
+    t->ready = true;
+    return MHS_OK;
+}
+
+int mhs_transport_solve_device(mhs_transport *t, double *saturation, double time, const double *gravity,
+                               const double *injection_rates, const mhs_transport_options *opt,
+                               mhs_transport_report *report) {
+    int rc = transport_ready(t);
+    if (rc != MHS_OK) return rc;
+    if (!saturation || !gravity) return tfail(t, MHS_ERR_INVALID, "transport_solve: saturation and gravity required");
+    if (!(time >= 0.0)) return tfail(t, MHS_ERR_INVALID, "transport_solve: time must be non-negative");
+    mhs_solver *s = t->flow;
+    TCHECK(hipSetDevice(s->device));
+    const mhs_transport_options o = opt ? *opt : default_transport_options();
+    if (o.minimum_small_steps < 1 || o.maximum_small_steps < o.minimum_small_steps)
+        return tfail(t, MHS_ERR_INVALID, "transport_solve: need 1 <= minimum_small_steps <= maximum_small_steps");
+    if (s->user_stream) {
+        TCHECK(hipEventRecord(s->ev_in, s->user_stream));
+        TCHECK(hipStreamWaitEvent(s->stream, s->ev_in, 0));
+    }
+    if (report) std::memset(report, 0, sizeof(*report));
+    rc = transport_run(t, saturation, time, gravity, injection_rates, o, report);
+    if (rc != MHS_OK) return rc;
+    if (s->user_stream) {
+        TCHECK(hipEventRecord(s->ev_out, s->stream));
+        TCHECK(hipStreamWaitEvent(s->user_stream, s->ev_out, 0));
+    }
+    return MHS_OK;
+}
+
+int mhs_transport_solve(mhs_transport *t, double *saturation, double time, const double *gravity,
+                        const double *injection_rates, const mhs_transport_options *opt,
+                        mhs_transport_report *report) {
+    int rc = transport_ready(t);
+    if (rc != MHS_OK) return rc;
+    if (!saturation || !gravity) return tfail(t, MHS_ERR_INVALID, "transport_solve: saturation and gravity required");
+    if (!(time >= 0.0)) return tfail(t, MHS_ERR_INVALID, "transport_solve: time must be non-negative");
+    mhs_solver *s = t->flow;
+    TCHECK(hipSetDevice(s->device));
+    const mhs_transport_options o = opt ? *opt : default_transport_options();
+    if (o.minimum_small_steps < 1 || o.maximum_small_steps < o.minimum_small_steps)
+        return tfail(t, MHS_ERR_INVALID, "transport_solve: need 1 <= minimum_small_steps <= maximum_small_steps");
+    const size_t nc = (size_t)s->nc;
+    TCHECK(hipMemcpy(t->s_host, saturation, nc * sizeof(double), hipMemcpyHostToDevice));
+    if (injection_rates) TCHECK(hipMemcpy(t->inj, injection_rates, nc * sizeof(double), hipMemcpyHostToDevice));
+    if (report) std::memset(report, 0, sizeof(*report));
+    rc = transport_run(t, t->s_host, time, gravity, injection_rates ? t->inj : nullptr, o, report);
+    if (rc != MHS_OK && rc != MHS_ERR_RANGE) return rc;
+    TCHECK(hipMemcpy(saturation, t->s_host, nc * sizeof(double), hipMemcpyDeviceToHost));
+    return rc;
+}
+
+int mhs_transport_residual(mhs_transport *t, const double *saturation, const double *gravity,
+                           const double *injection_rates, const mhs_transport_options *opt, double *residual) {
+    int rc = transport_ready(t);
+    if (rc != MHS_OK) return rc;
+    if (!saturation || !gravity || !residual)
+        return tfail(t, MHS_ERR_INVALID, "transport_residual: saturation, gravity and residual required");
+    mhs_solver *s = t->flow;
+    if (!s->solved) return tfail(t, MHS_ERR_STATE, "transport: the solver has no solution (call solve)");
+    TCHECK(hipSetDevice(s->device));
+    const mhs_transport_options o = opt ? *opt : default_transport_options();
+    t->D.method_viscous = o.method_viscous != 0;
+    t->D.method_gravity = o.method_gravity != 0;
+    t->D.method_capillary = o.method_capillary != 0;
+    const size_t nc = (size_t)s->nc;
+    TCHECK(hipMemcpy(t->s_host, saturation, nc * sizeof(double), hipMemcpyHostToDevice));
+    if (injection_rates) TCHECK(hipMemcpy(t->inj, injection_rates, nc * sizeof(double), hipMemcpyHostToDevice));
+    TCHECK(launch_tr_faces(s->stream, t->D, s->dT, s->dG, s->perm, s->outflux, gravity));
+    TCHECK(launch_tr_residual(s->stream, t->D, s->dT, t->s_host, injection_rates ? t->inj : nullptr, t->res));
+    TCHECK(hipStreamSynchronize(s->stream));
+    TCHECK(hipMemcpy(residual, t->res, nc * sizeof(double), hipMemcpyDeviceToHost));
+    return MHS_OK;
+}
+
+int mhs_transport_mobilities_device(mhs_transport *t, const double *saturation, double *totmob, double *rhomob) {
+    if (!t) return MHS_ERR_INVALID;
+    if (!t->ready) return tfail(t, MHS_ERR_STATE, "transport: call mhs_transport_init first");
+    if (!saturation) return tfail(t, MHS_ERR_INVALID, "transport_mobilities: saturation required");
+    mhs_solver *s = t->flow;
+    TCHECK(hipSetDevice(s->device));
+    if (s->user_stream) {
+        TCHECK(hipEventRecord(s->ev_in, s->user_stream));
+        TCHECK(hipStreamWaitEvent(s->stream, s->ev_in, 0));
+    }
+    TCHECK(launch_tr_mobilities(s->stream, t->D, saturation, totmob, rhomob));
+    if (s->user_stream) {
+        TCHECK(hipEventRecord(s->ev_out, s->stream));
+        TCHECK(hipStreamWaitEvent(s->user_stream, s->ev_out, 0));
     }
     return MHS_OK;
 }

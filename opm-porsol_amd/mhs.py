@@ -21,7 +21,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libmhs.so")
 
 MHS_OK, MHS_ERR_INVALID, MHS_ERR_STATE, MHS_ERR_NOT_CONVERGED, MHS_ERR_HIP, MHS_ERR_ALLOC, \
-    MHS_ERR_BREAKDOWN = range(7)
+    MHS_ERR_BREAKDOWN, MHS_ERR_RANGE = range(8)
 NEUMANN, DIRICHLET, PERIODIC = 0, 1, 2
 FP64, MIXED = 0, 1
 

@@ -1,0 +1,222 @@
+"""The explicit upstream transport on the GPU (include/euler_upstream.h)
+against the oracle (oracle/euler_upstream.py) on the fluxes of a GPU
+pressure solve: the residual term by term, whole transport solves with the
+CFL step count, periodic and corner-point grids, the retry and clamp rules,
+the out-of-range error, and an IMPES loop (pressure, then transport, with
+mobilities from the new saturation) against the oracle's own loop.
+
+Bars: residuals to 1e-12 of their largest entry (FP64, same fluxes, only the
+summation order differs); saturations to 1e-12 after a transport solve on the
+same fluxes; 1e-8 through an IMPES loop whose pressure solves stop at a
+relative residual of 1e-12 (GPU) and 1e-14 (oracle)."""
+import numpy as np
+import pytest
+
+import cases
+import euler_upstream as tr
+
+pytestmark = pytest.mark.gpu
+
+VISC = (1.0e-3, 3.0e-3)
+RHO = (1013.9, 834.7)
+DRHO = RHO[0] - RHO[1]
+GRAV = (0.0, 0.0, 9.81)
+BAR = 1e5
+
+
+def _fluid(sat):
+    lw, lo = tr.mobilities(sat, VISC)
+    return lw + lo, RHO[0] * lw + RHO[1] * lo
+
+
+def _flow(pkg, g, perm, bt, bv, sat, partner=None, grav=GRAV, src=None):
+    s = pkg.HybridSolver(0)
+    s.init(g, perm, grav, bt, partner)
+    tm, rm = _fluid(sat)
+    s.solve(tm, rm, np.zeros(g.num_cells) if src is None else src, bt, bv, residual_tolerance=1e-12)
+    return s, np.asarray(s.getSolution()[2]).reshape(-1)
+
+
+def _cart_case(pkg, dims=(12, 10, 6), seed=3):
+    g = pkg.grids.cartesian(*dims, 2.0, 2.0, 1.0)
+    nc, nhf = g.num_cells, g.num_half_faces
+    rng = np.random.default_rng(seed)
+    perm = pkg.fields.perm_c1(nc, seed)
+    bt, bv = cases.xdrop_bc(g, 2 * BAR, 0.0)
+    sat = rng.uniform(0.0, 1.0, nc)
+    sat_bc = rng.uniform(0.0, 1.0, nhf)
+    poro = rng.uniform(0.1, 0.3, nc)
+    inj = np.zeros(nc)
+    inj[5], inj[nc - 7] = 3e-8, -2e-8
+    return g, perm, bt, bv, sat, sat_bc, poro, inj
+
+
+@pytest.mark.parametrize("methods", [(1, 1, 1), (1, 0, 0), (0, 1, 0), (0, 0, 1)])
+def test_residual_matches_oracle(pkg, methods):
+    g, perm, bt, bv, sat, sat_bc, poro, inj = _cart_case(pkg)
+    s, flux = _flow(pkg, g, perm, bt, bv, sat)
+    t = pkg.EulerUpstream(s, poro, sat_bc, VISC, RHO)
+    mv, mg, mc = map(bool, methods)
+    r_gpu = t.computeResidual(sat, GRAV, inj, mv, mg, mc)
+    r_orc = tr.residual(g, perm, sat, flux, GRAV, DRHO, VISC, inj, sat_bc, None, mv, mg, mc)
+    assert np.max(np.abs(r_orc)) > 0
+    assert cases.rel_err(r_gpu, r_orc) < 1e-12
+
+
+def test_residual_corner_point_full_tensor(pkg):
+    g = pkg.grids.corner_point(10, 8, 5, 3.0, 3.0, 1.0, seed=4)
+    nc, nhf = g.num_cells, g.num_half_faces
+    rng = np.random.default_rng(9)
+    perm = pkg.fields.perm_c4(nc, 4)
+    bt, bv = cases.xdrop_bc(g, 3 * BAR, BAR)
+    sat = rng.uniform(0.0, 1.0, nc)
+    sat_bc = rng.uniform(0.0, 1.0, nhf)
+    s, flux = _flow(pkg, g, perm, bt, bv, sat)
+    t = pkg.EulerUpstream(s, np.full(nc, 0.2), sat_bc, VISC, RHO)
+    r_gpu = t.computeResidual(sat, GRAV)
+    r_orc = tr.residual(g, perm, sat, flux, GRAV, DRHO, VISC, None, sat_bc)
+    assert cases.rel_err(r_gpu, r_orc) < 1e-12
+
+
+def test_transport_solve_matches_oracle(pkg):
+    g, perm, bt, bv, sat, sat_bc, poro, inj = _cart_case(pkg, seed=5)
+    sat = np.clip(sat, 0.3, 0.7)
+    s, flux = _flow(pkg, g, perm, bt, bv, sat)
+    t = pkg.EulerUpstream(s, poro, sat_bc, VISC, RHO)
+    cfl = tr.cfl_time(g, perm, poro, flux, GRAV, DRHO, VISC)
+    T = 24.5 * cfl
+    s_orc, n_orc, rep_orc, cfl_orc = tr.transport_solve(g, perm, poro, sat, T, GRAV, flux, DRHO, VISC, inj, sat_bc)
+    s_gpu = sat.copy()
+    t.transportSolve(s_gpu, T, GRAV, inj)
+    r = t.report
+    assert r.cfl_dt == pytest.approx(cfl_orc, rel=1e-13)
+    # the capillary CFL time is 1e100 for this fluid (as in the reference), so
+    # the capillary term can push a step out of range and force a retry
+    assert (r.small_steps, r.repeats) == (n_orc, rep_orc)
+    assert n_orc == 25 * 2 ** rep_orc
+    fac = tr.cfl_factors(VISC)
+    assert (r.cfl_factor, r.cfl_factor_gravity, r.cfl_factor_capillary) == fac
+    np.testing.assert_allclose(s_gpu, s_orc, rtol=0, atol=1e-12)
+    assert np.max(np.abs(s_gpu - sat)) > 1e-3
+
+
+def test_periodic_transport(pkg):
+    """Periodic in x and y with a pressure drop along x: the saturation
+    crosses the periodic faces through the partner cells."""
+    g = pkg.grids.cartesian(8, 6, 4, 2.0, 2.0, 1.0)
+    nc, nhf = g.num_cells, g.num_half_faces
+    bt, bv, part = cases.periodic_bc(g, (0, 1), 0, BAR)
+    perm = pkg.fields.perm_c1(nc, 7)
+    rng = np.random.default_rng(1)
+    sat = rng.uniform(0.2, 0.8, nc)
+    poro = np.full(nc, 0.25)
+    s, flux = _flow(pkg, g, perm, bt, bv, sat, partner=part, grav=(0.0, 0.0, 0.0))
+    sat_bc = np.full(nhf, 0.5)
+    t = pkg.EulerUpstream(s, poro, sat_bc, VISC, RHO)
+    r_gpu = t.computeResidual(sat, (0.0, 0.0, 0.0))
+    r_orc = tr.residual(g, perm, sat, flux, (0, 0, 0), DRHO, VISC, None, sat_bc, part)
+    assert cases.rel_err(r_gpu, r_orc) < 1e-12
+    cfl = tr.cfl_time(g, perm, poro, flux, (0, 0, 0), DRHO, VISC)
+    s_orc, n_orc, _, _ = tr.transport_solve(g, perm, poro, sat, 10.2 * cfl, (0, 0, 0), flux, DRHO, VISC, None,
+                                            sat_bc, part)
+    s_gpu = sat.copy()
+    t.transportSolve(s_gpu, 10.2 * cfl, (0.0, 0.0, 0.0))
+    assert t.report.small_steps == n_orc and n_orc % 11 == 0
+    np.testing.assert_allclose(s_gpu, s_orc, rtol=0, atol=1e-12)
+    # closed system: water is conserved up to what the no-flow faces carry (the
+    # pressure solve's residual, stopped at 1e-12)
+    assert np.sum(poro * s_gpu) == pytest.approx(np.sum(poro * sat), rel=1e-10)
+
+
+@pytest.mark.parametrize("clamp", [False, True])
+def test_retry_and_clamp(pkg, clamp):
+    """Courant number 8: without clamping the pass overshoots and restarts
+    with doubled steps; with clamping it runs once.  Both as the oracle."""
+    g, perm, bt, bv, sat, sat_bc, poro, inj = _cart_case(pkg, (16, 4, 2), seed=11)
+    s, flux = _flow(pkg, g, perm, bt, bv, sat, grav=(0.0, 0.0, 0.0))
+    t = pkg.EulerUpstream(s, poro, sat_bc, VISC, RHO)
+    t.init({"courant_number": 8.0, "clamp_sat": clamp, "method_gravity": False})
+    cfl = tr.cfl_time(g, perm, poro, flux, (0, 0, 0), DRHO, VISC, 8.0, use_gravity=False)
+    T = 6.0 * cfl
+    s_orc, n_orc, rep_orc, _ = tr.transport_solve(g, perm, poro, sat, T, (0, 0, 0), flux, DRHO, VISC, None, sat_bc,
+                                                  courant_number=8.0, method_gravity=False, clamp_sat=clamp)
+    s_gpu = sat.copy()
+    t.transportSolve(s_gpu, T, (0.0, 0.0, 0.0))
+    assert (t.report.small_steps, t.report.repeats) == (n_orc, rep_orc)
+    assert (rep_orc == 0) if clamp else (rep_orc >= 1)
+    np.testing.assert_allclose(s_gpu, s_orc, rtol=0, atol=1e-12)
+
+
+def test_out_of_range_raises_and_restores(pkg):
+    """An injection that overfills a closed box fails for every step count:
+    RuntimeError after the retries, the saturation given back unchanged."""
+    g = pkg.grids.cartesian(4, 3, 2)
+    nc, nhf = g.num_cells, g.num_half_faces
+    bt = np.zeros(nhf, np.int32)
+    bv = np.zeros(nhf)
+    src = np.zeros(nc)
+    s, _ = _flow(pkg, g, pkg.fields.perm_c1(nc, 2), bt, bv, np.full(nc, 0.1), grav=(0.0, 0.0, 0.0), src=src)
+    t = pkg.EulerUpstream(s, np.full(nc, 0.2), None, VISC, RHO)
+    inj = np.zeros(nc)
+    inj[0] = 1e-3
+    sat = np.full(nc, 0.1)
+    with pytest.raises(RuntimeError, match="Saturation out of range"):
+        t.transportSolve(sat, 1e3, (0.0, 0.0, 0.0), inj)
+    assert np.all(sat == 0.1)
+    with pytest.raises(tr.SaturationOutOfRange):
+        tr.transport_solve(g, pkg.fields.perm_c1(nc, 2), np.full(nc, 0.2), np.full(nc, 0.1), 1e3, (0, 0, 0),
+                           np.zeros(nhf), DRHO, VISC, inj)
+
+
+def test_impes_loop_matches_oracle(pkg, orc):
+    """SimulatorTester::run (examples/SimulatorTester.hpp:59-107): pressure
+    with the mobilities of the current saturation, then transport over the
+    step; three steps, same structure, changing matrix."""
+    g = pkg.grids.cartesian(10, 8, 4, 2.0, 2.0, 1.0)
+    nc, nhf = g.num_cells, g.num_half_faces
+    perm = pkg.fields.perm_c1(nc, 12)
+    bt, bv = cases.xdrop_bc(g, 2 * BAR, 0.0)
+    sat_bc = np.zeros(nhf)
+    sat_bc[g.side == 0] = 1.0           # water enters through x-
+    poro = np.full(nc, 0.2)
+    sat0 = np.zeros(nc)
+    s = pkg.HybridSolver(0)
+    s.init(g, perm, GRAV, bt, None)
+    t = pkg.EulerUpstream(s, poro, sat_bc, VISC, RHO)
+    step = 2.0e5
+    s_gpu = sat0.copy()
+    s_orc = sat0.copy()
+    for _ in range(3):
+        tm, rm = _fluid(s_gpu)
+        s.solve(tm, rm, np.zeros(nc), bt, bv, residual_tolerance=1e-12)
+        t.transportSolve(s_gpu, step, GRAV)
+        tm, rm = _fluid(s_orc)
+        ref = orc.solve(g, perm, tm, rm, np.zeros(nc), bt, bv, grav=GRAV, tol=1e-14)
+        s_orc, *_ = tr.transport_solve(g, perm, poro, s_orc, step, GRAV, ref.outflux, DRHO, VISC, None, sat_bc)
+    assert np.max(s_orc) > 0.3
+    np.testing.assert_allclose(s_gpu, s_orc, rtol=0, atol=1e-8)
+
+
+def test_device_path_matches_host_path(pkg):
+    """transportSolve_device and mobilities_device on torch tensors give the
+    host path's saturation and the host mobilities bit for bit."""
+    import torch
+    g, perm, bt, bv, sat, sat_bc, poro, inj = _cart_case(pkg, (8, 8, 4), seed=13)
+    sat = np.clip(sat, 0.3, 0.7)
+    s, flux = _flow(pkg, g, perm, bt, bv, sat)
+    t = pkg.EulerUpstream(s, poro, sat_bc, VISC, RHO)
+    cfl = tr.cfl_time(g, perm, poro, flux, GRAV, DRHO, VISC)
+    host = sat.copy()
+    t.transportSolve(host, 7.5 * cfl, GRAV, inj)
+    d_sat = torch.tensor(sat, dtype=torch.float64, device="cuda")
+    d_inj = torch.tensor(inj, dtype=torch.float64, device="cuda")
+    t.transportSolve_device(d_sat.data_ptr(), 7.5 * cfl, GRAV, d_inj.data_ptr())
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(d_sat.cpu().numpy(), host)
+    tm = torch.empty_like(d_sat)
+    rm = torch.empty_like(d_sat)
+    t.mobilities_device(d_sat.data_ptr(), tm.data_ptr(), rm.data_ptr())
+    torch.cuda.synchronize()
+    tm_h, rm_h = _fluid(host)
+    np.testing.assert_allclose(tm.cpu().numpy(), tm_h, rtol=1e-15, atol=0)
+    np.testing.assert_allclose(rm.cpu().numpy(), rm_h, rtol=1e-15, atol=0)
