@@ -120,6 +120,12 @@ int mhs_transport_residual(mhs_transport *t, const double *saturation, const dou
 int mhs_transport_mobilities_device(mhs_transport *t, const double *saturation, double *totmob,
                                     double *rhomob);
 
+/* Profiling hook, no counterpart in the reference: average device time (ms)
+ * of `reps` back-to-back small-step kernels (saturation update of every
+ * cell) on the solver's stream, timed with HIP events, on the face data and
+ * initial saturation of the last successful mhs_transport_solve*. */
+int mhs_transport_time_step(mhs_transport *t, int reps, double *avg_ms);
+
 #ifdef __cplusplus
 }
 #endif

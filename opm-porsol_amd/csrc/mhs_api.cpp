@@ -83,7 +83,8 @@ struct mhs_solver {
     // static inner products
     double *binv = nullptr, *gflux = nullptr;
     double grav[3] = {0.0, 0.0, 0.0};
-    double *perm = nullptr;             // K per cell (row-major 3x3), kept for the adjoint
+    double *perm = nullptr;             // K per cell (row-major 3x3), kept for the adjoint and transport
+    long long perm_gen = 0;             // counts permeability uploads
 
     // multigrid hierarchy
     std::vector<int> ijk;
@@ -533,6 +534,7 @@ int inner_products_impl(mhs_solver *s, const double *perm, const double *gravity
         HIPCHECK(hipEventRecord(s->ev_in, s->user_stream));
         HIPCHECK(hipStreamWaitEvent(s->stream, s->ev_in, 0));
     }
+    ++s->perm_gen;
     hipError_t e = hipMemcpyAsync(s->perm, perm, n9 * sizeof(double),
                                   device_perm ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s->stream);
     if (e == hipSuccess)
@@ -1639,6 +1641,13 @@ struct mhs_transport {
     unsigned long long *mins = nullptr;
     int *flags = nullptr;               // [0] saturation out of range, [1] zero viscous CFL time
     double fac[3] = {0.0, 0.0, 0.0};
+    bool timed_ok = false;              // a transport solve set up the face coefficients
+    double last_dt = 0.0;
+    // key of the gravity / capillary coefficients and the gravity CFL time
+    long long coeff_gen = -1;
+    int coeff_cap = -1;
+    double coeff_grav[3] = {0.0, 0.0, 0.0};
+    double dt_g = 1e100;
     ~mhs_transport() {
         for (auto &b : bufs) (void)hipFree(b.p);
     }
@@ -1719,36 +1728,69 @@ int transport_ready(mhs_transport *t) {
     return MHS_OK;
 }
 
-// Per-face coefficients from the last outflux and the CFL time
-// (EulerUpstream::computeCflTime, EulerUpstream_impl.hpp:265-330).
-int transport_prepare(mhs_transport *t, const double *grav, const mhs_transport_options &o, double *cfl_dt) {
+// Gravity and capillary coefficients and the gravity CFL time, recomputed
+// only when the permeability, the gravity vector or the capillary switch
+// changed since the last call.
+int transport_coeffs(mhs_transport *t, const double *grav, int cap) {
+    mhs_solver *s = t->flow;
+    if (t->coeff_gen == s->perm_gen && t->coeff_cap == cap && t->coeff_grav[0] == grav[0] &&
+        t->coeff_grav[1] == grav[1] && t->coeff_grav[2] == grav[2])
+        return MHS_OK;
+    const int has_g = grav[0] != 0.0 || grav[1] != 0.0 || grav[2] != 0.0;
+    TCHECK(launch_tr_coeffs(s->stream, t->D, s->dT, s->dG, s->perm, grav, has_g, cap));
+    double dt_g = 1e100;
+    if (has_g) {
+        // with a zero gravity vector every local time is +inf (or NaN) and the
+        // reference's minimum stays 1e100
+        TCHECK(hipMemcpyAsync(t->mins + 1, &dt_g, sizeof(double), hipMemcpyHostToDevice, s->stream));
+        TCHECK(launch_tr_cfl_gravity(s->stream, t->D, s->dT, s->dG, s->perm, grav, t->fac[1], t->mins + 1));
+        TCHECK(hipMemcpyAsync(&dt_g, t->mins + 1, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    }
+    TCHECK(hipStreamSynchronize(s->stream));
+    t->dt_g = dt_g;
+    t->coeff_gen = s->perm_gen;
+    t->coeff_cap = cap;
+    for (int d = 0; d < 3; ++d) t->coeff_grav[d] = grav[d];
+    return MHS_OK;
+}
+
+// Face fluxes of the last pressure solve and the viscous CFL time.
+int transport_flux(mhs_transport *t, bool check_zero, double *dt_v) {
     mhs_solver *s = t->flow;
     if (!s->solved) return tfail(t, MHS_ERR_STATE, "transport: the solver has no solution (call solve)");
+    double init = 1e100;
+    int zero = 0;
+    TCHECK(hipMemcpyAsync(t->mins, &init, sizeof(double), hipMemcpyHostToDevice, s->stream));
+    TCHECK(hipMemcpyAsync(t->flags + 1, &zero, sizeof(int), hipMemcpyHostToDevice, s->stream));
+    TCHECK(launch_tr_flux(s->stream, t->D, s->dT, s->outflux, t->fac[0], t->mins, t->flags + 1));
+    double v;
+    int z;
+    TCHECK(hipMemcpyAsync(&v, t->mins, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    TCHECK(hipMemcpyAsync(&z, t->flags + 1, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    TCHECK(hipStreamSynchronize(s->stream));
+    if (check_zero && z) return tfail(t, MHS_ERR_INVALID, "Cfl computation gave dt = 0.0");
+    *dt_v = v;
+    return MHS_OK;
+}
+
+// Coefficients and fluxes for a transport solve, and its CFL time
+// (EulerUpstream::computeCflTime, EulerUpstream_impl.hpp:265-330).
+int transport_prepare(mhs_transport *t, const double *grav, const mhs_transport_options &o, double *cfl_dt) {
     t->D.method_viscous = o.method_viscous != 0;
     t->D.method_gravity = o.method_gravity != 0;
     t->D.method_capillary = o.method_capillary != 0;
-    TCHECK(launch_tr_faces(s->stream, t->D, s->dT, s->dG, s->perm, s->outflux, grav));
-    const int use_v = o.method_viscous && o.use_cfl_viscous;
-    const int use_g = o.method_gravity && o.use_cfl_gravity;
-    const int use_c = o.method_capillary && o.use_cfl_capillary;
-    double init[2] = {1e100, 1e100};
-    int zero[2] = {0, 0};
-    TCHECK(hipMemcpyAsync(t->mins, init, sizeof(init), hipMemcpyHostToDevice, s->stream));
-    TCHECK(hipMemcpyAsync(t->flags, zero, sizeof(zero), hipMemcpyHostToDevice, s->stream));
-    TCHECK(launch_tr_cfl(s->stream, t->D, s->dT, s->dG, s->perm, s->outflux, grav, t->fac[0], t->fac[1], use_v,
-                         use_g, t->mins, t->flags));
-    double mins[2];
-    int fl[2];
-    TCHECK(hipMemcpyAsync(mins, t->mins, sizeof(mins), hipMemcpyDeviceToHost, s->stream));
-    TCHECK(hipMemcpyAsync(fl, t->flags, sizeof(fl), hipMemcpyDeviceToHost, s->stream));
-    TCHECK(hipStreamSynchronize(s->stream));
-    if (fl[1]) return tfail(t, MHS_ERR_INVALID, "Cfl computation gave dt = 0.0");
+    int rc = transport_coeffs(t, grav, o.method_capillary != 0);
+    if (rc != MHS_OK) return rc;
+    const bool use_v = o.method_viscous && o.use_cfl_viscous;
+    const bool use_g = o.method_gravity && o.use_cfl_gravity;
+    double dt_v = 1e100;
+    rc = transport_flux(t, use_v, &dt_v);
+    if (rc != MHS_OK) return rc;
     // findCFLtimeCapillary (CflCalculator.hpp:143-170) divides by the
     // capillary factor, which is 0 for this fluid: every local time is +inf
     // (or NaN) and the time stays 1e100.
-    double dt_c = 1e100;
-    (void)use_c;
-    *cfl_dt = std::min(std::min(mins[0], mins[1]), dt_c) * o.courant_number;
+    const double dt_c = 1e100;
+    *cfl_dt = std::min(std::min(use_v ? dt_v : 1e100, use_g ? t->dt_g : 1e100), dt_c) * o.courant_number;
     return MHS_OK;
 }
 
@@ -1784,6 +1826,8 @@ int transport_run(mhs_transport *t, double *sat, double time, const double *grav
         TCHECK(hipMemcpyAsync(&bad, t->flags, sizeof(int), hipMemcpyDeviceToHost, s->stream));
         TCHECK(hipStreamSynchronize(s->stream));
         if (!bad) {
+            t->timed_ok = true;
+            t->last_dt = dt;
             if (rep) {
                 rep->small_steps = nsteps;
                 rep->repeats = repeats;
@@ -1875,10 +1919,18 @@ int mhs_transport_init(mhs_transport *t, const double *porosity, const double *s
         owner[h] = o;
         nbf[h] = kind[o] == 2 ? T.hf_ppart[o] : o;
     }
-    std::vector<double> pv(nc), sbc(nhf, 1.0);
-    for (int c = 0; c < nc; ++c) pv[c] = s->cvol[c] * porosity[c];
-    if (sat_bc)
-        for (int h = 0; h < nhf; ++h) sbc[h] = sat_bc[h];
+    // slot-major copies for the small step: face j of cell c at j * nc + c
+    const size_t nslot = (size_t)std::max(T.max_ncf, 1) * nc;
+    std::vector<double> pv(nc), sbc(nslot, 1.0);
+    std::vector<int> other_s(nslot, -1);
+    for (int c = 0; c < nc; ++c) {
+        pv[c] = s->cvol[c] * porosity[c];
+        for (int h = T.facepos[c]; h < T.facepos[c + 1]; ++h) {
+            const size_t sl = (size_t)(h - T.facepos[c]) * nc + c;
+            other_s[sl] = other[h];
+            if (sat_bc) sbc[sl] = sat_bc[h];
+        }
+    }
     for (auto &b : t->bufs) (void)hipFree(b.p);
     t->bufs.clear();
     t->ready = false;
@@ -1887,11 +1939,12 @@ int mhs_transport_init(mhs_transport *t, const double *porosity, const double *s
     int *d_owner = e == hipSuccess ? talloc<int>(t, nhf, &e) : nullptr;
     int *d_nbf = e == hipSuccess ? talloc<int>(t, nhf, &e) : nullptr;
     uint8_t *d_kind = e == hipSuccess ? talloc<uint8_t>(t, nhf, &e) : nullptr;
-    if (e == hipSuccess) t->sat_bc = talloc<double>(t, nhf, &e);
+    int *d_other_s = e == hipSuccess ? talloc<int>(t, nslot, &e) : nullptr;
+    if (e == hipSuccess) t->sat_bc = talloc<double>(t, nslot, &e);
     if (e == hipSuccess) t->pv = talloc<double>(t, nc, &e);
-    if (e == hipSuccess) t->F = talloc<double>(t, nhf, &e);
-    if (e == hipSuccess) t->G = talloc<double>(t, nhf, &e);
-    if (e == hipSuccess) t->C = talloc<double>(t, nhf, &e);
+    if (e == hipSuccess) t->F = talloc<double>(t, nslot, &e);
+    if (e == hipSuccess) t->G = talloc<double>(t, nslot, &e);
+    if (e == hipSuccess) t->C = talloc<double>(t, nslot, &e);
     if (e == hipSuccess) t->s_tmp = talloc<double>(t, nc, &e);
     if (e == hipSuccess) t->s_init = talloc<double>(t, nc, &e);
     if (e == hipSuccess) t->s_host = talloc<double>(t, nc, &e);
@@ -1904,7 +1957,8 @@ int mhs_transport_init(mhs_transport *t, const double *porosity, const double *s
     TCHECK(hipMemcpy(d_owner, owner.data(), nhf * sizeof(int), hipMemcpyHostToDevice));
     TCHECK(hipMemcpy(d_nbf, nbf.data(), nhf * sizeof(int), hipMemcpyHostToDevice));
     TCHECK(hipMemcpy(d_kind, kind.data(), nhf * sizeof(uint8_t), hipMemcpyHostToDevice));
-    TCHECK(hipMemcpy(t->sat_bc, sbc.data(), nhf * sizeof(double), hipMemcpyHostToDevice));
+    TCHECK(hipMemcpy(d_other_s, other_s.data(), nslot * sizeof(int), hipMemcpyHostToDevice));
+    TCHECK(hipMemcpy(t->sat_bc, sbc.data(), nslot * sizeof(double), hipMemcpyHostToDevice));
     TCHECK(hipMemcpy(t->pv, pv.data(), nc * sizeof(double), hipMemcpyHostToDevice));
     DevTransport &D = t->D;
     D = DevTransport{};
@@ -1914,7 +1968,8 @@ int mhs_transport_init(mhs_transport *t, const double *porosity, const double *s
     D.kind = d_kind;
     D.owner_hf = d_owner;
     D.owner_nbf = d_nbf;
-    D.sat_bc = t->sat_bc;
+    D.other_s = d_other_s;
+    D.sat_bc_s = t->sat_bc;
     D.pv = t->pv;
     D.F = t->F;
     D.G = t->G;
@@ -1926,6 +1981,8 @@ int mhs_transport_init(mhs_transport *t, const double *porosity, const double *s
     D.drho = fluid->density[0] - fluid->density[1];
     cfl_factors(D.visc_w, D.visc_o, t->fac);
     t->topo_key = s->dT.hf_cell;
+    t->timed_ok = false;
+    t->coeff_gen = -1;
     t->ready = true;
     return MHS_OK;
 }
@@ -1985,7 +2042,6 @@ int mhs_transport_residual(mhs_transport *t, const double *saturation, const dou
     if (!saturation || !gravity || !residual)
         return tfail(t, MHS_ERR_INVALID, "transport_residual: saturation, gravity and residual required");
     mhs_solver *s = t->flow;
-    if (!s->solved) return tfail(t, MHS_ERR_STATE, "transport: the solver has no solution (call solve)");
     TCHECK(hipSetDevice(s->device));
     const mhs_transport_options o = opt ? *opt : default_transport_options();
     t->D.method_viscous = o.method_viscous != 0;
@@ -1994,10 +2050,45 @@ int mhs_transport_residual(mhs_transport *t, const double *saturation, const dou
     const size_t nc = (size_t)s->nc;
     TCHECK(hipMemcpy(t->s_host, saturation, nc * sizeof(double), hipMemcpyHostToDevice));
     if (injection_rates) TCHECK(hipMemcpy(t->inj, injection_rates, nc * sizeof(double), hipMemcpyHostToDevice));
-    TCHECK(launch_tr_faces(s->stream, t->D, s->dT, s->dG, s->perm, s->outflux, gravity));
+    rc = transport_coeffs(t, gravity, o.method_capillary != 0);
+    if (rc != MHS_OK) return rc;
+    double dt_v;
+    rc = transport_flux(t, false, &dt_v);
+    if (rc != MHS_OK) return rc;
     TCHECK(launch_tr_residual(s->stream, t->D, s->dT, t->s_host, injection_rates ? t->inj : nullptr, t->res));
     TCHECK(hipStreamSynchronize(s->stream));
     TCHECK(hipMemcpy(residual, t->res, nc * sizeof(double), hipMemcpyDeviceToHost));
+    return MHS_OK;
+}
+
+int mhs_transport_time_step(mhs_transport *t, int reps, double *avg_ms) {
+    int rc = transport_ready(t);
+    if (rc != MHS_OK) return rc;
+    if (!avg_ms || reps < 1) return tfail(t, MHS_ERR_INVALID, "transport_time_step: bad arguments");
+    if (!t->timed_ok) return tfail(t, MHS_ERR_STATE, "transport_time_step: call transport_solve first");
+    mhs_solver *s = t->flow;
+    TCHECK(hipSetDevice(s->device));
+    int zero = 0;
+    TCHECK(hipMemcpyAsync(t->flags, &zero, sizeof(int), hipMemcpyHostToDevice, s->stream));
+    hipEvent_t e0, e1;
+    TCHECK(hipEventCreate(&e0));
+    TCHECK(hipEventCreate(&e1));
+    hipError_t e = hipSuccess;
+    // one untimed launch, then reps timed launches between two events on the
+    // solver's stream; input: the saturation the last transport solve started from
+    for (int it = 0; it <= reps && e == hipSuccess; ++it) {
+        if (it == 1) e = hipEventRecord(e0, s->stream);
+        if (e != hipSuccess) break;
+        e = launch_tr_step(s->stream, t->D, s->dT, t->s_init, t->s_tmp, nullptr, t->last_dt, 0, 0, t->flags);
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, s->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return tfail(t, MHS_ERR_HIP, std::string("transport_time_step: ") + hipGetErrorString(e));
+    *avg_ms = ms / reps;
     return MHS_OK;
 }
 

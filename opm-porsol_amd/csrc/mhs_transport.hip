@@ -56,121 +56,139 @@ __device__ __forceinline__ double face_ds(const DevTransport &D, double F, doubl
     return dS;
 }
 
-__global__ void k_tr_faces(DevTransport D, DevTopo T, DevGeom Gm, const double *__restrict__ perm,
-                           const double *__restrict__ outflux, double gx, double gy, double gz) {
-    const int h = blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= D.nhf) return;
-    const int o = D.owner_hf[h];
+// Gravity and capillary coefficients of every face (they depend on K, the
+// gravity vector and the geometry, not on the flux): one thread per
+// half-face; the owner's thread writes the face's slot on both sides.
+__global__ void k_tr_coeffs(DevTransport D, DevTopo T, DevGeom Gm, const double *__restrict__ perm, double gx,
+                            double gy, double gz, int has_g, int cap) {
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= D.nhf || D.owner_hf[o] != o) return;
     const int c0 = T.hf_cell[o];
     const int oc = D.other[o];
     const int c1 = oc < 0 ? c0 : oc;
-    double K[9];
+    double G = 0.0, C = 0.0;
+    if (has_g || (cap && oc >= 0)) {
+        double K[9];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) K[i] = 0.5 * (perm[(size_t)9 * c0 + i] + perm[(size_t)9 * c1 + i]);
-    const double a = Gm.farea[o];
-    const double n0 = Gm.fnrm[3 * (size_t)o], n1 = Gm.fnrm[3 * (size_t)o + 1], n2 = Gm.fnrm[3 * (size_t)o + 2];
-    const double gi0 = (K[0] * gx + K[1] * gy + K[2] * gz) * D.drho;
-    const double gi1 = (K[3] * gx + K[4] * gy + K[5] * gz) * D.drho;
-    const double gi2 = (K[6] * gx + K[7] * gy + K[8] * gz) * D.drho;
-    const double G = a * (n0 * gi0 + n1 * gi1 + n2 * gi2);
-    double C = 0.0;
-    if (oc >= 0) {
-        // estimateCapPressureGradient (:513-550): the finite difference between
-        // the cell centroids through the face centroid(s)
-        const int nbf = D.owner_nbf[h];
-        const double *cc = Gm.ccen + 3 * (size_t)c0, *nc = Gm.ccen + 3 * (size_t)c1;
-        const double *fc = Gm.fcen + 3 * (size_t)o, *nfc = Gm.fcen + 3 * (size_t)nbf;
-        double e0 = cc[0] - fc[0], e1 = cc[1] - fc[1], e2 = cc[2] - fc[2];
-        const double d0 = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
-        e0 = nc[0] - nfc[0]; e1 = nc[1] - nfc[1]; e2 = nc[2] - nfc[2];
-        const double d1 = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
-        double u0 = nc[0] - nfc[0] + fc[0] - cc[0];
-        double u1 = nc[1] - nfc[1] + fc[1] - cc[1];
-        double u2 = nc[2] - nfc[2] + fc[2] - cc[2];
-        const double un = sqrt(u0 * u0 + u1 * u1 + u2 * u2);
-        u0 /= un; u1 /= un; u2 /= un;
-        const double k0 = K[0] * u0 + K[1] * u1 + K[2] * u2;
-        const double k1 = K[3] * u0 + K[4] * u1 + K[5] * u2;
-        const double k2 = K[6] * u0 + K[7] * u1 + K[8] * u2;
-        C = a * (n0 * k0 + n1 * k1 + n2 * k2) / (d0 + d1);
+        for (int i = 0; i < 9; ++i) K[i] = 0.5 * (perm[(size_t)9 * c0 + i] + perm[(size_t)9 * c1 + i]);
+        const double a = Gm.farea[o];
+        const double n0 = Gm.fnrm[3 * (size_t)o], n1 = Gm.fnrm[3 * (size_t)o + 1], n2 = Gm.fnrm[3 * (size_t)o + 2];
+        if (has_g) {
+            const double gi0 = (K[0] * gx + K[1] * gy + K[2] * gz) * D.drho;
+            const double gi1 = (K[3] * gx + K[4] * gy + K[5] * gz) * D.drho;
+            const double gi2 = (K[6] * gx + K[7] * gy + K[8] * gz) * D.drho;
+            G = a * (n0 * gi0 + n1 * gi1 + n2 * gi2);
+        }
+        if (cap && oc >= 0) {
+            // estimateCapPressureGradient (:513-550): the finite difference
+            // between the cell centroids through the face centroid(s)
+            const int nbf = D.owner_nbf[o];
+            const double *cc = Gm.ccen + 3 * (size_t)c0, *nc = Gm.ccen + 3 * (size_t)c1;
+            const double *fc = Gm.fcen + 3 * (size_t)o, *nfc = Gm.fcen + 3 * (size_t)nbf;
+            double e0 = cc[0] - fc[0], e1 = cc[1] - fc[1], e2 = cc[2] - fc[2];
+            const double d0 = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+            e0 = nc[0] - nfc[0]; e1 = nc[1] - nfc[1]; e2 = nc[2] - nfc[2];
+            const double d1 = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+            double u0 = nc[0] - nfc[0] + fc[0] - cc[0];
+            double u1 = nc[1] - nfc[1] + fc[1] - cc[1];
+            double u2 = nc[2] - nfc[2] + fc[2] - cc[2];
+            const double un = sqrt(u0 * u0 + u1 * u1 + u2 * u2);
+            u0 /= un; u1 /= un; u2 /= un;
+            const double k0 = K[0] * u0 + K[1] * u1 + K[2] * u2;
+            const double k1 = K[3] * u0 + K[4] * u1 + K[5] * u2;
+            const double k2 = K[6] * u0 + K[7] * u1 + K[8] * u2;
+            C = a * (n0 * k0 + n1 * k1 + n2 * k2) / (d0 + d1);
+        }
     }
-    D.F[h] = outflux[o];
-    D.G[h] = G;
-    D.C[h] = C;
+    const size_t so = (size_t)(o - T.facepos[c0]) * D.nc + c0;
+    D.G[so] = G;
+    D.C[so] = C;
+    if (oc >= 0) {
+        const int t = T.hf_twin[o];
+        const size_t st = (size_t)(t - T.facepos[oc]) * D.nc + oc;
+        D.G[st] = G;
+        D.C[st] = C;
+    }
 }
 
 __device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
 
-// Per-cell CFL times (cfl_calculator::findCFLtimeVelocity :55-85 and
-// findCFLtimeGravity :91-137), reduced to their minima over the grid as
-// ordered bit patterns of positive doubles.  A zero viscous time is flagged.
-__global__ void k_tr_cfl(DevTransport D, DevTopo T, DevGeom Gm, const double *__restrict__ perm,
-                         const double *__restrict__ outflux, double gx, double gy, double gz, double fac_v,
-                         double fac_g, int use_v, int use_g, unsigned long long *mins, int *zero_dt) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    double dv = 1e100, dg = 1e100;
-    if (c < D.nc) {
-        const int h0 = T.facepos[c], h1 = T.facepos[c + 1];
-        if (use_v) {
-            double fp = 0.0, fn = 0.0;
-            for (int h = h0; h < h1; ++h) {
-                const double q = outflux[h];
-                if (q > 0) fp += q;
-                else fn -= q;
-            }
-            const double loc = (fac_v * D.pv[c]) / fmax(fn, fp);
-            if (loc == 0.0) zero_dt[0] = 1;
-            if (loc < dv) dv = loc;
-        }
-        if (use_g) {
-            const double *Kc = perm + (size_t)9 * c;
-            double flux = 0.0;
-            for (int h = h0; h < h1; ++h) {
-                double K[9];
-                if (D.kind[h] == 1) {
-                    const double *Kn = perm + (size_t)9 * D.other[h];
-#pragma unroll
-                    for (int i = 0; i < 9; ++i) K[i] = 0.5 * (Kc[i] + Kn[i]);
-                } else {
-#pragma unroll
-                    for (int i = 0; i < 9; ++i) K[i] = Kc[i];
-                }
-                const double *n = Gm.fnrm + 3 * (size_t)h;
-                const double g3[3] = {gx, gy, gz};
-                double f = 0.0;
-#pragma unroll
-                for (int k = 0; k < 3; ++k)
-#pragma unroll
-                    for (int q = 0; q < 3; ++q) f += n[q] * (K[3 * q + k] * g3[k] * D.drho);
-                f *= Gm.farea[h];
-                if (f > 0) flux += f;
-            }
-            const double loc = (fac_g * D.pv[c]) / flux;
-            if (loc < dg) dg = loc;
-        }
-    }
-    // minima of positive doubles (NaN and +inf order above 1e100 and never win)
-    unsigned long long bv = dbits(dv), bg = dbits(dg);
+// Block minimum of positive doubles as ordered bit patterns (NaN and +inf
+// order above 1e100 and never win), then one atomic per block.
+__device__ __forceinline__ void block_min(unsigned long long b, unsigned long long *out) {
     for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long ov = __shfl_down(bv, off, 64), og = __shfl_down(bg, off, 64);
-        bv = ov < bv ? ov : bv;
-        bg = og < bg ? og : bg;
+        const unsigned long long o = __shfl_down(b, off, 64);
+        b = o < b ? o : b;
     }
-    __shared__ unsigned long long sv[kTB / 64], sg[kTB / 64];
+    __shared__ unsigned long long sm[kTB / 64];
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    if (l == 0) {
-        sv[w] = bv;
-        sg[w] = bg;
-    }
+    if (l == 0) sm[w] = b;
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int i = 1; i < kTB / 64; ++i) {
-            bv = sv[i] < bv ? sv[i] : bv;
-            bg = sg[i] < bg ? sg[i] : bg;
-        }
-        atomicMin(&mins[0], bv);
-        atomicMin(&mins[1], bg);
+        for (int i = 1; i < kTB / 64; ++i) b = sm[i] < b ? sm[i] : b;
+        atomicMin(out, b);
     }
+}
+
+// Gravity CFL time (cfl_calculator::findCFLtimeGravity, CflCalculator.hpp:
+// 91-137): per cell the positive parts of area n.(K g drho), K averaged over
+// internal faces and the cell's own on boundary (also periodic) faces.
+__global__ void k_tr_cfl_gravity(DevTransport D, DevTopo T, DevGeom Gm, const double *__restrict__ perm, double gx,
+                                 double gy, double gz, double fac_g, unsigned long long *min_dt) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    double dg = 1e100;
+    if (c < D.nc) {
+        const int h0 = T.facepos[c], h1 = T.facepos[c + 1];
+        const double *Kc = perm + (size_t)9 * c;
+        const double g3[3] = {gx, gy, gz};
+        double flux = 0.0;
+        for (int h = h0; h < h1; ++h) {
+            double K[9];
+            if (D.kind[h] == 1) {
+                const double *Kn = perm + (size_t)9 * D.other[h];
+#pragma unroll
+                for (int i = 0; i < 9; ++i) K[i] = 0.5 * (Kc[i] + Kn[i]);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 9; ++i) K[i] = Kc[i];
+            }
+            const double *n = Gm.fnrm + 3 * (size_t)h;
+            double f = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int q = 0; q < 3; ++q) f += n[q] * (K[3 * q + k] * g3[k] * D.drho);
+            f *= Gm.farea[h];
+            if (f > 0) flux += f;
+        }
+        const double loc = (fac_g * D.pv[c]) / flux;
+        if (loc < dg) dg = loc;
+    }
+    block_min(dbits(dg), min_dt);
+}
+
+// Per transport solve: the owner's flux into the face slots of each cell,
+// and the viscous CFL time (findCFLtimeVelocity, CflCalculator.hpp:55-85)
+// from the cell's own outfluxes.  A zero time is flagged.
+__global__ void k_tr_flux(DevTransport D, DevTopo T, const double *__restrict__ outflux, double fac_v,
+                          unsigned long long *min_dt, int *zero_dt) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    double dv = 1e100;
+    if (c < D.nc) {
+        const int h0 = T.facepos[c], h1 = T.facepos[c + 1];
+        double fp = 0.0, fn = 0.0;
+        for (int h = h0; h < h1; ++h) {
+            const double q = outflux[h];
+            if (q > 0) fp += q;
+            else fn -= q;
+            const int o = D.owner_hf[h];
+            D.F[(size_t)(h - h0) * D.nc + c] = o == h ? q : outflux[o];
+        }
+        const double loc = (fac_v * D.pv[c]) / fmax(fn, fp);
+        if (loc == 0.0) zero_dt[0] = 1;
+        if (loc < dv) dv = loc;
+    }
+    block_min(dbits(dv), min_dt);
 }
 
 // Residual of cell c (EulerUpstreamResidual UpdateForCell::operator()
@@ -180,13 +198,14 @@ __device__ __forceinline__ double cell_residual(const DevTransport &D, const Dev
                                                 const double *__restrict__ s_in, const double *__restrict__ inj,
                                                 int c, double sc) {
     double r = 0.0;
-    const int h0 = T.facepos[c], h1 = T.facepos[c + 1];
-    for (int h = h0; h < h1; ++h) {
-        const int oc = D.other[h];
+    const int n = T.facepos[c + 1] - T.facepos[c];
+    for (int j = 0; j < n; ++j) {
+        const size_t sl = (size_t)j * D.nc + c;
+        const int oc = D.other_s[sl];
         const bool own = oc < 0 || c < oc;
-        const double sn = oc < 0 ? D.sat_bc[h] : s_in[oc];
-        const double dS = own ? face_ds(D, D.F[h], D.G[h], D.C[h], sc, sn, oc < 0)
-                              : face_ds(D, D.F[h], D.G[h], D.C[h], sn, sc, false);
+        const double sn = oc < 0 ? D.sat_bc_s[sl] : s_in[oc];
+        const double F = D.F[sl], G = D.G[sl], C = D.C[sl];
+        const double dS = own ? face_ds(D, F, G, C, sc, sn, oc < 0) : face_ds(D, F, G, C, sn, sc, false);
         r += own ? -dS : dS;
     }
     double rate = inj ? inj[c] : 0.0;
@@ -246,19 +265,24 @@ inline int blocks(long long n) { return (int)((n + kTB - 1) / kTB); }
 
 }  // namespace
 
-hipError_t launch_tr_faces(hipStream_t st, const DevTransport &D, const DevTopo &T, const DevGeom &G,
-                           const double *perm, const double *outflux, const double *grav) {
+hipError_t launch_tr_coeffs(hipStream_t st, const DevTransport &D, const DevTopo &T, const DevGeom &G,
+                            const double *perm, const double *grav, int has_g, int cap) {
     if (D.nhf == 0) return hipSuccess;
-    k_tr_faces<<<blocks(D.nhf), kTB, 0, st>>>(D, T, G, perm, outflux, grav[0], grav[1], grav[2]);
+    k_tr_coeffs<<<blocks(D.nhf), kTB, 0, st>>>(D, T, G, perm, grav[0], grav[1], grav[2], has_g, cap);
     return hipGetLastError();
 }
 
-hipError_t launch_tr_cfl(hipStream_t st, const DevTransport &D, const DevTopo &T, const DevGeom &G,
-                         const double *perm, const double *outflux, const double *grav, double fac_v, double fac_g,
-                         int use_v, int use_g, unsigned long long *mins, int *zero_dt) {
+hipError_t launch_tr_cfl_gravity(hipStream_t st, const DevTransport &D, const DevTopo &T, const DevGeom &G,
+                                 const double *perm, const double *grav, double fac_g, unsigned long long *min_dt) {
     if (D.nc == 0) return hipSuccess;
-    k_tr_cfl<<<blocks(D.nc), kTB, 0, st>>>(D, T, G, perm, outflux, grav[0], grav[1], grav[2], fac_v, fac_g, use_v,
-                                           use_g, mins, zero_dt);
+    k_tr_cfl_gravity<<<blocks(D.nc), kTB, 0, st>>>(D, T, G, perm, grav[0], grav[1], grav[2], fac_g, min_dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_tr_flux(hipStream_t st, const DevTransport &D, const DevTopo &T, const double *outflux,
+                          double fac_v, unsigned long long *min_dt, int *zero_dt) {
+    if (D.nc == 0) return hipSuccess;
+    k_tr_flux<<<blocks(D.nc), kTB, 0, st>>>(D, T, outflux, fac_v, min_dt, zero_dt);
     return hipGetLastError();
 }
 

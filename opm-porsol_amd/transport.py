@@ -22,6 +22,7 @@ EXPORTS = (
     "mhs_transport_default_options", "mhs_transport_create", "mhs_transport_destroy",
     "mhs_transport_last_error", "mhs_transport_init", "mhs_transport_solve",
     "mhs_transport_solve_device", "mhs_transport_residual", "mhs_transport_mobilities_device",
+    "mhs_transport_time_step",
 )
 
 # ReservoirPropertyCommon defaults (common/ReservoirPropertyCommon_impl.hpp:257-262)
@@ -72,6 +73,7 @@ def lib():
                                                  ctypes.POINTER(Report)]
         L.mhs_transport_residual.argtypes = [_P, _D, _D, _D, ctypes.POINTER(Options), _D]
         L.mhs_transport_mobilities_device.argtypes = [_P, _P, _P, _P]
+        L.mhs_transport_time_step.argtypes = [_P, ctypes.c_int, _D]
         for name in EXPORTS:
             if name not in ("mhs_transport_default_options", "mhs_transport_last_error"):
                 getattr(L, name).restype = ctypes.c_int
@@ -190,3 +192,9 @@ class EulerUpstream:
         per-cell inputs of the next pressure solve."""
         self._check(self._L.mhs_transport_mobilities_device(self._h, _P(saturation_ptr), _P(totmob_ptr or 0),
                                                             _P(rhomob_ptr or 0)))
+
+    def time_step(self, reps: int = 50) -> float:
+        """Average device time (ms) of one small-step kernel launch."""
+        ms = ctypes.c_double(0.0)
+        self._check(self._L.mhs_transport_time_step(self._h, int(reps), ctypes.byref(ms)))
+        return ms.value

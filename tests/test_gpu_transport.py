@@ -220,3 +220,24 @@ def test_device_path_matches_host_path(pkg):
     tm_h, rm_h = _fluid(host)
     np.testing.assert_allclose(tm.cpu().numpy(), tm_h, rtol=1e-15, atol=0)
     np.testing.assert_allclose(rm.cpu().numpy(), rm_h, rtol=1e-15, atol=0)
+
+
+def test_coefficients_follow_permeability_and_gravity(pkg):
+    """The gravity and capillary coefficients are kept between calls: a new
+    gravity vector, the capillary switch and a new permeability must each
+    reach the residual."""
+    g, perm, bt, bv, sat, sat_bc, poro, inj = _cart_case(pkg, (8, 6, 5), seed=17)
+    s, flux = _flow(pkg, g, perm, bt, bv, sat)
+    t = pkg.EulerUpstream(s, poro, sat_bc, VISC, RHO)
+    for grav, cap in ((GRAV, True), ((0.0, 3.0, -9.81), True), ((0.0, 3.0, -9.81), False), ((0.0, 0.0, 0.0), True)):
+        r_gpu = t.computeResidual(sat, grav, inj, method_capillary=cap)
+        r_orc = tr.residual(g, perm, sat, flux, grav, DRHO, VISC, inj, sat_bc, None, method_capillary=cap)
+        assert cases.rel_err(r_gpu, r_orc) < 1e-12, (grav, cap)
+    perm2 = pkg.fields.perm_c1(g.num_cells, 23)
+    s.computeInnerProducts(perm2, GRAV)
+    tm, rm = _fluid(sat)
+    s.solve(tm, rm, np.zeros(g.num_cells), bt, bv, residual_tolerance=1e-12)
+    flux2 = np.asarray(s.getSolution()[2]).reshape(-1)
+    r_gpu = t.computeResidual(sat, GRAV, inj)
+    r_orc = tr.residual(g, perm2, sat, flux2, GRAV, DRHO, VISC, inj, sat_bc)
+    assert cases.rel_err(r_gpu, r_orc) < 1e-12
