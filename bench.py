@@ -75,11 +75,12 @@ def _alg_bytes(g, nnz, nf, mg_rows=0, mg_nnz=0):
     return out
 
 
-def _measured_traffic(tag):
-    """HBM bytes per launch of the FP32 SpMV as measured by the FETCH_SIZE /
-    WRITE_SIZE counter runs summarised in profiles/ (calibrated against a
-    copy of known size); None when no such measurement is committed."""
-    path = os.path.join(ROOT, "profiles", "k_spmv_pq_128x128x64_hbm_traffic.txt")
+def _measured_traffic(tag, name="k_spmv_pq_128x128x64_hbm_traffic.txt"):
+    """HBM bytes per launch of the dominant kernel (the FP32 SpMV, or the
+    transport small step) as measured by the FETCH_SIZE / WRITE_SIZE counter
+    runs summarised in profiles/ (calibrated against a copy of known size);
+    None when no such measurement is committed."""
+    path = os.path.join(ROOT, "profiles", name)
     if tag != "32" or not os.path.exists(path):
         return None
     for line in open(path):
@@ -531,7 +532,8 @@ def bench_impes(args, pkg, torch, dev):
     ach = alg / (ms_k * 1e-3) / 1e9
     roof = {"kernel": "k_tr_step (explicit upstream small step, one thread per cell)", "bound": "hbm",
             "achieved": round(ach, 1), "peak": 8000.0, "unit": "GB/s", "frac": round(ach / 8000.0, 4),
-            "traffic": None, "alg_bytes_per_launch": alg, "avg_launch_us": round(ms_k * 1e3, 2)}
+            "traffic": _measured_traffic("32", "k_tr_step_128x128x64_hbm_traffic.txt"),
+            "alg_bytes_per_launch": alg, "avg_launch_us": round(ms_k * 1e3, 2)}
     s_h = sat.cpu().numpy()
     config = {
         "workload": "IMPES on 128x128x64 Cartesian, anisotropic K (kz/kx=0.1, sigma ln k 1.5, l=8), water "

@@ -22,5 +22,16 @@ struct Params {
         auto it = kv.find(k);
         return it == kv.end() ? def : std::atoi(it->second.c_str());
     }
+    std::string gets(const std::string &k, const std::string &def) const {
+        auto it = kv.find(k);
+        return it == kv.end() ? def : it->second;
+    }
+    // ParameterGroup::getDefault
+    double getDefault(const std::string &k, double def) const { return get(k, def); }
+    int getDefault(const std::string &k, int def) const { return geti(k, def); }
+    bool getDefault(const std::string &k, bool def) const {
+        auto it = kv.find(k);
+        return it == kv.end() ? def : (it->second == "1" || it->second == "true");
+    }
 };
 #endif

@@ -113,13 +113,19 @@ public:
             return outflux_[facepos_[cellno_[f->cellIndex()]] + f->localIndex()];
         }
         Scalar outflux(int hf) const { return outflux_[hf]; }
+        /// No counterpart in the reference: the solver handle that holds this
+        /// solution on the GPU, for consumers that read the fluxes in device
+        /// memory (the GPU EulerUpstream); null before the first solve.
+        mhs_solver *deviceHandle() const { return handle_; }
 
     private:
+        mhs_solver *handle_ = nullptr;
         std::vector<int> cellno_;
         std::vector<int> facepos_;
         std::vector<Scalar> pressure_;
         std::vector<Scalar> outflux_;
         void clear() {
+            handle_ = nullptr;
             std::vector<int>().swap(cellno_);
             std::vector<int>().swap(facepos_);
             std::vector<Scalar>().swap(pressure_);
@@ -331,6 +337,7 @@ public:
         check(mhs_get_solution(h_, flowSolution_.pressure_.data(), face_pressure_.data(),
                                flowSolution_.outflux_.data()),
               "getSolution");
+        flowSolution_.handle_ = h_;
     }
 
     /// Antisymmetric twin fluxes (:754-807); returns the largest change.

@@ -84,3 +84,35 @@ def test_periodic_driver_closed_form():
     assert abs(q - expect) <= 1e-9 * abs(expect)
     mm = float(re.search(r"max twin change:\s+([0-9.eE+-]+)", out).group(1))
     assert mm <= 1e-6 * abs(expect) / 32
+
+
+@pytest.mark.parametrize("capillary", [0, 1])
+def test_impes_example_matches_oracle(orc, pkg, tmp_path, capillary):
+    """examples/impes_example.cpp (IncompFlowSolverHybrid + EulerUpstream
+    drop-ins, SimulatorTester::run's loop) against the oracle's pressure solve
+    and transport, three steps."""
+    import euler_upstream as tr
+    nx, ny, nz, steps, dt = 12, 8, 4, 3, 2.0e5
+    out = tmp_path / "sat.txt"
+    _run("impes_example", f"nx={nx}", f"ny={ny}", f"nz={nz}", f"steps={steps}", f"stepsize={dt}",
+         f"method_capillary={capillary}", f"out={out}")
+    got = np.loadtxt(out)
+    g = pkg.grids.cartesian(nx, ny, nz, 2.0, 2.0, 1.0)
+    nc = g.num_cells
+    k = 100.0 * (1e-3 * 9.869233e-13) * np.exp(np.sin(0.37 * np.arange(nc)))
+    perm = np.zeros((nc, 9))
+    perm[:, [0, 4, 8]] = k[:, None]
+    bt = np.zeros(g.num_half_faces, np.int32)
+    bv = np.zeros(g.num_half_faces)
+    bt[(g.side == 0) | (g.side == 1)] = 1
+    bv[g.side == 0] = 2.0e5
+    sat_bc = np.where(g.side == 0, 1.0, 0.0)
+    visc, rho, grav = (1e-3, 3e-3), (1013.9, 834.7), (0.0, 0.0, 9.81)
+    s = np.zeros(nc)
+    for _ in range(steps):
+        lw, lo = tr.mobilities(s, visc)
+        ref = orc.solve(g, perm, lw + lo, rho[0] * lw + rho[1] * lo, np.zeros(nc), bt, bv, grav=grav, tol=1e-14)
+        s, *_ = tr.transport_solve(g, perm, np.full(nc, 0.2), s, dt, grav, ref.outflux, rho[0] - rho[1], visc,
+                                   np.zeros(nc), sat_bc, method_capillary=bool(capillary))
+    assert s.max() > 0.3
+    np.testing.assert_allclose(got, s, rtol=0, atol=1e-8)
