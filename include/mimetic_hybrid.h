@@ -286,6 +286,13 @@ int mhs_adjoint_device(mhs_solver *s, const double *dJ_dv, const mhs_solve_optio
                        double *grad_perm, double *grad_totmob, double *grad_rhomob,
                        double *grad_src, mhs_solve_report *report);
 
+/* estimateCellVelocity (opm/porsol/common/SimulatorUtilities.hpp:59-81) on
+ * the last solution (right-hand side k): cell_velocity[3*c + d] = sum over the
+ * faces of cell c of (x_f - x_c)[d] * outflux(f) / |c|.  The _device variant
+ * writes a device buffer, ordered on the stream given to mhs_create. */
+int mhs_estimate_cell_velocity(mhs_solver *s, int k, double *cell_velocity);
+int mhs_estimate_cell_velocity_device(mhs_solver *s, int k, double *cell_velocity);
+
 /* Profiling hook, no counterpart in the reference: average device time (ms)
  * of `reps` back-to-back launches of kernel `which` (0: the PCG SpMV with
  * <p, Ap>, in the precision of the last solve) on the solver's own stream,

@@ -1497,6 +1497,41 @@ int mhs_solution_device(mhs_solver *s, const double **cell_pressure, const doubl
     return MHS_OK;
 }
 
+static int cell_velocity_impl(mhs_solver *s, int k, double *vel, bool device) {
+    if (!s || !vel) return MHS_ERR_INVALID;
+    if (!s->solved) return fail(s, MHS_ERR_STATE, "estimate_cell_velocity: no solution (call solve)");
+    if (k < 0 || k >= s->last_nrhs) return fail(s, MHS_ERR_INVALID, "estimate_cell_velocity: bad rhs index");
+    HIPCHECK(hipSetDevice(s->device));
+    const size_t n3 = 3 * (size_t)s->nc;
+    double *out = vel;
+    TempBufs tmp;
+    if (!device) {
+        hipError_t he = hipSuccess;
+        out = dalloc<double>(s, tmp.v, n3, &he);
+        if (he != hipSuccess) return fail(s, MHS_ERR_ALLOC, std::string("device allocation: ") + hipGetErrorString(he));
+    } else if (s->user_stream) {
+        HIPCHECK(hipEventRecord(s->ev_in, s->user_stream));
+        HIPCHECK(hipStreamWaitEvent(s->stream, s->ev_in, 0));
+    }
+    HIPCHECK(launch_cell_velocity(s->stream, s->nc, s->dT.facepos, s->dG, s->outflux + (size_t)k * s->nhf, out));
+    if (!device) {
+        HIPCHECK(hipMemcpyAsync(vel, out, n3 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+        HIPCHECK(hipStreamSynchronize(s->stream));
+    } else if (s->user_stream) {
+        HIPCHECK(hipEventRecord(s->ev_out, s->stream));
+        HIPCHECK(hipStreamWaitEvent(s->user_stream, s->ev_out, 0));
+    }
+    return MHS_OK;
+}
+
+int mhs_estimate_cell_velocity(mhs_solver *s, int k, double *cell_velocity) {
+    return cell_velocity_impl(s, k, cell_velocity, false);
+}
+
+int mhs_estimate_cell_velocity_device(mhs_solver *s, int k, double *cell_velocity) {
+    return cell_velocity_impl(s, k, cell_velocity, true);
+}
+
 int mhs_post_process_fluxes(mhs_solver *s, double *max_modification) {
     if (!s) return MHS_ERR_INVALID;
     if (!s->solved) return fail(s, MHS_ERR_STATE, "post_process_fluxes: no solution (call solve)");

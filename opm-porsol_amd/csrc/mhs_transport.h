@@ -42,6 +42,9 @@ hipError_t launch_tr_residual(hipStream_t st, const DevTransport &D, const DevTo
                               const double *inj, double *res);
 hipError_t launch_tr_mobilities(hipStream_t st, const DevTransport &D, const double *s, double *totmob,
                                 double *rhomob);
+// estimateCellVelocity: vel[3 * nc] from the half-face outfluxes.
+hipError_t launch_cell_velocity(hipStream_t st, int nc, const int *facepos, const DevGeom &G, const double *outflux,
+                                double *vel);
 hipError_t launch_tr_pore_volume(hipStream_t st, int nc, const double *vol, const double *poro, double *pv);
 
 }  // namespace mhs

@@ -314,3 +314,38 @@ hipError_t launch_tr_pore_volume(hipStream_t st, int nc, const double *vol, cons
 }
 
 }  // namespace mhs
+
+namespace mhs {
+
+namespace {
+
+// estimateCellVelocity (common/SimulatorUtilities.hpp:59-81): per cell the
+// sum over its faces of (x_f - x_c) flux_f / |c|, written as 3 doubles.
+__global__ void k_cell_velocity(int nc, const int *__restrict__ facepos, DevGeom Gm,
+                                const double *__restrict__ outflux, double *__restrict__ vel) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const double cx = Gm.ccen[3 * (size_t)c], cy = Gm.ccen[3 * (size_t)c + 1], cz = Gm.ccen[3 * (size_t)c + 2];
+    const double vol = Gm.cvol[c];
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+    for (int h = facepos[c]; h < facepos[c + 1]; ++h) {
+        const double s = outflux[h] / vol;
+        v0 += (Gm.fcen[3 * (size_t)h] - cx) * s;
+        v1 += (Gm.fcen[3 * (size_t)h + 1] - cy) * s;
+        v2 += (Gm.fcen[3 * (size_t)h + 2] - cz) * s;
+    }
+    vel[3 * (size_t)c] = v0;
+    vel[3 * (size_t)c + 1] = v1;
+    vel[3 * (size_t)c + 2] = v2;
+}
+
+}  // namespace
+
+hipError_t launch_cell_velocity(hipStream_t st, int nc, const int *facepos, const DevGeom &G, const double *outflux,
+                                double *vel) {
+    if (nc == 0) return hipSuccess;
+    k_cell_velocity<<<(nc + 255) / 256, 256, 0, st>>>(nc, facepos, G, outflux, vel);
+    return hipGetLastError();
+}
+
+}  // namespace mhs

@@ -32,7 +32,8 @@ EXPORTS = (
     "mhs_solution_device", "mhs_post_process_fluxes", "mhs_get_dims", "mhs_get_cell_faces",
     "mhs_get_face_rows", "mhs_get_system", "mhs_time_kernel", "mhs_compute_inner_products_device",
     "mhs_get_solution_device", "mhs_adjoint", "mhs_adjoint_device", "mhs_sum_outflux",
-    "mhs_gravity_term", "mhs_topology_is_sane",
+    "mhs_gravity_term", "mhs_topology_is_sane", "mhs_estimate_cell_velocity",
+    "mhs_estimate_cell_velocity_device",
 )
 
 
@@ -130,6 +131,8 @@ def lib():
         L.mhs_sum_outflux.argtypes = [_P, ctypes.c_int, _I, _I, _D]
         L.mhs_gravity_term.argtypes = [_P, _D, _D, _D]
         L.mhs_topology_is_sane.argtypes = [ctypes.c_int, _I, _I, ctypes.POINTER(ctypes.c_int)]
+        L.mhs_estimate_cell_velocity.argtypes = [_P, ctypes.c_int, _D]
+        L.mhs_estimate_cell_velocity_device.argtypes = [_P, ctypes.c_int, _P]
         L.mhs_adjoint.argtypes = [_P, _D, ctypes.POINTER(Options), _D, _D, _D, _D,
                                   ctypes.POINTER(Report)]
         L.mhs_adjoint_device.argtypes = [_P, _P, ctypes.POINTER(Options), _P, _P, _P, _P,
@@ -411,6 +414,16 @@ class HybridSolver:
         self.adjoint_report = rep
         self._check(rc)
         return rep
+
+    def estimateCellVelocity(self, k: int = 0) -> np.ndarray:
+        """estimateCellVelocity (common/SimulatorUtilities.hpp:59-81) of the last
+        solution: [nc, 3], sum over faces of (x_f - x_c) outflux / |c|."""
+        v = np.zeros((self.grid.num_cells, 3))
+        self._check(self._L.mhs_estimate_cell_velocity(self._h, int(k), v.ctypes.data_as(_D)))
+        return v
+
+    def estimate_cell_velocity_device(self, out_ptr: int, k: int = 0):
+        self._check(self._L.mhs_estimate_cell_velocity_device(self._h, int(k), _P(out_ptr)))
 
     def postProcessFluxes(self) -> float:
         mm = ctypes.c_double(0.0)

@@ -243,3 +243,14 @@ def gravity_term(g, grav, omega):
     d = np.asarray(g.face_centroid, dtype=np.float64).reshape(-1, 3) - \
         np.asarray(g.cell_centroid, dtype=np.float64).reshape(-1, 3)[cell]
     return np.asarray(omega, dtype=np.float64)[cell] * (d @ np.asarray(grav, dtype=np.float64))
+
+
+def estimate_cell_velocity(g, outflux):
+    """estimateCellVelocity (opm/porsol/common/SimulatorUtilities.hpp:59-81):
+    per cell the sum over its faces of (x_f - x_c) * outflux(f) / |c|."""
+    hc = g.hf_cell()
+    d = np.asarray(g.face_centroid) - np.asarray(g.cell_centroid)[hc]
+    contrib = d * (np.asarray(outflux, dtype=np.float64) / np.asarray(g.cell_volume)[hc])[:, None]
+    v = np.zeros((g.num_cells, 3))
+    np.add.at(v, hc, contrib)
+    return v

@@ -241,3 +241,36 @@ def test_coefficients_follow_permeability_and_gravity(pkg):
     r_gpu = t.computeResidual(sat, GRAV, inj)
     r_orc = tr.residual(g, perm2, sat, flux2, GRAV, DRHO, VISC, inj, sat_bc)
     assert cases.rel_err(r_gpu, r_orc) < 1e-12
+
+
+def test_estimate_cell_velocity(pkg, orc):
+    """estimateCellVelocity on the GPU against the oracle on the same fluxes
+    (faulted corner-point grid), and the closed form of a uniform flow: a
+    linear pressure drop on a box with uniform K gives u = lambda K dp / L in
+    every cell."""
+    import torch
+    g = pkg.grids.corner_point(10, 8, 5, 3.0, 3.0, 1.0, seed=6)
+    nc = g.num_cells
+    perm = pkg.fields.perm_c4(nc, 6)
+    bt, bv = cases.xdrop_bc(g, 3 * BAR, BAR)
+    s, flux = _flow(pkg, g, perm, bt, bv, np.full(nc, 0.4))
+    v_gpu = s.estimateCellVelocity()
+    v_orc = orc.estimate_cell_velocity(g, flux)
+    assert cases.rel_err(v_gpu, v_orc) < 1e-13
+    d = torch.empty(3 * nc, dtype=torch.float64, device="cuda")
+    s.estimate_cell_velocity_device(d.data_ptr())
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(d.cpu().numpy().reshape(nc, 3), v_gpu)
+    gb = pkg.grids.cartesian(12, 5, 4, 2.0, 1.0, 1.0)
+    k = 150 * 9.869233e-16
+    K = np.zeros((gb.num_cells, 9))
+    K[:, [0, 4, 8]] = k
+    bt, bv = cases.xdrop_bc(gb, 2 * BAR, 0.0)
+    sb = pkg.HybridSolver(0)
+    sb.init(gb, K, (0.0, 0.0, 0.0), bt, None)
+    sb.solve(np.full(gb.num_cells, 1e3), np.zeros(gb.num_cells), np.zeros(gb.num_cells), bt, bv,
+             residual_tolerance=1e-13)
+    u = 1e3 * k * 2 * BAR / 24.0
+    vb = sb.estimateCellVelocity()
+    np.testing.assert_allclose(vb[:, 0], u, rtol=1e-9)
+    assert np.max(np.abs(vb[:, 1:])) < 1e-9 * u
