@@ -214,6 +214,27 @@ int mhs_solve_device(mhs_solver *s, const double *totmob, const double *rhomob,
                      const double *hf_bc_value, const mhs_solve_options *opt,
                      mhs_solve_report *report);
 
+/* solve() with the second inner-product plug-in, MimeticIPAnisoRelpermEvaluator
+ * (opm/porsol/mimetic/MimeticIPAnisoRelpermEvaluator.hpp:85-430): tensor
+ * mobilities per cell instead of scalars.  lambda_t[9*nc] is the total
+ * mobility tensor sum_i lambda_i and rho_lambda[9*nc] is sum_i rho_i lambda_i,
+ * both row-major like perm (computeDynamicParams :299-330).  Per cell
+ * B^-1 = (N (lambda_t K) N' + t A (I - QQ') A) / |c| with t = 6 tr(lambda_t K)/3
+ * (getInverseMatrix :363-400) and the gravity flux N (rho_lambda K g)
+ * (gravityFlux :406-416), g as given to compute_inner_products.  lambda_t K
+ * enters through its symmetric part, as K does in the scalar path: equal to
+ * the reference's product whenever lambda_t and K share principal axes (the
+ * reference's symmetric Schur update assumes a symmetric B^-1).  The rest is
+ * mhs_solve; the static inner products are kept for later scalar solves.
+ * An adjoint of such a solve is refused. */
+int mhs_solve_aniso(mhs_solver *s, const double *lambda_t, const double *rho_lambda,
+                    const double *src, const int *hf_bc_type, const double *hf_bc_value,
+                    const mhs_solve_options *opt, mhs_solve_report *report);
+/* Same with lambda_t, rho_lambda, src and hf_bc_value in device memory. */
+int mhs_solve_aniso_device(mhs_solver *s, const double *lambda_t, const double *rho_lambda,
+                           const double *src, const int *hf_bc_type, const double *hf_bc_value,
+                           const mhs_solve_options *opt, mhs_solve_report *report);
+
 /* IncompFlowSolverHybrid::getSolution (:828-831) -> FlowSolution::pressure
  * (:411-414) and ::outflux (:426-433), plus the face (contact) pressures the
  * reference keeps in the private soln_ (:935).  Any pointer may be NULL.

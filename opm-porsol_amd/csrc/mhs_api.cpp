@@ -85,6 +85,10 @@ struct mhs_solver {
     double grav[3] = {0.0, 0.0, 0.0};
     double *perm = nullptr;             // K per cell (row-major 3x3), kept for the adjoint and transport
     long long perm_gen = 0;             // counts permeability uploads
+    // anisotropic (tensor-mobility) inner products, allocated on the first such solve
+    double *an_lam = nullptr, *an_rlam = nullptr, *an_lamK = nullptr, *an_dynKg = nullptr,
+           *an_binv = nullptr, *an_gflux = nullptr, *an_ones = nullptr;
+    bool last_aniso = false;            // the last solve used tensor mobilities
 
     // multigrid hierarchy
     std::vector<int> ijk;
@@ -184,6 +188,7 @@ void mhs_solver::free_topology() {
     for (auto &b : topo_bufs) (void)hipFree(b.p);
     topo_bufs.clear();
     binv = gflux = perm = nullptr;
+    an_lam = an_rlam = an_lamK = an_dynKg = an_binv = an_gflux = an_ones = nullptr;
     dT = DevTopo{};
     shapes.clear();
     dG = DevGeom{};
@@ -1107,7 +1112,8 @@ int linear_solve(mhs_solver *s, const mhs_solve_options &opt, int nrhs, bool mix
 
 int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const double *src,
                const int *hf_bc_type, const double *bcval, const mhs_solve_options *opt_in,
-               mhs_solve_report *rep, bool device_inputs) {
+               mhs_solve_report *rep, bool device_inputs, const double *binv_use = nullptr,
+               const double *gflux_use = nullptr) {
     if (!s) return MHS_ERR_INVALID;
     mhs_solve_options opt;
     mhs_default_options(&opt);
@@ -1176,7 +1182,9 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
         d_tm = s->in_totmob; d_rm = s->in_rhomob;
     }
     // --- assembly -----------------------------------------------------------
-    HIPCHECK(launch_assemble(s->stream, s->dT, T.max_ncf, nrhs, s->binv, s->gflux, d_tm, d_rm, d_src,
+    s->last_aniso = binv_use != nullptr;
+    const double *binv = binv_use ? binv_use : s->binv, *gflux = gflux_use ? gflux_use : s->gflux;
+    HIPCHECK(launch_assemble(s->stream, s->dT, T.max_ncf, nrhs, binv, gflux, d_tm, d_rm, d_src,
                              s->ftype, d_bv, s->C));
     const int dguess = opt.linsolver_type != 0;  // AMG types start from the Dirichlet guess (:1521-1532)
     HIPCHECK(launch_fill(s->stream, s->dT, nrhs, s->C, s->ftype, d_bv, s->inst_reg, dguess, s->val64,
@@ -1254,7 +1262,7 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
     auto t2 = std::chrono::steady_clock::now();
 
     // --- recovery -------------------------------------------------------------
-    HIPCHECK(launch_recover(s->stream, s->dT, T.max_ncf, nrhs, s->binv, s->gflux, d_tm, d_rm, s->C,
+    HIPCHECK(launch_recover(s->stream, s->dT, T.max_ncf, nrhs, binv, gflux, d_tm, d_rm, s->C,
                             s->x64, s->press, s->outflux));
     HIPCHECK(hipStreamSynchronize(s->stream));
     if (s->user_stream) {
@@ -1309,6 +1317,7 @@ int adjoint_impl(mhs_solver *s, const double *dJ_dv, const mhs_solve_options *op
     if (!dJ_dv) return fail(s, MHS_ERR_INVALID, "adjoint: dJ_dv is null");
     if (s->T.has_periodic) return fail(s, MHS_ERR_INVALID, "adjoint: periodic boundaries are not supported");
     if (!s->perm) return fail(s, MHS_ERR_STATE, "adjoint: no permeability (call compute_inner_products)");
+    if (s->last_aniso) return fail(s, MHS_ERR_STATE, "adjoint: not available after a tensor-mobility solve");
     mhs_solve_options opt;
     mhs_default_options(&opt);
     if (opt_in) opt = *opt_in;
@@ -1465,6 +1474,69 @@ int mhs_solve_device(mhs_solver *s, const double *totmob, const double *rhomob, 
                      const int *hf_bc_type, const double *hf_bc_value,
                      const mhs_solve_options *opt, mhs_solve_report *report) {
     return solve_impl(s, totmob, rhomob, src, hf_bc_type, hf_bc_value, opt, report, true);
+}
+
+static int solve_aniso_impl(mhs_solver *s, const double *lambda_t, const double *rho_lambda, const double *src,
+                            const int *hf_bc_type, const double *bcval, const mhs_solve_options *opt,
+                            mhs_solve_report *report, bool device) {
+    if (!s) return MHS_ERR_INVALID;
+    if (!s->structure_valid || !s->ip_valid || !s->perm)
+        return fail(s, MHS_ERR_STATE, "solve: init() (structure and inner products) must come first");
+    if (!lambda_t || !rho_lambda) return fail(s, MHS_ERR_INVALID, "solve_aniso: null mobility tensor");
+    HIPCHECK(hipSetDevice(s->device));
+    const Topology &T = s->T;
+    const size_t nc = (size_t)T.nc;
+    if (!s->an_binv) {
+        hipError_t he = hipSuccess;
+        auto &L = s->topo_bufs;
+        s->an_lam = dalloc<double>(s, L, 9 * nc, &he);
+        if (he == hipSuccess) s->an_rlam = dalloc<double>(s, L, 9 * nc, &he);
+        if (he == hipSuccess) s->an_lamK = dalloc<double>(s, L, 9 * nc, &he);
+        if (he == hipSuccess) s->an_dynKg = dalloc<double>(s, L, 3 * nc, &he);
+        if (he == hipSuccess) s->an_gflux = dalloc<double>(s, L, (size_t)T.nhf, &he);
+        if (he == hipSuccess) s->an_ones = dalloc<double>(s, L, nc, &he);
+        if (he == hipSuccess)
+            s->an_binv = dalloc<double>(s, L, (size_t)T.max_ncf * (T.max_ncf + 1) / 2 * nc, &he);
+        if (he != hipSuccess) {
+            s->an_binv = nullptr;
+            return fail(s, MHS_ERR_ALLOC, std::string("device allocation: ") + hipGetErrorString(he));
+        }
+        std::vector<double> ones(nc, 1.0);
+        HIPCHECK(hipMemcpy(s->an_ones, ones.data(), nc * sizeof(double), hipMemcpyHostToDevice));
+    }
+    const double *lam = lambda_t, *rlam = rho_lambda;
+    if (device) {
+        if (s->user_stream) {
+            HIPCHECK(hipEventRecord(s->ev_in, s->user_stream));
+            HIPCHECK(hipStreamWaitEvent(s->stream, s->ev_in, 0));
+        }
+    } else {
+        HIPCHECK(hipMemcpyAsync(s->an_lam, lambda_t, 9 * nc * sizeof(double), hipMemcpyHostToDevice, s->stream));
+        HIPCHECK(hipMemcpyAsync(s->an_rlam, rho_lambda, 9 * nc * sizeof(double), hipMemcpyHostToDevice, s->stream));
+        lam = s->an_lam;
+        rlam = s->an_rlam;
+    }
+    HIPCHECK(launch_aniso_ip(s->stream, s->dT, s->dG, T.max_ncf, s->perm, lam, rlam, s->grav, s->an_lamK,
+                             s->an_dynKg, s->an_binv, s->an_gflux));
+    if (device) {
+        return solve_impl(s, s->an_ones, s->an_ones, src, hf_bc_type, bcval, opt, report, true, s->an_binv,
+                          s->an_gflux);
+    }
+    std::vector<double> ones(nc, 1.0);
+    return solve_impl(s, ones.data(), ones.data(), src, hf_bc_type, bcval, opt, report, false, s->an_binv,
+                      s->an_gflux);
+}
+
+int mhs_solve_aniso(mhs_solver *s, const double *lambda_t, const double *rho_lambda, const double *src,
+                    const int *hf_bc_type, const double *hf_bc_value, const mhs_solve_options *opt,
+                    mhs_solve_report *report) {
+    return solve_aniso_impl(s, lambda_t, rho_lambda, src, hf_bc_type, hf_bc_value, opt, report, false);
+}
+
+int mhs_solve_aniso_device(mhs_solver *s, const double *lambda_t, const double *rho_lambda, const double *src,
+                           const int *hf_bc_type, const double *hf_bc_value, const mhs_solve_options *opt,
+                           mhs_solve_report *report) {
+    return solve_aniso_impl(s, lambda_t, rho_lambda, src, hf_bc_type, hf_bc_value, opt, report, true);
 }
 
 int mhs_get_solution(mhs_solver *s, double *cell_pressure, double *face_pressure, double *outflux) {

@@ -119,6 +119,14 @@ hipError_t launch_static_ip(hipStream_t st, const DevTopo &T, const DevGeom &G, 
                             const double *perm, const double *grav, double *binv_pk,
                             double *gflux);
 
+// Inner products of MimeticIPAnisoRelpermEvaluator for per-cell mobility
+// tensors lam (lambda_t) and rlam (sum_i rho_i lambda_i), row-major 3x3:
+// B^-1 from lam K (through its symmetric part, as k_static_ip) into binv_pk,
+// gflux = N (rlam K g).  lamK[9 nc] and dynKg[3 nc] are work arrays.
+hipError_t launch_aniso_ip(hipStream_t st, const DevTopo &T, const DevGeom &G, int maxn, const double *perm,
+                           const double *lam, const double *rlam, const double *grav, double *lamK, double *dynKg,
+                           double *binv_pk, double *gflux);
+
 // gterm[h] = omega[cell(h)] * (x_f - x_c) . grav (MimeticIPEvaluator::gravityTerm).
 hipError_t launch_gravity_term(hipStream_t st, const DevTopo &T, const DevGeom &G, const double *grav,
                                const double *omega, double *gterm);

@@ -2312,6 +2312,60 @@ hipError_t launch_gravity_term(hipStream_t st, const DevTopo &T, const DevGeom &
     return hipGetLastError();
 }
 
+// MimeticIPAnisoRelpermEvaluator::computeDynamicParams
+// (MimeticIPAnisoRelpermEvaluator.hpp:299-330): per cell lambda K =
+// lambda_t K (row-major 3x3 products) and the dynamic gravity vector
+// (sum_i rho_i lambda_i) (K g).
+__global__ void k_aniso_products(int nc, const double *__restrict__ perm, const double *__restrict__ lam,
+                                 const double *__restrict__ rlam, double gx, double gy, double gz,
+                                 double *__restrict__ lamK, double *__restrict__ dynKg) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const double *K = perm + 9 * (size_t)c, *L = lam + 9 * (size_t)c, *R = rlam + 9 * (size_t)c;
+    double Kr[9], Lr[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        Kr[i] = K[i];
+        Lr[i] = L[i];
+    }
+#pragma unroll
+    for (int l = 0; l < 3; ++l)
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+            lamK[9 * (size_t)c + 3 * l + m] = Lr[3 * l] * Kr[m] + Lr[3 * l + 1] * Kr[3 + m] + Lr[3 * l + 2] * Kr[6 + m];
+    const double kg0 = Kr[0] * gx + Kr[1] * gy + Kr[2] * gz;
+    const double kg1 = Kr[3] * gx + Kr[4] * gy + Kr[5] * gz;
+    const double kg2 = Kr[6] * gx + Kr[7] * gy + Kr[8] * gz;
+#pragma unroll
+    for (int l = 0; l < 3; ++l) dynKg[3 * (size_t)c + l] = R[3 * l] * kg0 + R[3 * l + 1] * kg1 + R[3 * l + 2] * kg2;
+}
+
+// gravityFlux (MimeticIPAnisoRelpermEvaluator.hpp:406-416): N dyn_Kg per
+// half-face, N's rows being area times the unit normal.
+__global__ void k_aniso_gflux(DevTopo T, DevGeom G, const double *__restrict__ dynKg, double *__restrict__ gflux) {
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= T.nhf) return;
+    const int c = T.hf_cell[h];
+    const double a = G.farea[h];
+    gflux[h] = G.fnrm[3 * (size_t)h] * a * dynKg[3 * (size_t)c] + G.fnrm[3 * (size_t)h + 1] * a * dynKg[3 * (size_t)c + 1] +
+               G.fnrm[3 * (size_t)h + 2] * a * dynKg[3 * (size_t)c + 2];
+}
+
+hipError_t launch_aniso_ip(hipStream_t st, const DevTopo &T, const DevGeom &G, int maxn, const double *perm,
+                           const double *lam, const double *rlam, const double *grav, double *lamK, double *dynKg,
+                           double *binv_pk, double *gflux) {
+    if (T.nc == 0) return hipSuccess;
+    k_aniso_products<<<(T.nc + 255) / 256, 256, 0, st>>>(T.nc, perm, lam, rlam, grav[0], grav[1], grav[2], lamK,
+                                                          dynKg);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // B^-1 from lambda K; the static kernel's gravity flux is replaced below
+    e = launch_static_ip(st, T, G, maxn, lamK, grav, binv_pk, gflux);
+    if (e != hipSuccess) return e;
+    k_aniso_gflux<<<(T.nhf + 255) / 256, 256, 0, st>>>(T, G, dynKg, gflux);
+    return hipGetLastError();
+}
+
 hipError_t launch_static_ip(hipStream_t st, const DevTopo &T, const DevGeom &G, int maxn,
                             const double *perm, const double *grav, double *binv_pk,
                             double *gflux) {

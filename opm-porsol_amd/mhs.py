@@ -33,7 +33,7 @@ EXPORTS = (
     "mhs_get_face_rows", "mhs_get_system", "mhs_time_kernel", "mhs_compute_inner_products_device",
     "mhs_get_solution_device", "mhs_adjoint", "mhs_adjoint_device", "mhs_sum_outflux",
     "mhs_gravity_term", "mhs_topology_is_sane", "mhs_estimate_cell_velocity",
-    "mhs_estimate_cell_velocity_device",
+    "mhs_estimate_cell_velocity_device", "mhs_solve_aniso", "mhs_solve_aniso_device",
 )
 
 
@@ -132,6 +132,9 @@ def lib():
         L.mhs_gravity_term.argtypes = [_P, _D, _D, _D]
         L.mhs_topology_is_sane.argtypes = [ctypes.c_int, _I, _I, ctypes.POINTER(ctypes.c_int)]
         L.mhs_estimate_cell_velocity.argtypes = [_P, ctypes.c_int, _D]
+        L.mhs_solve_aniso.argtypes = [_P, _D, _D, _D, _I, _D, ctypes.POINTER(Options), ctypes.POINTER(Report)]
+        L.mhs_solve_aniso_device.argtypes = [_P, _P, _P, _P, _I, _P, ctypes.POINTER(Options),
+                                             ctypes.POINTER(Report)]
         L.mhs_estimate_cell_velocity_device.argtypes = [_P, ctypes.c_int, _P]
         L.mhs_adjoint.argtypes = [_P, _D, ctypes.POINTER(Options), _D, _D, _D, _D,
                                   ctypes.POINTER(Report)]
@@ -170,6 +173,13 @@ def topology_is_sane(g):
     if rc != 0:
         raise MhsError(rc, "topology_is_sane: invalid arguments")
     return bool(ok.value)
+
+
+def _solve_kw(kw):
+    """solve()'s keyword names mapped onto options()'s."""
+    names = {"residual_tolerance": "tol", "linsolver_verbosity": "linsolver_verbosity",
+             "linsolver_maxit": "linsolver_maxit"}
+    return {names.get(k, k): v for k, v in kw.items()}
 
 
 class HybridSolver:
@@ -331,6 +341,39 @@ class HybridSolver:
         self.solve_count += 1
         if rc == MHS_ERR_NOT_CONVERGED and not raise_on_failure:
             return rep
+        self._check(rc)
+        return rep
+
+    def solve_aniso(self, lambda_t, rho_lambda, src, bc_type, bc_value, **kw):
+        """solve() with MimeticIPAnisoRelpermEvaluator: per-cell mobility tensors
+        lambda_t [nc, 9] (total) and rho_lambda [nc, 9] (sum rho_i lambda_i),
+        row-major like the permeability.  Keyword options as solve()."""
+        g = self.grid
+        src = np.asarray(src, dtype=np.float64)
+        nrhs = 1 if src.ndim == 1 else src.shape[0]
+        bv = np.asarray(bc_value, dtype=np.float64).reshape(nrhs, g.num_half_faces)
+        lt, plt = _d(np.asarray(lambda_t, dtype=np.float64).reshape(-1))
+        rl, prl = _d(np.asarray(rho_lambda, dtype=np.float64).reshape(-1))
+        sr, psr = _d(src.reshape(nrhs, -1))
+        bt, pbt = _i(bc_type); bvv, pbv = _d(bv)
+        o = self.options(nrhs=nrhs, **_solve_kw(kw))
+        rep = Report()
+        rc = self._L.mhs_solve_aniso(self._h, plt, prl, psr, pbt, pbv, ctypes.byref(o), ctypes.byref(rep))
+        self.report = rep
+        self.nrhs = nrhs
+        self.solve_count += 1
+        self._check(rc)
+        return rep
+
+    def solve_aniso_device(self, lambda_ptr, rho_lambda_ptr, src_ptr, bc_type, bcval_ptr, nrhs=1, **kw):
+        bt, pbt = _i(bc_type)
+        o = self.options(nrhs=nrhs, **_solve_kw(kw))
+        rep = Report()
+        rc = self._L.mhs_solve_aniso_device(self._h, _P(lambda_ptr), _P(rho_lambda_ptr), _P(src_ptr), pbt,
+                                            _P(bcval_ptr), ctypes.byref(o), ctypes.byref(rep))
+        self.report = rep
+        self.nrhs = nrhs
+        self.solve_count += 1
         self._check(rc)
         return rep
 

@@ -219,6 +219,50 @@ def solve(g, perm, totmob, rhomob, src, bctype, bcval, partner=None, grav=(0.0, 
     return r
 
 
+def aniso_inner_products(g, perm, lambda_t, rho_lambda, grav=(0.0, 0.0, 0.0)):
+    """MimeticIPAnisoRelpermEvaluator (MimeticIPAnisoRelpermEvaluator.hpp):
+    computeDynamicParams (:299-330) forms lambda K = lambda_t K and
+    dyn_Kg = (sum rho_i lambda_i) (K g); getInverseMatrix (:363-400) is the
+    static formula with lambda K in place of K (t = 6 tr(lambda K) / 3), so
+    the static evaluator applied to lambda K gives B^-1; gravityFlux
+    (:406-416) is N dyn_Kg.  Returns (binv, pos, gflux)."""
+    nc = g.num_cells
+    K = np.asarray(perm, dtype=np.float64).reshape(nc, 3, 3)
+    L = np.asarray(lambda_t, dtype=np.float64).reshape(nc, 3, 3)
+    R = np.asarray(rho_lambda, dtype=np.float64).reshape(nc, 3, 3)
+    lamK = np.einsum("cij,cjk->cik", L, K)
+    binv, pos, _ = static_ip(g, lamK.reshape(nc, 9))
+    dyn = np.einsum("cij,cj->ci", R, np.einsum("cij,j->ci", K, np.asarray(grav, dtype=np.float64)))
+    hc = g.hf_cell()
+    gfl = np.einsum("hi,hi->h", np.asarray(g.face_normal) * np.asarray(g.face_area)[:, None], dyn[hc])
+    return binv, pos, gfl
+
+
+def solve_aniso(g, perm, lambda_t, rho_lambda, src, bctype, bcval, partner=None, grav=(0.0, 0.0, 0.0),
+                tol=1e-14, maxit=0, precond=2):
+    """solve() with the tensor-mobility inner products: assembly and recovery
+    with unit scalar mobilities on the dynamic B^-1 and gravity flux."""
+    hf_dof, nint, nf, maxf = enumerate_dof(g)
+    bctype = _ci(bctype)
+    if partner is None:
+        partner = np.full(g.num_half_faces, -1, dtype=np.int32)
+    pp, nper = periodic_dof(g, hf_dof, nf, bctype, _ci(partner))
+    binv, pos, gfl = aniso_inner_products(g, perm, lambda_t, rho_lambda, grav)
+    ones = np.ones(g.num_cells)
+    s = assemble(g, hf_dof, nf, pp, binv, gfl, ones, ones, src, bctype, bcval)
+    if s.do_regularization:
+        regularize(s)
+    x, rc, its, red = cg(s, tol, maxit, precond, False)
+    if rc != 0:
+        raise RuntimeError(f"oracle CG failed rc={rc} its={its} reduction={red}")
+    p, v = recover(g, hf_dof, binv, gfl, ones, ones, s, x)
+    r = Result()
+    r.face_pressure, r.cell_pressure, r.outflux = x, p, v
+    r.binv, r.binv_pos, r.gflux = binv, pos, gfl
+    r.iterations, r.reduction, r.rc = its, red, rc
+    return r
+
+
 def topology_is_sane(num_cells, facepos, nb):
     """topologyIsSane (IncompFlowSolverHybrid.hpp:108-130): true when, for
     every cell, its internal neighbours (non-boundary faces) are pairwise
