@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "mimetic_hybrid.h"
+#include "MimeticIPAnisoRelpermEvaluator.hpp"
 #include "MimeticIPEvaluator.hpp"
 
 namespace Opm {
@@ -93,9 +94,14 @@ bool topologyIsSane(const GI &g) {
 template <class GridInterface, class RockInterface, class BCInterface,
           template <class GridIF, class RockIF> class InnerProduct>
 class IncompFlowSolverHybrid {
+    static constexpr bool kTensorMobility =
+        std::is_same<InnerProduct<GridInterface, RockInterface>,
+                     MimeticIPAnisoRelpermEvaluator<GridInterface, RockInterface>>::value;
     static_assert(std::is_same<InnerProduct<GridInterface, RockInterface>,
-                               MimeticIPEvaluator<GridInterface, RockInterface>>::value,
-                  "the MI355X path implements the MimeticIPEvaluator inner product");
+                               MimeticIPEvaluator<GridInterface, RockInterface>>::value ||
+                      kTensorMobility,
+                  "the MI355X path implements the MimeticIPEvaluator and MimeticIPAnisoRelpermEvaluator "
+                  "inner products");
     typedef typename GridInterface::Scalar Scalar;
     enum FaceType { Internal, Dirichlet, Neumann, Periodic };
 
@@ -284,17 +290,33 @@ public:
         assert(matrix_structure_valid_);
         const int nc = pgrid_->numberOfCells();
         std::vector<double> totmob(nc), rhomob(nc), s(nc);
+        std::vector<double> lam_t(kTensorMobility ? 9 * (size_t)nc : 0), rho_lam(lam_t.size());
         std::vector<int> bctype(nhf_, MHS_BC_NEUMANN);
         std::vector<double> bcval(nhf_, 0.0);
         int h = 0, pos = 0;
         do_regularization_ = true;
         for (CI c = pgrid_->cellbegin(); c != pgrid_->cellend(); ++c, ++pos) {
             const int ci = c->index();
-            std::array<double, FluidInterface::NumberOfPhases> mob, rho;
-            r.phaseMobilities(ci, sat[ci], mob);
+            std::array<double, FluidInterface::NumberOfPhases> rho;
             r.phaseDensities(ci, rho);
-            totmob[pos] = std::accumulate(mob.begin(), mob.end(), 0.0);
-            rhomob[pos] = std::inner_product(rho.begin(), rho.end(), mob.begin(), 0.0);
+            if constexpr (kTensorMobility) {
+                // MimeticIPAnisoRelpermEvaluator::computeDynamicParams (:299-330):
+                // lambda_t = sum of the phase mobility tensors, and sum rho_i lambda_i
+                for (int ph = 0; ph < FluidInterface::NumberOfPhases; ++ph) {
+                    Mat3 m;
+                    r.phaseMobility(ph, ci, sat[ci], m);
+                    for (int l = 0; l < 3; ++l)
+                        for (int k = 0; k < 3; ++k) {
+                            lam_t[9 * (size_t)pos + 3 * l + k] += m(l, k);
+                            rho_lam[9 * (size_t)pos + 3 * l + k] += rho[ph] * m(l, k);
+                        }
+                }
+            } else {
+                std::array<double, FluidInterface::NumberOfPhases> mob;
+                r.phaseMobilities(ci, sat[ci], mob);
+                totmob[pos] = std::accumulate(mob.begin(), mob.end(), 0.0);
+                rhomob[pos] = std::inner_product(rho.begin(), rho.end(), mob.begin(), 0.0);
+            }
             s[pos] = src[ci];
             for (FI f = c->facebegin(); f != c->faceend(); ++f, ++h) {
                 if (!f->boundary()) continue;
@@ -322,8 +344,11 @@ public:
         opt.prolongate_factor = prolongate_factor;
         opt.smooth_steps = smooth_steps;
         opt.precision = precision_;
-        const int rc = mhs_solve(h_, totmob.data(), rhomob.data(), s.data(), bctype.data(),
-                                 bcval.data(), &opt, &report_);
+        const int rc = kTensorMobility
+                           ? mhs_solve_aniso(h_, lam_t.data(), rho_lam.data(), s.data(), bctype.data(),
+                                             bcval.data(), &opt, &report_)
+                           : mhs_solve(h_, totmob.data(), rhomob.data(), s.data(), bctype.data(),
+                                       bcval.data(), &opt, &report_);
         if (rc == MHS_ERR_INVALID && (linsolver_type < 0 || linsolver_type > 3)) {
             std::cerr << "Unknown linsolver_type: " << linsolver_type << '\n';
             throw std::runtime_error("Unknown linsolver_type");
