@@ -281,7 +281,7 @@ def _overlap_integrals(aL1, aL2, aH1, aH2, bL1, bL2, bH1, bH2):
 
 def corner_point(nx: int, ny: int, nz: int, dx: float = 1.0, dy: float = 1.0, dz: float = 1.0,
                  perturb: float = 0.3, fault_frac: float = 0.05, fault_shift: float = 0.5,
-                 seed: int = 0) -> Grid:
+                 seed: int = 0, return_depths: bool = False) -> Grid:
     """Faulted corner-point grid on vertical pillars (z is depth).
 
     Every pillar node depth is k dz + U(-perturb, perturb) dz (seeded); a
@@ -294,6 +294,9 @@ def corner_point(nx: int, ny: int, nz: int, dx: float = 1.0, dy: float = 1.0, dz
     unit normal) and the mean of their corner depths; lateral faces are
     planar and exact; cell volumes and centroids are exact (2x2 Gauss
     quadrature of the bilinear thickness).
+
+    return_depths=True returns the corner depths zc[j, i, c, l] instead of
+    the grid (the input of corner_point_from_depths and deck.write_grdecl).
 
     Local face order per cell: x- side, x+, y-, y+ (sub-faces by neighbour
     layer, exposed parts first/last), top (z-), bottom (z+).  Boundary ids
@@ -315,6 +318,31 @@ def corner_point(nx: int, ny: int, nz: int, dx: float = 1.0, dy: float = 1.0, dz
     # zc[j, i, c, l]: depth of corner c (0:(i,j) 1:(i+1,j) 2:(i,j+1) 3:(i+1,j+1)) at level l
     zc = np.stack([Z[pid(I, J)], Z[pid(I + 1, J)], Z[pid(I, J + 1)], Z[pid(I + 1, J + 1)]], axis=2)
     zc = zc + off[:, :, None, None]
+    if return_depths:
+        return zc
+    return corner_point_from_depths(nx, ny, nz, dx, dy, zc, dz)
+
+
+def corner_point_from_depths(nx: int, ny: int, nz: int, dx: float, dy: float, zc: np.ndarray,
+                             dz: float | None = None, origin=(0.0, 0.0)) -> Grid:
+    """Corner-point geometry on vertical pillars at (x0 + i dx, y0 + j dy).
+
+    zc[j, i, c, l] is the depth of corner c (0: pillar (i, j), 1: (i+1, j),
+    2: (i, j+1), 3: (i+1, j+1)) of column (i, j) at level l (the top of
+    layer l; l = nz is the bottom of the last layer); layers within a column
+    have no gaps.  Two neighbouring columns whose corner depths coincide on
+    their shared pillars are matching (1:1 faces); otherwise every overlap of
+    their layers on the shared vertical plane is a face, and parts facing no
+    cell are no-flow boundary (id 7, side 6).  dz scales the overlap
+    tolerance (default: the mean layer thickness).  Faces, ids and order as
+    described in corner_point().
+    """
+    zc = np.asarray(zc, dtype=np.float64)
+    if zc.shape != (ny, nx, 4, nz + 1):
+        raise ValueError(f"corner_point_from_depths: zc must have shape {(ny, nx, 4, nz + 1)}, got {zc.shape}")
+    if dz is None:
+        dz = float(np.mean(zc[..., -1] - zc[..., 0])) / nz
+    J, I = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")      # [ny, nx]
     zmin, zmax = zc.min() - 10.0 * dz, zc.max() + 10.0 * dz
     nc = nx * ny * nz
 
@@ -397,7 +425,7 @@ def corner_point(nx: int, ny: int, nz: int, dx: float = 1.0, dy: float = 1.0, dz
                     nrm, sideA + 1, sideA)
             return
         zB1, zB2 = zc[jB, iB, cB[0]], zc[jB, iB, cB[1]]
-        same = off[jA, iA] == off[jB, iB]
+        same = np.all(zA1 == zB1, axis=1) & np.all(zA2 == zB2, axis=1)
         # matching columns: 1:1 faces
         sel = np.nonzero(same)[0]
         for k in range(nz):
@@ -485,6 +513,8 @@ def corner_point(nx: int, ny: int, nz: int, dx: float = 1.0, dy: float = 1.0, dz
     nhf_per = np.bincount(cell, minlength=nc)
     facepos = np.concatenate([[0], np.cumsum(nhf_per)]).astype(np.int32)
     cat = lambda name: np.concatenate(rec[name])[order]
+    shift = np.array([origin[0], origin[1], 0.0])
+    ccen = ccen + shift
     return Grid(
         facepos=facepos,
         nb=cat("nb").astype(np.int32),
@@ -492,7 +522,7 @@ def corner_point(nx: int, ny: int, nz: int, dx: float = 1.0, dy: float = 1.0, dz
         cell_volume=cvol,
         cell_centroid=ccen,
         face_area=cat("area"),
-        face_centroid=cat("cen"),
+        face_centroid=cat("cen") + shift,
         face_normal=cat("nrm"),
         side=cat("side").astype(np.int8),
         dims=None,
