@@ -4,7 +4,7 @@ The directory name is not a Python identifier; load it with ``load_package``
 (in __graft_entry__.py, tests/conftest.py and bench.py), which registers it as
 ``opm_porsol_amd``.
 """
-from . import deck, fields, grids, mhs, transport  # noqa: F401
+from . import deck, fields, grids, mhs, output, transport  # noqa: F401
 from .mhs import HybridSolver, MhsError  # noqa: F401
 from .transport import EulerUpstream  # noqa: F401
 

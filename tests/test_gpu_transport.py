@@ -274,3 +274,21 @@ def test_estimate_cell_velocity(pkg, orc):
     vb = sb.estimateCellVelocity()
     np.testing.assert_allclose(vb[:, 0], u, rtol=1e-9)
     assert np.max(np.abs(vb[:, 1:])) < 1e-9 * u
+
+
+def test_simulator_output_fields(pkg, tmp_path):
+    """writeVtkOutput's cell data after a pressure solve: the phase
+    velocities split the cell velocity by the fractional flow, the pressure
+    is the solution's, and the VTU file holds every field."""
+    g, perm, bt, bv, sat, sat_bc, poro, inj = _cart_case(pkg, (6, 5, 4), seed=21)
+    s, flux = _flow(pkg, g, perm, bt, bv, sat)
+    f = pkg.output.simulator_fields(s, sat, VISC)
+    np.testing.assert_allclose(f["phase velocity [water]"] + f["phase velocity [oil]"], f["velocity"],
+                               rtol=1e-14, atol=1e-30)
+    np.testing.assert_allclose(f["fractional flow [water]"], tr.fractional_flow(sat, VISC), rtol=1e-15)
+    np.testing.assert_array_equal(f["pressure"], s.getSolution()[0])
+    zc = np.broadcast_to(np.arange(5) * 1.0, (5, 6, 4, 5)).copy()
+    P, conn = pkg.output.hexahedra(6, 5, 4, 2.0, 2.0, zc)
+    pkg.output.write_vtu(str(tmp_path / "o.vtu"), P, conn, f)
+    txt = (tmp_path / "o.vtu").read_text()
+    assert all(f'Name="{k}"' in txt for k in f)
