@@ -75,7 +75,8 @@ typedef struct mhs_fluid {
 void mhs_transport_default_options(mhs_transport_options *opt);
 
 /* EulerUpstream(g, r, b) (EulerUpstream_impl.hpp:76-89): binds the transport
- * to the solver `flow` (same device and stream). */
+ * to the solver `flow` (same device and stream).  The solver must outlive
+ * every call but mhs_transport_destroy, which may come after mhs_destroy. */
 int mhs_transport_create(mhs_transport **out, mhs_solver *flow);
 int mhs_transport_destroy(mhs_transport *t);
 const char *mhs_transport_last_error(const mhs_transport *t);

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -60,6 +61,11 @@ struct Shape {
 
 }  // namespace
 
+static long long next_generation() {
+    static std::atomic<long long> counter{0};
+    return ++counter;
+}
+
 struct mhs_solver {
     int device = 0;
     hipStream_t stream = nullptr;       // our own stream (default priority)
@@ -84,7 +90,10 @@ struct mhs_solver {
     double *binv = nullptr, *gflux = nullptr;
     double grav[3] = {0.0, 0.0, 0.0};
     double *perm = nullptr;             // K per cell (row-major 3x3), kept for the adjoint and transport
-    long long perm_gen = 0;             // counts permeability uploads
+    // generations of the permeability and of the structure, unique in the
+    // process (a later solver at the same address never repeats them)
+    long long perm_gen = 0;
+    long long struct_gen = 0;
     // anisotropic (tensor-mobility) inner products, allocated on the first such solve
     double *an_lam = nullptr, *an_rlam = nullptr, *an_lamK = nullptr, *an_dynKg = nullptr,
            *an_binv = nullptr, *an_gflux = nullptr, *an_ones = nullptr;
@@ -515,6 +524,7 @@ int mhs_init_system_structure(mhs_solver *s, const int *hf_bc_type, const int *h
         if (rc) return rc;
     }
     s->structure_valid = true;
+    s->struct_gen = next_generation();
     return MHS_OK;
 }
 
@@ -539,7 +549,7 @@ int inner_products_impl(mhs_solver *s, const double *perm, const double *gravity
         HIPCHECK(hipEventRecord(s->ev_in, s->user_stream));
         HIPCHECK(hipStreamWaitEvent(s->stream, s->ev_in, 0));
     }
-    ++s->perm_gen;
+    s->perm_gen = next_generation();
     hipError_t e = hipMemcpyAsync(s->perm, perm, n9 * sizeof(double),
                                   device_perm ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s->stream);
     if (e == hipSuccess)
@@ -1740,7 +1750,8 @@ struct mhs_transport {
     mhs_solver *flow = nullptr;
     std::string err;
     bool ready = false;
-    const int *topo_key = nullptr;      // the solver's device hf_cell when init ran
+    int device = 0;                     // the bound solver's device
+    long long topo_key = -1;            // the solver's structure generation when init ran
     std::vector<DevBuf> bufs;
     DevTransport D{};
     double *F = nullptr, *G = nullptr, *C = nullptr, *pv = nullptr, *sat_bc = nullptr;
@@ -1829,7 +1840,7 @@ int transport_ready(mhs_transport *t) {
     if (!t) return MHS_ERR_INVALID;
     if (!t->ready) return tfail(t, MHS_ERR_STATE, "transport: call mhs_transport_init first");
     mhs_solver *s = t->flow;
-    if (!s->structure_valid || s->dT.hf_cell != t->topo_key)
+    if (!s->structure_valid || s->struct_gen != t->topo_key)
         return tfail(t, MHS_ERR_STATE, "transport: the solver's structure changed; call mhs_transport_init again");
     if (!s->perm) return tfail(t, MHS_ERR_STATE, "transport: no permeability (call compute_inner_products)");
     return MHS_OK;
@@ -1975,13 +1986,14 @@ int mhs_transport_create(mhs_transport **out, mhs_solver *flow) {
     mhs_transport *t = new (std::nothrow) mhs_transport();
     if (!t) return MHS_ERR_ALLOC;
     t->flow = flow;
+    t->device = flow->device;
     *out = t;
     return MHS_OK;
 }
 
 int mhs_transport_destroy(mhs_transport *t) {
     if (!t) return MHS_OK;
-    (void)hipSetDevice(t->flow->device);
+    (void)hipSetDevice(t->device);      // the bound solver may already be gone
     delete t;
     return MHS_OK;
 }
@@ -2087,7 +2099,7 @@ int mhs_transport_init(mhs_transport *t, const double *porosity, const double *s
     D.rho_o = fluid->density[1];
     D.drho = fluid->density[0] - fluid->density[1];
     cfl_factors(D.visc_w, D.visc_o, t->fac);
-    t->topo_key = s->dT.hf_cell;
+    t->topo_key = s->struct_gen;
     t->timed_ok = false;
     t->coeff_gen = -1;
     t->ready = true;

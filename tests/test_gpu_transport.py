@@ -292,3 +292,29 @@ def test_simulator_output_fields(pkg, tmp_path):
     pkg.output.write_vtu(str(tmp_path / "o.vtu"), P, conn, f)
     txt = (tmp_path / "o.vtu").read_text()
     assert all(f'Name="{k}"' in txt for k in f)
+
+
+def test_transport_handle_lifetime_and_stale_structure(pkg):
+    """A new structure on the bound solver makes the transport ask for a new
+    init; destroying the solver before the transport handle is safe."""
+    import ctypes
+    g, perm, bt, bv, sat, sat_bc, poro, inj = _cart_case(pkg, (6, 4, 3), seed=31)
+    s, flux = _flow(pkg, g, perm, bt, bv, sat)
+    t = pkg.EulerUpstream(s, poro, sat_bc, VISC, RHO)
+    t.computeResidual(sat, GRAV)
+    s.initSystemStructure(bt)
+    s.computeInnerProducts(perm, GRAV)
+    tm, rm = _fluid(sat)
+    s.solve(tm, rm, np.zeros(g.num_cells), bt, bv, residual_tolerance=1e-12)
+    with pytest.raises(pkg.MhsError, match="mhs_transport_init again"):
+        t.computeResidual(sat, GRAV)
+    t.initObj(poro, sat_bc, VISC, RHO)
+    r_gpu = t.computeResidual(sat, GRAV)
+    r_orc = tr.residual(g, perm, sat, np.asarray(s.getSolution()[2]).reshape(-1), GRAV, DRHO, VISC, None, sat_bc)
+    assert cases.rel_err(r_gpu, r_orc) < 1e-12
+    L = pkg.transport.lib()
+    h = ctypes.c_void_p()
+    assert L.mhs_transport_create(ctypes.byref(h), s._h) == 0
+    L.mhs_destroy(s._h)
+    s._h = None
+    assert L.mhs_transport_destroy(h) == 0
