@@ -130,7 +130,11 @@ struct mhs_solver {
     double *h_sc = nullptr;
     int *h_act = nullptr, *h_iters = nullptr, *h_upd = nullptr, *h_status = nullptr;
     size_t h_nsys = 0;
-    std::vector<uint8_t> last_ftype;
+    // boundary half-faces of the structure (their instance), and the face
+    // types last uploaded for them; internal half-faces are always FT_INTERNAL
+    std::vector<int> bnd_hf, bnd_inst;
+    long long bnd_gen = -1;
+    std::vector<uint8_t> last_bft;
     int last_nrhs = 0;
     bool last_amg = false;
     long long solve_count = 0;
@@ -222,7 +226,7 @@ void mhs_solver::free_solve_buffers() {
     h_sc = nullptr;
     h_act = h_iters = h_upd = h_status = nullptr;
     h_nsys = 0;
-    last_ftype.clear();
+    last_bft.clear();
 }
 
 mhs_solver::~mhs_solver() {
@@ -1146,34 +1150,48 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
     if (rc != MHS_OK) return rc;
 
     // Face types (FaceType, :378) and per-instance regularisation flags.
-    std::vector<uint8_t> ft(T.nhf, FT_INTERNAL);
-    std::vector<int> reg(T.ninst, 1);
-    {
+    // Only boundary half-faces can change type between solves: their list is
+    // built once per structure, and the device types are rewritten only when
+    // a boundary type changed.
+    if (s->bnd_gen != s->struct_gen) {
+        s->bnd_hf.clear();
+        s->bnd_inst.clear();
         int t = 0;
         for (int c = 0; c < T.nc; ++c) {
             while (c >= T.inst_cellpos[t + 1]) ++t;
-            for (int h = T.facepos[c]; h < T.facepos[c + 1]; ++h) {
-                if (T.nb[h] >= 0) continue;
-                int bt = hf_bc_type[h];
-                if (bt == MHS_BC_DIRICHLET) { ft[h] = FT_DIRICHLET; reg[t] = 0; }
-                else if (bt == MHS_BC_PERIODIC) ft[h] = FT_PERIODIC;
-                else if (bt == MHS_BC_NEUMANN) ft[h] = FT_NEUMANN;
-                else return fail(s, MHS_ERR_INVALID, "solve: unknown boundary condition type");
-                if ((bt == MHS_BC_PERIODIC) != (T.hf_periodic[h] != 0))
-                    return fail(s, MHS_ERR_INVALID,
-                                "solve: periodic faces differ from those given to init_system_structure");
-            }
+            for (int h = T.facepos[c]; h < T.facepos[c + 1]; ++h)
+                if (T.nb[h] < 0) {
+                    s->bnd_hf.push_back(h);
+                    s->bnd_inst.push_back(t);
+                }
         }
+        s->bnd_gen = s->struct_gen;
+        s->last_bft.clear();
+    }
+    const size_t nbnd = s->bnd_hf.size();
+    std::vector<uint8_t> bft(nbnd);
+    std::vector<int> reg(T.ninst, 1);
+    for (size_t e = 0; e < nbnd; ++e) {
+        const int h = s->bnd_hf[e];
+        const int bt = hf_bc_type[h];
+        if (bt == MHS_BC_DIRICHLET) { bft[e] = FT_DIRICHLET; reg[s->bnd_inst[e]] = 0; }
+        else if (bt == MHS_BC_PERIODIC) bft[e] = FT_PERIODIC;
+        else if (bt == MHS_BC_NEUMANN) bft[e] = FT_NEUMANN;
+        else return fail(s, MHS_ERR_INVALID, "solve: unknown boundary condition type");
+        if ((bt == MHS_BC_PERIODIC) != (T.hf_periodic[h] != 0))
+            return fail(s, MHS_ERR_INVALID, "solve: periodic faces differ from those given to init_system_structure");
     }
     auto t0 = std::chrono::steady_clock::now();
     if (s->user_stream) {
         HIPCHECK(hipEventRecord(s->ev_in, s->user_stream));
         HIPCHECK(hipStreamWaitEvent(s->stream, s->ev_in, 0));
     }
-    if (ft != s->last_ftype) {
+    if (bft != s->last_bft) {
+        std::vector<uint8_t> ft(T.nhf, FT_INTERNAL);
+        for (size_t e = 0; e < nbnd; ++e) ft[s->bnd_hf[e]] = bft[e];
         HIPCHECK(hipMemcpyAsync(s->ftype, ft.data(), T.nhf, hipMemcpyHostToDevice, s->stream));
         HIPCHECK(hipStreamSynchronize(s->stream));
-        s->last_ftype = ft;
+        s->last_bft = bft;
     }
     HIPCHECK(hipMemcpyAsync(s->inst_reg, reg.data(), sizeof(int) * T.ninst, hipMemcpyHostToDevice, s->stream));
     const double *d_tm = totmob, *d_rm = rhomob, *d_src = src, *d_bv = bcval;
