@@ -376,6 +376,21 @@ int build_amg_device(mhs_solver *s, const Strength &strength) {
             A.max_rows[l] = 0;
             for (size_t t = 0; t + 1 < H.inst_ptr.size(); ++t)
                 A.max_rows[l] = std::max(A.max_rows[l], H.inst_ptr[t + 1] - H.inst_ptr[t]);
+            A.max_children[l] = 0;
+            for (size_t r = 0; r + 1 < H.child_ptr.size(); ++r)
+                A.max_children[l] = std::max(A.max_children[l], H.child_ptr[r + 1] - H.child_ptr[r]);
+            A.covers[l] = 0;
+            if (l >= 2 && !H.child_ptr.empty() && H.child_ptr.back() == s->H.lev[l - 1].n) {
+                std::vector<int> seen(s->H.lev[l - 1].n, 0);
+                bool ok = true;
+                for (int c : H.child) {
+                    if (c < 0 || c >= (int)seen.size() || seen[c]++) {
+                        ok = false;
+                        break;
+                    }
+                }
+                A.covers[l] = ok ? 1 : 0;
+            }
             D.nnz = H.row_ptr.empty() ? 0 : H.row_ptr.back();
             D.nsrc = H.src_ptr.empty() ? 0 : H.src_ptr.back();
             D.inst_ptr = dupload(s, L, H.inst_ptr, &he);

@@ -104,6 +104,8 @@ struct DevAmg {
     const DevAmg *dev;              // this structure's copy in device memory (refreshed by the host)
     int nu;                         // smoothing sweeps before and after the coarse correction
     int max_rows[kMaxLevels];       // rows of the largest instance on each level
+    int max_children[kMaxLevels];   // most finer rows aggregated into one row of each level
+    int covers[kMaxLevels];         // 1: every row of the finer level is a child of exactly one row
     // A P with the level-1 aggregates (SELL-64 like the face matrix), refilled
     // every solve from the face matrix through the gather lists
     const long long *ap_slice_ptr;

@@ -1343,6 +1343,51 @@ __global__ __launch_bounds__(TB) void k_vc0_res(Gate gt, DevTopo T, const VT *__
     }
 }
 
+// Down-sweep of a level and restriction to the next, for a tentative
+// (piecewise-constant) transfer: one coarse row per 8 G lanes, G lanes per
+// child row (at most 8 children), the children's residuals summed across the
+// child slots by shuffles.  Writes x and res of the finer level as
+// k_vcc_down and the coarse right-hand side as k_restrict, in one launch.
+template <typename VT, int NRHS, int G>
+__global__ __launch_bounds__(256) void k_vcc_down_restrict(DevAmgLevel F, DevAmgLevel C, const VT *__restrict__ b,
+                                                           VT *__restrict__ x, VT *__restrict__ res,
+                                                           VT *__restrict__ bc, VT w) {
+    static_assert(8 * G <= 64 && 64 % (8 * G) == 0, "a coarse row's lanes must tile a wavefront");
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int J = t / (8 * G), slot = (t / G) % 8, sub = t % G;
+    if (J >= C.n) return;            // whole 8G-lane groups leave together
+    const int c0 = C.child_ptr[J], nch = C.child_ptr[J + 1] - c0;
+    VT part[NRHS];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) part[k] = VT(0);
+    if (slot < nch) {
+        const int i = C.child[c0 + slot];
+        const VT *dinv = lvl_dinv<VT>(F);
+        VT acc[NRHS];
+        row_down<VT, NRHS, G>(F, lvl_val<VT>(F), dinv, b, w, i, acc, sub);
+        if (sub == 0) {
+            const VT d = w * dinv[i];
+#pragma unroll
+            for (int k = 0; k < NRHS; ++k) {
+                const size_t q = (size_t)i * NRHS + k;
+                const VT r = b[q] - acc[k];
+                x[q] = d * b[q];
+                res[q] = r;
+                part[k] = r;
+            }
+        }
+    }
+    // lanes with sub != 0 hold 0; sum the 8 slots of this coarse row
+#pragma unroll
+    for (int m = G; m < 8 * G; m <<= 1)
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) part[k] += __shfl_xor(part[k], m, 8 * G);
+    if (slot == 0 && sub == 0) {
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) bc[(size_t)J * NRHS + k] = part[k];
+    }
+}
+
 // Coarse levels (CSR), G lanes per row.
 template <typename VT, int NRHS, int G = 1>
 __global__ __launch_bounds__(256) void k_vcc_down(DevAmgLevel L, const VT *__restrict__ b,
@@ -2662,7 +2707,18 @@ static void vcycle_impl(hipStream_t st, const Gate &gt, const DevTopo &T, const 
         for (int l = 1; l < lt; ++l) {
             const DevAmgLevel &F = A.lev[l];
             const DevAmgLevel &C = A.lev[l + 1];
-            with_lanes(lanes_per_row(F.nnz, F.n), [&](auto g) {
+            const int gd = lanes_per_row(F.nnz, F.n);
+            if (!F.sa_out && A.covers[l + 1] && A.max_children[l + 1] <= 8 && gd <= 8) {
+                // down-sweep and restriction in one launch
+                with_lanes(gd, [&](auto g) {
+                    constexpr int Gv = decltype(g)::value;
+                    if constexpr (Gv <= 8)
+                        k_vcc_down_restrict<VT, NR, Gv><<<row_blocks(C.n, 8 * Gv), 256, 0, st>>>(
+                            F, C, (const VT *)F.b, (VT *)F.x, (VT *)F.res, (VT *)C.b, wl(l));
+                });
+                continue;
+            }
+            with_lanes(gd, [&](auto g) {
                 constexpr int Gv = decltype(g)::value;
                 k_vcc_down<VT, NR, Gv><<<row_blocks(F.n, Gv), 256, 0, st>>>(F, (const VT *)F.b, (VT *)F.x,
                                                                           (VT *)F.res, wl(l));
