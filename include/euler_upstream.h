@@ -96,7 +96,10 @@ int mhs_transport_init(mhs_transport *t, const double *porosity, const double *s
  * time / n with n from the CFL time; when a step leaves a saturation outside
  * [-0.001, 1.001] (check_sat without clamp_sat) the pass restarts from the
  * initial saturation with twice the steps, at most 10 times, and then fails
- * with MHS_ERR_RANGE.  opt and report may be NULL. */
+ * with MHS_ERR_RANGE.  On a solver holding several instances the transport
+ * advances all of them together: one CFL time, the smallest over the batch,
+ * sets the small steps of every instance.  With several right-hand sides it
+ * uses the fluxes of the first.  opt and report may be NULL. */
 int mhs_transport_solve(mhs_transport *t, double *saturation, double time, const double *gravity,
                         const double *injection_rates, const mhs_transport_options *opt,
                         mhs_transport_report *report);

@@ -318,3 +318,46 @@ def test_transport_handle_lifetime_and_stale_structure(pkg):
     L.mhs_destroy(s._h)
     s._h = None
     assert L.mhs_transport_destroy(h) == 0
+
+
+def test_transport_on_a_batch_with_two_rhs(pkg):
+    """A handle holding two uneven instances, solved with two right-hand
+    sides: the transport runs on every cell of the batch with the fluxes of
+    right-hand side 0.  One CFL time (the smallest over the batch) sets the
+    small steps of every instance, so the batch equals the oracle's transport
+    on the concatenated grid, and an instance with that CFL time equals the
+    oracle on its own grid."""
+    gs = [pkg.grids.cartesian(8, 6, 3, 2.0, 2.0, 1.0), pkg.grids.cartesian(5, 7, 4, 2.0, 2.0, 1.0)]
+    G, icp = pkg.grids.concat(gs)
+    nc, nhf = G.num_cells, G.num_half_faces
+    perm = pkg.fields.perm_c1(nc, 41)
+    bt, bv = cases.xdrop_bc(G, 2 * BAR, 0.0)
+    bv2 = np.stack([bv, 0.5 * bv])
+    rng = np.random.default_rng(41)
+    sat = rng.uniform(0.3, 0.7, nc)
+    sat_bc = np.where(G.side == 0, 1.0, 0.0)
+    poro = np.full(nc, 0.2)
+    s = pkg.HybridSolver(0)
+    s.init(G, perm, GRAV, bt, None, inst_cellpos=icp)
+    tm, rm = _fluid(sat)
+    s.solve(tm, rm, np.zeros((2, nc)), bt, bv2, residual_tolerance=1e-12)
+    flux = np.asarray(s.getSolution()[2])[0]
+    t = pkg.EulerUpstream(s, poro, sat_bc, VISC, RHO)
+    t.init({"method_capillary": False})
+    T = 2.0 * tr.cfl_time(G, perm, poro, flux, GRAV, DRHO, VISC, use_capillary=False)
+    s_gpu = sat.copy()
+    t.transportSolve(s_gpu, T, GRAV)
+    whole, n, _, _ = tr.transport_solve(G, perm, poro, sat, T, GRAV, flux, DRHO, VISC, None, sat_bc,
+                                        method_capillary=False)
+    assert t.report.small_steps == n
+    np.testing.assert_allclose(s_gpu, whole, rtol=0, atol=1e-12)
+    same = 0
+    for i, g in enumerate(gs):
+        c0, c1 = icp[i], icp[i + 1]
+        h0, h1 = G.facepos[c0], G.facepos[c1]
+        ref, n_i, _, _ = tr.transport_solve(g, perm[c0:c1], poro[c0:c1], sat[c0:c1], T, GRAV, flux[h0:h1], DRHO,
+                                            VISC, None, sat_bc[h0:h1], method_capillary=False)
+        if n_i == n:
+            np.testing.assert_allclose(s_gpu[c0:c1], ref, rtol=0, atol=1e-12)
+            same += 1
+    assert same >= 1
