@@ -567,9 +567,9 @@ __global__ __launch_bounds__(TB) void k_fill(DevTopo T, DevCellSys C,
 // the row's FP64 residual replaces the FP32 recursive one, and the partials
 // carry |r64|^2, r.M^-1 r and |r|^2 of the replaced residual.
 template <typename MT, typename VT, int NRHS, bool REL, int TB, int CH>
-__global__ __launch_bounds__(TB) void k_spmv_pq(DevTopo T, const MT *__restrict__ val,
-                                                const VT *__restrict__ p, VT *__restrict__ q,
-                                                DevRed R, DevScal S, RelArgs L) {
+__device__ __forceinline__ void spmv_pq_body(const DevTopo &T, const MT *__restrict__ val,
+                                             const VT *__restrict__ p, VT *__restrict__ q, const DevRed &R,
+                                             const DevScal &S, const RelArgs &L) {
     const int b = T.spmv_xcd ? xcd_block(gridDim.x) : (int)blockIdx.x;
     const int s = T.block_seg[b];
     int act[NRHS];
@@ -635,6 +635,32 @@ __global__ __launch_bounds__(TB) void k_spmv_pq(DevTopo T, const MT *__restrict_
     } else {
         block_partials<NQ, TB / 64>(part, b, R);
     }
+}
+
+template <typename MT, typename VT, int NRHS, bool REL, int TB, int CH>
+__global__ __launch_bounds__(TB) void k_spmv_pq(DevTopo T, const MT *__restrict__ val, const VT *__restrict__ p,
+                                                VT *__restrict__ q, DevRed R, DevScal S, RelArgs L) {
+    spmv_pq_body<MT, VT, NRHS, REL, TB, CH>(T, val, p, q, R, S, L);
+}
+
+// The same held to 8 waves per SIMD: the FP32 single-system kernel of the
+// mixed-precision PCG fits in 64 VGPRs this way (70 otherwise, for the
+// FP64 replacement loop that most iterations skip), without spills.
+template <typename MT, typename VT, int NRHS, bool REL, int TB, int CH>
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_spmv_pq8(
+    DevTopo T, const MT *__restrict__ val, const VT *__restrict__ p, VT *__restrict__ q, DevRed R, DevScal S,
+    RelArgs L) {
+    spmv_pq_body<MT, VT, NRHS, REL, TB, CH>(T, val, p, q, R, S, L);
+}
+
+// FP32 single-system instantiations launch k_spmv_pq8, the others k_spmv_pq.
+template <typename MT, typename VT, int NRHS, bool REL, int TB, int CH>
+static void launch_spmv_kernel(hipStream_t st, unsigned nblk, const DevTopo &T, const MT *val, const VT *p, VT *q,
+                               const DevRed &R, const DevScal &S, const RelArgs &L) {
+    if constexpr (std::is_same<MT, float>::value && NRHS == 1)
+        k_spmv_pq8<MT, VT, NRHS, REL, TB, CH><<<nblk, TB, 0, st>>>(T, val, p, q, R, S, L);
+    else
+        k_spmv_pq<MT, VT, NRHS, REL, TB, CH><<<nblk, TB, 0, st>>>(T, val, p, q, R, S, L);
 }
 
 template <typename VT, int NRHS>
@@ -2474,7 +2500,7 @@ hipError_t launch_pcg_iteration(hipStream_t st, const DevTopo &T, int nrhs, cons
         with_tb(T.spmv_tb, [&](auto tb) {
             constexpr int TBv = decltype(tb)::value;
             with_chunk(T.max_w, [&](auto ch) {
-                k_spmv_pq<MT, VT, NR, REL, TBv, decltype(ch)::value><<<T.nblk, TBv, 0, st>>>(T, val, p, q, R, S, L);
+                launch_spmv_kernel<MT, VT, NR, REL, TBv, decltype(ch)::value>(st, T.nblk, T, val, p, q, R, S, L);
             });
         });
         k_finish<REL ? 4 * NR : NR, FinPQ<NR, REL>><<<T.ninst, 256, 0, st>>>(R, S, A);
@@ -2504,7 +2530,8 @@ hipError_t launch_spmv(hipStream_t st, const DevTopo &T, int nrhs, const MT *val
         with_tb(T.spmv_tb, [&](auto tb) {
             constexpr int TBv = decltype(tb)::value;
             with_chunk(T.max_w, [&](auto ch) {
-                k_spmv_pq<MT, VT, NR, false, TBv, decltype(ch)::value><<<T.nblk, TBv, 0, st>>>(T, val, p, q, R, S, RelArgs{});
+                launch_spmv_kernel<MT, VT, NR, false, TBv, decltype(ch)::value>(st, T.nblk, T, val, p, q, R, S,
+                                                                              RelArgs{});
             });
         });
     });
@@ -2860,7 +2887,7 @@ hipError_t launch_pcg_iteration_amg(hipStream_t st, const DevTopo &T, const DevA
         with_tb(T.spmv_tb, [&](auto tb) {
             constexpr int TBv = decltype(tb)::value;
             with_chunk(T.max_w, [&](auto ch) {
-                k_spmv_pq<MT, VT, NR, REL, TBv, decltype(ch)::value><<<T.nblk, TBv, 0, st>>>(T, val, p, q, R, S, L);
+                launch_spmv_kernel<MT, VT, NR, REL, TBv, decltype(ch)::value>(st, T.nblk, T, val, p, q, R, S, L);
             });
         });
         k_finish<REL ? 4 * NR : NR, FinPQ<NR, REL>><<<T.ninst, 256, 0, st>>>(R, S, Af);
