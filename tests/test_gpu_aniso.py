@@ -85,3 +85,34 @@ def test_isotropic_tensors_equal_scalar_solve_and_device_path(pkg):
     np.testing.assert_array_equal(s.getSolution()[2], v1)
     with pytest.raises(pkg.MhsError, match="tensor-mobility"):
         s.adjoint(np.ones(g.num_half_faces))
+
+
+def test_tensor_mobility_batch_two_rhs_and_cell_velocity(pkg, orc):
+    """Two uneven instances and two right-hand sides in one tensor-mobility
+    solve: every (instance, right-hand side) as the oracle on its own grid,
+    and estimateCellVelocity of the second right-hand side as the oracle's."""
+    gs = [pkg.grids.corner_point(7, 6, 4, 3.0, 3.0, 1.0, seed=11), pkg.grids.cartesian(6, 5, 5, 2.0, 2.0, 1.0)]
+    G, icp = pkg.grids.concat(gs)
+    nc = G.num_cells
+    perm = pkg.fields.perm_c4(nc, 11)
+    rng = np.random.default_rng(11)
+    lam = _commuting(perm, rng.uniform(200.0, 600.0, nc), rng.uniform(100.0, 800.0, nc))
+    rl = _commuting(perm, rng.uniform(2e5, 6e5, nc), rng.uniform(1e5, 8e5, nc))
+    bt, bv = cases.xdrop_bc(G, 3e5, 1e5)
+    bv2 = np.stack([bv, 0.3 * bv])
+    s = pkg.HybridSolver(0)
+    s.init(G, perm, GRAV, bt, None, inst_cellpos=icp)
+    s.solve_aniso(lam, rl, np.zeros((2, nc)), bt, bv2, residual_tolerance=1e-12, precision="mixed")
+    p, pi, v = s.getSolution()
+    vel = s.estimateCellVelocity(1)
+    for t, g in enumerate(gs):
+        c0, c1 = icp[t], icp[t + 1]
+        h0, h1 = G.facepos[c0], G.facepos[c1]
+        for k in range(2):
+            ref = orc.solve_aniso(g, perm[c0:c1], lam[c0:c1], rl[c0:c1], np.zeros(c1 - c0), bt[h0:h1],
+                                  bv2[k, h0:h1], grav=GRAV, tol=1e-14)
+            assert cases.rel_err(v[k, h0:h1], ref.outflux) < 1e-8, (t, k)
+            assert cases.rel_err(p[k, c0:c1], ref.cell_pressure) < 1e-8, (t, k)
+            if k == 1:
+                ve = orc.estimate_cell_velocity(g, v[1, h0:h1])
+                assert cases.rel_err(vel[c0:c1], ve) < 1e-13
