@@ -417,12 +417,20 @@ void orc_regularize(const int *row_ptr, const int *col, double *val)
 }
 
 /* ------------------------------------------------------------------------ */
+/* Host threads of the CG's SpMV, dot products and vector updates (OpenMP);
+ * 1 (the default) keeps the single-threaded order of every sum. */
+static int orc_threads = 1;
+
+void orc_set_threads(int n) { orc_threads = n > 1 ? n : 1; }
+
 static void spmv(int n, const int *rp, const int *ci, const double *v,
                  const double *x, double *y)
 {
-    int i, k;
+    int i;
+#pragma omp parallel for schedule(static) num_threads(orc_threads) if (orc_threads > 1)
     for (i = 0; i < n; ++i) {
         double s = 0.0;
+        int k;
         for (k = rp[i]; k < rp[i + 1]; ++k) s += v[k] * x[ci[k]];
         y[i] = s;
     }
@@ -432,6 +440,7 @@ static double dot(int n, const double *a, const double *b)
 {
     double s = 0.0;
     int i;
+#pragma omp parallel for schedule(static) reduction(+ : s) num_threads(orc_threads) if (orc_threads > 1)
     for (i = 0; i < n; ++i) s += a[i] * b[i];
     return s;
 }
@@ -542,14 +551,18 @@ int orc_cg(int n, const int *rp, const int *ci, const double *a,
         alpha = dot(n, p, q);
         if (!(alpha > 0.0)) { rc = -1; break; }
         lambda = rholast / alpha;
+#pragma omp parallel for schedule(static) num_threads(orc_threads) if (orc_threads > 1)
         for (i = 0; i < n; ++i) { x[i] += lambda * p[i]; r[i] -= lambda * q[i]; }
         def = sqrt(dot(n, r, r));
         if (def / def0 < tol || def < 1e-30) { rc = 0; break; }
         if (precond == 2) ilu0_apply(n, rp, ci, lu, diag, r, z);
-        else if (precond == 1) for (i = 0; i < n; ++i) z[i] = dinv[i] * r[i];
-        else memcpy(z, r, sizeof(double) * (size_t)n);
+        else if (precond == 1) {
+#pragma omp parallel for schedule(static) num_threads(orc_threads) if (orc_threads > 1)
+            for (i = 0; i < n; ++i) z[i] = dinv[i] * r[i];
+        } else memcpy(z, r, sizeof(double) * (size_t)n);
         rho = dot(n, z, r);
         beta = rho / rholast;
+#pragma omp parallel for schedule(static) num_threads(orc_threads) if (orc_threads > 1)
         for (i = 0; i < n; ++i) p[i] = beta * p[i] + z[i];
         rholast = rho;
     }

@@ -50,8 +50,16 @@ def lib():
         L.orc_post_process.argtypes = [ctypes.c_int, _i, _i, _i, ctypes.c_int, _i, ctypes.c_int,
                                        _d]
         L.orc_post_process.restype = ctypes.c_double
+        L.orc_set_threads.argtypes = [ctypes.c_int]
         _LIB = L
     return _LIB
+
+
+def set_threads(n: int) -> None:
+    """Host threads of the CG's SpMV, dot products and vector updates.  1 (the
+    default) sums in the single-threaded order every pinned result uses; more
+    threads reorder the dot-product sums (results then differ in the last bits)."""
+    lib().orc_set_threads(int(n))
 
 
 def _pi(a):

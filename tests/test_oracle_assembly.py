@@ -194,6 +194,24 @@ def test_cg_matches_dense_and_preconditioners_agree(pkg, orc):
         assert r.reduction < 1e-14
 
 
+def test_threaded_cg_agrees_and_one_thread_is_unchanged(pkg, orc):
+    """The OpenMP CG (multi-core CPU figure) solves the same system; with one
+    thread it is bit-identical to the single-threaded summation order."""
+    g, perm, tm, rm, src, bt, bv = cases.case_c1()
+    base = [orc.solve(g, perm, tm, rm, src, bt, bv, tol=1e-14, precond=pc) for pc in (0, 1)]
+    orc.set_threads(4)
+    try:
+        thr = [orc.solve(g, perm, tm, rm, src, bt, bv, tol=1e-14, precond=pc) for pc in (0, 1)]
+    finally:
+        orc.set_threads(1)
+    again = [orc.solve(g, perm, tm, rm, src, bt, bv, tol=1e-14, precond=pc) for pc in (0, 1)]
+    for b, t, a in zip(base, thr, again):
+        assert cases.rel_err(t.face_pressure, b.face_pressure) < 1e-11
+        assert abs(t.iterations - b.iterations) <= 2
+        np.testing.assert_array_equal(a.face_pressure, b.face_pressure)
+        assert a.iterations == b.iterations
+
+
 def test_enumeration_rules(pkg, orc):
     g = pkg.grids.cartesian(3, 2, 2)
     hf, nint, nf, mx = orc.enumerate_dof(g)
