@@ -138,6 +138,27 @@ def _cpu_baseline(pkg, g, perm, bt, bv, iters_needed, sample_iters):
 
 
 
+def _copy_gbps(dev, mib=1024, reps=20):
+    """Read + write bandwidth of a plain device-to-device copy of `mib` MiB
+    (torch's copy kernel, timed with events on torch's current stream): the
+    streaming rate HBM reaches in practice, beside the 8 TB/s peak."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    return 2.0 * n * 4 / (ms * 1e-3) / 1e9
+
+
 def _spmv_roofline(s, nnz, nf, nrhs, tag):
     """Roofline entry of the PCG SpMV of the last solve (HIP-event timing)."""
     v = 4 if tag == "32" else 8
@@ -807,6 +828,9 @@ def main():
             "bound": "hbm", "achieved": round(ach, 1), "peak": 8000.0, "unit": "GB/s",
             "frac": round(ach / 8000.0, 4), "traffic": _measured_traffic(tag),
             "alg_bytes_per_launch": B["spmv" + tag], "avg_launch_us": round(kern_ms * 1e3, 2)}
+    copy_gbps = _copy_gbps(dev)
+    roof["device_copy_gbps"] = round(copy_gbps, 1)
+    roof["frac_of_device_copy"] = round(ach / copy_gbps, 4)
 
     cpu = None
     if not args.no_cpu_baseline and "fp64_jacobi" in cmp:
