@@ -126,7 +126,8 @@ struct mhs_solver {
     DevRed R{};
     DevScal S{};
     unsigned long long *maxbits = nullptr;
-    // pinned host mirrors of the small per-system state
+    // pinned host mirror of the small per-system state: one block laid out as
+    // the device block (sc, then act, iters, status), read back by one copy
     double *h_sc = nullptr;
     int *h_act = nullptr, *h_iters = nullptr, *h_upd = nullptr, *h_status = nullptr;
     size_t h_nsys = 0;
@@ -164,6 +165,10 @@ int fail(mhs_solver *s, int code, const std::string &msg) {
         if (e_ != hipSuccess)                                                          \
             return fail(s, MHS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
+
+// Bytes of the per-system state block: sc (SC_N doubles per system), then
+// act and iters (one int per system) and status (4 ints).
+size_t state_bytes(size_t nsys) { return nsys * SC_N * sizeof(double) + (2 * nsys + 4) * sizeof(int); }
 
 template <class T>
 T *dalloc(mhs_solver *s, std::vector<DevBuf> &list, size_t n, hipError_t *e) {
@@ -218,11 +223,8 @@ void mhs_solver::free_solve_buffers() {
     adj_pi = adj_b = adj_w = adj_y = adj_gK = adj_gtm = adj_grm = adj_gsrc = nullptr;
     cap_nrhs = 0;
     cap_mixed = false;
-    if (h_sc) hipHostFree(h_sc);
-    if (h_act) hipHostFree(h_act);
-    if (h_iters) hipHostFree(h_iters);
+    if (h_sc) hipHostFree(h_sc);  // also holds h_act, h_iters, h_status
     if (h_upd) hipHostFree(h_upd);
-    if (h_status) hipHostFree(h_status);
     h_sc = nullptr;
     h_act = h_iters = h_upd = h_status = nullptr;
     h_nsys = 0;
@@ -723,11 +725,17 @@ int ensure_solve_buffers(mhs_solver *s, int nrhs, bool mixed) {
     if (ok()) s->inst_reg = dalloc<int>(s, L, (size_t)T.ninst, &e);
     if (ok()) s->R.partials = dalloc<double>(s, L, (size_t)(nfp / kRowBlock) * 16, &e);
     if (ok()) s->R.counters = dalloc<int>(s, L, (size_t)T.ninst, &e);
-    if (ok()) s->S.sc = dalloc<double>(s, L, nsys * SC_N, &e);
-    if (ok()) s->S.act = dalloc<int>(s, L, nsys, &e);
-    if (ok()) s->S.iters = dalloc<int>(s, L, nsys, &e);
+    // sc, act, iters and status share one block so that the host reads the
+    // solver's progress with a single copy (read_state)
+    unsigned char *state = nullptr;
+    if (ok()) state = dalloc<unsigned char>(s, L, state_bytes(nsys), &e);
+    if (ok()) {
+        s->S.sc = reinterpret_cast<double *>(state);
+        s->S.act = reinterpret_cast<int *>(state + nsys * SC_N * sizeof(double));
+        s->S.iters = s->S.act + nsys;
+        s->S.status = s->S.iters + nsys;
+    }
     if (ok()) s->S.upd = dalloc<int>(s, L, nsys, &e);
-    if (ok()) s->S.status = dalloc<int>(s, L, 4, &e);
     if (ok()) s->maxbits = dalloc<unsigned long long>(s, L, 1, &e);
     s->z64 = nullptr;
     s->z32 = nullptr;
@@ -753,11 +761,14 @@ int ensure_solve_buffers(mhs_solver *s, int nrhs, bool mixed) {
     }
     if (ok()) e = hipMemset(s->R.counters, 0, sizeof(int) * T.ninst);
     if (ok()) e = hipMemset(s->x64, 0, nv * sizeof(double));
-    if (ok()) e = hipHostMalloc(reinterpret_cast<void **>(&s->h_sc), nsys * SC_N * sizeof(double), 0);
-    if (ok()) e = hipHostMalloc(reinterpret_cast<void **>(&s->h_act), nsys * sizeof(int), 0);
-    if (ok()) e = hipHostMalloc(reinterpret_cast<void **>(&s->h_iters), nsys * sizeof(int), 0);
+    if (ok()) e = hipHostMalloc(reinterpret_cast<void **>(&s->h_sc), state_bytes(nsys), 0);
+    if (ok()) {
+        s->h_act = reinterpret_cast<int *>(reinterpret_cast<unsigned char *>(s->h_sc) +
+                                           nsys * SC_N * sizeof(double));
+        s->h_iters = s->h_act + nsys;
+        s->h_status = s->h_iters + nsys;
+    }
     if (ok()) e = hipHostMalloc(reinterpret_cast<void **>(&s->h_upd), nsys * sizeof(int), 0);
-    if (ok()) e = hipHostMalloc(reinterpret_cast<void **>(&s->h_status), 4 * sizeof(int), 0);
     if (!ok()) {
         s->free_solve_buffers();
         return fail(s, MHS_ERR_ALLOC, std::string("solve buffers: ") + hipGetErrorString(e));
@@ -833,11 +844,12 @@ int run_chunk(mhs_solver *s, int nrhs, int chunk, bool use_graph, const MT *val,
 }
 
 int read_state(mhs_solver *s, size_t nsys, bool sc) {
-    HIPCHECK(hipMemcpyAsync(s->h_act, s->S.act, nsys * sizeof(int), hipMemcpyDeviceToHost, s->stream));
-    HIPCHECK(hipMemcpyAsync(s->h_iters, s->S.iters, nsys * sizeof(int), hipMemcpyDeviceToHost, s->stream));
-    HIPCHECK(hipMemcpyAsync(s->h_status, s->S.status, 4 * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    // act, iters and status follow sc in one block on both sides
     if (sc)
-        HIPCHECK(hipMemcpyAsync(s->h_sc, s->S.sc, nsys * SC_N * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+        HIPCHECK(hipMemcpyAsync(s->h_sc, s->S.sc, state_bytes(nsys), hipMemcpyDeviceToHost, s->stream));
+    else
+        HIPCHECK(hipMemcpyAsync(s->h_act, s->S.act, (2 * nsys + 4) * sizeof(int), hipMemcpyDeviceToHost,
+                                s->stream));
     HIPCHECK(hipStreamSynchronize(s->stream));
     return MHS_OK;
 }
