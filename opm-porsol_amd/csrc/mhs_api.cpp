@@ -843,13 +843,9 @@ int run_chunk(mhs_solver *s, int nrhs, int chunk, bool use_graph, const MT *val,
     return MHS_OK;
 }
 
-int read_state(mhs_solver *s, size_t nsys, bool sc) {
-    // act, iters and status follow sc in one block on both sides
-    if (sc)
-        HIPCHECK(hipMemcpyAsync(s->h_sc, s->S.sc, state_bytes(nsys), hipMemcpyDeviceToHost, s->stream));
-    else
-        HIPCHECK(hipMemcpyAsync(s->h_act, s->S.act, (2 * nsys + 4) * sizeof(int), hipMemcpyDeviceToHost,
-                                s->stream));
+int read_state(mhs_solver *s, size_t nsys) {
+    // sc, act, iters and status: one block on both sides, one copy
+    HIPCHECK(hipMemcpyAsync(s->h_sc, s->S.sc, state_bytes(nsys), hipMemcpyDeviceToHost, s->stream));
     HIPCHECK(hipStreamSynchronize(s->stream));
     return MHS_OK;
 }
@@ -1098,7 +1094,7 @@ int linear_solve(mhs_solver *s, const mhs_solve_options &opt, int nrhs, bool mix
             HIPCHECK(launch_amg_start<float>(s->stream, s->dT, s->dA, nrhs, s->val32, s->dinv32, s->r32,
                                              s->z32, s->p32, s->R, s->S, nullptr, s->x32));
     }
-    int rc = read_state(s, nsys, true);
+    int rc = read_state(s, nsys);
     if (rc) return rc;
     while (s->h_status[0] > 0) {
         int mx = 0;
@@ -1122,7 +1118,7 @@ int linear_solve(mhs_solver *s, const mhs_solve_options &opt, int nrhs, bool mix
             rc = run_chunk<float, float>(s, nrhs, n, graph, s->val32, s->dinv32, s->x32, s->r32, s->p32,
                                          s->q32, s->z32, amg, MHS_MIXED, tol2);
         if (rc) return rc;
-        rc = read_state(s, nsys, true);
+        rc = read_state(s, nsys);
         if (rc) return rc;
         if (opt.verbosity >= 2)
             for (size_t i = 0; i < std::min<size_t>(nsys, 4); ++i) {
@@ -1139,7 +1135,7 @@ int linear_solve(mhs_solver *s, const mhs_solve_options &opt, int nrhs, bool mix
     for (size_t i = 0; i < nsys; ++i) out.iterations = std::max(out.iterations, s->h_iters[i]);
     if (mixed) HIPCHECK(launch_merge_all(s->stream, s->dT, nrhs, s->x64, s->x32));
     HIPCHECK(launch_resnorm(s->stream, s->dT, nrhs, s->val64, s->b, s->x64, s->R, s->S));
-    rc = read_state(s, nsys, true);
+    rc = read_state(s, nsys);
     if (rc) return rc;
     out.replacements = s->h_status[2];
     double red = 0.0;
