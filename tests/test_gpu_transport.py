@@ -100,6 +100,35 @@ def test_transport_solve_matches_oracle(pkg):
     assert np.max(np.abs(s_gpu - sat)) > 1e-3
 
 
+def test_transport_solve_corner_point_matches_oracle(pkg):
+    """A whole transport solve on a faulted corner-point grid (cells with up to
+    11 faces, rotated full tensors, gravity, a well pair): CFL time, step and
+    retry counts, and saturations against the oracle on the same fluxes."""
+    g = pkg.grids.corner_point(10, 8, 5, 3.0, 3.0, 1.0, seed=12, fault_frac=0.3)
+    nc, nhf = g.num_cells, g.num_half_faces
+    assert g.ncf().max() > 6
+    rng = np.random.default_rng(21)
+    perm = pkg.fields.perm_c4(nc, 12)
+    bt, bv = cases.xdrop_bc(g, 3 * BAR, BAR)
+    sat = rng.uniform(0.3, 0.7, nc)
+    sat_bc = rng.uniform(0.0, 1.0, nhf)
+    poro = rng.uniform(0.1, 0.3, nc)
+    inj = np.zeros(nc)
+    inj[3], inj[nc - 5] = 3e-8, -2e-8
+    s, flux = _flow(pkg, g, perm, bt, bv, sat)
+    t = pkg.EulerUpstream(s, poro, sat_bc, VISC, RHO)
+    cfl = tr.cfl_time(g, perm, poro, flux, GRAV, DRHO, VISC)
+    s_orc, n_orc, rep_orc, cfl_orc = tr.transport_solve(g, perm, poro, sat, 12.5 * cfl, GRAV, flux, DRHO, VISC,
+                                                        inj, sat_bc)
+    s_gpu = sat.copy()
+    t.transportSolve(s_gpu, 12.5 * cfl, GRAV, inj)
+    r = t.report
+    assert r.cfl_dt == pytest.approx(cfl_orc, rel=1e-13)
+    assert (r.small_steps, r.repeats) == (n_orc, rep_orc)
+    np.testing.assert_allclose(s_gpu, s_orc, rtol=0, atol=1e-12)
+    assert np.max(np.abs(s_gpu - sat)) > 1e-2
+
+
 def test_periodic_transport(pkg):
     """Periodic in x and y with a pressure drop along x: the saturation
     crosses the periodic faces through the partner cells."""
