@@ -83,6 +83,54 @@ def test_adjoint_matches_oracle_finite_differences(pkg, orc):
         assert np.max(np.abs(gr[key] - fd)) <= 2e-6 * np.abs(fd).max(), (key, np.max(np.abs(gr[key] - fd)))
 
 
+def test_adjoint_corner_point_matches_oracle_finite_differences(pkg, orc):
+    """The same gradients on a faulted corner-point grid (cells with up to 9
+    faces, non-orthogonal faces), against central differences of the oracle."""
+    g = pkg.grids.corner_point(4, 3, 3, 1.0, 1.2, 0.6, seed=8, fault_frac=0.4)
+    assert g.ncf().max() > 6
+    nc, nhf = g.num_cells, g.num_half_faces
+    rng = np.random.default_rng(4)
+    M = 0.3 * rng.standard_normal((nc, 3, 3))
+    K = (np.einsum("nij,nkj->nik", M, M) + np.exp(rng.normal(0.0, 0.5, (nc, 1, 1))) * np.eye(3)).reshape(nc, 9)
+    tm, rm, src = rng.uniform(0.5, 2.0, nc), rng.uniform(0.2, 1.5, nc), rng.normal(0.0, 0.1, nc)
+    grav = (0.3, -0.2, 1.5)
+    bt = np.zeros(nhf, np.int32)
+    bv = np.zeros(nhf)
+    bt[g.side == 0] = 1
+    bv[g.side == 0] = rng.uniform(0.5, 1.5, int(np.sum(g.side == 0)))
+    bv[g.side == 3] = 0.05
+    c, d = rng.uniform(0.5, 1.5, nhf), rng.normal(0.0, 1.0, nhf)
+    s = pkg.HybridSolver(0)
+    s.init(g, K, grav, bt, None)
+    s.solve(tm, rm, src, bt, bv, residual_tolerance=1e-13, precision="fp64")
+    gr = s.adjoint(2.0 * c * s.getSolution()[2] + d, tol=1e-13)
+
+    def J(K_=K, tm_=tm, rm_=rm, src_=src):
+        return _objective(orc.solve(g, K_, tm_, rm_, src_, bt, bv, grav=grav, tol=1e-15).outflux, c, d)
+
+    rel = 1e-6
+    fdK = np.zeros((nc, 9))
+    for cell in range(nc):
+        for (l, m) in ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)):
+            h = rel * np.abs(K[cell]).max()
+            Kp, Km = K.copy(), K.copy()
+            for (a, b) in {(l, m), (m, l)}:
+                Kp[cell, 3 * a + b] += h
+                Km[cell, 3 * a + b] -= h
+            dj = (J(K_=Kp) - J(K_=Km)) / (2 * h)
+            fdK[cell, 3 * l + m] = fdK[cell, 3 * m + l] = dj if l == m else dj / 2
+    assert np.max(np.abs(gr["grad_perm"] - fdK)) <= 2e-6 * np.abs(fdK).max()
+    for key, name, base in (("grad_totmob", "tm_", tm), ("grad_rhomob", "rm_", rm), ("grad_src", "src_", src)):
+        fd = np.zeros(nc)
+        for cell in range(nc):
+            h = rel * max(abs(base[cell]), 1.0)
+            p, m = base.copy(), base.copy()
+            p[cell] += h
+            m[cell] -= h
+            fd[cell] = (J(**{name: p}) - J(**{name: m})) / (2 * h)
+        assert np.max(np.abs(gr[key] - fd)) <= 2e-6 * np.abs(fd).max(), key
+
+
 def _c5_like(pkg, ninst=2, dims=(12, 10, 6), seed=11):
     grids, perms = [], []
     for t in range(ninst):
