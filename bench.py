@@ -45,7 +45,7 @@ def _problem(pkg, seed):
     return g, perm, bt, bv
 
 
-def _alg_bytes(g, nnz, nf, mg_rows=0, mg_nnz=0):
+def _alg_bytes(g, nnz, nf, mg_rows=0, mg_nnz=0, ap_nnz=0):
     """Algorithmic HBM bytes per unit of work for one right-hand side: each
     array counted once per kernel at its stored precision, 4-byte indices,
     gathered vectors counted once.  v is the value size of the PCG (4 in
@@ -60,11 +60,13 @@ def _alg_bytes(g, nnz, nf, mg_rows=0, mg_nnz=0):
         out["spmv" + tag] = mat + 2 * nf * v
         # Jacobi-PCG iteration: SpMV + x,r,p,q,dinv reads and x,r,p writes
         out["iter" + tag] = mat + 4 * (nf + 1) + 13 * nf * v
-        # multigrid-PCG iteration: three level-0 operator passes (SpMV,
-        # pre-smoothed residual, post-smoothing sweep), 23 level-0 vector
-        # passes, aggregate and child maps, and on the coarse levels two
-        # operator passes and ten vector passes per row
-        out["amg_iter" + tag] = (3 * mat + 23 * nf * v + 8 * nf
+        # multigrid-PCG iteration: two passes over the face matrix (SpMV,
+        # pre-smoothed residual), one over A P with the level-1 aggregates
+        # (post-smoothing sweep), 22 level-0 vector passes (SpMV 2, x/r update
+        # with the pre-smoothing 8, residual 3, restriction 1, post-smoothing
+        # 5, p update 3), the aggregate map twice, and on the coarse levels
+        # two operator passes and ten vector passes per row
+        out["amg_iter" + tag] = (2 * mat + ap_nnz * (v + 4) + 22 * nf * v + 8 * nf
                                  + 2 * mg_nnz * (v + 4) + mg_rows * (12 + 10 * v))
     # FP64 residual replacement: FP64 operator + x, b read, r written
     out["refine64"] = nnz * (8 + 4) + 4 * nf * 8
@@ -136,6 +138,89 @@ def _cpu_baseline(pkg, g, perm, bt, bv, iters_needed, sample_iters):
                                     f"{threads} threads, single-core assembly and recovery")}
     return out
 
+
+
+def _c2_fixture():
+    """The oracle's solution of the headline case (tests/golden/c2_128x128x64.npz,
+    written by tests/golden/make_golden.py with the oracle's ILU0-CG to 1e-14)."""
+    return np.load(os.path.join(ROOT, "tests", "golden", "c2_128x128x64.npz"))
+
+
+def _err_vs_oracle(g, perm, fx, p, pi, v):
+    """Max-norm relative error of the GPU solution against the oracle's at the
+    fixture's 32768 seeded sample positions per field (the same positions the
+    parity test checks)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from cases import c2_sample_index
+    import opm_porsol_amd
+    if str(fx["perm_checksum"]) != opm_porsol_amd.fields.checksum(perm):
+        return None
+    out = {}
+    for k, (name, a) in enumerate((("face_pressure", pi), ("cell_pressure", p), ("outflux", v))):
+        idx = c2_sample_index(a.shape[0], k)
+        out[name] = float(np.max(np.abs(a[idx] - fx[name + "_sample"])) / fx[name + "_norms"][0])
+    return out
+
+
+def _cpu_baseline_ilu0(pkg, g, perm, bt, bv, fx, sample_iters):
+    """The oracle (oracle/oracle.c, FP64) with ILU0-CG, the reference's
+    linsolver_type 0 (SeqILU0 + CGSolver, IncompFlowSolverHybrid.hpp:1387-1420),
+    on the host: init (dof numbering, static inner products), full assembly,
+    the ILU0 factorisation, `sample_iters` CG iterations and full recovery,
+    timed; the iterations extrapolated to the count the oracle's complete
+    ILU0-CG run needed to reach 1e-10 (recorded in the fixture).  One core,
+    then the CG's SpMV, dot products and vector updates on all host cores
+    (OpenMP; the ILU0 triangular solves stay sequential)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    nc = g.num_cells
+    tm, rm, src = np.ones(nc), np.zeros(nc), np.zeros(nc)
+    its_needed = int(fx["ilu0_iterations_1e10"])
+    t0 = time.perf_counter()
+    hf, nint, nf, _ = oracle.enumerate_dof(g)
+    pp, _ = oracle.periodic_dof(g, hf, nf, bt, np.full(g.num_half_faces, -1, np.int32))
+    binv, pos, gfl = oracle.static_ip(g, perm)
+    t1 = time.perf_counter()
+    S = oracle.assemble(g, hf, nf, pp, binv, gfl, tm, rm, src, bt, bv)
+    t2 = time.perf_counter()
+
+    def cg_time(n):
+        t = time.perf_counter()
+        x, rc, its, red = oracle.cg(S, 1e-30, maxit=n, precond=2)
+        return time.perf_counter() - t, its, x
+
+    t_one, _, _ = cg_time(1)                      # factorisation + one iteration
+    t_n, its, x = cg_time(1 + sample_iters)
+    per_it = (t_n - t_one) / max(its - 1, 1)
+    t_fact = max(t_one - per_it, 0.0)
+    t3 = time.perf_counter()
+    oracle.recover(g, hf, binv, gfl, tm, rm, S, x)
+    t_rec = time.perf_counter() - t3
+    t_solve = (t2 - t1) + t_fact + its_needed * per_it + t_rec
+    out = {
+        "value": 1.0 / t_solve, "unit": "solves/s", "cores": 1, "kind": "port",
+        "sample": (f"oracle/oracle.c FP64 ILU0-CG (reference linsolver_type 0) on 1 host core "
+                   f"({platform.processor() or 'cpu'}, {os.cpu_count()} visible): assembly {t2 - t1:.2f} s, "
+                   f"ILU0 factorisation {t_fact:.2f} s, {its - 1} CG iterations at {per_it * 1e3:.1f} ms "
+                   f"extrapolated to the {its_needed} the oracle's full ILU0-CG run needs to reach 1e-10 "
+                   f"(tests/golden/c2_128x128x64.npz), recovery {t_rec:.2f} s"),
+        "seconds_per_solve": t_solve, "init_s": round(t1 - t0, 3), "iterations": its_needed,
+    }
+    threads = min(int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1), 16)
+    if threads > 1:
+        oracle.set_threads(threads)
+        try:
+            t_one_mt, _, _ = cg_time(1)
+            t_n_mt, its_mt, _ = cg_time(1 + sample_iters)
+        finally:
+            oracle.set_threads(1)
+        per_it_mt = (t_n_mt - t_one_mt) / max(its_mt - 1, 1)
+        t_mt = (t2 - t1) + t_fact + its_needed * per_it_mt + t_rec
+        out["openmp"] = {"value": 1.0 / t_mt, "unit": "solves/s", "cores": threads,
+                         "sample": (f"{its_mt - 1} ILU0-CG iterations at {per_it_mt * 1e3:.1f} ms with the SpMV, "
+                                    f"dot products and updates on {threads} threads, the triangular solves, "
+                                    f"assembly and recovery on one core")}
+    return out
 
 
 def _copy_gbps(dev, mib=1024, reps=20):
@@ -806,7 +891,7 @@ def main():
     cell = g.hf_cell()
     mass = float(np.max(np.abs(np.bincount(cell, weights=v, minlength=nc))) / np.max(np.abs(v)))
 
-    B = _alg_bytes(g, nnz, nf, rep["mg_rows"], rep["mg_nnz"])
+    B = _alg_bytes(g, nnz, nf, rep["mg_rows"], rep["mg_nnz"], rep["ap_nnz"])
     tag = "32" if args.precision == "mixed" else "64"
     it_key = ("amg_iter" if rep["precond"] == 1 else "iter") + tag
     n_ref = rep["replacements"]
@@ -832,10 +917,14 @@ def main():
     roof["device_copy_gbps"] = round(copy_gbps, 1)
     roof["frac_of_device_copy"] = round(ach / copy_gbps, 4)
 
+    fx = _c2_fixture() if args.seed == 2 else None
     cpu = None
-    if not args.no_cpu_baseline and "fp64_jacobi" in cmp:
-        cpu = _cpu_baseline(pkg, g, perm, bt, bv, cmp["fp64_jacobi"][1]["iterations"],
-                            args.cpu_sample_iters)
+    if not args.no_cpu_baseline:
+        if fx is not None:
+            cpu = _cpu_baseline_ilu0(pkg, g, perm, bt, bv, fx, args.cpu_sample_iters)
+        elif "fp64_jacobi" in cmp:
+            cpu = _cpu_baseline(pkg, g, perm, bt, bv, cmp["fp64_jacobi"][1]["iterations"],
+                                args.cpu_sample_iters)
 
     config = {
         "workload": "mimetic hybrid solve, 128x128x64 Cartesian, anisotropic K (kz/kx=0.1, "
@@ -850,6 +939,10 @@ def main():
         "launch_shape": {k: rep[k] for k in ("spmv_threads", "spmv_xcd", "sweep_threads", "sweep_xcd",
                                              "tail_rows")},
     }
+    if fx is not None:
+        config["err_vs_oracle"] = _err_vs_oracle(g, perm, fx, p, pi, v)
+        config["err_vs_oracle_note"] = ("max-norm relative error at 32768 seeded positions per field against "
+                                        "the oracle's ILU0-CG solution to 1e-14 (tests/golden/c2_128x128x64.npz)")
     for key, (t, r, sol) in cmp.items():
         p64, pi64, v64 = sol
         config[key + "_gpu"] = {
@@ -859,6 +952,8 @@ def main():
         if key == "fp64":
             config["err_vs_fp64_gpu"] = {"face_pressure": rel(pi, pi64), "cell_pressure": rel(p, p64),
                                          "outflux": rel(v, v64)}
+        if fx is not None:
+            config[key + "_gpu"]["err_vs_oracle"] = _err_vs_oracle(g, perm, fx, p64, pi64, v64)
 
     out = {
         "metric": "face-pressure solves/sec + HBM GB/s at 128x128x64, rel. residual 1e-10",

@@ -68,8 +68,14 @@ enum mhs_precision {
 typedef struct {
     double tol;                /* residual_tolerance (:662): ||r||/||r0|| bound */
     int maxit;                 /* linsolver_maxit (:666); 0 -> number of unknowns */
-    int linsolver_type;        /* (:664) 0: Jacobi-PCG (stands in for ILU0-CG);
-                                  1..3: the AMG family, served by the same PCG */
+    int linsolver_type;        /* (:664) 0: Jacobi-PCG, standing in for SeqILU0-CG
+                                  (:1387-1420); it needs more iterations than ILU0,
+                                  so a maxit tuned for ILU0 can end in
+                                  MHS_ERR_NOT_CONVERGED; 1 (AMG, :1481-1541), 2 (KAMG,
+                                  :1609-1667) and 3 (FastAMG, :1546-1605): the same
+                                  multigrid-preconditioned CG (Jacobi-PCG without
+                                  logical cell indices).  All converge to the
+                                  reference's discrete solution. */
     int verbosity;             /* linsolver_verbosity (:663) */
     int same_matrix;           /* (:665) accepted; the matrix is always rebuilt */
     double prolongate_factor;  /* (:667) multigrid coarse-correction factor (default 1.6) */
@@ -121,6 +127,8 @@ typedef struct {
     double smoother_weights[8];   /* multigrid Jacobi weight of levels 0..7 (0 beyond) */
     int coarsening[8][3];         /* logical block factors of the transfers from levels
                                      0..7 (1 or 2 along i, j, k; 0 beyond) */
+    long long ap_nnz;             /* stored entries of A P (face rows x level-1 aggregates)
+                                     read by the post-smoothing sweep; 0 without multigrid */
 } mhs_solve_report;
 
 typedef struct {

@@ -107,6 +107,8 @@ struct mhs_solver {
     std::vector<DevBuf> amg_bufs;       // structure of the hierarchy on the device
     bool amg_planned = false;           // aggregate shapes chosen from the permeability
     std::vector<double> rho;            // largest eigenvalue of D^-1 A per level (empty: not estimated)
+    long long rho_gen = -1;             // permeability generation the estimate was made for
+    bool rho_aniso = false;             // ... and whether that solve had tensor mobilities
     double *z64 = nullptr;
     float *z32 = nullptr;
 
@@ -1244,6 +1246,13 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
     auto t1 = std::chrono::steady_clock::now();
 
     // --- preconditioner ---------------------------------------------------------
+    // The smoother weights come from rho(D^-1 A) of an earlier solve of this
+    // hierarchy.  They are estimated again for a new permeability or inner
+    // product, and, should a multigrid solve whose estimate came from another
+    // solve fail (changed mobilities can raise rho past the 1.5 rho_est at
+    // which the weighted Jacobi smoother stops converging), once more for this
+    // system before the solve is repeated from its initial guess.
+    if (s->rho_gen != s->perm_gen || s->rho_aniso != (binv_use != nullptr)) s->rho.clear();
     bool amg = false;
     if (opt.precond == 1) {
         if (!s->has_amg)
@@ -1254,58 +1263,75 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
     } else if (opt.precond != 0) {
         return fail(s, MHS_ERR_INVALID, "solve: unknown preconditioner");
     }
-    if (amg) {
-        s->dA.alpha = opt.prolongate_factor;
-        s->dA.nu = std::max(1, opt.smooth_steps);
-        HIPCHECK(launch_amg_setup(s->stream, s->dT, s->dA, s->val64, mixed));
-        // Jacobi weights: the caller's, or 4 / (3 rho(D^-1 A)) per level with
-        // rho estimated on the first multigrid solve of this hierarchy
-        if (opt.smoother_weight > 0.0) {
-            for (int l = 0; l < kMaxLevels; ++l) s->dA.omega_l[l] = opt.smoother_weight;
-        } else {
-            if (s->rho.empty()) {
-                rc = estimate_rho(s);
-                if (rc) return rc;
-            }
-            for (int l = 0; l < kMaxLevels; ++l) {
-                const double r = l < (int)s->rho.size() ? s->rho[l] : 0.0;
-                s->dA.omega_l[l] = r > 0.0 ? std::min(1.0, std::max(0.2, 4.0 / (3.0 * r))) : 0.7;
-            }
-        }
-        // the kernels that walk the whole hierarchy read this copy (nothing of
-        // the previous solve is still running: it ended with a synchronise;
-        // the set-up kernels above do not read it)
-        HIPCHECK(hipMemcpy(const_cast<DevAmg *>(s->dA.dev), &s->dA, sizeof(DevAmg), hipMemcpyHostToDevice));
-    }
-
-    // --- launch shapes -------------------------------------------------------
+    bool rho_fresh = false;   // the weights were estimated on this very system
     double tune_ms = 0.0;
-    if (opt.autotune) {
-        rc = autotune(s, nrhs, mixed, amg, &tune_ms);
-        if (rc) return rc;
-    } else {
-        const Shape d;
-        s->dT.spmv_tb = d.spmv_tb; s->dT.spmv_xcd = d.spmv_xcd;
-        s->dT.sweep_tb = d.sweep_tb; s->dT.sweep_xcd = d.sweep_xcd;
-        s->dT.tail_rows = d.tail_rows;
-    }
-    auto valid_tb = [](int t) { return t == 256 || t == 512 || t == 1024; };
-    if (opt.spmv_threads) {
-        if (!valid_tb(opt.spmv_threads)) return fail(s, MHS_ERR_INVALID, "solve: spmv_threads must be 256, 512 or 1024");
-        s->dT.spmv_tb = opt.spmv_threads;
-        s->dT.spmv_xcd = 1;
-    }
-    if (opt.tail_rows >= 0) s->dT.tail_rows = opt.tail_rows;
-    if (opt.sweep_threads) {
-        if (!valid_tb(opt.sweep_threads)) return fail(s, MHS_ERR_INVALID, "solve: sweep_threads must be 256, 512 or 1024");
-        s->dT.sweep_tb = opt.sweep_threads;
-        s->dT.sweep_xcd = 1;
-    }
-
-    // --- linear solve ----------------------------------------------------------
     LinearResult lr;
-    rc = linear_solve(s, opt, nrhs, mixed, amg, lr);
-    if (rc) return rc;
+    for (int attempt = 0;; ++attempt) {
+        if (attempt > 0) {
+            // repeat with weights from this system: the assembled values and the
+            // initial guess are written again (the failed attempt changed x64)
+            s->rho.clear();
+            HIPCHECK(launch_fill(s->stream, s->dT, nrhs, s->C, s->ftype, d_bv, s->inst_reg, dguess, s->val64,
+                                 mixed ? s->val32 : nullptr, s->b, s->dinv64, s->dinv32, s->x64));
+        }
+        if (amg) {
+            s->dA.alpha = opt.prolongate_factor;
+            s->dA.nu = std::max(1, opt.smooth_steps);
+            HIPCHECK(launch_amg_setup(s->stream, s->dT, s->dA, s->val64, mixed));
+            // Jacobi weights: the caller's, or 4 / (3 rho(D^-1 A)) per level with
+            // rho estimated on the first multigrid solve of this hierarchy
+            if (opt.smoother_weight > 0.0) {
+                for (int l = 0; l < kMaxLevels; ++l) s->dA.omega_l[l] = opt.smoother_weight;
+            } else {
+                if (s->rho.empty()) {
+                    rc = estimate_rho(s);
+                    if (rc) return rc;
+                    s->rho_gen = s->perm_gen;
+                    s->rho_aniso = binv_use != nullptr;
+                    rho_fresh = true;
+                }
+                for (int l = 0; l < kMaxLevels; ++l) {
+                    const double r = l < (int)s->rho.size() ? s->rho[l] : 0.0;
+                    s->dA.omega_l[l] = r > 0.0 ? std::min(1.0, std::max(0.2, 4.0 / (3.0 * r))) : 0.7;
+                }
+            }
+            // the kernels that walk the whole hierarchy read this copy (nothing of
+            // the previous solve is still running: it ended with a synchronise;
+            // the set-up kernels above do not read it)
+            HIPCHECK(hipMemcpy(const_cast<DevAmg *>(s->dA.dev), &s->dA, sizeof(DevAmg), hipMemcpyHostToDevice));
+        }
+
+        // --- launch shapes -------------------------------------------------------
+        if (opt.autotune) {
+            rc = autotune(s, nrhs, mixed, amg, &tune_ms);
+            if (rc) return rc;
+        } else {
+            const Shape d;
+            s->dT.spmv_tb = d.spmv_tb; s->dT.spmv_xcd = d.spmv_xcd;
+            s->dT.sweep_tb = d.sweep_tb; s->dT.sweep_xcd = d.sweep_xcd;
+            s->dT.tail_rows = d.tail_rows;
+        }
+        auto valid_tb = [](int t) { return t == 256 || t == 512 || t == 1024; };
+        if (opt.spmv_threads) {
+            if (!valid_tb(opt.spmv_threads)) return fail(s, MHS_ERR_INVALID, "solve: spmv_threads must be 256, 512 or 1024");
+            s->dT.spmv_tb = opt.spmv_threads;
+            s->dT.spmv_xcd = 1;
+        }
+        if (opt.tail_rows >= 0) s->dT.tail_rows = opt.tail_rows;
+        if (opt.sweep_threads) {
+            if (!valid_tb(opt.sweep_threads)) return fail(s, MHS_ERR_INVALID, "solve: sweep_threads must be 256, 512 or 1024");
+            s->dT.sweep_tb = opt.sweep_threads;
+            s->dT.sweep_xcd = 1;
+        }
+
+        // --- linear solve ----------------------------------------------------------
+        lr = LinearResult{};
+        rc = linear_solve(s, opt, nrhs, mixed, amg, lr);
+        const bool failed = rc == MHS_ERR_BREAKDOWN || (rc == MHS_OK && !lr.converged);
+        if (failed && amg && !rho_fresh && opt.smoother_weight <= 0.0 && attempt == 0) continue;
+        if (rc) return rc;
+        break;
+    }
     const int total_iters = lr.iterations, replacements = lr.replacements;
     const bool converged = lr.converged;
     const double red = lr.reduction;
@@ -1352,6 +1378,7 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
         rep->sweep_xcd = s->dT.sweep_xcd;
         rep->tail_rows = s->dT.tail_rows;
         rep->tune_ms = tune_ms;
+        rep->ap_nnz = amg ? s->dA.ap_slots : 0;
     }
     if (!converged)
         return fail(s, MHS_ERR_NOT_CONVERGED,
@@ -1460,20 +1487,40 @@ int mhs_adjoint_device(mhs_solver *s, const double *dJ_dv, const mhs_solve_optio
     return adjoint_impl(s, dJ_dv, opt, grad_perm, grad_totmob, grad_rhomob, grad_src, report, true);
 }
 
-int mhs_sum_outflux(mhs_solver *s, int ngroups, const int *group_ptr, const int *group_hf, double *sums) {
+}  // extern "C"
+
+namespace {
+
+// Arguments of the group sums: *ne = entries of group_hf.  MHS_OK with
+// ngroups == 0 means nothing to do.
+int check_group_args(mhs_solver *s, const char *who, int ngroups, const int *group_ptr, const int *group_hf,
+                     const double *sums, size_t *ne) {
     if (!s) return MHS_ERR_INVALID;
-    if (!s->solved) return fail(s, MHS_ERR_STATE, "sum_outflux: no solution (call solve)");
+    *ne = 0;
+    if (!s->solved) return fail(s, MHS_ERR_STATE, std::string(who) + ": no solution (call solve)");
     if (ngroups < 0 || (ngroups > 0 && (!group_ptr || !group_hf || !sums)))
-        return fail(s, MHS_ERR_INVALID, "sum_outflux: bad arguments");
+        return fail(s, MHS_ERR_INVALID, std::string(who) + ": bad arguments");
+    if (ngroups == 0) return MHS_OK;
+    if (group_ptr[0] != 0) return fail(s, MHS_ERR_INVALID, std::string(who) + ": group_ptr[0] must be 0");
+    for (int g = 0; g < ngroups; ++g)
+        if (group_ptr[g + 1] < group_ptr[g])
+            return fail(s, MHS_ERR_INVALID, std::string(who) + ": group_ptr not monotone");
+    *ne = (size_t)group_ptr[ngroups];
+    for (size_t e = 0; e < *ne; ++e)
+        if (group_hf[e] < 0 || group_hf[e] >= s->T.nhf)
+            return fail(s, MHS_ERR_INVALID, std::string(who) + ": half-face index out of range");
+    return MHS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mhs_sum_outflux(mhs_solver *s, int ngroups, const int *group_ptr, const int *group_hf, double *sums) {
+    size_t ne = 0;
+    if (int rc = check_group_args(s, "sum_outflux", ngroups, group_ptr, group_hf, sums, &ne)) return rc;
     if (ngroups == 0) return MHS_OK;
     const Topology &T = s->T;
-    if (group_ptr[0] != 0) return fail(s, MHS_ERR_INVALID, "sum_outflux: group_ptr[0] must be 0");
-    for (int g = 0; g < ngroups; ++g)
-        if (group_ptr[g + 1] < group_ptr[g]) return fail(s, MHS_ERR_INVALID, "sum_outflux: group_ptr not monotone");
-    const size_t ne = (size_t)group_ptr[ngroups];
-    for (size_t e = 0; e < ne; ++e)
-        if (group_hf[e] < 0 || group_hf[e] >= T.nhf)
-            return fail(s, MHS_ERR_INVALID, "sum_outflux: half-face index out of range");
     HIPCHECK(hipSetDevice(s->device));
     const int nrhs = s->last_nrhs;
     int *dptr = nullptr, *dhf = nullptr;

@@ -4,9 +4,13 @@
 // face, source -Lu * volume.  Prints the L1 / L2 / Linf error densities in the
 // reference's format (examples/timing_results.txt records them for N = 50, 60).
 //   known_answer_test nx=60 tolerance=1e-8 linear_solver_type=1 precision=0
+// print_system=<prefix> writes the assembled system after the solve
+// (IncompFlowSolverHybrid::printSystem: <prefix>-mat.dat, <prefix>-rhs.dat);
+// print_stats=1 prints IncompFlowSolverHybrid::printStats to stdout.
 #include <chrono>
 #include <cmath>
 #include <iostream>
+#include <string>
 #include <vector>
 
 #include <opm/porsol/common/BoundaryConditions.hpp>
@@ -84,6 +88,10 @@ int main(int argc, char **argv) try {
                   << (r.precond ? "multigrid" : "Jacobi") << ", levels: " << r.levels
                   << ", reduction: " << r.reduction << std::endl;
     }
+
+    if (param.geti("print_stats", 0)) solver.printStats(std::cout);
+    const std::string prefix = param.gets("print_system", "");
+    if (!prefix.empty()) solver.printSystem(prefix);
 
     std::vector<double> cell_pressure;
     Opm::getCellPressure(cell_pressure, g, solver.getSolution());

@@ -2,9 +2,11 @@
 // periodic in x and y with a 1 bar drop along x, no-flow in z, unique boundary
 // ids, unit permeability.  Prints the net flux through the x- side and the
 // largest twin-flux change of postProcessFluxes().
-//   mimetic_periodic_test nx=8 ny=8 nz=4
+//   mimetic_periodic_test nx=8 ny=8 nz=4 linear_solver_type=1
+// print_system=<prefix> and print_stats=1 as in known_answer_test.
 #include <cmath>
 #include <iostream>
+#include <string>
 #include <vector>
 
 #include <opm/porsol/common/BoundaryConditions.hpp>
@@ -39,7 +41,10 @@ int main(int argc, char **argv) try {
     Vec3 gravity(0.0);
     solver.init(g, r, gravity, fbc);
     std::vector<double> src(g.numberOfCells(), 0.0), sat(g.numberOfCells(), 0.0);
-    solver.solve(r, sat, fbc, src, param.get("tolerance", 1e-12), 0);
+    solver.solve(r, sat, fbc, src, param.get("tolerance", 1e-12), 0, param.geti("linear_solver_type", 1));
+    if (param.geti("print_stats", 0)) solver.printStats(std::cout);
+    const std::string prefix = param.gets("print_system", "");
+    if (!prefix.empty()) solver.printSystem(prefix);
     const auto &soln = solver.getSolution();
     double qx = 0.0;
     for (GI::CellIterator c = g.cellbegin(); c != g.cellend(); ++c)

@@ -59,7 +59,7 @@ class Report(ctypes.Structure):
                 ("spmv_xcd", ctypes.c_int), ("sweep_threads", ctypes.c_int),
                 ("sweep_xcd", ctypes.c_int), ("tail_rows", ctypes.c_int),
                 ("tune_ms", ctypes.c_double), ("smoother_weights", ctypes.c_double * 8),
-                ("coarsening", (ctypes.c_int * 3) * 8)]
+                ("coarsening", (ctypes.c_int * 3) * 8), ("ap_nnz", ctypes.c_longlong)]
 
     def as_dict(self):
         out = {}

@@ -55,3 +55,39 @@ def case_c1(seed=1):
 
 def rel_err(a, b):
     return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / max(np.max(np.abs(b)), 1e-300))
+
+
+# --- the 128x128x64 headline case (bench.py's default workload) -------------------------------
+C2_DIMS = (128, 128, 64)
+C2_SAMPLES = 32768
+
+
+def c2_problem(seed=2):
+    """128x128x64 unit cells, K = diag(kx, kx, 0.1 kx) with ln kx = ln 100 mD
+    + 1.5 z (z Gaussian-filtered, l = 8), 1 bar on x-, 0 on x+, no-flow
+    elsewhere, unit mobility, no gravity, no sources."""
+    g = grids.cartesian(*C2_DIMS)
+    perm = fields.perm_c2(C2_DIMS, seed=seed)
+    bt = np.zeros(g.num_half_faces, np.int32)
+    bv = np.zeros(g.num_half_faces)
+    bt[g.side == 0] = DIR
+    bv[g.side == 0] = 1.0e5
+    bt[g.side == 1] = DIR
+    return g, perm, bt, bv
+
+
+def c2_sample_index(n, stream):
+    """Seeded sample positions (with repeats) into an array of length n."""
+    return np.minimum((fields.uniform(11, stream, C2_SAMPLES) * n).astype(np.int64), n - 1)
+
+
+def xplane_rates(g, v):
+    """Total flux along +x through each x-plane of a Cartesian grid: minus the
+    outflux of every x- face of cell column i, then the x+ faces of the last
+    column (mass conservation makes all nx + 1 rates equal without sources)."""
+    nx, ny, nz = g.dims
+    vv = np.asarray(v).reshape(-1, 6)
+    rates = np.empty(nx + 1)
+    rates[:nx] = -vv[:, 0].reshape(nz, ny, nx).sum(axis=(0, 1))
+    rates[nx] = vv[:, 1].reshape(nz, ny, nx)[:, :, -1].sum()
+    return rates

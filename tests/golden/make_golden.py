@@ -12,6 +12,15 @@ c1_20x20x10.npz           the 20x20x10 plain case (ln k ~ N(ln 100 mD, 1),
 periodic_4x4x4.npz        4^3 fully periodic, unit pressure drop along x
 known_answer.json         the error norms the reference records in
                           examples/timing_results.txt
+c2_128x128x64.npz         the 128x128x64 headline case (anisotropic Gaussian-
+                          filtered K, seed 2, 1 bar -> 0 along x) solved by the
+                          oracle's ILU0-CG to a 1e-14 relative residual: face
+                          and cell pressures and half-face fluxes at 32768
+                          seeded sample positions each, whole-field max, 2-norm
+                          and sum, the flow rate through every x-plane, and the
+                          ILU0-CG iteration counts to 1e-10 and 1e-14
+
+    python tests/golden/make_golden.py c2     (only the headline fixture, ~5 min)
 """
 import json
 import os
@@ -32,7 +41,33 @@ import oracle  # noqa: E402
 import cases  # noqa: E402
 
 
+def make_c2():
+    g, perm, bt, bv = cases.c2_problem()
+    nc = g.num_cells
+    tm, rm, src = np.ones(nc), np.zeros(nc), np.zeros(nc)
+    r = oracle.solve(g, perm, tm, rm, src, bt, bv, tol=1e-14, precond=2)
+    res = r.system.rhs - oracle.spmv(r.system, r.face_pressure)
+    true_res = float(np.linalg.norm(res) / np.linalg.norm(r.system.rhs))
+    r10 = oracle.solve(g, perm, tm, rm, src, bt, bv, tol=1e-10, precond=2)
+    out = {"perm_checksum": pkg.fields.checksum(perm), "ilu0_iterations_1e14": r.iterations,
+           "ilu0_iterations_1e10": r10.iterations, "true_residual": true_res,
+           "xplane_rates": cases.xplane_rates(g, r.outflux)}
+    for k, (name, a) in enumerate((("face_pressure", r.face_pressure), ("cell_pressure", r.cell_pressure),
+                                   ("outflux", r.outflux))):
+        idx = cases.c2_sample_index(a.shape[0], k)
+        out[name + "_sample"] = a[idx]
+        out[name + "_norms"] = np.array([np.max(np.abs(a)), np.linalg.norm(a), np.sum(a)])
+        e10 = np.max(np.abs(getattr(r10, name) - a)) / np.max(np.abs(a))
+        out[name + "_ilu0_1e10_err"] = e10
+        print(f"{name}: ILU0-CG at 1e-10 differs from 1e-14 by {e10:.2e} (max-norm relative)")
+    print(f"ILU0-CG iterations: {r10.iterations} to 1e-10, {r.iterations} to 1e-14; true residual {true_res:.2e}")
+    np.savez_compressed(os.path.join(HERE, "c2_128x128x64.npz"), **out)
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "c2":
+        make_c2()
+        return
     out = {}
     for n in (1, 2, 4):
         g = pkg.grids.cartesian(n, n, n, 1.0 / n, 1.0 / n, 1.0 / n)
@@ -65,6 +100,7 @@ def main():
     }
     with open(os.path.join(HERE, "known_answer.json"), "w") as f:
         json.dump(known, f, indent=1)
+    make_c2()
 
 
 if __name__ == "__main__":

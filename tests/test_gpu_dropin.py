@@ -128,3 +128,170 @@ def test_aniso_relperm_example_matches_scalar_plugin():
     dp = float(re.search(r"pressure difference \(relative\):\s+([0-9.eE+-]+)", out).group(1))
     vmax = float(re.search(r"max \|flux\|\s+([0-9.eE+-]+)", out).group(1))
     assert vmax > 0 and df < 1e-9 and dp < 1e-9
+
+
+# --- printSystem / printStats (IncompFlowSolverHybrid.hpp:848-871, 895-905) ------------------
+
+def _read_dump(prefix, nf):
+    """The MATLAB spconvert files of printSystem: 1-based (row, col, value)
+    triplets in <prefix>-mat.dat and one right-hand side value per line in
+    <prefix>-rhs.dat."""
+    import scipy.sparse as sp
+    m = np.loadtxt(f"{prefix}-mat.dat", ndmin=2)
+    rhs = np.loadtxt(f"{prefix}-rhs.dat", ndmin=1)
+    rows, cols = m[:, 0].astype(np.int64) - 1, m[:, 1].astype(np.int64) - 1
+    A = sp.coo_matrix((m[:, 2], (rows, cols)), shape=(nf, nf)).tocsr()
+    return A, rows, cols, rhs
+
+
+def _oracle_system(orc, g, perm, tm, rm, src, bt, bv, partner=None):
+    hf, nint, nf, maxf = orc.enumerate_dof(g)
+    if partner is None:
+        partner = np.full(g.num_half_faces, -1, np.int32)
+    pp, _ = orc.periodic_dof(g, hf, nf, np.asarray(bt, np.int32), partner)
+    binv, pos, gfl = orc.static_ip(g, perm)
+    S = orc.assemble(g, hf, nf, pp, binv, gfl, tm, rm, src, bt, bv)
+    if S.do_regularization:
+        orc.regularize(S)
+    return S, hf, nint, nf, maxf
+
+
+def _reference_pattern(g, hf, nf, ppartner=None):
+    """The stored pattern of the reference's BCRS matrix: the ncf x ncf dof
+    block of every cell (setGridConnections, IncompFlowSolverHybrid.hpp:
+    1223-1262), rows and columns of periodic faces redirected to the smaller
+    partner dof, plus the periodic couplings (setBCConnections, :1266-1319)."""
+    pat = set()
+    red = np.arange(nf)
+    if ppartner is not None:
+        for d in range(nf):
+            if ppartner[d] >= 0:
+                red[d] = min(d, ppartner[d])
+                pat.add((d, int(ppartner[d])))
+                pat.add((d, d))
+    for c in range(g.num_cells):
+        dofs = hf[g.facepos[c]:g.facepos[c + 1]]
+        for i in dofs:
+            for j in dofs:
+                pat.add((int(red[i]), int(red[j])))
+    return pat
+
+
+def _compare_dump(S, prefix, nf, pattern=None):
+    import scipy.sparse as sp
+    A, rows, cols, rhs = _read_dump(prefix, nf)
+    R = sp.csr_matrix((S.val, S.col, S.row_ptr), shape=(nf, nf))
+    got = set(zip(rows.tolist(), cols.tolist()))
+    assert len(got) == len(rows)            # no entry written twice
+    # every nonzero of the oracle's assembly is stored ...
+    Rc = R.tocoo()
+    nzr = Rc.data != 0
+    assert set(zip(Rc.row[nzr].tolist(), Rc.col[nzr].tolist())) <= got
+    # ... in the reference's stored pattern (explicit zeros kept)
+    if pattern == "oracle":
+        pattern = set(zip(Rc.row.tolist(), Rc.col.tolist()))
+    if pattern is not None:
+        assert got == pattern
+    D = (A - R).tocoo()
+    assert (np.max(np.abs(D.data)) if D.nnz else 0.0) <= 1e-13 * np.max(np.abs(R.data))
+    assert np.max(np.abs(rhs - S.rhs)) <= 1e-13 * max(np.max(np.abs(S.rhs)), 1e-300)
+
+
+def _check_stats(out, g, hf, nint, nf, maxf):
+    assert re.search(r"Maximum number of cell faces = (\d+)", out).group(1) == str(maxf)
+    assert re.search(r"Number of internal faces\s+= (\d+)", out).group(1) == str(nint)
+    assert re.search(r"Total number of faces\s+= (\d+)", out).group(1) == str(nf)
+    rows = re.findall(r"\t\[(\d+)\] -> \[([0-9,]*)", out)
+    assert len(rows) == g.num_cells
+    for c, faces in rows:
+        c = int(c)
+        got = [int(x) for x in faces.split(",") if x]
+        assert got == hf[g.facepos[c]:g.facepos[c + 1]].tolist(), c
+
+
+def test_print_system_dirichlet_matches_oracle(orc, pkg, tmp_path):
+    """known_answer_test's assembled system (all-Dirichlet, sources) written by
+    the drop-in's printSystem, entry by entry against the oracle's assembly."""
+    n = 5
+    prefix = str(tmp_path / "ka")
+    out = _run("known_answer_test", f"nx={n}", "tolerance=1e-12", "print_stats=1", f"print_system={prefix}")
+    g = pkg.grids.cartesian(n, n, n, 1.0 / n, 1.0 / n, 1.0 / n)
+    u = lambda x: np.sin(2 * np.pi * x[:, 0]) * np.cos(2 * np.pi * x[:, 1]) * x[:, 2]
+    lu = lambda x: -8 * np.pi ** 2 * np.sin(2 * np.pi * x[:, 0]) * np.cos(2 * np.pi * x[:, 1]) * x[:, 2]
+    nc = g.num_cells
+    perm = np.tile(np.eye(3).reshape(1, 9), (nc, 1))
+    bt = np.where(g.nb < 0, 1, 0).astype(np.int32)
+    bv = np.where(g.nb < 0, u(g.face_centroid), 0.0)
+    src = -lu(g.cell_centroid) * g.cell_volume
+    S, hf, nint, nf, maxf = _oracle_system(orc, g, perm, np.ones(nc), np.zeros(nc), src, bt, bv)
+    assert not S.do_regularization
+    _compare_dump(S, prefix, nf, _reference_pattern(g, hf, nf))
+    _check_stats(out, g, hf, nint, nf, maxf)
+
+
+def test_print_system_periodic_matches_oracle(orc, pkg, tmp_path):
+    """mimetic_periodic_test's system (periodic in x and y with a 1 bar drop,
+    no Dirichlet face, so the S[0][0] *= 2 regularisation of :1398-1400 is in
+    the dump, as the reference prints S_ after the solve)."""
+    import cases
+    nx, ny, nz = 4, 3, 2
+    prefix = str(tmp_path / "per")
+    out = _run("mimetic_periodic_test", f"nx={nx}", f"ny={ny}", f"nz={nz}", "print_stats=1",
+               f"print_system={prefix}")
+    g = pkg.grids.cartesian(nx, ny, nz)
+    nc = g.num_cells
+    perm = np.tile(np.eye(3).reshape(1, 9), (nc, 1))
+    bt, bv, partner = cases.periodic_bc(g, (0, 1), drop_dir=0, pdrop=1.0e5)
+    lam = 1.0 / 3.0e-3      # sat 0: kro(0) / mu_o with the default 3 cP
+    S, hf, nint, nf, maxf = _oracle_system(orc, g, perm, np.full(nc, lam), np.zeros(nc), np.zeros(nc),
+                                           bt, bv, partner)
+    assert S.do_regularization
+    _compare_dump(S, prefix, nf, "oracle")
+    _check_stats(out, g, hf, nint, nf, maxf)
+
+
+# --- linsolver_type 0..3 (IncompFlowSolverHybrid.hpp:683-700) -----------------------------------
+
+@pytest.mark.parametrize("ltype", [0, 2, 3])
+def test_known_answer_linsolver_types(orc, pkg, ltype):
+    """known_answer_test with the reference's other linear solver types: 0
+    (SeqILU0-CG in the reference, Jacobi-PCG here), 2 (KAMG) and 3 (FastAMG),
+    both served by the multigrid-preconditioned CG, give the oracle's
+    converged solution."""
+    n = 12
+    out = _run("known_answer_test", f"nx={n}", "tolerance=1e-12", f"linear_solver_type={ltype}")
+    assert ("preconditioner: Jacobi" if ltype == 0 else "preconditioner: multigrid") in out, out
+    got = _norms(out)
+    g = pkg.grids.cartesian(n, n, n, 1.0 / n, 1.0 / n, 1.0 / n)
+    u = lambda x: np.sin(2 * np.pi * x[:, 0]) * np.cos(2 * np.pi * x[:, 1]) * x[:, 2]
+    lu = lambda x: -8 * np.pi ** 2 * np.sin(2 * np.pi * x[:, 0]) * np.cos(2 * np.pi * x[:, 1]) * x[:, 2]
+    nc = g.num_cells
+    perm = np.tile(np.eye(3).reshape(1, 9), (nc, 1))
+    bt = np.where(g.nb < 0, 1, 0).astype(np.int32)
+    bv = np.where(g.nb < 0, u(g.face_centroid), 0.0)
+    src = -lu(g.cell_centroid) * g.cell_volume
+    ref = orc.solve(g, perm, np.ones(nc), np.zeros(nc), src, bt, bv, tol=1e-14)
+    d = u(g.cell_centroid) - ref.cell_pressure
+    v = g.cell_volume
+    exp = np.array([np.sum(np.abs(d * v)) / v.sum(), np.sqrt(np.sum(d * d * v)) / v.sum(),
+                    np.max(np.abs(d))])
+    assert np.all(np.abs(got - exp) / exp < 1e-5), (got, exp)
+
+
+@pytest.mark.parametrize("ltype", [0, 1, 2, 3])
+def test_periodic_driver_linsolver_types(ltype):
+    """The periodic driver (regularised, pure periodic/Neumann system) with
+    every linear solver type against the closed form of uniform flow."""
+    out = _run("mimetic_periodic_test", "nx=8", "ny=8", "nz=4", f"linear_solver_type={ltype}")
+    q = float(re.search(r"net x- outflux:\s+([0-9.eE+-]+)", out).group(1))
+    lam = 1.0 / 3.0e-3
+    expect = -lam * 1.0 * 1.0e5 * (8 * 4) / 8.0
+    assert abs(q - expect) <= 1e-9 * abs(expect)
+
+
+def test_unknown_linsolver_type_throws():
+    """An unknown type is rejected with the reference's text (:698-700)."""
+    path = os.path.join(BUILD, "known_answer_test")
+    res = subprocess.run([path, "nx=4", "linear_solver_type=4"], capture_output=True, text=True, timeout=120)
+    assert res.returncode != 0
+    assert "Unknown linsolver_type: 4" in res.stdout + res.stderr

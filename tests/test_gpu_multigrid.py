@@ -203,3 +203,47 @@ def test_face_level_weight_from_spectral_radius(pkg, orc):
     rho = spl.eigsh(M, k=1, which="LA", return_eigenvectors=False)[0]
     w_exact = 4.0 / (3.0 * rho)
     assert w_exact * 0.999 <= w0 <= w_exact * 1.05, (w0, w_exact, rho)
+
+
+def test_new_permeability_reestimates_smoother_weights(pkg, orc):
+    """The Jacobi weights follow the permeability: after computeInnerProducts
+    with a far more anisotropic K on the same handle (rho(D^-1 A) 2.14 ->
+    2.97 here), the next solve's face-level weight is 4/(3 rho) of the new
+    matrix, and its solution and iteration count equal a fresh handle's
+    that was given the same two permeabilities."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    g = pkg.grids.cartesian(16, 12, 10, 1.0, 1.0, 0.5)
+    nc = g.num_cells
+    p1 = pkg.fields.perm_c1(nc, 6)
+    p2 = pkg.fields.diag_perm(p1[:, 0], p1[:, 0], 0.01 * p1[:, 0])
+    bt, bv = cases.xdrop_bc(g)
+    tm, rm = cases.unit_fluid(nc)
+    s = pkg.HybridSolver(0)
+    s.init(g, p1, (0, 0, 0), bt, None)
+    s.solve(tm, rm, np.zeros(nc), bt, bv, residual_tolerance=1e-10, precision="mixed")
+    w1 = s.report.as_dict()["smoother_weights"][0]
+    s.computeInnerProducts(p2)
+    s.solve(tm, rm, np.zeros(nc), bt, bv, residual_tolerance=1e-10, precision="mixed")
+    w2 = s.report.as_dict()["smoother_weights"][0]
+    it2 = s.report.iterations
+    hf, nint, nf, _ = orc.enumerate_dof(g)
+    pp, _ = orc.periodic_dof(g, hf, nf, bt, np.full(g.num_half_faces, -1, np.int32))
+    binv, pos, gfl = orc.static_ip(g, p2)
+    S = orc.assemble(g, hf, nf, pp, binv, gfl, tm, rm, np.zeros(nc), bt, bv)
+    A = sp.csr_matrix((S.val, S.col, S.row_ptr), shape=(nf, nf))
+    d = 1.0 / np.sqrt(A.diagonal())
+    rho = spl.eigsh(sp.diags(d) @ A @ sp.diags(d), k=1, which="LA", return_eigenvectors=False)[0]
+    w_exact = 4.0 / (3.0 * rho)
+    assert w_exact * 0.999 <= w2 <= w_exact * 1.05, (w1, w2, w_exact, rho)
+    assert w2 < 0.8 * w1
+    # the same sequence on a fresh handle
+    f = pkg.HybridSolver(0)
+    f.init(g, p1, (0, 0, 0), bt, None)
+    f.computeInnerProducts(p2)
+    f.solve(tm, rm, np.zeros(nc), bt, bv, residual_tolerance=1e-10, precision="mixed")
+    assert f.report.as_dict()["smoother_weights"][0] == w2
+    assert abs(f.report.iterations - it2) <= 1
+    assert cases.rel_err(s.getSolution()[2], f.getSolution()[2]) < 1e-8
+    ref = orc.solve(g, p2, tm, rm, np.zeros(nc), bt, bv, tol=1e-14)
+    assert cases.rel_err(s.getSolution()[2], ref.outflux) < 1e-8

@@ -259,3 +259,21 @@ def test_gravity_term(pkg, orc, kind):
     got = s.gravityTerm(grav, om)
     want = orc.gravity_term(g, grav, om)
     assert np.max(np.abs(got - want)) <= 1e-14 * np.max(np.abs(want))
+
+
+@pytest.mark.parametrize("ltype", [0, 2, 3])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_c1_linsolver_types(pkg, orc, ltype, precision):
+    """linsolver_type 0 (Jacobi-PCG, standing in for SeqILU0-CG), 2 (KAMG)
+    and 3 (FastAMG) through the ctypes binding against the oracle: every type
+    converges to the same discrete solution (IncompFlowSolverHybrid.hpp:671-700)."""
+    g, perm, tm, rm, src, bt, bv = cases.case_c1()
+    ref = orc.solve(g, perm, tm, rm, src, bt, bv, tol=1e-14)
+    s = _solver(pkg, g, perm)
+    rep = s.solve(tm, rm, src, bt, bv, residual_tolerance=1e-12, precision=precision, linsolver_type=ltype)
+    assert rep.converged and rep.reduction <= 1e-12
+    assert rep.precond == (0 if ltype == 0 else 1)
+    p, pi, v = s.getSolution()
+    assert rel_err(pi, ref.face_pressure) < TOL_SOL
+    assert rel_err(p, ref.cell_pressure) < TOL_SOL
+    assert rel_err(v, ref.outflux) < TOL_SOL

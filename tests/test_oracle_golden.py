@@ -51,3 +51,27 @@ def test_known_answer_50(pkg, orc):
     got = _known_answer_norms(pkg, orc, 50)
     exp = np.array([ref["L1"], ref["L2"], ref["Linf"]])
     assert np.all(np.abs(got - exp) / exp < np.array([5e-5, 5e-5, 1e-4])), (got, exp)
+
+
+def test_c2_fixture_is_the_headline_case_and_conserves_mass(pkg):
+    """The 128x128x64 fixture (the oracle's ILU0-CG to 1e-14) belongs to the
+    headline inputs (permeability checksum), was solved to a true residual
+    below 1e-13, and carries the same flow through every x-plane (no sources:
+    conservation fixes the rate from x- to x+)."""
+    import cases
+    fx = np.load(cases.golden("c2_128x128x64.npz"))
+    g, perm, bt, bv = cases.c2_problem()
+    assert str(fx["perm_checksum"]) == pkg.fields.checksum(perm)
+    assert float(fx["true_residual"]) < 1e-13
+    rates = fx["xplane_rates"]
+    assert rates.shape == (129,) and rates.min() > 0
+    assert np.max(np.abs(rates - rates.mean())) <= 1e-10 * rates.mean()
+    assert int(fx["ilu0_iterations_1e10"]) < int(fx["ilu0_iterations_1e14"])
+    for name in ("face_pressure", "cell_pressure", "outflux"):
+        assert fx[name + "_sample"].shape == (cases.C2_SAMPLES,)
+        assert 0 < float(fx[name + "_ilu0_1e10_err"]) < 1e-7
+    # pressures lie between the two Dirichlet values (maximum principle of the
+    # face and cell pressures on this grid of unit cubes)
+    for name in ("face_pressure", "cell_pressure"):
+        a = fx[name + "_sample"]
+        assert a.min() >= -1e-9 and a.max() <= 1.0e5 * (1 + 1e-12)
