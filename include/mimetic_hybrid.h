@@ -290,6 +290,17 @@ int mhs_get_solution_device(mhs_solver *s, double *cell_pressure, double *outflu
  * the host. */
 int mhs_sum_outflux(mhs_solver *s, int ngroups, const int *group_ptr,
                     const int *group_hf, double *sums);
+/* mhs_sum_outflux without any device allocation.  No counterpart in the
+ * reference: for callers that must not allocate device memory (serving loops
+ * and harnesses that forbid hipMalloc).  The group lists and the sums are
+ * staged in a per-GPU scratch array of 16 MiB that the library holds until
+ * the process ends: (ngroups + 1) and ne ints, each rounded up to 256 bytes,
+ * plus nrhs * ngroups doubles must fit, else MHS_ERR_INVALID.  Every call on
+ * one GPU shares that array, so calls that overlap on one GPU (several host
+ * threads, several handles) race; calls on one handle, which are ordered,
+ * are safe.  Same results as mhs_sum_outflux. */
+int mhs_sum_outflux_noalloc(mhs_solver *s, int ngroups, const int *group_ptr,
+                            const int *group_hf, double *sums);
 
 /* Adjoint of the last solve.  No counterpart in the reference: this is the
  * differentiated form of solve() (:657-703) for gradient-based calibration.

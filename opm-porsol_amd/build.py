@@ -24,7 +24,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-u
           "-Wno-unused-result", f"--offload-arch={ARCH}",
           "-I", os.path.join(ROOT, "include"), "-I", CSRC]
 
-SOURCES = ["mhs_kernels.hip", "mhs_transport.hip", "mhs_api.cpp", "mhs_structure.cpp", "mhs_amg.cpp"]
+SOURCES = ["mhs_kernels.hip", "mhs_transport.hip", "mhs_scratch.hip", "mhs_api.cpp", "mhs_structure.cpp",
+           "mhs_amg.cpp"]
 
 
 def _compile(src: str) -> tuple[subprocess.Popen, str]:

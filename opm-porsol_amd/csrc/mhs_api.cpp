@@ -9,6 +9,7 @@
 #include "../../include/euler_upstream.h"
 #include "mhs_amg.h"
 #include "mhs_kernels.h"
+#include "mhs_scratch.h"
 #include "mhs_structure.h"
 #include "mhs_transport.h"
 
@@ -72,6 +73,10 @@ struct mhs_solver {
     hipStream_t user_stream = nullptr;  // caller's stream, ordered against ours
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     std::string err;
+    // this device's scratch array (mhs_sum_outflux_noalloc); null when the
+    // address lookup at creation failed, with the reason in scratch_err
+    void *scratch = nullptr;
+    hipError_t scratch_err = hipSuccess;
 
     // grid (host copies)
     bool grid_set = false;
@@ -279,6 +284,8 @@ int mhs_create(mhs_solver **out, int device, void *stream) {
         delete s;
         return MHS_ERR_HIP;
     }
+    // once per device, with the device current: the address of its scratch array
+    s->scratch_err = scratch_address(device, &s->scratch);
     s->user_stream = static_cast<hipStream_t>(stream);
     *out = s;
     return MHS_OK;
@@ -1512,6 +1519,8 @@ int check_group_args(mhs_solver *s, const char *who, int ngroups, const int *gro
     return MHS_OK;
 }
 
+size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
+
 }  // namespace
 
 extern "C" {
@@ -1540,6 +1549,36 @@ int mhs_sum_outflux(mhs_solver *s, int ngroups, const int *group_ptr, const int 
     (void)hipFree(dhf);
     (void)hipFree(dout);
     if (e != hipSuccess) return fail(s, MHS_ERR_HIP, std::string("sum_outflux: ") + hipGetErrorString(e));
+    return MHS_OK;
+}
+
+int mhs_sum_outflux_noalloc(mhs_solver *s, int ngroups, const int *group_ptr, const int *group_hf,
+                            double *sums) {
+    size_t ne = 0;
+    if (int rc = check_group_args(s, "sum_outflux_noalloc", ngroups, group_ptr, group_hf, sums, &ne)) return rc;
+    if (ngroups == 0) return MHS_OK;
+    if (!s->scratch)
+        return fail(s, MHS_ERR_HIP, std::string("sum_outflux_noalloc: no scratch array on this device: ") +
+                                        hipGetErrorString(s->scratch_err));
+    const int nrhs = s->last_nrhs;
+    // staging in this device's scratch array: group_ptr, group_hf, then the sums
+    const size_t b_ptr = align256(sizeof(int) * ((size_t)ngroups + 1)), b_hf = align256(sizeof(int) * ne),
+                 b_out = sizeof(double) * (size_t)nrhs * ngroups;
+    if (b_ptr + b_hf + b_out > kScratchBytes)
+        return fail(s, MHS_ERR_INVALID, "sum_outflux_noalloc: " + std::to_string(b_ptr + b_hf + b_out) +
+                                            " bytes of staging exceed the " + std::to_string(kScratchBytes) +
+                                            "-byte scratch array");
+    HIPCHECK(hipSetDevice(s->device));
+    unsigned char *base = static_cast<unsigned char *>(s->scratch);
+    int *dptr = reinterpret_cast<int *>(base);
+    int *dhf = reinterpret_cast<int *>(base + b_ptr);
+    double *dout = reinterpret_cast<double *>(base + b_ptr + b_hf);
+    // every output entry is written by the kernel: nothing to clear
+    HIPCHECK(hipMemcpyAsync(dptr, group_ptr, sizeof(int) * ((size_t)ngroups + 1), hipMemcpyHostToDevice, s->stream));
+    if (ne > 0) HIPCHECK(hipMemcpyAsync(dhf, group_hf, sizeof(int) * ne, hipMemcpyHostToDevice, s->stream));
+    HIPCHECK(launch_group_sum(s->stream, nrhs, ngroups, s->T.nhf, dptr, dhf, s->outflux, dout));
+    HIPCHECK(hipMemcpyAsync(sums, dout, b_out, hipMemcpyDeviceToHost, s->stream));
+    HIPCHECK(hipStreamSynchronize(s->stream));
     return MHS_OK;
 }
 

@@ -34,6 +34,7 @@ EXPORTS = (
     "mhs_get_solution_device", "mhs_adjoint", "mhs_adjoint_device", "mhs_sum_outflux",
     "mhs_gravity_term", "mhs_topology_is_sane", "mhs_estimate_cell_velocity",
     "mhs_estimate_cell_velocity_device", "mhs_solve_aniso", "mhs_solve_aniso_device",
+    "mhs_sum_outflux_noalloc",
 )
 
 
@@ -129,6 +130,7 @@ def lib():
         L.mhs_compute_inner_products_device.argtypes = [_P, _P, _D]
         L.mhs_get_solution_device.argtypes = [_P, _P, _P]
         L.mhs_sum_outflux.argtypes = [_P, ctypes.c_int, _I, _I, _D]
+        L.mhs_sum_outflux_noalloc.argtypes = [_P, ctypes.c_int, _I, _I, _D]
         L.mhs_gravity_term.argtypes = [_P, _D, _D, _D]
         L.mhs_topology_is_sane.argtypes = [ctypes.c_int, _I, _I, ctypes.POINTER(ctypes.c_int)]
         L.mhs_estimate_cell_velocity.argtypes = [_P, ctypes.c_int, _D]
@@ -420,6 +422,16 @@ class HybridSolver:
         ng = gp.shape[0] - 1
         out = np.empty((self.nrhs, ng))
         self._check(self._L.mhs_sum_outflux(self._h, ng, pgp, pgh, out.ctypes.data_as(_D)))
+        return out
+
+    def sum_outflux_noalloc(self, group_ptr, group_hf):
+        """sum_outflux through the entry point that allocates no device memory
+        (its staging lives in the library's per-GPU scratch array)."""
+        gp, pgp = _i(group_ptr)
+        gh, pgh = _i(group_hf)
+        ng = gp.shape[0] - 1
+        out = np.empty((self.nrhs, ng))
+        self._check(self._L.mhs_sum_outflux_noalloc(self._h, ng, pgp, pgh, out.ctypes.data_as(_D)))
         return out
 
     # --- adjoint --------------------------------------------------------------------
