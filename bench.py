@@ -53,27 +53,45 @@ def _alg_bytes(g, nnz, nf, mg_rows=0, mg_nnz=0, ap_nnz=0):
     nc, nhf = g.num_cells, g.num_half_faces
     sq = int(np.sum(g.ncf().astype(np.int64) ** 2))
     pk = int(np.sum(g.ncf().astype(np.int64) * (g.ncf() + 1) // 2))
+    nint = int(np.sum(g.nb >= 0)) // 2
     out = {}
     for v, tag in ((4, "32"), (8, "64")):
         mat = nnz * (v + 4)
         # SpMV q = A p with the fused <p,q> partials: matrix, p, q
         out["spmv" + tag] = mat + 2 * nf * v
+        # pre-smoothing residual res = r - A x: matrix, x, r, res
+        out["resid" + tag] = mat + 3 * nf * v
         # Jacobi-PCG iteration: SpMV + x,r,p,q,dinv reads and x,r,p writes
         out["iter" + tag] = mat + 4 * (nf + 1) + 13 * nf * v
-        # multigrid-PCG iteration: two passes over the face matrix (SpMV,
-        # pre-smoothed residual), one over A P with the level-1 aggregates
-        # (post-smoothing sweep), 22 level-0 vector passes (SpMV 2, x/r update
-        # with the pre-smoothing 8, residual 3, restriction 1, post-smoothing
-        # 5, p update 3), the aggregate map twice, and on the coarse levels
-        # two operator passes and ten vector passes per row
-        out["amg_iter" + tag] = (2 * mat + ap_nnz * (v + 4) + 22 * nf * v + 8 * nf
-                                 + 2 * mg_nnz * (v + 4) + mg_rows * (12 + 10 * v))
+    # The same two operator passes element by element (FP32): the packed local
+    # matrices and the face rows of every cell, the gathered vector once, the
+    # output written once and, on every face with two cells, read and written
+    # again by the second colour.
+    cell_part = pk * 4 + nhf * 4
+    out["ebe_spmv"] = cell_part + nf * 4 + nf * 4 + nint * 8
+    out["ebe_resid"] = cell_part + 2 * nf * 4 + nf * 4 + nint * 8
+    for v, tag in ((4, "32"), (8, "64")):
+        # multigrid-PCG iteration: the SpMV and the pre-smoothed residual, one
+        # pass over A P with the level-1 aggregates (post-smoothing sweep), 17
+        # more level-0 vector passes (x/r update with the pre-smoothing 8,
+        # restriction 1, post-smoothing 5, p update 3), the aggregate map
+        # twice, and on the coarse levels two operator passes and ten vector
+        # passes per row
+        rest = (ap_nnz * (v + 4) + 17 * nf * v + 8 * nf + 2 * mg_nnz * (v + 4) + mg_rows * (12 + 10 * v))
+        out["amg_iter" + tag] = out["spmv" + tag] + out["resid" + tag] + rest
+        if tag == "32":
+            out["amg_iter_ebe"] = out["ebe_spmv"] + out["ebe_resid"] + rest
     # FP64 residual replacement: FP64 operator + x, b read, r written
     out["refine64"] = nnz * (8 + 4) + 4 * nf * 8
+    # element by element it rides on the SpMV: the FP64 values instead of
+    # the FP32 ones, x64 and b read, the FP64 sums of two-cell faces kept
+    # between the colours, the FP32 residual written
+    out["refine64_ebe"] = pk * 4 + 2 * nf * 8 + nint * 16 + nf * 4
     # assembly (static B^-1 read packed, values written in FP64 + FP32, rhs, F, L, g)
     out["assemble"] = pk * 8 + nc * 16 + nnz * (8 + 4) + nf * 8 + nhf * 8 + nc * 16 + sq
     # recovery: pi, F, L, g, B^-1 (packed), mobility, p, v
     out["recover"] = nf * 8 + nhf * 8 + nc * 16 + pk * 8 + nc * 8 + nc * 8 + nhf * 8
+    out["pk"] = pk
     return out
 
 
@@ -244,16 +262,24 @@ def _copy_gbps(dev, mib=1024, reps=20):
     return 2.0 * n * 4 / (ms * 1e-3) / 1e9
 
 
-def _spmv_roofline(s, nnz, nf, nrhs, tag):
-    """Roofline entry of the PCG SpMV of the last solve (HIP-event timing)."""
+def _spmv_roofline(s, nnz, nf, nrhs, tag, g=None):
+    """Roofline entry of the PCG product of the last solve (HIP-event timing):
+    the element-by-element product when the solve used it, else the SpMV
+    with the assembled face matrix."""
     v = 4 if tag == "32" else 8
-    alg = nnz * (v + 4) + 2 * nf * v * nrhs
+    rep = s.report
     ms = s.time_kernel(0, reps=50)
+    if rep.element_operator and g is not None:
+        pk = int(np.sum(g.ncf().astype(np.int64) * (g.ncf() + 1) // 2))
+        nint = int(np.sum(g.nb >= 0)) // 2
+        alg = pk * 4 + g.num_half_faces * 4 + (2 * nf * 4 + nint * 8) * nrhs
+        kname = f"k_ebe<nrhs={nrhs}> (element-by-element product, {rep.colours} colour launches)"
+    else:
+        alg = nnz * (v + 4) + 2 * nf * v * nrhs
+        kname = (f"{'k_spmv_pq8' if (tag == '32' and nrhs == 1) else 'k_spmv_pq'}"
+                 f"<{'float' if tag == '32' else 'double'}, nrhs={nrhs}> (face-matrix SpMV with fused <p,Ap>)")
     ach = alg / (ms * 1e-3) / 1e9
-    kname = "k_spmv_pq8" if (tag == "32" and nrhs == 1) else "k_spmv_pq"
-    return {"kernel": f"{kname}<{'float' if tag == '32' else 'double'}, nrhs={nrhs}> (face-matrix "
-                      f"SpMV with fused <p,Ap>)",
-            "bound": "hbm", "achieved": round(ach, 1), "peak": 8000.0, "unit": "GB/s",
+    return {"kernel": kname, "bound": "hbm", "achieved": round(ach, 1), "peak": 8000.0, "unit": "GB/s",
             "frac": round(ach / 8000.0, 4), "traffic": None,
             "alg_bytes_per_launch": alg, "avg_launch_us": round(ms * 1e3, 2)}
 
@@ -309,7 +335,7 @@ def bench_upscale(args, pkg, torch, dev):
     d = s.dims()
     nnz = int(s.system(0)[0][-1])
     tag = "32" if args.precision == "mixed" else "64"
-    roof = _spmv_roofline(s, nnz, d.num_faces, 3, tag)
+    roof = _spmv_roofline(s, nnz, d.num_faces, 3, tag, g)
     mD = 9.869233e-16
     kd = np.stack([keff[:, i, i] for i in range(3)], 1) / mD
     config = {
@@ -454,7 +480,7 @@ def bench_adjoint(args, pkg, torch, dev):
     rep, arep = s.report.as_dict(), s.adjoint_report.as_dict()
     d = s.dims()
     nnz = int(s.system(0)[0][-1])
-    roof = _spmv_roofline(s, nnz, d.num_faces, 1, "32" if args.precision == "mixed" else "64")
+    roof = _spmv_roofline(s, nnz, d.num_faces, 1, "32" if args.precision == "mixed" else "64", G)
     config = {
         "workload": "32 realizations of 64x64x32, lognormal isotropic K (l=6, sigma 1), J = sum over 16 "
                     "x+ producer faces per realization of (flux - target)^2, dJ/dln K by the adjoint",
@@ -539,7 +565,7 @@ def bench_faulted(args, pkg, torch, dev):
     p, pi, v = s.getSolution()
     d = s.dims()
     nnz = int(s.system(0)[0][-1])
-    roof = _spmv_roofline(s, nnz, d.num_faces, 1, "32" if args.precision == "mixed" else "64")
+    roof = _spmv_roofline(s, nnz, d.num_faces, 1, "32" if args.precision == "mixed" else "64", g)
     cell = g.hf_cell()
     config = {
         "workload": "200x200x50 faulted corner-point grid (6-14 faces per cell), rotated anisotropic "
@@ -751,7 +777,7 @@ def bench_aniso(args, pkg, torch, dev):
     v = s.getSolution()[2]
     d = s.dims()
     nnz = int(s.system(0)[0][-1])
-    roof = _spmv_roofline(s, nnz, d.num_faces, 1, "32" if args.precision == "mixed" else "64")
+    roof = _spmv_roofline(s, nnz, d.num_faces, 1, "32" if args.precision == "mixed" else "64", g)
     cell = g.hf_cell()
     config = {
         "workload": "mimetic hybrid solve with tensor mobilities, 128x128x64 Cartesian, anisotropic K "
@@ -893,9 +919,21 @@ def main():
 
     B = _alg_bytes(g, nnz, nf, rep["mg_rows"], rep["mg_nnz"], rep["ap_nnz"])
     tag = "32" if args.precision == "mixed" else "64"
-    it_key = ("amg_iter" if rep["precond"] == 1 else "iter") + tag
+    ebe = bool(rep["element_operator"])
+    if ebe:
+        it_key = "amg_iter_ebe" if rep["precond"] == 1 else None
+        ref_key = "refine64_ebe"
+    else:
+        it_key = ("amg_iter" if rep["precond"] == 1 else "iter") + tag
+        ref_key = "refine64"
     n_ref = rep["replacements"]
-    bytes_solve = B["assemble"] + rep["iterations"] * B[it_key] + n_ref * B["refine64"] + B["recover"]
+    if it_key is None:      # element-by-element Jacobi-PCG
+        it_bytes = B["ebe_spmv"] + B["iter32"] - B["spmv32"]
+    else:
+        it_bytes = B[it_key]
+    bytes_solve = B["assemble"] + rep["iterations"] * it_bytes + n_ref * B[ref_key] + B["recover"]
+    if ebe:   # the assembly also writes the FP64 and FP32 local matrices
+        bytes_solve += B["pk"] * 12
     gbps = bytes_solve / (ms_step * 1e-3) / 1e9
 
     if args.copy_probe_mib > 0:
@@ -907,12 +945,28 @@ def main():
         del a, b
 
     kern_ms = s.time_kernel(0, reps=50)
-    ach = B["spmv" + tag] / (kern_ms * 1e-3) / 1e9
-    roof = {"kernel": f"{'k_spmv_pq8<float>' if tag == '32' else 'k_spmv_pq<double>'} (face-matrix SpMV with "
-                      f"fused <p,Ap>)",
-            "bound": "hbm", "achieved": round(ach, 1), "peak": 8000.0, "unit": "GB/s",
-            "frac": round(ach / 8000.0, 4), "traffic": _measured_traffic(tag),
-            "alg_bytes_per_launch": B["spmv" + tag], "avg_launch_us": round(kern_ms * 1e3, 2)}
+    if ebe:
+        alg = B["ebe_spmv"]
+        roof = {"kernel": f"k_ebe<6, 1, EBE_SPMV, REL> (element-by-element PCG product q = A p with fused <p,Ap>, "
+                          f"{rep['colours']} colour launches per product)",
+                "traffic": _measured_traffic(tag, "k_ebe_spmv_128x128x64_hbm_traffic.txt")}
+    else:
+        alg = B["spmv" + tag]
+        roof = {"kernel": f"{'k_spmv_pq8<float>' if tag == '32' else 'k_spmv_pq<double>'} (face-matrix SpMV "
+                          f"with fused <p,Ap>)",
+                "traffic": _measured_traffic(tag)}
+    ach = alg / (kern_ms * 1e-3) / 1e9
+    roof.update({"bound": "hbm", "achieved": round(ach, 1), "peak": 8000.0, "unit": "GB/s",
+                 "frac": round(ach / 8000.0, 4), "alg_bytes_per_launch": alg,
+                 "avg_launch_us": round(kern_ms * 1e3, 2)})
+    if ebe:
+        roof["alg_bytes_note"] = ("per product (both colour launches): packed FP32 local matrices and face rows "
+                                  "of every cell, p gathered once, q written once and read+written again on "
+                                  "faces with two cells")
+        # the assembled FP32 matrix product the same solve no longer runs
+        sell_ms = s.time_kernel(1, reps=20)
+        roof["assembled_matrix_spmv"] = {"alg_bytes": B["spmv32"], "avg_us": round(sell_ms * 1e3, 2),
+                                         "achieved": round(B["spmv32"] / (sell_ms * 1e-3) / 1e9, 1)}
     copy_gbps = _copy_gbps(dev)
     roof["device_copy_gbps"] = round(copy_gbps, 1)
     roof["frac_of_device_copy"] = round(ach / copy_gbps, 4)
@@ -938,6 +992,8 @@ def main():
         "mass_balance": mass,
         "launch_shape": {k: rep[k] for k in ("spmv_threads", "spmv_xcd", "sweep_threads", "sweep_xcd",
                                              "tail_rows")},
+        "level0_operator": ("element by element (FP32 local matrices, %d colours)" % rep["colours"]) if ebe
+                           else "assembled face matrix",
     }
     if fx is not None:
         config["err_vs_oracle"] = _err_vs_oracle(g, perm, fx, p, pi, v)

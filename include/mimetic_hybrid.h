@@ -107,6 +107,12 @@ typedef struct {
     int tail_rows;             /* -1: from autotune (0 when autotune is 0); >= 0: coarse
                                   levels whose largest instance has at most this many
                                   rows run in one workgroup per instance */
+    int element_operator;      /* mixed precision: 1 (default) applies the face
+                                  operator of the PCG product, its FP64 residual
+                                  replacement and the multigrid pre-smoothing residual
+                                  cell by cell from the local Schur complements, when
+                                  the structure has no periodic faces and no instance
+                                  is regularised; 0: the assembled FP32 face matrix */
 } mhs_solve_options;
 
 typedef struct {
@@ -129,6 +135,9 @@ typedef struct {
                                      0..7 (1 or 2 along i, j, k; 0 beyond) */
     long long ap_nnz;             /* stored entries of A P (face rows x level-1 aggregates)
                                      read by the post-smoothing sweep; 0 without multigrid */
+    int element_operator;         /* 1: the solve applied the level-0 operator cell by cell */
+    int colours;                  /* cell colours of that operator (0 when not used) */
+    long long element_values;     /* packed local-matrix entries it stores (with padding) */
 } mhs_solve_report;
 
 typedef struct {
@@ -334,9 +343,12 @@ int mhs_estimate_cell_velocity(mhs_solver *s, int k, double *cell_velocity);
 int mhs_estimate_cell_velocity_device(mhs_solver *s, int k, double *cell_velocity);
 
 /* Profiling hook, no counterpart in the reference: average device time (ms)
- * of `reps` back-to-back launches of kernel `which` (0: the PCG SpMV with
- * <p, Ap>, in the precision of the last solve) on the solver's own stream,
- * timed with HIP events, on the data of the last solve. */
+ * of `reps` back-to-back launches of kernel `which` on the solver's own
+ * stream, timed with HIP events, on the data of the last solve: 0, the PCG
+ * product q = A p with <p, Ap> as the last solve ran it (in mixed precision
+ * element by element when it used that operator: all colour launches); 1,
+ * the product with the assembled face matrix in the precision of the last
+ * solve. */
 int mhs_time_kernel(mhs_solver *s, int which, int reps, double *avg_ms);
 
 #ifdef __cplusplus

@@ -8,6 +8,7 @@
 #include "../../include/mimetic_hybrid.h"
 #include "../../include/euler_upstream.h"
 #include "mhs_amg.h"
+#include "mhs_ebe.h"
 #include "mhs_kernels.h"
 #include "mhs_scratch.h"
 #include "mhs_structure.h"
@@ -46,7 +47,7 @@ struct TempBufs {
 
 struct GraphCache {
     hipGraphExec_t exec = nullptr;
-    int precision = -1, nrhs = 0, chunk = 0, amg = -1, nu = -1;
+    int precision = -1, nrhs = 0, chunk = 0, amg = -1, nu = -1, ebe = -1;
     int shape[5] = {-1, -1, -1, -1, -1};
     double tol2 = -1.0, rf2 = -1.0, alpha = -1.0;
     double omega[kMaxLevels] = {};
@@ -103,6 +104,14 @@ struct mhs_solver {
     double *an_lam = nullptr, *an_rlam = nullptr, *an_lamK = nullptr, *an_dynKg = nullptr,
            *an_binv = nullptr, *an_gflux = nullptr, *an_ones = nullptr;
     bool last_aniso = false;            // the last solve used tensor mobilities
+
+    // element-by-element level-0 operator (mixed precision): layout built with
+    // the structure, values written by the assembly
+    EbeLayout EL;
+    bool ebe_ok = false;                // the structure allows it
+    std::string ebe_why;                // else why not
+    DevEbe dE{};
+    bool last_ebe = false;              // the last solve used it
 
     // multigrid hierarchy
     std::vector<int> ijk;
@@ -212,6 +221,10 @@ void mhs_solver::free_topology() {
     free_amg();
     for (auto &b : topo_bufs) (void)hipFree(b.p);
     topo_bufs.clear();
+    dE = DevEbe{};
+    EL = EbeLayout();
+    ebe_ok = false;
+    last_ebe = false;
     binv = gflux = perm = nullptr;
     an_lam = an_rlam = an_lamK = an_dynKg = an_binv = an_gflux = an_ones = nullptr;
     dT = DevTopo{};
@@ -225,6 +238,9 @@ void mhs_solver::free_solve_buffers() {
     graphs.clear();
     for (auto &b : solve_bufs) (void)hipFree(b.p);
     solve_bufs.clear();
+    dE.val64 = nullptr;
+    dE.val32 = nullptr;
+    dE.R.partials = nullptr;
     for (auto &b : adj_bufs) (void)hipFree(b.p);
     adj_bufs.clear();
     adj_pi = adj_b = adj_w = adj_y = adj_gK = adj_gtm = adj_grm = adj_gsrc = nullptr;
@@ -268,6 +284,7 @@ void mhs_default_options(mhs_solve_options *o) {
     o->spmv_threads = 0;
     o->sweep_threads = 0;
     o->tail_rows = -1;
+    o->element_operator = 1;
 }
 
 int mhs_create(mhs_solver **out, int device, void *stream) {
@@ -549,6 +566,32 @@ int mhs_init_system_structure(mhs_solver *s, const int *hf_bc_type, const int *h
     if (he == hipSuccess) s->binv = dalloc<double>(s, L, (size_t)T.max_ncf * (T.max_ncf + 1) / 2 * T.nc, &he);
     if (he == hipSuccess) s->gflux = dalloc<double>(s, L, (size_t)T.nhf, &he);
     if (he != hipSuccess) return fail(s, MHS_ERR_ALLOC, std::string("device allocation: ") + hipGetErrorString(he));
+    s->ebe_why = build_ebe(s->EL, T);
+    s->ebe_ok = s->ebe_why.empty() && s->EL.ncolours <= kEbeColours && T.max_ncf <= kMaxFaces;
+    if (s->ebe_ok) {
+        const EbeLayout &E = s->EL;
+        DevEbe &D = s->dE;
+        D = DevEbe{};
+        D.ncolours = E.ncolours;
+        D.nblk = E.nblk;
+        for (int k = 0; k <= E.ncolours; ++k) D.colour_blk[k] = E.colour_blk[k];
+        D.blk_seg = dupload(s, L, E.blk_seg, &he);
+        if (he == hipSuccess) D.blk_slot = dupload(s, L, E.blk_slot, &he);
+        if (he == hipSuccess) D.seg_n = dupload(s, L, E.seg_n, &he);
+        if (he == hipSuccess) D.seg_inst = dupload(s, L, E.seg_inst, &he);
+        if (he == hipSuccess) D.seg_count = dupload(s, L, E.seg_count, &he);
+        if (he == hipSuccess) D.seg_len = dupload(s, L, E.seg_len, &he);
+        if (he == hipSuccess) D.seg_blk0 = dupload(s, L, E.seg_blk0, &he);
+        if (he == hipSuccess) D.seg_val = dupload(s, L, E.seg_val, &he);
+        if (he == hipSuccess) D.seg_dof = dupload(s, L, E.seg_dof, &he);
+        if (he == hipSuccess) D.cell_seg = dupload(s, L, E.cell_seg, &he);
+        if (he == hipSuccess) D.cell_local = dupload(s, L, E.cell_local, &he);
+        if (he == hipSuccess) D.dof = dupload(s, L, E.dof, &he);
+        if (he == hipSuccess) D.R.seg_blk = dupload(s, L, E.inst_slot, &he);
+        D.R.nblk = E.nslots;
+        if (he != hipSuccess)
+            return fail(s, MHS_ERR_ALLOC, std::string("device allocation: ") + hipGetErrorString(he));
+    }
     if (!s->ijk.empty()) {
         const int rc = build_amg_device(s, Strength{1.0, 1.0, 1.0});
         if (rc) return rc;
@@ -768,6 +811,15 @@ int ensure_solve_buffers(mhs_solver *s, int nrhs, bool mixed) {
         if (mixed) { if (ok()) s->z32 = dalloc<float>(s, L, nv, &e); }
         else { if (ok()) s->z64 = dalloc<double>(s, L, nv, &e); }
     }
+    if (mixed && s->ebe_ok) {
+        if (ok()) s->dE.val64 = dalloc<double>(s, L, (size_t)s->EL.nval, &e);
+        if (ok()) s->dE.val32 = dalloc<float>(s, L, (size_t)s->EL.nval, &e);
+        if (ok()) s->dE.R.partials = dalloc<double>(s, L, (size_t)s->EL.nslots * 16, &e);
+        // padding rows are never written by the element-by-element passes:
+        // their q and multigrid residual stay zero, like their p and r
+        if (ok()) e = hipMemset(s->q32, 0, nv * sizeof(float));
+        if (ok() && s->has_amg) e = hipMemset(s->dA.res0, 0, nv * sizeof(float));
+    }
     if (ok()) e = hipMemset(s->R.counters, 0, sizeof(int) * T.ninst);
     if (ok()) e = hipMemset(s->x64, 0, nv * sizeof(double));
     if (ok()) e = hipHostMalloc(reinterpret_cast<void **>(&s->h_sc), state_bytes(nsys), 0);
@@ -792,14 +844,19 @@ int ensure_solve_buffers(mhs_solver *s, int nrhs, bool mixed) {
 }
 
 // Launch `chunk` PCG iterations, through a cached hipGraph when enabled.
+// The element-by-element operator of the current solve, or null (assembled
+// matrix).  Only the FP32 iteration uses it.
+const DevEbe *cur_ebe(const mhs_solver *s) { return s->last_ebe ? &s->dE : nullptr; }
+
 template <typename MT, typename VT>
 hipError_t one_iteration(mhs_solver *s, int nrhs, bool amg, const MT *val, const VT *dinv, VT *x, VT *r,
                          VT *p, VT *q, VT *z, double tol2) {
+    const DevEbe *E = std::is_same<VT, float>::value ? cur_ebe(s) : nullptr;
     if (amg)
         return launch_pcg_iteration_amg<MT, VT>(s->stream, s->dT, s->dA, nrhs, val, dinv, x, r, p, q, z, s->R,
-                                                s->S, s->val64, s->b, s->x64, tol2);
+                                                s->S, s->val64, s->b, s->x64, tol2, E, s->r64);
     return launch_pcg_iteration<MT, VT>(s->stream, s->dT, nrhs, val, dinv, x, r, p, q, s->R, s->S, s->val64,
-                                        s->b, s->x64, tol2);
+                                        s->b, s->x64, tol2, E, s->r64);
 }
 
 template <typename MT, typename VT>
@@ -808,7 +865,8 @@ int run_chunk(mhs_solver *s, int nrhs, int chunk, bool use_graph, const MT *val,
     if (use_graph) {
         GraphCache &G = s->graphs[chunk];
         const int shape[5] = {s->dT.spmv_tb, s->dT.spmv_xcd, s->dT.sweep_tb, s->dT.sweep_xcd, s->dT.tail_rows};
-        if (!(G.exec && G.precision == precision && G.nrhs == nrhs && G.chunk == chunk &&
+        const int ebe = cur_ebe(s) ? 1 : 0;
+        if (!(G.exec && G.precision == precision && G.nrhs == nrhs && G.chunk == chunk && G.ebe == ebe &&
               G.key == (const void *)val && G.tol2 == tol2 && G.rf2 == s->S.rf2 && G.amg == int(amg) &&
               std::equal(G.omega, G.omega + kMaxLevels, s->dA.omega_l) && G.alpha == s->dA.alpha &&
               G.nu == s->dA.nu &&
@@ -835,6 +893,7 @@ int run_chunk(mhs_solver *s, int nrhs, int chunk, bool use_graph, const MT *val,
             G.precision = precision;
             G.nrhs = nrhs;
             G.chunk = chunk;
+            G.ebe = ebe;
             G.key = val;
             G.tol2 = tol2;
             G.rf2 = s->S.rf2;
@@ -955,7 +1014,8 @@ int estimate_rho(mhs_solver *s) {
 // once per structure and (nrhs, precision, preconditioner); the choice only
 // changes the order of floating-point partial sums.
 int autotune(mhs_solver *s, int nrhs, bool mixed, bool amg, double *tune_ms) {
-    const int key = nrhs * 4 + (mixed ? 2 : 0) + (amg ? 1 : 0);
+    const DevEbe *E = mixed ? cur_ebe(s) : nullptr;
+    const int key = (nrhs * 4 + (mixed ? 2 : 0) + (amg ? 1 : 0)) * 2 + (E ? 1 : 0);
     auto it = s->shapes.find(key);
     *tune_ms = 0.0;
     if (it == s->shapes.end()) {
@@ -980,8 +1040,10 @@ int autotune(mhs_solver *s, int nrhs, bool mixed, bool amg, double *tune_ms) {
         };
         Shape best;
         float bs = 1e30f, bw = 1e30f;
+        // the assembled-matrix SpMV (not run when the operator is applied cell by cell)
         for (int tb : {256, 512, 1024})
             for (int xcd : {1, 0}) {
+                if (E) break;
                 s->dT.spmv_tb = tb;
                 s->dT.spmv_xcd = xcd;
                 const float ms = time_it([&]() {
@@ -1021,7 +1083,7 @@ int autotune(mhs_solver *s, int nrhs, bool mixed, bool amg, double *tune_ms) {
                 s->dT.tail_rows = rows;
                 const float ms = time_it([&]() {
                     return mixed ? launch_vcycle<float>(s->stream, s->dT, s->dA, nrhs, s->val32, s->dinv32, s->r32,
-                                                        s->z32, s->R)
+                                                        s->z32, s->R, E)
                                  : launch_vcycle<double>(s->stream, s->dT, s->dA, nrhs, s->val64, s->dinv64,
                                                          s->r64, s->z64, s->R);
                 });
@@ -1241,11 +1303,16 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
         HIPCHECK(hipMemcpyAsync(s->in_rhomob, rhomob, sizeof(double) * T.nc, kind, s->stream));
         d_tm = s->in_totmob; d_rm = s->in_rhomob;
     }
+    // the level-0 operator of the FP32 iteration: cell by cell when the
+    // structure allows it and no instance is regularised (the doubled entry
+    // of :1398-1400 is not a cell contribution)
+    s->last_ebe = mixed && s->ebe_ok && opt.element_operator != 0 &&
+                  std::none_of(reg.begin(), reg.end(), [](int v) { return v != 0; });
     // --- assembly -----------------------------------------------------------
     s->last_aniso = binv_use != nullptr;
     const double *binv = binv_use ? binv_use : s->binv, *gflux = gflux_use ? gflux_use : s->gflux;
     HIPCHECK(launch_assemble(s->stream, s->dT, T.max_ncf, nrhs, binv, gflux, d_tm, d_rm, d_src,
-                             s->ftype, d_bv, s->C));
+                             s->ftype, d_bv, s->C, cur_ebe(s)));
     const int dguess = opt.linsolver_type != 0;  // AMG types start from the Dirichlet guess (:1521-1532)
     HIPCHECK(launch_fill(s->stream, s->dT, nrhs, s->C, s->ftype, d_bv, s->inst_reg, dguess, s->val64,
                          mixed ? s->val32 : nullptr, s->b, s->dinv64, s->dinv32, s->x64));
@@ -1386,6 +1453,9 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
         rep->tail_rows = s->dT.tail_rows;
         rep->tune_ms = tune_ms;
         rep->ap_nnz = amg ? s->dA.ap_slots : 0;
+        rep->element_operator = s->last_ebe ? 1 : 0;
+        rep->colours = s->last_ebe ? s->EL.ncolours : 0;
+        rep->element_values = s->last_ebe ? s->EL.nval : 0;
     }
     if (!converged)
         return fail(s, MHS_ERR_NOT_CONVERGED,
@@ -1800,7 +1870,8 @@ int mhs_get_face_rows(mhs_solver *s, int *face_row) {
 int mhs_time_kernel(mhs_solver *s, int which, int reps, double *avg_ms) {
     if (!s || !avg_ms || reps < 1) return MHS_ERR_INVALID;
     if (!s->solved) return fail(s, MHS_ERR_STATE, "time_kernel: call solve first");
-    if (which != 0) return fail(s, MHS_ERR_INVALID, "time_kernel: unknown kernel id");
+    if (which != 0 && which != 1) return fail(s, MHS_ERR_INVALID, "time_kernel: unknown kernel id");
+    const DevEbe *E = (which == 0 && s->cap_mixed) ? cur_ebe(s) : nullptr;
     HIPCHECK(hipSetDevice(s->device));
     const size_t nsys = (size_t)s->T.ninst * s->cap_nrhs;
     std::vector<int> ones(nsys, 1);
@@ -1813,7 +1884,9 @@ int mhs_time_kernel(mhs_solver *s, int which, int reps, double *avg_ms) {
     for (int it = 0; it <= reps && e == hipSuccess; ++it) {
         if (it == 1) e = hipEventRecord(e0, s->stream);
         if (e != hipSuccess) break;
-        if (s->cap_mixed)
+        if (E)
+            e = launch_ebe_spmv(s->stream, s->T.max_ncf, s->cap_nrhs, *E, s->p32, s->q32, s->S);
+        else if (s->cap_mixed)
             e = launch_spmv<float, float>(s->stream, s->dT, s->cap_nrhs, s->val32, s->p32, s->q32, s->R, s->S);
         else
             e = launch_spmv<double, double>(s->stream, s->dT, s->cap_nrhs, s->val64, s->p64, s->q64, s->R, s->S);

@@ -1,0 +1,102 @@
+// Element-by-element layout of the level-0 face operator.  See mhs_ebe.h.
+#include "mhs_ebe.h"
+
+#include <algorithm>
+#include <numeric>
+
+namespace mhs {
+
+std::string build_ebe(EbeLayout &E, const Topology &T) {
+    E = EbeLayout();
+    if (T.has_periodic) return "periodic faces";
+    if ((long long)T.nfp > (long long)kEbeRow) return "too many face rows";
+    const int nc = T.nc;
+    // greedy colouring of the cells in index order: the smallest colour that
+    // no already coloured face neighbour has (a checkerboard on box grids)
+    std::vector<int> colour(nc, -1);
+    int ncol = 0;
+    for (int c = 0; c < nc; ++c) {
+        unsigned used = 0;
+        for (int h = T.facepos[c]; h < T.facepos[c + 1]; ++h) {
+            const int o = T.nb[h];
+            if (o >= 0 && colour[o] >= 0) used |= 1u << colour[o];
+        }
+        int k = 0;
+        while (used & (1u << k)) ++k;
+        if (k >= kMaxColours) return "more than 8 colours";
+        colour[c] = k;
+        ncol = std::max(ncol, k + 1);
+    }
+    // instance of every cell
+    std::vector<int> inst(nc);
+    for (int t = 0; t < T.ninst; ++t)
+        for (int c = T.inst_cellpos[t]; c < T.inst_cellpos[t + 1]; ++c) inst[c] = t;
+    // segments: cells sorted by (colour, instance, face count, index)
+    auto key = [&](int c) {
+        return ((long long)colour[c] * T.ninst + inst[c]) * (kMaxFaces + 1) + (T.facepos[c + 1] - T.facepos[c]);
+    };
+    std::vector<int> order(nc);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key(a) < key(b); });
+    E.ncolours = ncol;
+    E.colour_blk.assign(ncol + 1, 0);
+    E.cell_seg.assign(nc, -1);
+    E.cell_local.assign(nc, -1);
+    std::vector<int> seg_colour;
+    for (size_t i = 0; i < order.size();) {
+        size_t j = i;
+        const long long k = key(order[i]);
+        while (j < order.size() && key(order[j]) == k) ++j;
+        const int c0 = order[i];
+        const int n = T.facepos[c0 + 1] - T.facepos[c0];
+        const int count = (int)(j - i);
+        const int len = (count + kEbeBlock - 1) / kEbeBlock * kEbeBlock;
+        const int s = E.nseg++;
+        E.seg_n.push_back(n);
+        E.seg_inst.push_back(inst[c0]);
+        E.seg_count.push_back(count);
+        E.seg_len.push_back(len);
+        E.seg_blk0.push_back(E.nblk);
+        E.seg_val.push_back(E.nval);
+        E.seg_dof.push_back(E.ndof);
+        seg_colour.push_back(colour[c0]);
+        for (int b = 0; b < len / kEbeBlock; ++b) E.blk_seg.push_back(s);
+        E.nblk += len / kEbeBlock;
+        E.nval += (long long)n * (n + 1) / 2 * len;
+        E.ndof += (long long)n * len;
+        for (size_t q = i; q < j; ++q) {
+            E.cell_seg[order[q]] = s;
+            E.cell_local[order[q]] = (int)(q - i);
+        }
+        i = j;
+    }
+    for (int k = 0, b = 0; k <= ncol; ++k) {
+        while (b < E.nblk && seg_colour[E.blk_seg[b]] < k) ++b;
+        E.colour_blk[k] = b;
+    }
+    // face rows with the first/last-writer flags, component-major per segment
+    E.dof.assign((size_t)E.ndof, 0u);
+    for (int c = 0; c < nc; ++c) {
+        const int s = E.cell_seg[c], l = E.cell_local[c], len = E.seg_len[s];
+        const int f0 = T.facepos[c], n = T.facepos[c + 1] - f0;
+        for (int j = 0; j < n; ++j) {
+            const int h = f0 + j, o = T.nb[h];
+            uint32_t d = (uint32_t)T.hf_dof[h];
+            if (o < 0 || colour[c] < colour[o]) d |= kEbeFirst;
+            if (o < 0 || colour[c] > colour[o]) d |= kEbeLast;
+            E.dof[(size_t)E.seg_dof[s] + (size_t)j * len + l] = d;
+        }
+    }
+    // partial-sum slots: each instance's blocks (over all colours) contiguous
+    std::vector<int> per_inst(T.ninst + 1, 0);
+    for (int b = 0; b < E.nblk; ++b) ++per_inst[E.seg_inst[E.blk_seg[b]] + 1];
+    E.inst_slot.assign(T.ninst + 1, 0);
+    for (int t = 0; t < T.ninst; ++t) E.inst_slot[t + 1] = E.inst_slot[t] + per_inst[t + 1];
+    std::vector<int> next(E.inst_slot.begin(), E.inst_slot.end() - 1);
+    E.blk_slot.resize(E.nblk);
+    for (int b = 0; b < E.nblk; ++b) E.blk_slot[b] = next[E.seg_inst[E.blk_seg[b]]]++;
+    E.nslots = E.nblk;
+    return "";
+}
+
+}  // namespace mhs

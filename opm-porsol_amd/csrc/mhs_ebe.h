@@ -1,0 +1,47 @@
+// Element-by-element layout of the level-0 face operator (host side).
+//
+// The face matrix is the sum over cells of each cell's local Schur complement
+// (after the Dirichlet rules).  Applying it cell by cell reads n(n+1)/2 packed
+// values and n face rows per cell (21 + 6 for a hexahedron) instead of every
+// face row's entries with their column indices (11 + 11 per face of a
+// hexahedral grid, three faces per cell).  Cells are coloured so that no two
+// cells of one colour share a face; one launch per colour then scatters to
+// face rows without atomics.  Cells are grouped into segments of one colour,
+// one instance and one face count, each padded to whole 256-cell blocks, and
+// their values are stored component-major inside the segment (entry q of the
+// cell in slot i at seg_val + q * len + i) so that a wavefront's loads
+// coalesce.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "mhs_structure.h"
+
+namespace mhs {
+
+constexpr int kEbeBlock = 256;      // cells per workgroup
+constexpr int kMaxColours = 8;
+// Face-row flags in the element dof table: the cell is the first (lowest
+// colour) of the face's cells to write it, and the last (highest colour).
+constexpr uint32_t kEbeFirst = 1u << 30, kEbeLast = 1u << 31, kEbeRow = (1u << 30) - 1;
+
+struct EbeLayout {
+    int ncolours = 0, nseg = 0, nblk = 0, nslots = 0;
+    long long nval = 0, ndof = 0;
+    std::vector<int> colour_blk;             // [ncolours + 1] first block of each colour
+    std::vector<int> blk_seg, blk_slot;      // [nblk] segment, partial-sum slot
+    std::vector<int> seg_n, seg_inst, seg_count, seg_len, seg_blk0;
+    std::vector<long long> seg_val, seg_dof; // value and dof bases of each segment
+    std::vector<int> inst_slot;              // [ninst + 1] partial slots per instance (contiguous)
+    std::vector<int> cell_seg, cell_local;   // [nc] segment and slot of each cell
+    std::vector<uint32_t> dof;               // [ndof] face row | kEbeFirst | kEbeLast
+};
+
+// Builds the layout for a structure; returns an empty string, or the reason
+// why the element-by-element operator does not apply (periodic faces, whose
+// constraint rows are not a sum of cell matrices; too many colours).
+std::string build_ebe(EbeLayout &E, const Topology &T);
+
+}  // namespace mhs

@@ -65,6 +65,24 @@ struct DevScal {
     double rf2;        // replacement trigger: |r|^2 <= rf2 * |r at last replacement|^2
 };
 
+// Element-by-element level-0 operator (see mhs_ebe.h): per segment of cells
+// of one colour, instance and face count, the packed local matrices S'_c
+// component-major and the face rows with first/last-writer flags.
+constexpr int kEbeColours = 8;
+struct DevEbe {
+    int ncolours = 0, nblk = 0;
+    int colour_blk[kEbeColours + 1] = {};
+    const int *blk_seg = nullptr, *blk_slot = nullptr;
+    const int *seg_n = nullptr, *seg_inst = nullptr, *seg_count = nullptr, *seg_len = nullptr,
+              *seg_blk0 = nullptr;
+    const long long *seg_val = nullptr, *seg_dof = nullptr;
+    const int *cell_seg = nullptr, *cell_local = nullptr;
+    const uint32_t *dof = nullptr;
+    double *val64 = nullptr;   // S'_c in FP64 (residual replacement)
+    float *val32 = nullptr;    // the same rounded to FP32 (PCG products)
+    DevRed R{};                // partial sums: one slot per block, instance-contiguous
+};
+
 // Multigrid hierarchy on the device (level 0 is the SELL face matrix).
 constexpr int kMaxLevels = 16;
 struct DevAmgLevel {
@@ -133,10 +151,18 @@ hipError_t launch_aniso_ip(hipStream_t st, const DevTopo &T, const DevGeom &G, i
 hipError_t launch_gravity_term(hipStream_t st, const DevTopo &T, const DevGeom &G, const double *grav,
                                const double *omega, double *gterm);
 
+// E (may be null): also write each cell's S'_c (the local Schur complement
+// with Dirichlet rows reduced to their diagonal and Dirichlet columns zero) in
+// the element-by-element layout, FP64 and FP32.
 hipError_t launch_assemble(hipStream_t st, const DevTopo &T, int maxn, int nrhs,
                            const double *binv_pk, const double *gflux, const double *totmob,
                            const double *rhomob, const double *src, const uint8_t *ftype,
-                           const double *bcval, DevCellSys C);
+                           const double *bcval, DevCellSys C, const DevEbe *E = nullptr);
+
+// q = A p through the element-by-element operator (one launch per colour),
+// with the <p, q> partials in E.R (profiling; the PCG launches it itself).
+hipError_t launch_ebe_spmv(hipStream_t st, int maxn, int nrhs, const DevEbe &E, const float *p, float *q,
+                           DevScal S);
 
 hipError_t launch_fill(hipStream_t st, const DevTopo &T, int nrhs, const DevCellSys &C,
                        const uint8_t *ftype, const double *bcval, const int *inst_reg,
@@ -145,11 +171,13 @@ hipError_t launch_fill(hipStream_t st, const DevTopo &T, int nrhs, const DevCell
 
 // One Jacobi-PCG iteration (spmv_pq, update_xr, update_p; in FP32 also the
 // fused FP64 residual replacement, which needs val64, b and x64).
+// E (FP32 only, may be null): the SpMV runs element by element; r64 is the
+// FP64 accumulator of its fused residual replacement.
 template <typename MT, typename VT>
 hipError_t launch_pcg_iteration(hipStream_t st, const DevTopo &T, int nrhs, const MT *val,
                                 const VT *dinv, VT *x, VT *r, VT *p, VT *q, DevRed R,
                                 DevScal S, const double *val64, const double *b, double *x64,
-                                double tol2);
+                                double tol2, const DevEbe *E = nullptr, double *r64 = nullptr);
 
 // The SpMV + <p, Ap> kernel alone (profiling).
 template <typename MT, typename VT>
@@ -194,14 +222,18 @@ hipError_t launch_sweep0(hipStream_t st, const DevTopo &T, int nrhs, const VT *v
 template <typename VT>
 hipError_t launch_post0(hipStream_t st, const DevTopo &T, const DevAmg &A, int nrhs, const VT *dinv0,
                         const VT *r, VT *z, DevRed R);
+// E (FP32 only, may be null): the SpMV and, with one smoothing sweep, the
+// level-0 pre-smoothing residual run element by element; r64 is the FP64
+// accumulator of the fused residual replacement (FP32 iteration).
 template <typename VT>
 hipError_t launch_vcycle(hipStream_t st, const DevTopo &T, const DevAmg &A, int nrhs, const VT *val0,
-                         const VT *dinv0, const VT *r, VT *z, DevRed R);
+                         const VT *dinv0, const VT *r, VT *z, DevRed R, const DevEbe *E = nullptr);
 template <typename MT, typename VT>
 hipError_t launch_pcg_iteration_amg(hipStream_t st, const DevTopo &T, const DevAmg &A, int nrhs,
                                     const MT *val, const VT *dinv, VT *x, VT *r, VT *p, VT *q, VT *z,
                                     DevRed R, DevScal S, const double *val64, const double *b,
-                                    double *x64, double tol2);
+                                    double *x64, double tol2, const DevEbe *E = nullptr,
+                                    double *r64 = nullptr);
 // z = M^-1 r, rz, p = z (start of the preconditioned iteration after
 // launch_pcg_start64 / launch_reliable(init)).
 template <typename VT>

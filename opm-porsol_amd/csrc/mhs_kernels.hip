@@ -379,7 +379,7 @@ __global__ __launch_bounds__(64) void k_assemble(DevTopo T, const double *__rest
                                                  const double *__restrict__ src,
                                                  const uint8_t *__restrict__ ftype,
                                                  const double *__restrict__ bcval,
-                                                 DevCellSys C) {
+                                                 DevCellSys C, DevEbe E) {
     constexpr int NP = MAXN * (MAXN + 1) / 2;
     const int c = blockIdx.x * 64 + threadIdx.x;
     if (c >= T.nc) return;
@@ -422,6 +422,23 @@ __global__ __launch_bounds__(64) void k_assemble(DevTopo T, const double *__rest
         if (i < n) C.F[f0 + i] = F[i];
     }
     C.L[c] = L;
+    if (E.val64) {
+        // S'_c for the element-by-element operator: a Dirichlet row keeps its
+        // diagonal only (:1832-1835), a Dirichlet column is dropped (:1880-1896)
+        const int sg = E.cell_seg[c], len = E.seg_len[sg];
+        const size_t vb = (size_t)E.seg_val[sg] + E.cell_local[c];
+#pragma unroll
+        for (int i = 0; i < MAXN; ++i)
+#pragma unroll
+            for (int j = i; j < MAXN; ++j) {
+                if (j >= n) continue;
+                const bool drop = i != j && (ft[i] == FT_DIRICHLET || ft[j] == FT_DIRICHLET);
+                const double v = drop ? 0.0 : S[pidx(i, j, MAXN)];
+                const size_t o = vb + (size_t)pidx(i, j, n) * len;
+                E.val64[o] = v;
+                E.val32[o] = (float)v;
+            }
+    }
 #pragma unroll
     for (int k = 0; k < NRHS; ++k) {
         const double gk = src[(size_t)k * T.nc + c] - gsum;
@@ -661,6 +678,163 @@ static void launch_spmv_kernel(hipStream_t st, unsigned nblk, const DevTopo &T, 
         k_spmv_pq8<MT, VT, NRHS, REL, TB, CH><<<nblk, TB, 0, st>>>(T, val, p, q, R, S, L);
     else
         k_spmv_pq<MT, VT, NRHS, REL, TB, CH><<<nblk, TB, 0, st>>>(T, val, p, q, R, S, L);
+}
+
+// ---------------------------------------------------------------------------
+// Element-by-element level-0 operator (FP32 iteration).  The face matrix is the
+// sum over cells of S'_c, the local Schur complement after the rules of
+// addCellContrib (IncompFlowSolverHybrid.hpp:1817-1901) that keep only the
+// diagonal of a Dirichlet row and drop Dirichlet columns.  One launch per
+// colour (no two cells of a colour share a face), one thread per cell:
+// y = S'_c x_c, then each face row is stored by the first of its cells to
+// write it, added to by the others, and finished by the last:
+//   EBE_SPMV   q = A p and the <p, q> partials.  With REL, when the instance's
+//              FP64 residual replacement is due, also r = b - A64 x64: the
+//              FP64 values are read once, the FP32 product uses their rounding
+//              (the values every other iteration reads), the FP64 sums build
+//              up in r64, and the finishing cell writes r32 and the partials
+//              |r64|^2, r.M^-1 r, |r|^2.
+//   EBE_RESID  res = r - A x (the multigrid pre-smoothing residual).
+// Per hexahedron: 21 packed values and 6 face rows, against 11 entries and 11
+// column indices per face row of the assembled matrix.
+enum { EBE_SPMV = 0, EBE_RESID = 1 };
+constexpr uint32_t kEbeFirstD = 1u << 30, kEbeLastD = 1u << 31, kEbeRowD = (1u << 30) - 1;
+
+template <int MAXN, int NRHS, int MODE, bool REL>
+__global__ __launch_bounds__(256) void k_ebe(DevEbe E, int colour, Gate gt, DevScal S,
+                                             const float *__restrict__ x, float *__restrict__ out,
+                                             const float *__restrict__ rin, RelArgs L,
+                                             double *__restrict__ r64) {
+    const int b = E.colour_blk[colour] + (int)blockIdx.x;
+    const int seg = E.blk_seg[b];
+    const int s = E.seg_inst[seg];
+    int act[NRHS];
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) {
+        act[k] = gt.act ? gt.act[s * NRHS + k] : 1;
+        any |= act[k] != 0;
+    }
+    if (!any) return;                       // block-uniform: one instance per block
+    int u[NRHS];
+    bool rel = false;
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) {
+        u[k] = (REL && act[k]) ? S.upd[s * NRHS + k] : 0;
+        rel |= u[k] != 0;
+    }
+    const int n = E.seg_n[seg], len = E.seg_len[seg];
+    const int local = (b - E.seg_blk0[seg]) * 256 + (int)threadIdx.x;
+    const bool valid = local < E.seg_count[seg];
+    constexpr int NQ = MODE == EBE_SPMV ? (REL ? 4 * NRHS : NRHS) : 1;
+    double part[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) part[q] = 0.0;
+    if (valid) {
+        const uint32_t *dp = E.dof + E.seg_dof[seg] + local;
+        uint32_t d[MAXN];
+        float xv[MAXN][NRHS], y[MAXN][NRHS];
+#pragma unroll
+        for (int j = 0; j < MAXN; ++j) d[j] = (j < n) ? __builtin_nontemporal_load(dp + (size_t)j * len) : 0u;
+#pragma unroll
+        for (int j = 0; j < MAXN; ++j)
+#pragma unroll
+            for (int k = 0; k < NRHS; ++k) {
+                xv[j][k] = (j < n) ? x[(size_t)(d[j] & kEbeRowD) * NRHS + k] : 0.0f;
+                y[j][k] = 0.0f;
+            }
+        const size_t vb = (size_t)E.seg_val[seg] + local;
+        double y64[REL ? MAXN : 1][NRHS];
+        if (!(REL && rel)) {
+            const float *v = E.val32 + vb;
+#pragma unroll
+            for (int i = 0; i < MAXN; ++i)
+#pragma unroll
+                for (int j = i; j < MAXN; ++j) {
+                    if (j >= n) continue;
+                    const float a = __builtin_nontemporal_load(v + (size_t)pidx(i, j, n) * len);
+#pragma unroll
+                    for (int k = 0; k < NRHS; ++k) {
+                        y[i][k] += a * xv[j][k];
+                        if (j != i) y[j][k] += a * xv[i][k];
+                    }
+                }
+        } else if constexpr (REL) {
+            const double *v = E.val64 + vb;
+            double xd[MAXN][NRHS];
+#pragma unroll
+            for (int j = 0; j < MAXN; ++j)
+#pragma unroll
+                for (int k = 0; k < NRHS; ++k) {
+                    xd[j][k] = (j < n && u[k]) ? L.x64[(size_t)(d[j] & kEbeRowD) * NRHS + k] : 0.0;
+                    y64[j][k] = 0.0;
+                }
+#pragma unroll
+            for (int i = 0; i < MAXN; ++i)
+#pragma unroll
+                for (int j = i; j < MAXN; ++j) {
+                    if (j >= n) continue;
+                    const double a = __builtin_nontemporal_load(v + (size_t)pidx(i, j, n) * len);
+                    const float af = (float)a;
+#pragma unroll
+                    for (int k = 0; k < NRHS; ++k) {
+                        y[i][k] += af * xv[j][k];
+                        y64[i][k] += a * xd[j][k];
+                        if (j != i) {
+                            y[j][k] += af * xv[i][k];
+                            y64[j][k] += a * xd[i][k];
+                        }
+                    }
+                }
+        }
+#pragma unroll
+        for (int j = 0; j < MAXN; ++j) {
+            if (j >= n) continue;
+            const uint32_t dj = d[j];
+            const size_t row = dj & kEbeRowD;
+            const bool first = (dj & kEbeFirstD) != 0, last = (dj & kEbeLastD) != 0;
+#pragma unroll
+            for (int k = 0; k < NRHS; ++k) {
+                const size_t i = row * NRHS + k;
+                if constexpr (MODE == EBE_SPMV) {
+                    const float qv = first ? y[j][k] : out[i] + y[j][k];
+                    out[i] = qv;
+                    if (last) part[k] += (double)xv[j][k] * (double)qv;
+                } else {
+                    out[i] = (first ? rin[i] : out[i]) - y[j][k];
+                }
+            }
+            if constexpr (REL && MODE == EBE_SPMV) {
+                if (rel) {
+#pragma unroll
+                    for (int k = 0; k < NRHS; ++k) {
+                        if (!u[k]) continue;
+                        const size_t i = row * NRHS + k;
+                        const double rv = (first ? L.b[i] : r64[i]) - y64[j][k];
+                        if (!last) {
+                            r64[i] = rv;
+                            continue;
+                        }
+                        const float rf = (float)rv;
+                        const float dg = L.dinv[row];
+                        L.r32[i] = rf;
+                        part[NRHS + 3 * k] += rv * rv;
+                        part[NRHS + 3 * k + 1] += (double)rf * (double)(dg * rf);
+                        part[NRHS + 3 * k + 2] += (double)rf * (double)rf;
+                    }
+                }
+            }
+        }
+    }
+    if constexpr (MODE == EBE_SPMV) {
+        const int slot = E.blk_slot[b];
+        if constexpr (REL) {
+            if (rel) block_partials<NQ>(part, slot, E.R);
+            else block_partials_n<NQ, NRHS>(part, slot, E.R);
+        } else {
+            block_partials<NQ>(part, slot, E.R);
+        }
+    }
 }
 
 template <typename VT, int NRHS>
@@ -2336,6 +2510,20 @@ static hipError_t with_maxn(int maxn, Fn fn) {
 
 static inline unsigned cell_blocks(int nc) { return (unsigned)((nc + 63) / 64); }
 
+// The element-by-element operator: one launch per colour.
+template <int NR, int MODE, bool REL>
+static void ebe_apply(hipStream_t st, int maxn, const DevEbe &E, const Gate &gt, const DevScal &S, const float *x,
+                      float *out, const float *rin, const RelArgs &L, double *r64) {
+    with_maxn(maxn, [&](auto mn) {
+        constexpr int MN = decltype(mn)::value;
+        for (int c = 0; c < E.ncolours; ++c) {
+            const int nb = E.colour_blk[c + 1] - E.colour_blk[c];
+            if (nb > 0) k_ebe<MN, NR, MODE, REL><<<nb, 256, 0, st>>>(E, c, gt, S, x, out, rin, L, r64);
+        }
+    });
+}
+
+
 // Workgroup size of the row-block kernels (each workgroup owns 1024 rows).
 template <class Fn>
 static hipError_t with_tb(int tb, Fn fn) {
@@ -2451,15 +2639,16 @@ hipError_t launch_static_ip(hipStream_t st, const DevTopo &T, const DevGeom &G, 
 hipError_t launch_assemble(hipStream_t st, const DevTopo &T, int maxn, int nrhs,
                            const double *binv_pk, const double *gflux, const double *totmob,
                            const double *rhomob, const double *src, const uint8_t *ftype,
-                           const double *bcval, DevCellSys C) {
+                           const double *bcval, DevCellSys C, const DevEbe *E) {
     if (T.nc == 0) return hipSuccess;
+    const DevEbe Ev = E ? *E : DevEbe{};
     hipError_t e = hipSuccess;
     hipError_t e2 = with_maxn(maxn, [&](auto mn) {
         constexpr int MN = decltype(mn)::value;
         e = with_nrhs(nrhs, [&](auto nr) {
             constexpr int NR = decltype(nr)::value;
             k_assemble<MN, NR><<<cell_blocks(T.nc), 64, 0, st>>>(T, binv_pk, gflux, totmob, rhomob,
-                                                                 src, ftype, bcval, C);
+                                                                 src, ftype, bcval, C, Ev);
         });
     });
     return e != hipSuccess ? e : e2;
@@ -2490,20 +2679,31 @@ template <typename MT, typename VT>
 hipError_t launch_pcg_iteration(hipStream_t st, const DevTopo &T, int nrhs, const MT *val,
                                 const VT *dinv, VT *x, VT *r, VT *p, VT *q, DevRed R,
                                 DevScal S, const double *val64, const double *b, double *x64,
-                                double tol2) {
+                                double tol2, const DevEbe *E, double *r64) {
     return with_nrhs(nrhs, [&](auto nr) {
         constexpr int NR = decltype(nr)::value;
         const FinArgs A{tol2, 0};
         constexpr bool REL = std::is_same<VT, float>::value;
         RelArgs L{};
         if constexpr (REL) L = RelArgs{val64, x64, b, dinv, r};
-        with_tb(T.spmv_tb, [&](auto tb) {
-            constexpr int TBv = decltype(tb)::value;
-            with_chunk(T.max_w, [&](auto ch) {
-                launch_spmv_kernel<MT, VT, NR, REL, TBv, decltype(ch)::value>(st, T.nblk, T, val, p, q, R, S, L);
+        bool ebe = false;
+        if constexpr (REL) {
+            if (E) {
+                ebe_apply<NR, EBE_SPMV, true>(st, T.max_ncf, *E, Gate{S.act}, S, p, q, nullptr, L, r64);
+                k_finish<4 * NR, FinPQ<NR, true>><<<T.ninst, 256, 0, st>>>(E->R, S, A);
+                ebe = true;
+            }
+        }
+        if (!ebe) {
+            with_tb(T.spmv_tb, [&](auto tb) {
+                constexpr int TBv = decltype(tb)::value;
+                with_chunk(T.max_w, [&](auto ch) {
+                    launch_spmv_kernel<MT, VT, NR, REL, TBv, decltype(ch)::value>(st, T.nblk, T, val, p, q, R, S,
+                                                                                  L);
+                });
             });
-        });
-        k_finish<REL ? 4 * NR : NR, FinPQ<NR, REL>><<<T.ninst, 256, 0, st>>>(R, S, A);
+            k_finish<REL ? 4 * NR : NR, FinPQ<NR, REL>><<<T.ninst, 256, 0, st>>>(R, S, A);
+        }
         k_update_xr<VT, NR><<<T.nblk, 256, 0, st>>>(T, dinv, x, r, p, q, R, S);
         k_finish<2 * NR, FinXR<NR>><<<T.ninst, 256, 0, st>>>(R, S, A);
         k_update_p<VT, NR><<<T.nblk, 256, 0, st>>>(T, dinv, r, p, x, REL ? x64 : nullptr, S);
@@ -2515,12 +2715,12 @@ template hipError_t launch_pcg_iteration<double, double>(hipStream_t, const DevT
                                                          double *, double *, double *,
                                                          double *, DevRed, DevScal,
                                                          const double *, const double *,
-                                                         double *, double);
+                                                         double *, double, const DevEbe *, double *);
 template hipError_t launch_pcg_iteration<float, float>(hipStream_t, const DevTopo &, int,
                                                        const float *, const float *, float *,
                                                        float *, float *, float *, DevRed,
                                                        DevScal, const double *, const double *,
-                                                       double *, double);
+                                                       double *, double, const DevEbe *, double *);
 
 template <typename MT, typename VT>
 hipError_t launch_spmv(hipStream_t st, const DevTopo &T, int nrhs, const MT *val, const VT *p,
@@ -2694,7 +2894,8 @@ static void sweep0(hipStream_t st, const DevTopo &T, const VT *val0, const VT *d
 // values nothing reads.
 template <typename VT, int NR>
 static void vcycle_impl(hipStream_t st, const Gate &gt, const DevTopo &T, const DevAmg &A, const VT *val0,
-                        const VT *dinv0, const VT *r, VT *z, DevRed R, bool x0_ready) {
+                        const VT *dinv0, const VT *r, VT *z, DevRed R, bool x0_ready,
+                        const DevEbe *E = nullptr) {
     const VT w = (VT)A.omega_l[0], al = (VT)A.alpha;   // w: level 0
     auto wl = [&](int l) { return (VT)A.omega_l[l]; };
     const int nu = A.nu < 1 ? 1 : A.nu;
@@ -2707,12 +2908,21 @@ static void vcycle_impl(hipStream_t st, const Gate &gt, const DevTopo &T, const 
         sweep0<VT, NR, false>(st, T, val0, dinv0, r, x0, xt0, w, R);
         std::swap(x0, xt0);
     }
-    with_chunk(T.max_w, [&](auto ch) {
-        with_tb(T.spmv_tb, [&](auto tb) {
-            constexpr int TBv = decltype(tb)::value;
-            k_vc0_res<VT, NR, decltype(ch)::value, TBv><<<T.nblk, TBv, 0, st>>>(gt, T, val0, r, x0, res0);
+    bool ebe = false;
+    if constexpr (std::is_same<VT, float>::value) {
+        if (E) {
+            ebe_apply<NR, EBE_RESID, false>(st, T.max_ncf, *E, gt, DevScal{}, x0, res0, r, RelArgs{}, nullptr);
+            ebe = true;
+        }
+    }
+    if (!ebe) {
+        with_chunk(T.max_w, [&](auto ch) {
+            with_tb(T.spmv_tb, [&](auto tb) {
+                constexpr int TBv = decltype(tb)::value;
+                k_vc0_res<VT, NR, decltype(ch)::value, TBv><<<T.nblk, TBv, 0, st>>>(gt, T, val0, r, x0, res0);
+            });
         });
-    });
+    }
     {
         const DevAmgLevel &C = A.lev[1];
         with_lanes(lanes_per_row(T.nfp, C.n), [&](auto g) {
@@ -2862,40 +3072,58 @@ template hipError_t launch_sweep0<float>(hipStream_t, const DevTopo &, int, cons
 
 template <typename VT>
 hipError_t launch_vcycle(hipStream_t st, const DevTopo &T, const DevAmg &A, int nrhs, const VT *val0,
-                         const VT *dinv0, const VT *r, VT *z, DevRed R) {
+                         const VT *dinv0, const VT *r, VT *z, DevRed R, const DevEbe *E) {
     return with_nrhs(nrhs, [&](auto nr) {
         constexpr int NR = decltype(nr)::value;
-        vcycle_impl<VT, NR>(st, Gate{}, T, A, val0, dinv0, r, z, R, false);
+        vcycle_impl<VT, NR>(st, Gate{}, T, A, val0, dinv0, r, z, R, false, E);
     });
 }
 template hipError_t launch_vcycle<double>(hipStream_t, const DevTopo &, const DevAmg &, int, const double *,
-                                          const double *, const double *, double *, DevRed);
+                                          const double *, const double *, double *, DevRed, const DevEbe *);
 template hipError_t launch_vcycle<float>(hipStream_t, const DevTopo &, const DevAmg &, int, const float *,
-                                         const float *, const float *, float *, DevRed);
+                                         const float *, const float *, float *, DevRed, const DevEbe *);
+
+hipError_t launch_ebe_spmv(hipStream_t st, int maxn, int nrhs, const DevEbe &E, const float *p, float *q,
+                           DevScal S) {
+    return with_nrhs(nrhs, [&](auto nr) {
+        constexpr int NR = decltype(nr)::value;
+        ebe_apply<NR, EBE_SPMV, false>(st, maxn, E, Gate{S.act}, S, p, q, nullptr, RelArgs{}, nullptr);
+    });
+}
 
 template <typename MT, typename VT>
 hipError_t launch_pcg_iteration_amg(hipStream_t st, const DevTopo &T, const DevAmg &A, int nrhs,
                                     const MT *val, const VT *dinv, VT *x, VT *r, VT *p, VT *q, VT *z,
                                     DevRed R, DevScal S, const double *val64, const double *b,
-                                    double *x64, double tol2) {
+                                    double *x64, double tol2, const DevEbe *E, double *r64) {
     return with_nrhs(nrhs, [&](auto nr) {
         constexpr int NR = decltype(nr)::value;
         const FinArgs Af{tol2, 0};
         constexpr bool REL = std::is_same<VT, float>::value;
         RelArgs L{};
         if constexpr (REL) L = RelArgs{val64, x64, b, dinv, r};
-        with_tb(T.spmv_tb, [&](auto tb) {
-            constexpr int TBv = decltype(tb)::value;
-            with_chunk(T.max_w, [&](auto ch) {
-                launch_spmv_kernel<MT, VT, NR, REL, TBv, decltype(ch)::value>(st, T.nblk, T, val, p, q, R, S, L);
+        bool ebe = false;
+        if constexpr (REL) {
+            if (E) {
+                ebe_apply<NR, EBE_SPMV, true>(st, T.max_ncf, *E, Gate{S.act}, S, p, q, nullptr, L, r64);
+                k_finish<4 * NR, FinPQ<NR, true>><<<T.ninst, 256, 0, st>>>(E->R, S, Af);
+                ebe = true;
+            }
+        }
+        if (!ebe) {
+            with_tb(T.spmv_tb, [&](auto tb) {
+                constexpr int TBv = decltype(tb)::value;
+                with_chunk(T.max_w, [&](auto ch) {
+                    launch_spmv_kernel<MT, VT, NR, REL, TBv, decltype(ch)::value>(st, T.nblk, T, val, p, q, R, S, L);
+                });
             });
-        });
-        k_finish<REL ? 4 * NR : NR, FinPQ<NR, REL>><<<T.ninst, 256, 0, st>>>(R, S, Af);
+            k_finish<REL ? 4 * NR : NR, FinPQ<NR, REL>><<<T.ninst, 256, 0, st>>>(R, S, Af);
+        }
         k_update_xr<VT, NR><<<T.nblk, 256, 0, st>>>(T, dinv, x, r, p, q, R, S, (VT *)A.x0, (VT)A.omega_l[0]);
         // the V-cycle's <r, z> partials after update_xr's: one finisher for both
         DevRed Rz = R;
         Rz.partials += (size_t)2 * NR * R.nblk;
-        vcycle_impl<VT, NR>(st, Gate{S.act}, T, A, val, dinv, r, z, Rz, true);
+        vcycle_impl<VT, NR>(st, Gate{S.act}, T, A, val, dinv, r, z, Rz, true, E);
         k_finish<3 * NR, FinXRZ<NR>><<<T.ninst, 256, 0, st>>>(R, S, Af);
         k_update_pz<VT, NR><<<T.nblk, 256, 0, st>>>(T, z, p, x, REL ? x64 : nullptr, S);
     });
@@ -2904,11 +3132,12 @@ template hipError_t launch_pcg_iteration_amg<double, double>(hipStream_t, const 
                                                              int, const double *, const double *, double *,
                                                              double *, double *, double *, double *, DevRed,
                                                              DevScal, const double *, const double *,
-                                                             double *, double);
+                                                             double *, double, const DevEbe *, double *);
 template hipError_t launch_pcg_iteration_amg<float, float>(hipStream_t, const DevTopo &, const DevAmg &, int,
                                                            const float *, const float *, float *, float *,
                                                            float *, float *, float *, DevRed, DevScal,
-                                                           const double *, const double *, double *, double);
+                                                           const double *, const double *, double *, double,
+                                                           const DevEbe *, double *);
 
 template <typename VT>
 hipError_t launch_amg_start(hipStream_t st, const DevTopo &T, const DevAmg &A, int nrhs, const VT *val0,

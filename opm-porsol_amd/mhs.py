@@ -47,7 +47,8 @@ class Options(ctypes.Structure):
                 ("replace_factor", ctypes.c_double), ("use_graph", ctypes.c_int),
                 ("precond", ctypes.c_int), ("smoother_weight", ctypes.c_double),
                 ("autotune", ctypes.c_int), ("spmv_threads", ctypes.c_int),
-                ("sweep_threads", ctypes.c_int), ("tail_rows", ctypes.c_int)]
+                ("sweep_threads", ctypes.c_int), ("tail_rows", ctypes.c_int),
+                ("element_operator", ctypes.c_int)]
 
 
 class Report(ctypes.Structure):
@@ -60,7 +61,9 @@ class Report(ctypes.Structure):
                 ("spmv_xcd", ctypes.c_int), ("sweep_threads", ctypes.c_int),
                 ("sweep_xcd", ctypes.c_int), ("tail_rows", ctypes.c_int),
                 ("tune_ms", ctypes.c_double), ("smoother_weights", ctypes.c_double * 8),
-                ("coarsening", (ctypes.c_int * 3) * 8), ("ap_nnz", ctypes.c_longlong)]
+                ("coarsening", (ctypes.c_int * 3) * 8), ("ap_nnz", ctypes.c_longlong),
+                ("element_operator", ctypes.c_int), ("colours", ctypes.c_int),
+                ("element_values", ctypes.c_longlong)]
 
     def as_dict(self):
         out = {}
@@ -300,7 +303,7 @@ class HybridSolver:
                 linsolver_maxit=0, prolongate_factor=1.6, smooth_steps=1, precision=FP64,
                 nrhs=1, chunk=32, replace_factor=0.01, use_graph=True, precond=-1,
                 smoother_weight=None, autotune=True, spmv_threads=0, sweep_threads=0,
-                tail_rows=-1):
+                tail_rows=-1, element_operator=True):
         o = Options()
         self._L.mhs_default_options(ctypes.byref(o))
         o.tol, o.maxit, o.linsolver_type, o.verbosity = tol, linsolver_maxit, linsolver_type, \
@@ -316,6 +319,7 @@ class HybridSolver:
         o.autotune = int(autotune)
         o.spmv_threads, o.sweep_threads = int(spmv_threads), int(sweep_threads)
         o.tail_rows = int(tail_rows)
+        o.element_operator = int(element_operator)
         return o
 
     def solve(self, totmob, rhomob, src, bc_type, bc_value, residual_tolerance=1e-8,
@@ -323,7 +327,7 @@ class HybridSolver:
               prolongate_factor=1.6, smooth_steps=1, precision="fp64", chunk=32,
               replace_factor=0.01, use_graph=True, raise_on_failure=True, precond=-1,
               smoother_weight=None, autotune=True, spmv_threads=0, sweep_threads=0,
-              tail_rows=-1):
+              tail_rows=-1, element_operator=True):
         """IncompFlowSolverHybrid::solve (:657-703) with per-cell mobilities."""
         g = self.grid
         src = np.asarray(src, dtype=np.float64)
@@ -334,7 +338,7 @@ class HybridSolver:
         o = self.options(residual_tolerance, linsolver_verbosity, linsolver_type, same_matrix,
                          linsolver_maxit, prolongate_factor, smooth_steps, precision, nrhs, chunk,
                          replace_factor, use_graph, precond, smoother_weight, autotune,
-                         spmv_threads, sweep_threads, tail_rows)
+                         spmv_threads, sweep_threads, tail_rows, element_operator)
         rep = Report()
         rc = self._L.mhs_solve(self._h, ptm, prm, psr, pbt, pbv, ctypes.byref(o),
                                ctypes.byref(rep))
