@@ -6,27 +6,31 @@
 
 namespace mhs {
 
+int greedy_colours(int nc, const int *facepos, const int *nb, std::vector<int> &colour) {
+    colour.assign(nc, -1);
+    int ncol = 0;
+    for (int c = 0; c < nc; ++c) {
+        unsigned long long used = 0;
+        for (int h = facepos[c]; h < facepos[c + 1]; ++h) {
+            const int o = nb[h];
+            if (o >= 0 && colour[o] >= 0 && colour[o] < 64) used |= 1ull << colour[o];
+        }
+        int k = 0;
+        while (k < 64 && (used & (1ull << k))) ++k;
+        colour[c] = k;
+        ncol = std::max(ncol, k + 1);
+    }
+    return ncol;
+}
+
 std::string build_ebe(EbeLayout &E, const Topology &T) {
     E = EbeLayout();
     if (T.has_periodic) return "periodic faces";
     if ((long long)T.nfp > (long long)kEbeRow) return "too many face rows";
     const int nc = T.nc;
-    // greedy colouring of the cells in index order: the smallest colour that
-    // no already coloured face neighbour has (a checkerboard on box grids)
-    std::vector<int> colour(nc, -1);
-    int ncol = 0;
-    for (int c = 0; c < nc; ++c) {
-        unsigned used = 0;
-        for (int h = T.facepos[c]; h < T.facepos[c + 1]; ++h) {
-            const int o = T.nb[h];
-            if (o >= 0 && colour[o] >= 0) used |= 1u << colour[o];
-        }
-        int k = 0;
-        while (used & (1u << k)) ++k;
-        if (k >= kMaxColours) return "more than 8 colours";
-        colour[c] = k;
-        ncol = std::max(ncol, k + 1);
-    }
+    std::vector<int> colour;
+    const int ncol = greedy_colours(nc, T.facepos.data(), T.nb.data(), colour);
+    if (ncol > kMaxColours) return "more than 8 colours";
     // instance of every cell
     std::vector<int> inst(nc);
     for (int t = 0; t < T.ninst; ++t)

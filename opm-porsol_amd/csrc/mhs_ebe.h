@@ -39,6 +39,11 @@ struct EbeLayout {
     std::vector<uint32_t> dof;               // [ndof] face row | kEbeFirst | kEbeLast
 };
 
+// Greedy colouring of the cells in index order: each cell takes the smallest
+// colour that none of its already coloured face neighbours has (the
+// checkerboard on box grids).  Returns the number of colours.
+int greedy_colours(int nc, const int *facepos, const int *nb, std::vector<int> &colour);
+
 // Builds the layout for a structure; returns an empty string, or the reason
 // why the element-by-element operator does not apply (periodic faces, whose
 // constraint rows are not a sum of cell matrices; too many colours).

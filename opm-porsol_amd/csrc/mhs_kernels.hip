@@ -705,7 +705,10 @@ __global__ __launch_bounds__(256) void k_ebe(DevEbe E, int colour, Gate gt, DevS
                                              const float *__restrict__ x, float *__restrict__ out,
                                              const float *__restrict__ rin, RelArgs L,
                                              double *__restrict__ r64) {
-    const int b = E.colour_blk[colour] + (int)blockIdx.x;
+    // XCD-contiguous block ranges: every colour pass gives an XCD the same
+    // cells' neighbourhood, so the second colour finds the rows the first
+    // one wrote in that XCD's L2
+    const int b = E.colour_blk[colour] + xcd_block(gridDim.x);
     const int seg = E.blk_seg[b];
     const int s = E.seg_inst[seg];
     int act[NRHS];
@@ -1815,6 +1818,9 @@ __global__ __launch_bounds__(1024) void k_vcycle_tail(Gate gt, const DevAmg *__r
         __syncthreads();
     }
     {
+        // coarsest level: x = A^-1 b with the dense inverse, one wavefront per
+        // (row, rhs) and four of them per pass: lane j takes column j (the
+        // row-major inverse is read coalesced), a shuffle sum finishes the row
         const DevAmgLevel &C = A.lev[L - 1];
         const int r0 = C.inst_ptr[s], m = C.inst_ptr[s + 1] - r0;
         const VT *inv;
@@ -1822,11 +1828,25 @@ __global__ __launch_bounds__(1024) void k_vcycle_tail(Gate gt, const DevAmg *__r
         else inv = A.cinv64 + A.coarse_off[s];
         const VT *b = (const VT *)C.b;
         VT *x = (VT *)C.x;
-        for (int e = tid; e < m * NRHS; e += nt) {
-            const int i = e / NRHS, k = e % NRHS;
-            VT acc = VT(0);
-            for (int j = 0; j < m; ++j) acc += inv[i * m + j] * b[(size_t)(r0 + j) * NRHS + k];
-            x[(size_t)(r0 + i) * NRHS + k] = acc;
+        const int lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
+        const int rows = m * NRHS;
+        for (int e0 = wid; e0 < rows; e0 += 4 * nw) {
+            VT acc[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                acc[u] = VT(0);
+                const int e = e0 + u * nw;
+                if (e >= rows) continue;
+                const int i = e / NRHS, k = e % NRHS;
+                for (int j = lane; j < m; j += 64) acc[u] += inv[i * m + j] * b[(size_t)(r0 + j) * NRHS + k];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) acc[u] += __shfl_xor(acc[u], off, 64);
+                const int e = e0 + u * nw;
+                if (lane == 0 && e < rows) x[(size_t)(r0 + e / NRHS) * NRHS + e % NRHS] = acc[u];
+            }
         }
         __syncthreads();
     }

@@ -99,12 +99,12 @@ std::string build_topology(Topology &T, int ninst, const int *inst_cellpos, int 
         std::vector<int> fam, len, order;
         for (int t = 0; t < ninst; ++t) {
             const int nf = T.inst_nf[t], fb = T.inst_facebase[t];
-            fam.assign(nf, 3);
+            fam.assign(nf, hf_family ? 1 << 30 : 0);
             len.assign(nf, 0);
             for (int c = T.inst_cellpos[t]; c < T.inst_cellpos[t + 1]; ++c)
                 for (int h = facepos[c]; h < facepos[c + 1]; ++h) {
                     const int d = hf_local[h];
-                    if (hf_family) fam[d] = std::min(fam[d], std::max(0, std::min(2, hf_family[h])));
+                    if (hf_family) fam[d] = std::min(fam[d], std::max(0, hf_family[h]));
                     len[d] += facepos[c + 1] - facepos[c];   // row length up to one
                 }
             order.resize(nf);

@@ -54,9 +54,10 @@ struct Topology {
 };
 
 // Builds the topology.  Rows are the faces of each instance in the order:
-// face family (hf_family, e.g. the dominant axis of the face normal; null:
-// one family), then, within windows of kRowBlock rows, estimated row length,
-// then reference dof number.  A wavefront then holds faces of one family
+// face family (hf_family, a non-negative sort key per half-face, e.g. the
+// dominant axis of the face normal; a face takes the smaller key of its two
+// half-faces; null: one family), then, within windows of kRowBlock rows,
+// estimated row length, then reference dof number.  A wavefront then holds faces of one family
 // (its gathers point the same way) and SELL slices hold rows of similar
 // length (little padding); everything defined by the reference numbering
 // (periodic redirection, the regularised dof 0, face outputs) goes through
