@@ -525,10 +525,12 @@ int mhs_init_system_structure(mhs_solver *s, const int *hf_bc_type, const int *h
     s->free_topology();
     s->structure_valid = s->ip_valid = s->solved = false;
     // face family of every half-face: the dominant axis of its normal; without
-    // periodic faces (the element-by-element operator applies) also the colour
-    // of the face's lower-numbered cell, so that the faces one colour's cells
+    // periodic faces (the element-by-element operator applies) also a class of
+    // the face's lower-numbered cell, so that the faces one colour's cells
     // touch are contiguous rows within each family (coalesced gathers and
-    // stores in the colour passes)
+    // stores in the colour passes): with logical indices the parities of
+    // (i, j, k), which also fix the face's place in the multigrid blocks (equal
+    // A P row lengths inside a class: no slice padding), else the colour
     std::vector<int> family(s->nhf, 0);
     bool periodic = false;
     for (int h = 0; h < s->nhf && hf_bc_type; ++h)
@@ -536,13 +538,20 @@ int mhs_init_system_structure(mhs_solver *s, const int *hf_bc_type, const int *h
     std::vector<int> colour;
     int ncol = 1;
     if (!periodic) ncol = greedy_colours(s->nc, s->facepos.data(), s->nb.data(), colour);
+    const bool by_parity = !s->ijk.empty();
     for (int c = 0; c < s->nc; ++c)
         for (int h = s->facepos[c]; h < s->facepos[c + 1]; ++h) {
             const double *n = &s->fnrm[3 * (size_t)h];
             const double ax = std::fabs(n[0]), ay = std::fabs(n[1]), az = std::fabs(n[2]);
             const int fam = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
             const int lower = s->nb[h] >= 0 ? std::min(c, s->nb[h]) : c;
-            family[h] = (periodic || ncol > kEbeColours) ? fam : fam * kEbeColours + colour[lower];
+            int cls = 0;
+            if (periodic || ncol > kEbeColours) cls = 0;
+            else if (by_parity) {
+                const int *q = &s->ijk[3 * (size_t)lower];
+                cls = (q[0] & 1) + 2 * (q[1] & 1) + 4 * (q[2] & 1);
+            } else cls = colour[lower];
+            family[h] = fam * 8 + cls;
         }
     std::string e = build_topology(s->T, s->ninst, s->inst_cellpos.data(), s->nc, s->facepos.data(),
                                    s->nb.data(), hf_bc_type, hf_periodic_partner, family.data());
