@@ -25,7 +25,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-u
           "-I", os.path.join(ROOT, "include"), "-I", CSRC]
 
 SOURCES = ["mhs_kernels.hip", "mhs_transport.hip", "mhs_scratch.hip", "mhs_api.cpp", "mhs_structure.cpp",
-           "mhs_amg.cpp", "mhs_ebe.cpp"]
+           "mhs_amg.cpp", "mhs_ebe.cpp", "mhs_deck.cpp"]
 
 
 def _compile(src: str) -> tuple[subprocess.Popen, str]:

@@ -3,12 +3,14 @@
 ReservoirPropertyCommon::init(deck, global_cell, perm_threshold, ...)
 (opm/porsol/common/ReservoirPropertyCommon_impl.hpp:275-300).
 
-- ``read_grdecl``: keyword / value records of a GRDECL file (values ended
-  by ``/``, ``n*v`` repeats, ``--`` comments, section keywords without data).
+- ``read_grdecl``: keyword / value records of a GRDECL or DATA file (values
+  ended by ``/``, ``n*v`` repeats, ``--`` comments; keywords the grid and
+  rock do not need are skipped).
 - ``grid_from_deck``: SPECGRID (or DIMENS), COORD and ZCORN into the solver's
   grid.  The geometry is exact for vertical pillars on a regular lattice with
-  gap-free layers (faults between columns allowed); other decks, inactive
-  cells (ACTNUM 0) and FIELD units are rejected with a ValueError.  The
+  gap-free layers (faults between columns allowed); other decks and
+  inactive cells (ACTNUM 0) are rejected with a ValueError.  FIELD lengths
+  are converted from feet.  The
   reference reads decks through dune-cornerpoint's processEclipseFormat,
   which is not in its tree: its geometry and face order are not reproduced
   here (parity unpinned, as for the corner-point generator).
@@ -28,9 +30,13 @@ from . import grids
 # opm-core units: milli * darcy (9.869233e-13 m^2)
 MILLIDARCY = 1e-3 * 9.869233e-13
 
-SECTION_KEYWORDS = {"RUNSPEC", "GRID", "EDIT", "PROPS", "REGIONS", "SOLUTION", "SUMMARY", "SCHEDULE",
-                    "NOECHO", "ECHO", "METRIC", "FIELD", "LAB", "END", "NONNC", "OLDTRAN", "NEWTRAN"}
+SECTION_KEYWORDS = {"RUNSPEC", "GRID", "EDIT", "PROPS", "REGIONS", "SOLUTION", "SUMMARY", "SCHEDULE"}
 INT_KEYWORDS = {"SPECGRID", "DIMENS", "ACTNUM"}
+# the arrays the grid and the rock need; every other keyword's data is skipped
+ARRAY_KEYWORDS = {"SPECGRID", "DIMENS", "COORD", "ZCORN", "ACTNUM", "PORO", "PERMX", "PERMY", "PERMZ",
+                  "PERMXY", "PERMXZ", "PERMYX", "PERMYZ", "PERMZX", "PERMZY"}
+FOOT = 0.3048
+_KW = re.compile(r"[A-Z][A-Z0-9_]{0,7}")
 
 
 def _values(tokens, keyword):
@@ -56,36 +62,74 @@ def _parse(rest: str, kw: str, dtype):
         raise ValueError(f"read_grdecl: unreadable value in {kw}: {e}") from None
 
 
+def _keyword_line(toks):
+    """A keyword stands alone on its line; a section keyword may carry a
+    banner after it (RUNSPEC ====) and an array keyword the first of its
+    values."""
+    if not toks or not _KW.fullmatch(toks[0]):
+        return False
+    return len(toks) == 1 or toks[0] in SECTION_KEYWORDS or toks[0] in ARRAY_KEYWORDS
+
+
 def read_grdecl(text: str) -> dict:
-    """Keyword records of a GRDECL text: {keyword: numpy array} (raw values,
-    deck units); section keywords map to an empty array."""
+    """Keyword records of a GRDECL or DATA text: {keyword: numpy array} (raw
+    values, deck units) for the arrays of ARRAY_KEYWORDS; the unit keyword
+    (METRIC, FIELD or LAB) maps to an empty array.  Other keywords (dataless
+    ones such as OIL or NONNC, TITLE with its text line, tables, RUNSPEC
+    records) are skipped up to the next keyword line; reading stops at the
+    first SOLUTION, SUMMARY, SCHEDULE or REGIONS section.  LAB units are
+    rejected."""
     deck = {}
-    text = re.sub(r"--[^\n]*", " ", text)
-    parts = text.split("/")
-    for n, part in enumerate(parts):
-        rest = part
-        while True:
-            head = rest.split(None, 1)
-            if not head or head[0] not in SECTION_KEYWORDS:
-                break
-            deck[head[0]] = np.zeros(0)
-            rest = head[1] if len(head) > 1 else ""
-        if not head:
+    lines = [ln.split("--", 1)[0] for ln in text.splitlines()]
+    li = 0
+    while li < len(lines):
+        toks = lines[li].split()
+        if not toks:
+            li += 1
             continue
-        kw = head[0]
-        rest = head[1] if len(head) > 1 else ""
-        if not re.fullmatch(r"[A-Z][A-Z0-9_]{0,7}", kw):
-            raise ValueError(f"read_grdecl: expected a keyword, got '{kw}'")
-        if n == len(parts) - 1:
-            raise ValueError(f"read_grdecl: keyword {kw} is not terminated by '/'")
-        if kw == "SPECGRID":
-            deck[kw] = np.array(rest.split()[:3], dtype=np.int64)     # NX NY NZ (then NUMRES, radial flag)
-        elif kw in INT_KEYWORDS:
-            deck[kw] = _parse(rest, kw, np.int64)
-        else:
-            deck[kw] = _parse(rest, kw, np.float64)
-    if "FIELD" in deck or "LAB" in deck:
-        raise ValueError("read_grdecl: only METRIC units are supported")
+        if not _keyword_line(toks):
+            raise ValueError(f"read_grdecl: expected a keyword, got '{toks[0]}' (line {li + 1})")
+        kw = toks[0]
+        first = lines[li].split(None, 1)[1] if len(toks) > 1 and kw in ARRAY_KEYWORDS else ""
+        li += 1
+        if kw in SECTION_KEYWORDS:
+            deck[kw] = np.zeros(0)
+        if kw in ("SOLUTION", "SUMMARY", "SCHEDULE", "REGIONS"):
+            break
+        if kw in ("METRIC", "FIELD", "LAB"):
+            if kw == "LAB":
+                raise ValueError("read_grdecl: only METRIC and FIELD units are supported")
+            deck[kw] = np.zeros(0)
+            continue
+        if kw == "TITLE":
+            li += 1
+            continue
+        if kw in ARRAY_KEYWORDS:
+            parts, done = [], False
+            if first:
+                if "/" in first:
+                    first = first.split("/", 1)[0]
+                    done = True
+                parts.append(first)
+            while not done and li < len(lines):
+                line = lines[li]
+                if "/" in line:
+                    line = line.split("/", 1)[0]
+                    done = True
+                parts.append(line)
+                li += 1
+            if not done:
+                raise ValueError(f"read_grdecl: keyword {kw} is not terminated by '/'")
+            rest = " ".join(parts)
+            if kw == "SPECGRID":
+                deck[kw] = np.array(rest.split()[:3], dtype=np.int64)     # NX NY NZ (then NUMRES, radial flag)
+            elif kw in INT_KEYWORDS:
+                deck[kw] = _parse(rest, kw, np.int64)
+            else:
+                deck[kw] = _parse(rest, kw, np.float64)
+            continue
+        while li < len(lines) and not _keyword_line(lines[li].split()):
+            li += 1
     return deck
 
 
@@ -105,6 +149,8 @@ def grid_from_deck(deck: dict, rtol: float = 1e-9) -> grids.Grid:
     zcorn = deck.get("ZCORN")
     if coord is None or zcorn is None:
         raise ValueError("deck needs COORD and ZCORN")
+    if "FIELD" in deck:                      # lengths in feet
+        coord, zcorn = coord * FOOT, zcorn * FOOT
     if coord.size != 6 * (nx + 1) * (ny + 1):
         raise ValueError(f"COORD has {coord.size} values, expected {6 * (nx + 1) * (ny + 1)}")
     if zcorn.size != 8 * nx * ny * nz:
@@ -262,3 +308,80 @@ def write_grdecl(nx, ny, nz, dx, dy, zc, origin=(0.0, 0.0), props=None) -> str:
     for name, vals in (props or {}).items():
         out += [name, _fmt(vals), "/"]
     return "\n".join(out) + "\n"
+
+
+# --- the same input through the library's C-ABI (include/eclipse_grid.h) ------------------------
+EXPORTS = ("mhs_deck_read_file", "mhs_deck_read_text", "mhs_deck_destroy", "mhs_deck_last_error", "mhs_deck_dims",
+           "mhs_deck_build_grid", "mhs_deck_grid_arrays", "mhs_deck_rock")
+
+
+class NativeDeck:
+    """A deck read by libmhs (mhs_deck_*), the reader the C++ drop-in uses."""
+
+    def __init__(self, path: str | None = None, text: str | None = None):
+        import ctypes
+        from . import mhs
+        L = mhs.lib()
+        self._L, self._c = L, ctypes
+        P = ctypes.c_void_p
+        L.mhs_deck_read_file.argtypes = [ctypes.POINTER(P), ctypes.c_char_p]
+        L.mhs_deck_read_text.argtypes = [ctypes.POINTER(P), ctypes.c_char_p]
+        L.mhs_deck_destroy.argtypes = [P]
+        L.mhs_deck_last_error.argtypes = [P]
+        L.mhs_deck_last_error.restype = ctypes.c_char_p
+        L.mhs_deck_dims.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
+        L.mhs_deck_build_grid.argtypes = [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.mhs_deck_grid_arrays.argtypes = [P] + [P] * 10
+        L.mhs_deck_rock.argtypes = [P, ctypes.c_int, P, ctypes.c_double, P, P, ctypes.POINTER(ctypes.c_int)]
+        self._h = P()
+        if path is not None:
+            rc = L.mhs_deck_read_file(ctypes.byref(self._h), path.encode())
+        else:
+            rc = L.mhs_deck_read_text(ctypes.byref(self._h), text.encode())
+        self._check(rc)
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = self._L.mhs_deck_last_error(self._h).decode() if self._h else "deck read failed"
+            raise ValueError(msg)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._L.mhs_deck_destroy(self._h)
+            self._h = None
+
+    def dims(self):
+        d = (self._c.c_int * 3)()
+        self._check(self._L.mhs_deck_dims(self._h, d))
+        return tuple(d)
+
+    def grid(self):
+        """(grids.Grid, corners [nc, 8, 3]) of the deck."""
+        c = self._c
+        nc, nhf = c.c_int(), c.c_int()
+        self._check(self._L.mhs_deck_build_grid(self._h, c.byref(nc), c.byref(nhf)))
+        nc, nhf = nc.value, nhf.value
+        a = {"facepos": np.empty(nc + 1, np.int32), "nb": np.empty(nhf, np.int32), "bid": np.empty(nhf, np.int32),
+             "cvol": np.empty(nc), "ccen": np.empty((nc, 3)), "farea": np.empty(nhf), "fcen": np.empty((nhf, 3)),
+             "fnrm": np.empty((nhf, 3)), "ijk": np.empty((nc, 3), np.int32), "corners": np.empty((nc, 8, 3))}
+        ptr = lambda x: x.ctypes.data_as(c.c_void_p)
+        self._check(self._L.mhs_deck_grid_arrays(self._h, *(ptr(a[k]) for k in (
+            "facepos", "nb", "bid", "cvol", "ccen", "farea", "fcen", "fnrm", "ijk", "corners"))))
+        side = np.where(a["bid"] == 0, -1, np.where(a["bid"] <= 6, a["bid"] - 1, 6)).astype(np.int8)
+        g = grids.Grid(facepos=a["facepos"], nb=a["nb"], bid=a["bid"], cell_volume=a["cvol"], cell_centroid=a["ccen"],
+                       face_area=a["farea"], face_centroid=a["fcen"], face_normal=a["fnrm"], side=side, dims=None,
+                       ijk=a["ijk"])
+        return g, a["corners"]
+
+    def rock(self, global_cell=None, perm_threshold: float = 0.0):
+        """(perm [nc, 9] in m^2, porosity [nc], kind) as rock_from_deck."""
+        c = self._c
+        nx, ny, nz = self.dims()
+        gc = np.arange(nx * ny * nz, dtype=np.int32) if global_cell is None else np.asarray(global_cell, np.int32)
+        perm = np.empty((gc.size, 9))
+        poro = np.empty(gc.size)
+        kind = c.c_int()
+        self._check(self._L.mhs_deck_rock(self._h, gc.size, gc.ctypes.data_as(c.c_void_p), perm_threshold,
+                                          perm.ctypes.data_as(c.c_void_p), poro.ctypes.data_as(c.c_void_p),
+                                          c.byref(kind)))
+        return perm, poro, ("none", "scalar", "diagonal", "tensor")[kind.value]

@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol(pkg):
     assert len(names) >= 20
     for n in names:
         assert hasattr(lib, n), n
-    assert set(names) == set(pkg.mhs.EXPORTS) | set(pkg.transport.EXPORTS)
+    assert set(names) == set(pkg.mhs.EXPORTS) | set(pkg.transport.EXPORTS) | set(pkg.deck.EXPORTS)
 
 
 def test_default_options_match_reference_defaults(pkg):

@@ -24,8 +24,8 @@ PERMX 100 3.5E2 /
     assert "GRID" in d
     with pytest.raises(ValueError, match="not terminated"):
         pkg.deck.read_grdecl("PORO 0.1 0.2")
-    with pytest.raises(ValueError, match="METRIC"):
-        pkg.deck.read_grdecl("FIELD\nPORO 0.1 /")
+    with pytest.raises(ValueError, match="METRIC and FIELD"):
+        pkg.deck.read_grdecl("LAB\nPORO 0.1 /")
 
 
 def test_flat_deck_is_the_cartesian_grid(pkg):
@@ -137,3 +137,99 @@ def test_permeability_errors(pkg):
 def test_reader_rejects_unreadable_values(pkg):
     with pytest.raises(ValueError, match="unreadable"):
         pkg.deck.read_grdecl("PORO 0.1 abc 0.2 /")
+
+
+# --- the C-ABI reader (include/eclipse_grid.h), the one the C++ drop-in uses ---------------------
+GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden")
+
+
+def _same_grid(a, b, rtol=1e-13):
+    np.testing.assert_array_equal(a.facepos, b.facepos)
+    np.testing.assert_array_equal(a.nb, b.nb)
+    np.testing.assert_array_equal(a.bid, b.bid)
+    np.testing.assert_array_equal(a.ijk, b.ijk)
+    for f in ("cell_volume", "cell_centroid", "face_area", "face_centroid", "face_normal"):
+        x, y = np.asarray(getattr(a, f)), np.asarray(getattr(b, f))
+        np.testing.assert_allclose(x, y, rtol=rtol, atol=rtol * np.max(np.abs(y)), err_msg=f)
+
+
+def test_native_reader_matches_python_on_faulted_deck(pkg):
+    """The library's reader and geometry give the grid, boundary ids, logical
+    indices and rock of the Python reader on a faulted deck with an origin
+    shift and PERMX/PERMZ/PORO; the VTK corners rebuild the cell heights."""
+    nx, ny, nz, dx, dy = 7, 4, 5, 2.5, 1.5
+    zc = pkg.grids.corner_point(nx, ny, nz, dx, dy, 1.0, seed=11, fault_frac=0.4, return_depths=True)
+    rng = np.random.default_rng(3)
+    n = nx * ny * nz
+    props = {"PERMX": rng.uniform(10, 500, n), "PERMZ": rng.uniform(1, 50, n), "PORO": rng.uniform(0.1, 0.3, n)}
+    txt = pkg.deck.write_grdecl(nx, ny, nz, dx, dy, zc, origin=(20.0, 10.0), props=props)
+    py = pkg.deck.read_grdecl(txt)
+    gp = pkg.deck.grid_from_deck(py)
+    nd = pkg.deck.NativeDeck(text=txt)
+    assert nd.dims() == (nx, ny, nz)
+    gn, corners = nd.grid()
+    _same_grid(gn, gp)
+    assert np.any(gn.ncf() > 6) and np.any(gn.bid == 7)
+    Kp, php, kp = pkg.deck.rock_from_deck(py, perm_threshold=1e-18)
+    Kn, phn, kn = nd.rock(perm_threshold=1e-18)
+    assert kp == kn == "diagonal"
+    np.testing.assert_array_equal(Kn, Kp)
+    np.testing.assert_array_equal(phn, php)
+    # corners: top ring then bottom ring; the mean height of the four pillars
+    # times the column footprint is the cell volume of a straight-pillar cell
+    h = (corners[:, 4:, 2] - corners[:, :4, 2]).mean(axis=1)
+    np.testing.assert_allclose(h * dx * dy, gn.cell_volume, rtol=1e-12)
+    ij = np.asarray(gn.ijk)[:, :2]
+    np.testing.assert_allclose(corners[:, :, :2].min(axis=1), [20.0, 10.0] + ij * [dx, dy], rtol=1e-14)
+    np.testing.assert_allclose(corners[:, :, :2].max(axis=1), [20.0, 10.0] + (ij + 1) * [dx, dy], rtol=1e-14)
+
+
+def test_native_reader_spe9_deck(pkg):
+    """SPE9small.DATA (the reference's black-oil deck: FIELD units, RUNSPEC
+    keywords, PVT tables after the GRID arrays, reading stops at SOLUTION):
+    both readers agree, 24 x 1 x 15 cells, lengths converted from feet,
+    permeability PERMX/Y/Z in millidarcy."""
+    path = __import__("os").path.join(GOLDEN, "spe9small.data")
+    txt = open(path).read()
+    py = pkg.deck.read_grdecl(txt)
+    nd = pkg.deck.NativeDeck(path=path)
+    assert nd.dims() == (24, 1, 15) == pkg.deck.deck_dims(py)
+    gn, corners = nd.grid()
+    gp = pkg.deck.grid_from_deck(py)
+    _same_grid(gn, gp)
+    assert gn.num_cells == 360 and np.all(gn.cell_volume > 0)
+    zc = np.asarray(py["ZCORN"]).reshape(15, 2, 2, 48) * 0.3048     # [k, top/bottom, j-side, 2 nx]
+    thick = (zc[:, 1] - zc[:, 0]).mean(axis=1).reshape(15, 24, 2).mean(axis=2)
+    coord = np.asarray(py["COORD"]).reshape(2, 25, 6) * 0.3048
+    dx, dy = np.diff(coord[0, :, 0]), coord[1, 0, 1] - coord[0, 0, 1]
+    np.testing.assert_allclose(gn.cell_volume.reshape(15, 24), thick * dx[None, :] * dy, rtol=1e-12)
+    Kn, phn, kind = nd.rock()
+    Kp, php, _ = pkg.deck.rock_from_deck(py)
+    assert kind == "diagonal"
+    np.testing.assert_array_equal(Kn, Kp)
+    np.testing.assert_array_equal(phn, php)
+    np.testing.assert_allclose(Kn[:, 0], np.asarray(py["PERMX"]) * MD, rtol=1e-15)
+    np.testing.assert_allclose(Kn[:, 8], np.asarray(py["PERMZ"]) * MD, rtol=1e-15)
+
+
+def test_native_reader_errors(pkg):
+    """Errors of the C-ABI reader carry the same texts as the Python one."""
+    with pytest.raises(ValueError, match="not terminated"):
+        pkg.deck.NativeDeck(text="PORO 0.1 0.2")
+    with pytest.raises(ValueError, match="METRIC and FIELD"):
+        pkg.deck.NativeDeck(text="LAB\nPORO 0.1 /")
+    with pytest.raises(ValueError, match="unreadable"):
+        pkg.deck.NativeDeck(text="PORO 0.1 abc 0.2 /")
+    with pytest.raises(ValueError, match="Could not open"):
+        pkg.deck.NativeDeck(path="/nonexistent/deck.grdecl")
+    d = pkg.deck.NativeDeck(text="SPECGRID\n2 1 1 1 F /\nPERMY 1 1 /\nPERMZ 1 1 /\nPERMXY 1 1 /\n")
+    with pytest.raises(ValueError, match="Invalid set of permeability"):
+        d.rock()
+    d = pkg.deck.NativeDeck(text="SPECGRID\n2 1 1 1 F /\nPERMX 1 1 /\nPORO 0.2 /\n")
+    with pytest.raises(ValueError, match="PORO"):
+        d.rock()
+    nx, ny, nz = 2, 2, 2
+    zc = np.broadcast_to(np.arange(nz + 1) * 1.0, (ny, nx, 4, nz + 1)).copy()
+    txt = pkg.deck.write_grdecl(nx, ny, nz, 1.0, 1.0, zc)
+    with pytest.raises(ValueError, match="inactive"):
+        pkg.deck.NativeDeck(text=txt + "ACTNUM\n1 1 0 1 1 1 1 1 /\n").grid()
