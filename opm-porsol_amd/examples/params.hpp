@@ -3,7 +3,10 @@
 #define MHS_EXAMPLE_PARAMS_HPP
 #include <cstdlib>
 #include <map>
+#include <sstream>
+#include <stdexcept>
 #include <string>
+#include <type_traits>
 
 struct Params {
     std::map<std::string, std::string> kv;
@@ -32,6 +35,34 @@ struct Params {
     bool getDefault(const std::string &k, bool def) const {
         auto it = kv.find(k);
         return it == kv.end() ? def : (it->second == "1" || it->second == "true");
+    }
+    // ParameterGroup::has, get<T>, getDefault<T>
+    bool has(const std::string &k) const { return kv.count(k) != 0; }
+    template <class T>
+    T get(const std::string &k) const {
+        auto it = kv.find(k);
+        if (it == kv.end()) throw std::runtime_error("missing parameter " + k);
+        return convert<T>(it->second);
+    }
+    template <class T>
+    T getDefault(const std::string &k, const T &def) const {
+        auto it = kv.find(k);
+        return it == kv.end() ? def : convert<T>(it->second);
+    }
+
+private:
+    template <class T>
+    static T convert(const std::string &v) {
+        if constexpr (std::is_same<T, std::string>::value) {
+            return v;
+        } else if constexpr (std::is_same<T, bool>::value) {
+            return v == "1" || v == "true";
+        } else {
+            std::istringstream is(v);
+            T x{};
+            if (!(is >> x)) throw std::runtime_error("unreadable parameter value " + v);
+            return x;
+        }
     }
 };
 #endif

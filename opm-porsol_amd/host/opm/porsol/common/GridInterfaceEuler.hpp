@@ -107,6 +107,7 @@ public:
         }
     }
     const int *dims() const { return dims_; }
+    const double *cellSize() const { return h_; }
 
 private:
     int dims_[3] = {0, 0, 0};

@@ -7,14 +7,19 @@
 // Defaults follow ReservoirPropertyCommon_impl.hpp:257-262 (densities
 // 1013.9 / 834.7 kg/m^3, viscosities 1 / 3 cP).  Without rock tables the
 // relative permeabilities fall back to krw = s^2, kro = (1-s)^2
-// (ReservoirPropertyCapillary_impl.hpp:117-168).  Reading ECLIPSE decks is
-// not part of this drop-in.
+// (ReservoirPropertyCapillary_impl.hpp:117-168) and the capillary pressure to
+// 1e5 (1 - s) (RPC_impl :466-490).  Rock and fluid tables (rock_list) are not
+// read; permeability and porosity come from an ECLIPSE deck through
+// init(deck, global_cell, ...) (RPC_impl :275-300, over mhs_deck_rock).
 #ifndef OPM_PORSOL_AMD_RESERVOIRPROPERTYCOMMON_HPP
 #define OPM_PORSOL_AMD_RESERVOIRPROPERTYCOMMON_HPP
 
 #include <cassert>
+#include <stdexcept>
+#include <string>
 #include <vector>
 
+#include "EclipseDeck.hpp"
 #include "Units.hpp"
 
 namespace Opm {
@@ -74,6 +79,32 @@ public:
         porosity_.assign(num_cells, uniform_poro);
         permeability_ = perm_rowmajor;
     }
+    /// Permeability and porosity of the deck's cells global_cell (RPC_impl
+    /// :275-300): the keyword rules of classifyPermeability / fillTensor /
+    /// assignPermeability (:52-244, 682-736; values in mD, diagonal raised to
+    /// perm_threshold) and assignPorosity (:592-611; 1 without PORO).  Rock
+    /// tables are not supported: a rock_list_filename throws.  The J-function
+    /// arguments only act on rock tables and are accepted for the signature.
+    void init(const DeckConstPtr &deck, const std::vector<int> &global_cell, double perm_threshold = 0.0,
+              const std::string *rock_list_filename = nullptr, bool use_jfunction_scaling = true,
+              double sigma = 1.0, double theta = 0.0) {
+        (void)use_jfunction_scaling; (void)sigma; (void)theta;
+        static_assert(dim == 3, "ECLIPSE decks describe three-dimensional grids");
+        if (!deck) throw std::invalid_argument("ReservoirPropertyCommon::init: null deck");
+        if (rock_list_filename)
+            throw std::runtime_error("ReservoirPropertyCommon::init: rock tables (rock_list) are not supported");
+        const int nc = int(global_cell.size());
+        porosity_.resize(nc);
+        permeability_.resize(9 * (size_t)nc);
+        int kind = 0;
+        deck->check(mhs_deck_rock(deck->handle(), nc, global_cell.data(), perm_threshold, permeability_.data(),
+                                  porosity_.data(), &kind),
+                    "reading the rock properties");
+        perm_kind_ = kind;
+    }
+    /// 0: no permeability keyword, 1: scalar, 2: diagonal, 3: full tensor (the
+    /// classification of RPC_impl :52-104); -1 when not read from a deck.
+    int permeabilityKind() const { return perm_kind_; }
     void setViscosities(double v1, double v2) { viscosity1_ = v1; viscosity2_ = v2; }
     void setDensities(double d1, double d2) { density1_ = d1; density2_ = d2; }
 
@@ -95,9 +126,14 @@ public:
         density[0] = density1_;
         density[1] = density2_;
     }
+    /// Without rock tables the reference falls back to 1e5 (1 - s) Pa and its
+    /// derivative -1e5 (RPC_impl :466-490).
+    double capillaryPressure(int /*cell_index*/, double saturation) const { return 1e5 * (1.0 - saturation); }
+    double capillaryPressureDeriv(int /*cell_index*/, double /*saturation*/) const { return -1.0e5; }
 
 protected:
     double density1_, density2_, viscosity1_, viscosity2_;
+    int perm_kind_ = -1;
     std::vector<double> porosity_;
     std::vector<double> permeability_;
 };

@@ -21,6 +21,7 @@
 #include <fstream>
 #include <iostream>
 #include <iterator>
+#include <memory>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -121,17 +122,23 @@ public:
         Scalar outflux(int hf) const { return outflux_[hf]; }
         /// No counterpart in the reference: the solver handle that holds this
         /// solution on the GPU, for consumers that read the fluxes in device
-        /// memory (the GPU EulerUpstream); null before the first solve.
-        mhs_solver *deviceHandle() const { return handle_; }
+        /// memory (the GPU EulerUpstream, estimateCellVelocity); null before
+        /// the first solve, and for a copy of the solution once the solver
+        /// has solved, post-processed or been cleared since the copy was made.
+        mhs_solver *deviceHandle() const { return (handle_ && live_ && *live_ == gen_) ? handle_ : nullptr; }
 
     private:
         mhs_solver *handle_ = nullptr;
+        std::shared_ptr<const long long> live_;   // the solver's solve counter
+        long long gen_ = -1;                       // its value for this solution
         std::vector<int> cellno_;
         std::vector<int> facepos_;
         std::vector<Scalar> pressure_;
         std::vector<Scalar> outflux_;
         void clear() {
             handle_ = nullptr;
+            live_.reset();
+            gen_ = -1;
             std::vector<int>().swap(cellno_);
             std::vector<int>().swap(facepos_);
             std::vector<Scalar>().swap(pressure_);
@@ -166,6 +173,7 @@ public:
         matrix_structure_valid_ = false;
         do_regularization_ = true;
         flowSolution_.clear();
+        ++*solve_count_;
         periodic_hf_.clear();
         face_pressure_.clear();
         if (h_) check(mhs_clear(h_), "clear");
@@ -362,7 +370,7 @@ public:
         check(mhs_get_solution(h_, flowSolution_.pressure_.data(), face_pressure_.data(),
                                flowSolution_.outflux_.data()),
               "getSolution");
-        flowSolution_.handle_ = h_;
+        mark_solution_live();
     }
 
     /// Antisymmetric twin fluxes (:754-807); returns the largest change.
@@ -370,6 +378,7 @@ public:
         double mm = 0.0;
         check(mhs_post_process_fluxes(h_, &mm), "postProcessFluxes");
         check(mhs_get_solution(h_, nullptr, nullptr, flowSolution_.outflux_.data()), "getSolution");
+        mark_solution_live();
         return mm;
     }
 
@@ -432,8 +441,15 @@ private:
     std::vector<int> periodic_hf_;
     std::vector<double> face_pressure_;
     mhs_solve_report report_{};
+    std::shared_ptr<long long> solve_count_ = std::make_shared<long long>(0);
     FlowSolution flowSolution_;
 
+    // the device data now hold flowSolution_; earlier copies lose their handle
+    void mark_solution_live() {
+        flowSolution_.handle_ = h_;
+        flowSolution_.live_ = solve_count_;
+        flowSolution_.gen_ = ++*solve_count_;
+    }
     void ensure_handle() {
         if (!h_ && mhs_create(&h_, device_, nullptr) != MHS_OK)
             throw std::runtime_error("IncompFlowSolverHybrid: cannot create the GPU solver");

@@ -1,5 +1,6 @@
 """Problem set-ups shared by the CPU and GPU tests (synthetic, seeded)."""
 import os
+import re
 import sys
 
 import numpy as np
@@ -91,3 +92,21 @@ def xplane_rates(g, v):
     rates[:nx] = -vv[:, 0].reshape(nz, ny, nx).sum(axis=(0, 1))
     rates[nx] = vv[:, 1].reshape(nz, ny, nx)[:, :, -1].sum()
     return rates
+
+
+def read_vtu(path):
+    """{name: array} of the cell data and the point / cell arrays of an ASCII
+    .vtu file, plus '_sizes' = (points, cells)."""
+    txt = open(path).read()
+    out = {}
+    for m in re.finditer(r'<DataArray type="(\w+)"(?: Name="([^"]*)")?(?: NumberOfComponents="(\d+)")?'
+                         r' format="ascii">\n(.*?)</DataArray>', txt, flags=re.S):
+        _, name, ncomp, body = m.groups()
+        vals = np.array(body.split(), dtype=np.float64)
+        if name is None:
+            name = "points"
+        ncomp = int(ncomp) if ncomp else 1
+        out[name] = vals.reshape(-1, ncomp) if ncomp > 1 else vals
+    head = re.search(r'NumberOfPoints="(\d+)" NumberOfCells="(\d+)"', txt)
+    out["_sizes"] = (int(head.group(1)), int(head.group(2)))
+    return out

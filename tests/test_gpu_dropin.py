@@ -295,3 +295,92 @@ def test_unknown_linsolver_type_throws():
     res = subprocess.run([path, "nx=4", "linear_solver_type=4"], capture_output=True, text=True, timeout=120)
     assert res.returncode != 0
     assert "Unknown linsolver_type: 4" in res.stdout + res.stderr
+
+
+# --- ECLIPSE deck -> solve -> VTU (examples/mimetic_solver_test.cpp) -------------------------
+
+def _deck_case(pkg, kind, tmp_path):
+    """(deck path, Python grid, perm) of a faulted GRDECL deck with PERMX /
+    PERMY / PERMZ / PORO, or the reference's SPE9small.DATA."""
+    if kind == "faulted":
+        nx, ny, nz, dx, dy = 16, 12, 8, 25.0, 20.0
+        zc = pkg.grids.corner_point(nx, ny, nz, dx, dy, 4.0, seed=21, fault_frac=0.3, return_depths=True)
+        rng = np.random.default_rng(4)
+        n = nx * ny * nz
+        props = {"PERMX": np.exp(rng.normal(4.5, 1.0, n)), "PERMY": np.exp(rng.normal(4.5, 1.0, n)),
+                 "PERMZ": np.exp(rng.normal(2.5, 1.0, n)), "PORO": rng.uniform(0.1, 0.3, n)}
+        txt = pkg.deck.write_grdecl(nx, ny, nz, dx, dy, zc, origin=(1000.0, 2000.0), props=props)
+        path = str(tmp_path / "faulted.grdecl")
+        open(path, "w").write(txt)
+    else:
+        path = os.path.join(ROOT, "tests", "golden", "spe9small.data")
+        txt = open(path).read()
+    d = pkg.deck.read_grdecl(txt)
+    g = pkg.deck.grid_from_deck(d)
+    K, _, _ = pkg.deck.rock_from_deck(d)
+    return path, g, K
+
+
+def _deck_oracle(orc, g, K):
+    """The example's problem on the oracle: Dirichlet 100 bar on the top
+    (id 5) and 0 on the bottom (id 6), no-flow elsewhere (the fault-exposed
+    faces included), saturation 0 (total mobility 1 / 3 cP), no gravity."""
+    nc = g.num_cells
+    bt = np.where((g.bid == 5) | (g.bid == 6), 1, 0).astype(np.int32)
+    bv = np.where(g.bid == 5, 1e7, 0.0)
+    lam = np.full(nc, 1.0 / 3e-3)
+    ref = orc.solve(g, K, lam, 834.7 * lam, np.zeros(nc), bt, bv, tol=1e-14)
+    hc = g.hf_cell()
+    vel = np.zeros((nc, 3))
+    np.add.at(vel, hc, (np.asarray(g.face_centroid) - np.asarray(g.cell_centroid)[hc]) * ref.outflux[:, None])
+    return ref.cell_pressure, vel / np.asarray(g.cell_volume)[:, None]
+
+
+@pytest.mark.parametrize("kind", ["faulted", "spe9"])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_deck_example_matches_oracle(orc, pkg, tmp_path, kind, precision):
+    """The C++ drop-in reads a deck (setupGridAndProps with
+    fileformat=eclipse: CornerPointGrid + ReservoirPropertyCommon::init),
+    solves on the GPU and writes the VTU of cell velocity (estimateCellVelocity
+    on the GPU) and pressure; both agree with the oracle's solve of the same
+    deck read by the Python reader.  The full writeVtkOutput file carries the
+    reference's seven field names."""
+    from cases import read_vtu
+    deck, g, K = _deck_case(pkg, kind, tmp_path)
+    out = _run("mimetic_solver_test", "fileformat=eclipse", f"filename={deck}", "tolerance=1e-13",
+               f"precision={precision}", f"output={tmp_path / 'sol'}", f"full_output={tmp_path / 'full'}")
+    assert re.search(rf"cells: {g.num_cells} ", out), out
+    v = read_vtu(str(tmp_path / "sol.vtu"))
+    p_ref, vel_ref = _deck_oracle(orc, g, K)
+    assert v["_sizes"] == (8 * g.num_cells, g.num_cells)
+    np.testing.assert_allclose(v["pressure"], p_ref, rtol=0, atol=1e-9 * 1e7)
+    np.testing.assert_allclose(v["velocity"], vel_ref, rtol=0, atol=1e-8 * np.abs(vel_ref).max())
+    f = read_vtu(str(tmp_path / "full.vtu"))
+    assert set(k for k in f if not k.startswith("_")) >= {
+        "saturation", "pressure", "capillary pressure", "fractional flow [water]", "velocity",
+        "phase velocity [water]", "phase velocity [oil]"}
+    np.testing.assert_array_equal(f["pressure"], v["pressure"])
+    np.testing.assert_array_equal(f["velocity"], v["velocity"])
+    np.testing.assert_array_equal(f["saturation"], 0.0)
+    np.testing.assert_array_equal(f["capillary pressure"], 1e5)
+    np.testing.assert_array_equal(f["fractional flow [water]"], 0.0)
+    np.testing.assert_array_equal(f["phase velocity [water]"], 0.0)
+    np.testing.assert_array_equal(f["phase velocity [oil]"], v["velocity"])
+    fld = np.loadtxt(str(tmp_path / "full-pressure.txt"))
+    assert int(fld[0]) == g.num_cells
+    np.testing.assert_allclose(fld[1:], p_ref, rtol=1e-5)
+
+
+def test_deck_example_reference_settings(orc, pkg, tmp_path):
+    """The example as the reference runs it (tolerance 5e-9, linear solver
+    type 3, testsolution-0.vtu in the working directory)."""
+    from cases import read_vtu
+    deck, g, K = _deck_case(pkg, "faulted", tmp_path)
+    path = os.path.join(BUILD, "mimetic_solver_test")
+    res = subprocess.run([path, "fileformat=eclipse", f"filename={deck}"], capture_output=True, text=True,
+                         timeout=300, cwd=str(tmp_path))
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "preconditioner: multigrid" in res.stdout, res.stdout
+    v = read_vtu(str(tmp_path / "testsolution-0.vtu"))
+    p_ref, _ = _deck_oracle(orc, g, K)
+    np.testing.assert_allclose(v["pressure"], p_ref, rtol=0, atol=1e-5 * 1e7)
