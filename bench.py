@@ -45,11 +45,47 @@ def _problem(pkg, seed):
     return g, perm, bt, bv
 
 
-def _alg_bytes(g, nnz, nf, mg_rows=0, mg_nnz=0, ap_nnz=0):
+def _vcycle_kernels(nf, v, lev_rows, lev_nnz, p_nnz, t_nnz, ap_nnz, sa_level):
+    """Algorithmic bytes of the kernels of one multigrid V(1,1) cycle outside
+    the level-0 operator pass (the pre-smoothing residual is counted by the
+    caller), as [(kernel, bytes)]: each array once per kernel at its stored
+    precision v, 4-byte indices, gathered vectors once.  lev_rows / lev_nnz
+    hold levels 0 (faces) .. L-1; sa_level is the level whose transfer to the
+    next is smoothed (-1: none)."""
+    L = len(lev_rows)
+    out = []
+    n1 = lev_rows[1]
+    # faces -> level 1 (children lists): child pointers, children, res0, b1
+    out.append(("k_restrict (faces -> level 1)", nf * (4 + v) + n1 * (4 + v)))
+    for l in range(1, L - 1):
+        n, nnz, nc = lev_rows[l], lev_nnz[l], lev_rows[l + 1]
+        # down-sweep from zero: row pointers, matrix, b, D^-1, x and res written
+        out.append((f"down-sweep level {l}", 8 * nnz + n * (4 + 4 * v)))
+        if l == sa_level:
+            # P' rows (pointers, columns, values), res gathered, b_c written
+            out.append((f"restriction P' level {l}", 8 * p_nnz + nc * (4 + v) + n * v))
+            # up-sweep across the smoothed transfer: P and T = A P rows, res, x,
+            # D^-1 read, x written, x_c gathered
+            out.append((f"up-sweep (P, T) level {l}", 8 * (p_nnz + t_nnz) + n * (8 + 4 * v) + nc * v))
+        else:
+            out.append((f"restriction level {l}", n * (4 + v) + nc * (4 + v)))
+            # up-sweep: matrix, aggregate map, b, D^-1 read, x written, x_c gathered
+            out.append((f"up-sweep level {l}", 8 * nnz + n * (8 + 3 * v) + nc * v))
+    m = lev_rows[L - 1]
+    out.append(("coarsest dense solve", m * m * v + 2 * m * v))
+    # post-smoothing on level 0 with A P: its SELL entries, the aggregate map,
+    # x0, res0, r, D^-1 read, z written, x_1 gathered
+    out.append(("k_vc0_post", ap_nnz * (v + 4) + nf * (4 + 5 * v) + n1 * v))
+    return out
+
+
+def _alg_bytes(g, nnz, nf, rep=None):
     """Algorithmic HBM bytes per unit of work for one right-hand side: each
     array counted once per kernel at its stored precision, 4-byte indices,
     gathered vectors counted once.  v is the value size of the PCG (4 in
-    mixed precision, 8 in FP64)."""
+    mixed precision, 8 in FP64).  With the solve report of a multigrid solve,
+    "amg_iter*" are the sums over the kernels of one iteration, listed per
+    kernel in "kernels*"."""
     nc, nhf = g.num_cells, g.num_half_faces
     sq = int(np.sum(g.ncf().astype(np.int64) ** 2))
     pk = int(np.sum(g.ncf().astype(np.int64) * (g.ncf() + 1) // 2))
@@ -70,17 +106,25 @@ def _alg_bytes(g, nnz, nf, mg_rows=0, mg_nnz=0, ap_nnz=0):
     cell_part = pk * 4 + nhf * 4
     out["ebe_spmv"] = cell_part + nf * 4 + nf * 4 + nint * 8
     out["ebe_resid"] = cell_part + 2 * nf * 4 + nf * 4 + nint * 8
-    for v, tag in ((4, "32"), (8, "64")):
-        # multigrid-PCG iteration: the SpMV and the pre-smoothed residual, one
-        # pass over A P with the level-1 aggregates (post-smoothing sweep), 17
-        # more level-0 vector passes (x/r update with the pre-smoothing 8,
-        # restriction 1, post-smoothing 5, p update 3), the aggregate map
-        # twice, and on the coarse levels two operator passes and ten vector
-        # passes per row
-        rest = (ap_nnz * (v + 4) + 17 * nf * v + 8 * nf + 2 * mg_nnz * (v + 4) + mg_rows * (12 + 10 * v))
-        out["amg_iter" + tag] = out["spmv" + tag] + out["resid" + tag] + rest
-        if tag == "32":
-            out["amg_iter_ebe"] = out["ebe_spmv"] + out["ebe_resid"] + rest
+    if rep is not None and rep.get("precond") == 1:
+        lev_rows = [int(x) for x in rep["level_rows"]]
+        lev_nnz = [int(x) for x in rep["level_nnz"]]
+        p_nnz, t_nnz = (int(x) for x in rep["transfer_nnz"])
+        sa_level = int(rep["smoothed_level"])
+        for v, tag in ((4, "32"), (8, "64")):
+            # x += a p, r -= a q with the <r, D^-1 r>, |r|^2 partials and the
+            # first smoothing step x0 = w D^-1 r: x, r, p, q, D^-1 read, x, r, x0 written
+            upd_xr = ("k_update_xr", 8 * nf * v)
+            upd_pz = ("k_update_pz", 3 * nf * v)        # p = z + b p
+            vc = _vcycle_kernels(nf, v, lev_rows, lev_nnz, p_nnz, t_nnz, int(rep["ap_nnz"]), sa_level)
+            ks = [("SpMV q = A p", out["spmv" + tag]), upd_xr, ("pre-smoothing residual", out["resid" + tag])]
+            out["kernels" + tag] = ks + vc + [upd_pz]
+            out["amg_iter" + tag] = sum(b for _, b in out["kernels" + tag])
+            if tag == "32":
+                ke = [("k_ebe product (2 colour launches)", out["ebe_spmv"]), upd_xr,
+                      ("k_ebe pre-smoothing residual (2 colour launches)", out["ebe_resid"])]
+                out["kernels_ebe"] = ke + vc + [upd_pz]
+                out["amg_iter_ebe"] = sum(b for _, b in out["kernels_ebe"])
     # FP64 residual replacement: FP64 operator + x, b read, r written
     out["refine64"] = nnz * (8 + 4) + 4 * nf * 8
     # element by element it rides on the SpMV: the FP64 values instead of
@@ -917,7 +961,7 @@ def main():
     cell = g.hf_cell()
     mass = float(np.max(np.abs(np.bincount(cell, weights=v, minlength=nc))) / np.max(np.abs(v)))
 
-    B = _alg_bytes(g, nnz, nf, rep["mg_rows"], rep["mg_nnz"], rep["ap_nnz"])
+    B = _alg_bytes(g, nnz, nf, rep)
     tag = "32" if args.precision == "mixed" else "64"
     ebe = bool(rep["element_operator"])
     if ebe:
@@ -995,6 +1039,10 @@ def main():
         "level0_operator": ("element by element (FP32 local matrices, %d colours)" % rep["colours"]) if ebe
                            else "assembled face matrix",
     }
+    kkey = "kernels_ebe" if (ebe and rep["precond"] == 1) else ("kernels" + tag)
+    if kkey in B:
+        # the per-iteration part of alg_bytes_per_solve, kernel by kernel
+        config["alg_bytes_per_iteration"] = {k: int(b) for k, b in B[kkey]}
     if fx is not None:
         config["err_vs_oracle"] = _err_vs_oracle(g, perm, fx, p, pi, v)
         config["err_vs_oracle_note"] = ("max-norm relative error at 32768 seeded positions per field against "

@@ -138,6 +138,13 @@ typedef struct {
     int element_operator;         /* 1: the solve applied the level-0 operator cell by cell */
     int colours;                  /* cell colours of that operator (0 when not used) */
     long long element_values;     /* packed local-matrix entries it stores (with padding) */
+    long long level_rows[8];      /* multigrid: rows of levels 0..7 (level 0 the face rows,
+                                     unpadded; 0 beyond the coarsest) */
+    long long level_nnz[8];       /* stored entries of those level matrices (level 0: the
+                                     assembled face matrix) */
+    long long transfer_nnz[2];    /* entries of the smoothed transfer P and of T = A P
+                                     (0 when no transfer is smoothed) */
+    int smoothed_level;           /* level whose transfer to the next is smoothed, -1: none */
 } mhs_solve_report;
 
 typedef struct {

@@ -1050,15 +1050,34 @@ int autotune(mhs_solver *s, int nrhs, bool mixed, bool amg, double *tune_ms) {
         HIPCHECK(hipEventCreate(&e0));
         HIPCHECK(hipEventCreate(&e1));
         hipError_t err = hipSuccess;
+        // Each candidate runs as the solve runs it: ten launches captured in a
+        // hipGraph, replayed four times; the fastest of the last three
+        // replays counts.  Timing plain launches instead favours shapes with
+        // fewer kernels, since the host's launch cost then shows in the time.
         auto time_it = [&](auto launch) -> float {
-            for (int w = 0; w < 2 && err == hipSuccess; ++w) err = launch();
-            if (err == hipSuccess) err = hipEventRecord(e0, s->stream);
-            for (int r = 0; r < 10 && err == hipSuccess; ++r) err = launch();
-            if (err == hipSuccess) err = hipEventRecord(e1, s->stream);
-            if (err == hipSuccess) err = hipEventSynchronize(e1);
-            float ms = 0.0f;
-            if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
-            return err == hipSuccess ? ms / 10.0f : 1e30f;
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            if (err == hipSuccess) err = hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal);
+            if (err == hipSuccess) {
+                hipError_t el = hipSuccess;
+                for (int r = 0; r < 10 && el == hipSuccess; ++r) el = launch();
+                const hipError_t ee = hipStreamEndCapture(s->stream, &graph);
+                err = el != hipSuccess ? el : ee;
+            }
+            if (err == hipSuccess) err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            if (graph) (void)hipGraphDestroy(graph);
+            float best_ms = 1e30f;
+            for (int rep = 0; rep < 4 && err == hipSuccess; ++rep) {
+                err = hipEventRecord(e0, s->stream);
+                if (err == hipSuccess) err = hipGraphLaunch(exec, s->stream);
+                if (err == hipSuccess) err = hipEventRecord(e1, s->stream);
+                if (err == hipSuccess) err = hipEventSynchronize(e1);
+                float ms = 0.0f;
+                if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+                if (err == hipSuccess && rep > 0) best_ms = std::min(best_ms, ms / 10.0f);
+            }
+            if (exec) (void)hipGraphExecDestroy(exec);
+            return err == hipSuccess ? best_ms : 1e30f;
         };
         Shape best;
         float bs = 1e30f, bw = 1e30f;
@@ -1478,6 +1497,22 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
         rep->element_operator = s->last_ebe ? 1 : 0;
         rep->colours = s->last_ebe ? s->EL.ncolours : 0;
         rep->element_values = s->last_ebe ? s->EL.nval : 0;
+        for (int l = 0; l < 8; ++l) {
+            const bool have = amg && l < s->H.nlev();
+            rep->level_rows[l] = have ? (l == 0 ? (long long)s->T.nf_total : (long long)s->H.lev[l].n) : 0;
+            rep->level_nnz[l] = have ? (l == 0 ? (long long)s->T.slice_ptr[s->T.nslices]
+                                                : (long long)s->H.lev[l].col.size())
+                                      : 0;
+        }
+        rep->transfer_nnz[0] = rep->transfer_nnz[1] = 0;
+        rep->smoothed_level = -1;
+        if (amg)
+            for (int l = 0; l < s->H.nlev(); ++l)
+                if (s->H.lev[l].sa_out) {
+                    rep->transfer_nnz[0] = (long long)s->H.lev[l].p_col.size();
+                    rep->transfer_nnz[1] = (long long)s->H.lev[l].t_col.size();
+                    rep->smoothed_level = l;
+                }
     }
     if (!converged)
         return fail(s, MHS_ERR_NOT_CONVERGED,

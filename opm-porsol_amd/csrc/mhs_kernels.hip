@@ -1172,39 +1172,41 @@ __global__ __launch_bounds__(256) void k_pair_sum(int n, const int *__restrict__
 }
 
 // Dense inverse of each instance's coarsest operator (Gauss-Jordan, no
-// pivoting: the Galerkin operators are symmetric positive definite).
-__global__ __launch_bounds__(256) void k_coarse_inverse(DevAmgLevel L, const int *__restrict__ inst_ptr,
-                                                        const int *__restrict__ off,
-                                                        double *__restrict__ inv64,
-                                                        float *__restrict__ inv32) {
+// pivoting: the Galerkin operators are symmetric positive definite).  Per
+// pivot: the scaled pivot row and the pivot column go to their own arrays,
+// then every thread updates its own entries of the matrix; two barriers.
+__global__ __launch_bounds__(1024) void k_coarse_inverse(DevAmgLevel L, const int *__restrict__ inst_ptr,
+                                                         const int *__restrict__ off,
+                                                         double *__restrict__ inv64,
+                                                         float *__restrict__ inv32) {
     constexpr int M = kMaxCoarsest;
+    constexpr int NT = 1024;
     __shared__ double A[M][M + 1];
-    __shared__ double colk[M];
-    const int t = blockIdx.x;
+    __shared__ double rowk[M], colk[M];
+    const int t = blockIdx.x, tid = threadIdx.x;
     const int r0 = inst_ptr[t], m = inst_ptr[t + 1] - r0;
-    for (int e = threadIdx.x; e < M * M; e += 256) A[e / M][e % M] = 0.0;
+    for (int e = tid; e < M * M; e += NT) A[e / M][e % M] = 0.0;
     __syncthreads();
-    for (int i = threadIdx.x; i < m; i += 256)
+    for (int i = tid; i < m; i += NT)
         for (int k = L.row_ptr[r0 + i]; k < L.row_ptr[r0 + i + 1]; ++k)
             A[i][L.col[k] - r0] = L.val64[k];
     __syncthreads();
     for (int k = 0; k < m; ++k) {
-        const double piv = A[k][k];
+        if (tid < M) {
+            if (tid < m) rowk[tid] = (tid == k ? 1.0 : A[k][tid]) / A[k][k];
+        } else if (tid < 2 * M) {
+            const int i = tid - M;
+            if (i < m) colk[i] = A[i][k];
+        }
         __syncthreads();
-        for (int j = threadIdx.x; j < m; j += 256) A[k][j] = (j == k ? 1.0 : A[k][j]) / piv;
-        __syncthreads();
-        for (int i = threadIdx.x; i < m; i += 256) colk[i] = A[i][k];
-        __syncthreads();
-        // (i, j) from the compile-time width M: shifts, no integer division
-        for (int e = threadIdx.x; e < M * M; e += 256) {
+        for (int e = tid; e < M * M; e += NT) {
             const int i = e / M, j = e % M;
-            if (i >= m || j >= m || i == k) continue;
-            const double base = (j == k) ? 0.0 : A[i][j];
-            A[i][j] = base - colk[i] * A[k][j];
+            if (i >= m || j >= m) continue;
+            A[i][j] = (i == k) ? rowk[j] : (j == k ? 0.0 : A[i][j]) - colk[i] * rowk[j];
         }
         __syncthreads();
     }
-    for (int e = threadIdx.x; e < M * M; e += 256) {
+    for (int e = tid; e < M * M; e += NT) {
         const int i = e / M, j = e % M;
         if (i >= m || j >= m) continue;
         const double v = A[i][j];
@@ -2837,7 +2839,7 @@ hipError_t launch_amg_setup(hipStream_t st, const DevTopo &T, const DevAmg &A, c
                 (int)A.ap_slots, A.ap_src_ptr, A.ap_src, val64, A.ap64, mixed ? A.ap32 : nullptr);
         });
     const DevAmgLevel &C = A.lev[A.nlev - 1];
-    k_coarse_inverse<<<A.ninst, 256, 0, st>>>(C, C.inst_ptr, A.coarse_off, A.cinv64,
+    k_coarse_inverse<<<A.ninst, 1024, 0, st>>>(C, C.inst_ptr, A.coarse_off, A.cinv64,
                                               mixed ? A.cinv32 : nullptr);
     return hipGetLastError();
 }
@@ -2965,9 +2967,10 @@ static void vcycle_impl(hipStream_t st, const Gate &gt, const DevTopo &T, const 
             const DevAmgLevel &F = A.lev[l];
             const DevAmgLevel &C = A.lev[l + 1];
             const int gd = lanes_per_row(F.nnz, F.n);
-            if (!F.sa_out && A.covers[l + 1] && A.max_children[l + 1] <= 8 && gd <= 8) {
-                // down-sweep and restriction in one launch
-                with_lanes(gd, [&](auto g) {
+            if (!F.sa_out && A.covers[l + 1] && A.max_children[l + 1] <= 8) {
+                // down-sweep and restriction in one launch (at most 8 lanes per
+                // child row, so a coarse row and its children fill one wavefront)
+                with_lanes(std::min(gd, 8), [&](auto g) {
                     constexpr int Gv = decltype(g)::value;
                     if constexpr (Gv <= 8)
                         k_vcc_down_restrict<VT, NR, Gv><<<row_blocks(C.n, 8 * Gv), 256, 0, st>>>(

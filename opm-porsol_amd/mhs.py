@@ -63,7 +63,9 @@ class Report(ctypes.Structure):
                 ("tune_ms", ctypes.c_double), ("smoother_weights", ctypes.c_double * 8),
                 ("coarsening", (ctypes.c_int * 3) * 8), ("ap_nnz", ctypes.c_longlong),
                 ("element_operator", ctypes.c_int), ("colours", ctypes.c_int),
-                ("element_values", ctypes.c_longlong)]
+                ("element_values", ctypes.c_longlong), ("level_rows", ctypes.c_longlong * 8),
+                ("level_nnz", ctypes.c_longlong * 8), ("transfer_nnz", ctypes.c_longlong * 2),
+                ("smoothed_level", ctypes.c_int)]
 
     def as_dict(self):
         out = {}
@@ -73,6 +75,10 @@ class Report(ctypes.Structure):
                 v = [x for x in v if x > 0.0]
             elif k == "coarsening":
                 v = [tuple(b) for b in v if b[0] > 0]
+            elif k in ("level_rows", "level_nnz"):
+                v = [int(x) for x in v if x > 0]
+            elif k == "transfer_nnz":
+                v = [int(x) for x in v]
             out[k] = v
         return out
 

@@ -102,3 +102,30 @@ def test_cpp_dropin_compiles(pkg):
     spec.loader.exec_module(b)
     exes = b.build_examples()
     assert any(e.endswith("known_answer_test") for e in exes)
+
+
+def test_ctypes_structs_match_the_c_layout(pkg, tmp_path):
+    """The ctypes mirrors of the C-ABI structs have the C compiler's size and
+    field offsets (a field added on one side only would shift every later one)."""
+    import ctypes
+    pairs = [("mhs_solve_options", pkg.mhs.Options), ("mhs_solve_report", pkg.mhs.Report),
+             ("mhs_dims", pkg.mhs.Dims), ("mhs_transport_options", pkg.transport.Options),
+             ("mhs_transport_report", pkg.transport.Report), ("mhs_fluid", pkg.transport.Fluid)]
+    lines = ["#include <cstddef>", "#include <cstdio>", '#include "mimetic_hybrid.h"', '#include "euler_upstream.h"',
+             "int main() {"]
+    for cname, py in pairs:
+        lines.append(f'  std::printf("{cname} %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'  std::printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("  return 0;\n}")
+    src = tmp_path / "layout.cpp"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    res = subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                         capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
+    got = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True).stdout.splitlines())
+    for cname, py in pairs:
+        assert int(got[cname]) == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert int(got[f"{cname}.{f}"]) == getattr(py, f).offset, f"{cname}.{f}"
