@@ -597,15 +597,9 @@ int mhs_init_system_structure(mhs_solver *s, const int *hf_bc_type, const int *h
         D.ncolours = E.ncolours;
         D.nblk = E.nblk;
         for (int k = 0; k <= E.ncolours; ++k) D.colour_blk[k] = E.colour_blk[k];
-        D.blk_seg = dupload(s, L, E.blk_seg, &he);
-        if (he == hipSuccess) D.blk_slot = dupload(s, L, E.blk_slot, &he);
-        if (he == hipSuccess) D.seg_n = dupload(s, L, E.seg_n, &he);
-        if (he == hipSuccess) D.seg_inst = dupload(s, L, E.seg_inst, &he);
-        if (he == hipSuccess) D.seg_count = dupload(s, L, E.seg_count, &he);
-        if (he == hipSuccess) D.seg_len = dupload(s, L, E.seg_len, &he);
-        if (he == hipSuccess) D.seg_blk0 = dupload(s, L, E.seg_blk0, &he);
+        D.blk_info = reinterpret_cast<const int4 *>(dupload(s, L, E.blk_info, &he));
+        if (he == hipSuccess) D.blk_base = reinterpret_cast<const longlong2 *>(dupload(s, L, E.blk_base, &he));
         if (he == hipSuccess) D.seg_val = dupload(s, L, E.seg_val, &he);
-        if (he == hipSuccess) D.seg_dof = dupload(s, L, E.seg_dof, &he);
         if (he == hipSuccess) D.cell_seg = dupload(s, L, E.cell_seg, &he);
         if (he == hipSuccess) D.cell_local = dupload(s, L, E.cell_local, &he);
         if (he == hipSuccess) D.dof = dupload(s, L, E.dof, &he);

@@ -1,6 +1,8 @@
 // Element-by-element layout of the level-0 face operator.  See mhs_ebe.h.
 #include "mhs_ebe.h"
 
+#include "mhs_kernels.h"
+
 #include <algorithm>
 #include <numeric>
 
@@ -78,17 +80,17 @@ std::string build_ebe(EbeLayout &E, const Topology &T) {
         while (b < E.nblk && seg_colour[E.blk_seg[b]] < k) ++b;
         E.colour_blk[k] = b;
     }
-    // face rows with the first/last-writer flags, component-major per segment
+    // face rows with the first/last-writer flags, in the 64-cell tiles of the values
     E.dof.assign((size_t)E.ndof, 0u);
     for (int c = 0; c < nc; ++c) {
-        const int s = E.cell_seg[c], l = E.cell_local[c], len = E.seg_len[s];
+        const int s = E.cell_seg[c], l = E.cell_local[c];
         const int f0 = T.facepos[c], n = T.facepos[c + 1] - f0;
         for (int j = 0; j < n; ++j) {
             const int h = f0 + j, o = T.nb[h];
             uint32_t d = (uint32_t)T.hf_dof[h];
             if (o < 0 || colour[c] < colour[o]) d |= kEbeFirst;
             if (o < 0 || colour[c] > colour[o]) d |= kEbeLast;
-            E.dof[(size_t)E.seg_dof[s] + (size_t)j * len + l] = d;
+            E.dof[(size_t)E.seg_dof[s] + (size_t)ebe_slot(l, j, n)] = d;
         }
     }
     // partial-sum slots: each instance's blocks (over all colours) contiguous
@@ -100,6 +102,17 @@ std::string build_ebe(EbeLayout &E, const Topology &T) {
     E.blk_slot.resize(E.nblk);
     for (int b = 0; b < E.nblk; ++b) E.blk_slot[b] = next[E.seg_inst[E.blk_seg[b]]]++;
     E.nslots = E.nblk;
+    E.blk_info.resize(4 * (size_t)E.nblk);
+    E.blk_base.resize(2 * (size_t)E.nblk);
+    for (int b = 0; b < E.nblk; ++b) {
+        const int s = E.blk_seg[b], n = E.seg_n[s], local0 = (b - E.seg_blk0[s]) * kEbeBlock;
+        E.blk_info[4 * b] = n;
+        E.blk_info[4 * b + 1] = E.seg_inst[s];
+        E.blk_info[4 * b + 2] = std::min(kEbeBlock, E.seg_count[s] - local0);
+        E.blk_info[4 * b + 3] = E.blk_slot[b];
+        E.blk_base[2 * b] = E.seg_val[s] + ebe_slot(local0, 0, n * (n + 1) / 2);
+        E.blk_base[2 * b + 1] = E.seg_dof[s] + ebe_slot(local0, 0, n);
+    }
     return "";
 }
 

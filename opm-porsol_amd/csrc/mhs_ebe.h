@@ -8,9 +8,10 @@
 // cells of one colour share a face; one launch per colour then scatters to
 // face rows without atomics.  Cells are grouped into segments of one colour,
 // one instance and one face count, each padded to whole 256-cell blocks, and
-// their values are stored component-major inside the segment (entry q of the
-// cell in slot i at seg_val + q * len + i) so that a wavefront's loads
-// coalesce.
+// their values and face rows are stored in 64-cell tiles, component-major
+// inside a tile (entry q of the cell in slot i at seg_val + ebe_slot(i, q,
+// ncomp), mhs_kernels.h), so that a wavefront's loads coalesce and stay
+// within one contiguous range.
 #pragma once
 
 #include <cstdint>
@@ -35,6 +36,12 @@ struct EbeLayout {
     std::vector<int> seg_n, seg_inst, seg_count, seg_len, seg_blk0;
     std::vector<long long> seg_val, seg_dof; // value and dof bases of each segment
     std::vector<int> inst_slot;              // [ninst + 1] partial slots per instance (contiguous)
+    // Per block, what its kernel needs before its first data load, in two
+    // records read in one round trip: (face count, instance, cells in the
+    // block, partial-sum slot) and (value base, face-row base) of its first
+    // 64-cell tile.
+    std::vector<int> blk_info;               // [4 * nblk]
+    std::vector<long long> blk_base;         // [2 * nblk]
     std::vector<int> cell_seg, cell_local;   // [nc] segment and slot of each cell
     std::vector<uint32_t> dof;               // [ndof] face row | kEbeFirst | kEbeLast
 };

@@ -67,15 +67,26 @@ struct DevScal {
 
 // Element-by-element level-0 operator (see mhs_ebe.h): per segment of cells
 // of one colour, instance and face count, the packed local matrices S'_c
-// component-major and the face rows with first/last-writer flags.
+// and the face rows with first/last-writer flags, both in 64-cell tiles.
 constexpr int kEbeColours = 8;
+
+// Offset of component q (of ncomp per cell) of the cell in slot `local` of an
+// element-by-element segment.  Cells are stored in tiles of 64 (one
+// wavefront), component-major inside a tile: a wavefront's load of one
+// component is 64 consecutive elements, and all components of its cells lie
+// in one contiguous ncomp * 64-element range, so the loads of a wavefront stay
+// within a page instead of striding across the whole segment.
+__host__ __device__ constexpr long long ebe_slot(int local, int q, int ncomp) {
+    return (long long)(local >> 6) * ((long long)ncomp << 6) + ((long long)q << 6) + (local & 63);
+}
 struct DevEbe {
     int ncolours = 0, nblk = 0;
     int colour_blk[kEbeColours + 1] = {};
-    const int *blk_seg = nullptr, *blk_slot = nullptr;
-    const int *seg_n = nullptr, *seg_inst = nullptr, *seg_count = nullptr, *seg_len = nullptr,
-              *seg_blk0 = nullptr;
-    const long long *seg_val = nullptr, *seg_dof = nullptr;
+    // per block: (face count, instance, cells in the block, partial slot) and
+    // (value base, face-row base) of its first tile
+    const int4 *blk_info = nullptr;
+    const longlong2 *blk_base = nullptr;
+    const long long *seg_val = nullptr;     // per segment (assembly writes the values)
     const int *cell_seg = nullptr, *cell_local = nullptr;
     const uint32_t *dof = nullptr;
     double *val64 = nullptr;   // S'_c in FP64 (residual replacement)

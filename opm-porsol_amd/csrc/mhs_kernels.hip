@@ -54,6 +54,10 @@ __device__ __forceinline__ int xcd_block(int nblk) {
     return (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
+// Instance of a 1024-row block.  A handle with one instance skips the table
+// read, so the kernels' first data loads wait on one fewer round trip.
+__device__ __forceinline__ int block_inst(const DevTopo &T, int b) { return T.ninst == 1 ? 0 : T.block_seg[b]; }
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -425,8 +429,8 @@ __global__ __launch_bounds__(64) void k_assemble(DevTopo T, const double *__rest
     if (E.val64) {
         // S'_c for the element-by-element operator: a Dirichlet row keeps its
         // diagonal only (:1832-1835), a Dirichlet column is dropped (:1880-1896)
-        const int sg = E.cell_seg[c], len = E.seg_len[sg];
-        const size_t vb = (size_t)E.seg_val[sg] + E.cell_local[c];
+        const int sg = E.cell_seg[c], local = E.cell_local[c], np = n * (n + 1) / 2;
+        const size_t vb = (size_t)E.seg_val[sg];
 #pragma unroll
         for (int i = 0; i < MAXN; ++i)
 #pragma unroll
@@ -434,7 +438,7 @@ __global__ __launch_bounds__(64) void k_assemble(DevTopo T, const double *__rest
                 if (j >= n) continue;
                 const bool drop = i != j && (ft[i] == FT_DIRICHLET || ft[j] == FT_DIRICHLET);
                 const double v = drop ? 0.0 : S[pidx(i, j, MAXN)];
-                const size_t o = vb + (size_t)pidx(i, j, n) * len;
+                const size_t o = vb + (size_t)ebe_slot(local, pidx(i, j, n), np);
                 E.val64[o] = v;
                 E.val32[o] = (float)v;
             }
@@ -588,7 +592,7 @@ __device__ __forceinline__ void spmv_pq_body(const DevTopo &T, const MT *__restr
                                              const VT *__restrict__ p, VT *__restrict__ q, const DevRed &R,
                                              const DevScal &S, const RelArgs &L) {
     const int b = T.spmv_xcd ? xcd_block(gridDim.x) : (int)blockIdx.x;
-    const int s = T.block_seg[b];
+    const int s = block_inst(T, b);
     int act[NRHS];
     if (!load_act<NRHS>(S.act, s, act)) return;
     int u[NRHS];
@@ -700,45 +704,26 @@ static void launch_spmv_kernel(hipStream_t st, unsigned nblk, const DevTopo &T, 
 enum { EBE_SPMV = 0, EBE_RESID = 1 };
 constexpr uint32_t kEbeFirstD = 1u << 30, kEbeLastD = 1u << 31, kEbeRowD = (1u << 30) - 1;
 
-template <int MAXN, int NRHS, int MODE, bool REL>
-__global__ __launch_bounds__(256) void k_ebe(DevEbe E, int colour, Gate gt, DevScal S,
-                                             const float *__restrict__ x, float *__restrict__ out,
-                                             const float *__restrict__ rin, RelArgs L,
-                                             double *__restrict__ r64) {
-    // XCD-contiguous block ranges: every colour pass gives an XCD the same
-    // cells' neighbourhood, so the second colour finds the rows the first
-    // one wrote in that XCD's L2
-    const int b = E.colour_blk[colour] + xcd_block(gridDim.x);
-    const int seg = E.blk_seg[b];
-    const int s = E.seg_inst[seg];
-    int act[NRHS];
-    bool any = false;
+// One cell of a colour pass (see k_ebe).  NF is the face count when FIXED
+// (every cell of the block has NF faces: the loops unroll to immediate load
+// offsets), else the bound of the block's runtime face count nrt.  RP: the
+// block's instance has its FP64 residual replacement due (block-uniform), so
+// the FP32-only path is compiled without any of the FP64 work.
+template <int NF, bool FIXED, int NRHS, int MODE, bool RP, int NQ>
+__device__ __forceinline__ void ebe_cell(const DevEbe &E, int nrt, longlong2 bb, bool valid, bool any,
+                                         const int (&u)[NRHS], const float *__restrict__ x,
+                                         float *__restrict__ out, const float *__restrict__ rin, const RelArgs &L,
+                                         double *__restrict__ r64, double (&part)[NQ]) {
+    constexpr int MAXN = NF;
+    const int n = FIXED ? NF : nrt, np = n * (n + 1) / 2;
+    // component q of this cell at tile base + 64 q (64-cell tiles)
+    const uint32_t *dp = E.dof + bb.y + ebe_slot(threadIdx.x, 0, n);
+    uint32_t d[MAXN];
 #pragma unroll
-    for (int k = 0; k < NRHS; ++k) {
-        act[k] = gt.act ? gt.act[s * NRHS + k] : 1;
-        any |= act[k] != 0;
-    }
+    for (int j = 0; j < MAXN; ++j) d[j] = (valid && j < n) ? __builtin_nontemporal_load(dp + 64 * j) : 0u;
     if (!any) return;                       // block-uniform: one instance per block
-    int u[NRHS];
-    bool rel = false;
-#pragma unroll
-    for (int k = 0; k < NRHS; ++k) {
-        u[k] = (REL && act[k]) ? S.upd[s * NRHS + k] : 0;
-        rel |= u[k] != 0;
-    }
-    const int n = E.seg_n[seg], len = E.seg_len[seg];
-    const int local = (b - E.seg_blk0[seg]) * 256 + (int)threadIdx.x;
-    const bool valid = local < E.seg_count[seg];
-    constexpr int NQ = MODE == EBE_SPMV ? (REL ? 4 * NRHS : NRHS) : 1;
-    double part[NQ];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) part[q] = 0.0;
     if (valid) {
-        const uint32_t *dp = E.dof + E.seg_dof[seg] + local;
-        uint32_t d[MAXN];
         float xv[MAXN][NRHS], y[MAXN][NRHS];
-#pragma unroll
-        for (int j = 0; j < MAXN; ++j) d[j] = (j < n) ? __builtin_nontemporal_load(dp + (size_t)j * len) : 0u;
 #pragma unroll
         for (int j = 0; j < MAXN; ++j)
 #pragma unroll
@@ -746,23 +731,23 @@ __global__ __launch_bounds__(256) void k_ebe(DevEbe E, int colour, Gate gt, DevS
                 xv[j][k] = (j < n) ? x[(size_t)(d[j] & kEbeRowD) * NRHS + k] : 0.0f;
                 y[j][k] = 0.0f;
             }
-        const size_t vb = (size_t)E.seg_val[seg] + local;
-        double y64[REL ? MAXN : 1][NRHS];
-        if (!(REL && rel)) {
+        const size_t vb = (size_t)bb.x + (size_t)ebe_slot(threadIdx.x, 0, np);
+        double y64[RP ? MAXN : 1][NRHS];
+        if constexpr (!RP) {
             const float *v = E.val32 + vb;
 #pragma unroll
             for (int i = 0; i < MAXN; ++i)
 #pragma unroll
                 for (int j = i; j < MAXN; ++j) {
                     if (j >= n) continue;
-                    const float a = __builtin_nontemporal_load(v + (size_t)pidx(i, j, n) * len);
+                    const float a = __builtin_nontemporal_load(v + 64 * pidx(i, j, n));
 #pragma unroll
                     for (int k = 0; k < NRHS; ++k) {
                         y[i][k] += a * xv[j][k];
                         if (j != i) y[j][k] += a * xv[i][k];
                     }
                 }
-        } else if constexpr (REL) {
+        } else {
             const double *v = E.val64 + vb;
             double xd[MAXN][NRHS];
 #pragma unroll
@@ -777,7 +762,7 @@ __global__ __launch_bounds__(256) void k_ebe(DevEbe E, int colour, Gate gt, DevS
 #pragma unroll
                 for (int j = i; j < MAXN; ++j) {
                     if (j >= n) continue;
-                    const double a = __builtin_nontemporal_load(v + (size_t)pidx(i, j, n) * len);
+                    const double a = __builtin_nontemporal_load(v + 64 * pidx(i, j, n));
                     const float af = (float)a;
 #pragma unroll
                     for (int k = 0; k < NRHS; ++k) {
@@ -807,30 +792,80 @@ __global__ __launch_bounds__(256) void k_ebe(DevEbe E, int colour, Gate gt, DevS
                     out[i] = (first ? rin[i] : out[i]) - y[j][k];
                 }
             }
-            if constexpr (REL && MODE == EBE_SPMV) {
-                if (rel) {
+            if constexpr (RP && MODE == EBE_SPMV) {
 #pragma unroll
-                    for (int k = 0; k < NRHS; ++k) {
-                        if (!u[k]) continue;
-                        const size_t i = row * NRHS + k;
-                        const double rv = (first ? L.b[i] : r64[i]) - y64[j][k];
-                        if (!last) {
-                            r64[i] = rv;
-                            continue;
-                        }
-                        const float rf = (float)rv;
-                        const float dg = L.dinv[row];
-                        L.r32[i] = rf;
-                        part[NRHS + 3 * k] += rv * rv;
-                        part[NRHS + 3 * k + 1] += (double)rf * (double)(dg * rf);
-                        part[NRHS + 3 * k + 2] += (double)rf * (double)rf;
+                for (int k = 0; k < NRHS; ++k) {
+                    if (!u[k]) continue;
+                    const size_t i = row * NRHS + k;
+                    const double rv = (first ? L.b[i] : r64[i]) - y64[j][k];
+                    if (!last) {
+                        r64[i] = rv;
+                        continue;
                     }
+                    const float rf = (float)rv;
+                    const float dg = L.dinv[row];
+                    L.r32[i] = rf;
+                    part[NRHS + 3 * k] += rv * rv;
+                    part[NRHS + 3 * k + 1] += (double)rf * (double)(dg * rf);
+                    part[NRHS + 3 * k + 2] += (double)rf * (double)rf;
                 }
             }
         }
     }
+}
+
+template <int MAXN, int NRHS, int MODE, bool REL>
+__global__ __launch_bounds__(256) void k_ebe(DevEbe E, int colour, Gate gt, DevScal S,
+                                             const float *__restrict__ x, float *__restrict__ out,
+                                             const float *__restrict__ rin, RelArgs L,
+                                             double *__restrict__ r64) {
+    // XCD-contiguous block ranges: every colour pass gives an XCD the same
+    // cells' neighbourhood, so the second colour finds the rows the first
+    // one wrote in that XCD's L2
+    const int b = E.colour_blk[colour] + xcd_block(gridDim.x);
+    // The block's two records first; then the instance's flags and the
+    // cells' face rows in one round trip.  The rows are loaded before the
+    // gate is known, so a block of a finished instance reads them in vain.
+    const int4 bi = E.blk_info[b];          // face count, instance, cells, partial slot
+    const longlong2 bb = E.blk_base[b];     // value and face-row bases of the first tile
+    const int n = bi.x, s = bi.y;
+    const bool valid = (int)threadIdx.x < bi.z;
+    int act[NRHS], u[NRHS];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) {
+        act[k] = gt.act ? gt.act[s * NRHS + k] : 1;
+        u[k] = REL ? S.upd[s * NRHS + k] : 0;
+    }
+    bool any = false, rel = false;
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) {
+        any |= act[k] != 0;
+        u[k] = act[k] ? u[k] : 0;
+        rel |= u[k] != 0;
+    }
+    constexpr int NQ = MODE == EBE_SPMV ? (REL ? 4 * NRHS : NRHS) : 1;
+    double part[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) part[q] = 0.0;
+    // The face count is block-uniform: blocks of hexahedra (and of cells
+    // with the largest face count) take the path with compile-time offsets,
+    // which a runtime count costs a quarter of the kernel's time.
+    auto faces = [&](auto rp) {
+        constexpr bool RP = decltype(rp)::value;
+        if (n == MAXN)
+            ebe_cell<MAXN, true, NRHS, MODE, RP>(E, n, bb, valid, any, u, x, out, rin, L, r64, part);
+        else if (MAXN > 6 && n == 6)
+            ebe_cell<(MAXN > 6 ? 6 : MAXN), true, NRHS, MODE, RP>(E, n, bb, valid, any, u, x, out, rin, L, r64, part);
+        else
+            ebe_cell<MAXN, false, NRHS, MODE, RP>(E, n, bb, valid, any, u, x, out, rin, L, r64, part);
+    };
+    if (REL && rel)
+        faces(std::integral_constant<bool, REL>{});
+    else
+        faces(std::false_type{});
+    if (!any) return;
     if constexpr (MODE == EBE_SPMV) {
-        const int slot = E.blk_slot[b];
+        const int slot = bi.w;
         if constexpr (REL) {
             if (rel) block_partials<NQ>(part, slot, E.R);
             else block_partials_n<NQ, NRHS>(part, slot, E.R);
@@ -848,7 +883,7 @@ __global__ __launch_bounds__(256) void k_update_xr(DevTopo T, const VT *__restri
                                                    DevScal S, VT *__restrict__ x0s = nullptr,
                                                    VT w = VT(0)) {
     const int b = xcd_block(gridDim.x);
-    const int s = T.block_seg[b];
+    const int s = block_inst(T, b);
     int act[NRHS];
     if (!load_act<NRHS>(S.act, s, act)) return;
     VT al[NRHS];
@@ -894,7 +929,7 @@ __global__ __launch_bounds__(256) void k_update_p(DevTopo T, const VT *__restric
                                                   VT *__restrict__ p, VT *__restrict__ x,
                                                   double *__restrict__ x64, DevScal S) {
     const int b = xcd_block(gridDim.x);
-    const int s = T.block_seg[b];
+    const int s = block_inst(T, b);
     int act[NRHS];
     if (!load_act<NRHS>(S.act, s, act)) return;
     VT be[NRHS];
@@ -978,7 +1013,7 @@ __global__ __launch_bounds__(256) void k_reliable(DevTopo T, const double *__res
                                                   float *__restrict__ r32, float *__restrict__ p32,
                                                   int init, double tol2, DevRed R, DevScal S) {
     const int b = xcd_block(gridDim.x);
-    const int s = T.block_seg[b];
+    const int s = block_inst(T, b);
     int u[NRHS];
     if (init) {
 #pragma unroll
@@ -1486,7 +1521,7 @@ __global__ __launch_bounds__(TB) void k_vc0_post(Gate gt, DevTopo T, const long 
                                                  const VT *__restrict__ res0, const VT *__restrict__ xc,
                                                  VT *__restrict__ z, VT w, VT al, DevRed R) {
     const int blk = T.sweep_xcd ? xcd_block(gridDim.x) : (int)blockIdx.x;
-    if (gate_off<NRHS>(gt, T.block_seg[blk])) return;
+    if (gate_off<NRHS>(gt, block_inst(T, blk))) return;
     double part[NRHS];
 #pragma unroll
     for (int k = 0; k < NRHS; ++k) part[k] = 0.0;
@@ -1537,7 +1572,7 @@ __global__ __launch_bounds__(TB) void k_vc0_res(Gate gt, DevTopo T, const VT *__
                                                 const VT *__restrict__ b, const VT *__restrict__ x,
                                                 VT *__restrict__ res) {
     const int blk = T.spmv_xcd ? xcd_block(gridDim.x) : (int)blockIdx.x;
-    if (gate_off<NRHS>(gt, T.block_seg[blk])) return;
+    if (gate_off<NRHS>(gt, block_inst(T, blk))) return;
 #pragma unroll
     for (int j = 0; j < kRowBlock / TB; ++j) {
         const int f = blk * kRowBlock + j * TB + threadIdx.x;
@@ -1889,7 +1924,7 @@ __global__ __launch_bounds__(256) void k_update_pz(DevTopo T, const VT *__restri
                                                    VT *__restrict__ p, VT *__restrict__ x,
                                                    double *__restrict__ x64, DevScal S) {
     const int b = xcd_block(gridDim.x);
-    const int s = T.block_seg[b];
+    const int s = block_inst(T, b);
     int act[NRHS];
     if (!load_act<NRHS>(S.act, s, act)) return;
     VT be[NRHS];
