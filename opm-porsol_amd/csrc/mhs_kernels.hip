@@ -38,6 +38,9 @@ struct FinArgs {
     int init;
 };
 
+template <int V>
+using IC = std::integral_constant<int, V>;
+
 // Operands of the FP64 residual replacement fused into the FP32 SpMV.
 struct RelArgs {
     const double *val64, *x64, *b;
@@ -852,9 +855,12 @@ __global__ __launch_bounds__(256) void k_ebe(DevEbe E, int colour, Gate gt, DevS
     // which a runtime count costs a quarter of the kernel's time.
     auto faces = [&](auto rp) {
         constexpr bool RP = decltype(rp)::value;
-        if (n == MAXN)
+        // the replacement path stays on the runtime face count: unrolled to
+        // immediate offsets it loads all FP64 values at once and needs 100
+        // VGPRs, which would halve the occupancy of every launch
+        if (!RP && n == MAXN)
             ebe_cell<MAXN, true, NRHS, MODE, RP>(E, n, bb, valid, any, u, x, out, rin, L, r64, part);
-        else if (MAXN > 6 && n == 6)
+        else if (!RP && MAXN > 6 && n == 6)
             ebe_cell<(MAXN > 6 ? 6 : MAXN), true, NRHS, MODE, RP>(E, n, bb, valid, any, u, x, out, rin, L, r64, part);
         else
             ebe_cell<MAXN, false, NRHS, MODE, RP>(E, n, bb, valid, any, u, x, out, rin, L, r64, part);
@@ -1508,6 +1514,61 @@ __global__ __launch_bounds__(TB) void k_vc0_up(DevTopo T, const VT *__restrict__
     if (LAST) block_partials<NRHS, TB / 64>(part, blk, R);
 }
 
+// Width of an A P slice as a compile-time constant (0: wider than 8, taken
+// 8 entries at a time).
+template <class Fn>
+__device__ __forceinline__ void with_ap_width(int wd, Fn fn) {
+    switch (wd) {
+        case 1: fn(IC<1>{}); break;
+        case 2: fn(IC<2>{}); break;
+        case 3: fn(IC<3>{}); break;
+        case 4: fn(IC<4>{}); break;
+        case 5: fn(IC<5>{}); break;
+        case 6: fn(IC<6>{}); break;
+        case 7: fn(IC<7>{}); break;
+        case 8: fn(IC<8>{}); break;
+        default: fn(IC<0>{}); break;
+    }
+}
+
+// acc = sum over the W entries of one A P row (lane-major: entry e at 64 e)
+// of a * xc[col]; W = 0: the runtime width wd in chunks of kChunk.
+template <typename VT, int NRHS, int W>
+__device__ __forceinline__ void ap_row(const int *__restrict__ col, const VT *__restrict__ val, int wd,
+                                       const VT *__restrict__ xc, VT (&acc)[NRHS]) {
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) acc[k] = VT(0);
+    if constexpr (W > 0) {
+        int c[W];
+        VT a[W];
+#pragma unroll
+        for (int u = 0; u < W; ++u) {
+            c[u] = __builtin_nontemporal_load(col + 64 * u);
+            a[u] = __builtin_nontemporal_load(val + 64 * u);
+        }
+#pragma unroll
+        for (int u = 0; u < W; ++u)
+#pragma unroll
+            for (int k = 0; k < NRHS; ++k) acc[k] += a[u] * xc[(size_t)c[u] * NRHS + k];
+    } else {
+        for (int e0 = 0; e0 < wd; e0 += kChunk) {
+            int c[kChunk];
+            VT a[kChunk];
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u) {
+                const bool in = e0 + u < wd;
+                c[u] = in ? __builtin_nontemporal_load(col + 64LL * (e0 + u)) : 0;
+                a[u] = in ? __builtin_nontemporal_load(val + 64LL * (e0 + u)) : VT(0);
+            }
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u)
+#pragma unroll
+                for (int k = 0; k < NRHS; ++k)
+                    acc[k] += e0 + u < wd ? a[u] * xc[(size_t)c[u] * NRHS + k] : VT(0);
+        }
+    }
+}
+
 // Post-smoothing sweep after the coarse correction without reading A: with
 // x~ = x0 + alpha P xc and A x0 = r - res0 (from the pre-smoothing pass),
 //   z = x~ + w D^-1 (r - A x~) = x~ + w D^-1 (res0 - alpha (A P) xc),
@@ -1532,24 +1593,12 @@ __global__ __launch_bounds__(TB) void k_vc0_post(Gate gt, DevTopo T, const long 
         const long long s0 = ap_ptr[slice];
         const int wd = (int)((ap_ptr[slice + 1] - s0) >> 6);
         VT acc[NRHS];
-#pragma unroll
-        for (int k = 0; k < NRHS; ++k) acc[k] = VT(0);
-        for (int e0 = 0; e0 < wd; e0 += kChunk) {
-            int c[kChunk];
-            VT a[kChunk];
-#pragma unroll
-            for (int u = 0; u < kChunk; ++u) {
-                const bool in = e0 + u < wd;
-                const long long o = s0 + lane + 64LL * (e0 + u);
-                c[u] = in ? __builtin_nontemporal_load(ap_col + o) : 0;
-                a[u] = in ? __builtin_nontemporal_load(ap + o) : VT(0);
-            }
-#pragma unroll
-            for (int u = 0; u < kChunk; ++u)
-#pragma unroll
-                for (int k = 0; k < NRHS; ++k)
-                    acc[k] += e0 + u < wd ? a[u] * xc[(size_t)c[u] * NRHS + k] : VT(0);
-        }
+        // the slice width is wave-uniform: widths up to 8 (every slice of a
+        // hexahedral grid) unroll to immediate load offsets
+        with_ap_width(wd, [&](auto wc) {
+            constexpr int W = decltype(wc)::value;
+            ap_row<VT, NRHS, W>(ap_col + s0 + lane, ap + s0 + lane, wd, xc, acc);
+        });
         const int g = agg[f];
         const VT d = w * dinv[f];
 #pragma unroll
@@ -2540,8 +2589,6 @@ __global__ __launch_bounds__(256) void k_gravity_term(DevTopo T, DevGeom G, doub
 
 // ===========================================================================
 // Launchers.
-template <int V>
-using IC = std::integral_constant<int, V>;
 
 template <class Fn>
 static hipError_t with_nrhs(int nrhs, Fn fn) {
