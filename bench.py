@@ -202,6 +202,20 @@ def _cpu_baseline(pkg, g, perm, bt, bv, iters_needed, sample_iters):
 
 
 
+def _forced_shape(args):
+    """Launch shapes given on the command line (the profiling runs fix them so
+    that every counter pass replays the same kernels); the autotuner's choice
+    otherwise."""
+    out = {}
+    if args.spmv_threads:
+        out["spmv_threads"] = args.spmv_threads
+    if args.sweep_threads:
+        out["sweep_threads"] = args.sweep_threads
+    if args.tail_rows >= 0:
+        out["tail_rows"] = args.tail_rows
+    return out
+
+
 def _c2_fixture():
     """The oracle's solution of the headline case (tests/golden/c2_128x128x64.npz,
     written by tests/golden/make_golden.py with the oracle's ILU0-CG to 1e-14)."""
@@ -876,6 +890,12 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--chunk", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--spmv-threads", type=int, default=0,
+                    help="force the SpMV workgroup size (256, 512, 1024; default: autotuned)")
+    ap.add_argument("--sweep-threads", type=int, default=0,
+                    help="force the post-smoothing sweep's workgroup size (default: autotuned)")
+    ap.add_argument("--tail-rows", type=int, default=-1,
+                    help="force the one-workgroup tail's level size limit (default: autotuned)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch PCG kernels one by one instead of replaying captured graphs")
     ap.add_argument("--no-compare", action="store_true",
@@ -923,7 +943,7 @@ def main():
     def step(precision, precond=args.precond):
         return s.solve_device(tm.data_ptr(), rm.data_ptr(), src.data_ptr(), bt, bvd.data_ptr(),
                               nrhs=1, tol=TOL, precision=precision, chunk=args.chunk,
-                              precond=precond, use_graph=not args.no_graph)
+                              precond=precond, use_graph=not args.no_graph, **_forced_shape(args))
 
     def timed(precision, precond):
         step(precision, precond)                 # first call builds graphs / hierarchy

@@ -152,6 +152,8 @@ void sa_transfer(AmgLevel &F, AmgLevel &C) {
                 F.pt_col[e] = i;
                 F.pt_map[e] = q;
             }
+        F.p_tpos.assign(F.p_col.size(), 0);
+        for (size_t e = 0; e < F.pt_map.size(); ++e) F.p_tpos[F.pt_map[e]] = int(e);
     }
     // T = A P
     std::vector<std::array<int, 3>> tb;

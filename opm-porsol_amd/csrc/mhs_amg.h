@@ -44,6 +44,7 @@ struct AmgLevel {
     std::vector<int> p_ptr, p_col, p_own;      // P rows; p_own[i]: entry of row i in its own aggregate
     std::vector<int> p_src_ptr, p_src;         // per P entry: positions of this level's values
     std::vector<int> pt_ptr, pt_col, pt_map;   // rows of the coarser level: fine row, P position
+    std::vector<int> p_tpos;                   // per P entry: its position in the P' rows (inverse of pt_map)
     std::vector<int> t_ptr, t_col;             // T = A P
     std::vector<int> t_pair_ptr, t_pa, t_pb;   // per T entry: (value position, P position)
 };

@@ -445,6 +445,9 @@ int build_amg_device(mhs_solver *s, const Strength &strength) {
                 if (he == hipSuccess) D.pt_ptr = dupload(s, L, H.pt_ptr, &he);
                 if (he == hipSuccess) D.pt_col = dupload(s, L, H.pt_col, &he);
                 if (he == hipSuccess) D.pt_map = dupload(s, L, H.pt_map, &he);
+                if (he == hipSuccess) D.p_tpos = dupload(s, L, H.p_tpos, &he);
+                if (he == hipSuccess) D.pt_val64 = dalloc<double>(s, L, (size_t)D.p_nnz, &he);
+                if (he == hipSuccess) D.pt_val32 = dalloc<float>(s, L, (size_t)D.p_nnz, &he);
                 if (he == hipSuccess) D.t_ptr = dupload(s, L, H.t_ptr, &he);
                 if (he == hipSuccess) D.t_col = dupload(s, L, H.t_col, &he);
                 if (he == hipSuccess) D.t_pair_ptr = dupload(s, L, H.t_pair_ptr, &he);

@@ -114,6 +114,9 @@ struct DevAmgLevel {
     const int *src2;
     const int *p_ptr, *p_col, *p_own, *p_src_ptr, *p_src;
     const int *pt_ptr, *pt_col, *pt_map;
+    const int *p_tpos;              // per P entry: its position in the P' rows
+    double *pt_val64;               // P values in P' row order (refilled with P)
+    float *pt_val32;
     const int *t_ptr, *t_col, *t_pair_ptr, *t_pa, *t_pb;
     double *p_val64, *t_val64;
     float *p_val32, *t_val32;

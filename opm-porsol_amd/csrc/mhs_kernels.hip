@@ -1179,6 +1179,10 @@ __global__ __launch_bounds__(256) void k_sa_p(DevAmgLevel F, double wp) {
         const double v = (q == F.p_own[i] ? 1.0 : 0.0) - wp * d * s;
         F.p_val64[q] = v;
         if (F.p_val32) F.p_val32[q] = (float)v;
+        // the same value in P' row order, for the restriction's contiguous rows
+        const int e = F.p_tpos[q];
+        F.pt_val64[e] = v;
+        if (F.pt_val32) F.pt_val32[e] = (float)v;
     }
 }
 
@@ -1371,16 +1375,21 @@ __device__ __forceinline__ const VT *lvl_p(const DevAmgLevel &L) {
     if constexpr (std::is_same<VT, float>::value) return L.p_val32; else return L.p_val64;
 }
 template <typename VT>
+__device__ __forceinline__ const VT *lvl_pt(const DevAmgLevel &L) {
+    if constexpr (std::is_same<VT, float>::value) return L.pt_val32; else return L.pt_val64;
+}
+template <typename VT>
 __device__ __forceinline__ const VT *lvl_t(const DevAmgLevel &L) {
     if constexpr (std::is_same<VT, float>::value) return L.t_val32; else return L.t_val64;
 }
 
-// acc = (P' res)_I across a smoothed transfer (P' row I: fine rows and P positions).
+// acc = (P' res)_I across a smoothed transfer (P' row I: fine rows and the P
+// values stored in P' row order, so a row's values are contiguous).
 template <typename VT, int NRHS, int G = 1>
 __device__ __forceinline__ void row_restrict_sa(const DevAmgLevel &F, const VT *__restrict__ res, int I,
                                                 VT (&acc)[NRHS], int sub = 0) {
     constexpr int KC = chunk_for<G>();
-    const VT *pval = lvl_p<VT>(F);
+    const VT *ptval = lvl_pt<VT>(F);
 #pragma unroll
     for (int k = 0; k < NRHS; ++k) acc[k] = VT(0);
     const int e1 = F.pt_ptr[I + 1];
@@ -1392,7 +1401,7 @@ __device__ __forceinline__ void row_restrict_sa(const DevAmgLevel &F, const VT *
             const int e = base + G * u;
             const bool in = e < e1;
             c[u] = in ? F.pt_col[e] : 0;
-            pv[u] = in ? pval[F.pt_map[e]] : VT(0);
+            pv[u] = in ? ptval[e] : VT(0);
         }
         VT r[KC][NRHS];
 #pragma unroll
@@ -1865,13 +1874,49 @@ __global__ __launch_bounds__(256) void k_coarse_solve(const int *__restrict__ in
     }
 }
 
+// Coarse-level up-sweep from an iterate the coarser level's kernel already
+// prolonged in place (x~ = w D^-1 b + al P xc): xo = x~ + w D^-1 (b - A x~),
+// one gather per entry (k_vcc_up1 forms x~ of every column on the fly, four
+// dependent gathers).  PRO: the transfer from this level to the finer one is
+// tentative and covers it, so the launch also prolongs xo into the finer
+// level's iterate in place, xf[c] += al xo[i] for the children c of row i.
+template <typename VT, int NRHS, int G, bool PRO>
+__global__ __launch_bounds__(256) void k_vcc_up_plain(DevAmgLevel F, const VT *__restrict__ x, VT *__restrict__ xo,
+                                                      VT *__restrict__ xf, VT w, VT al) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int i = t / G, sub = t % G;
+    if (i >= F.n) return;
+    VT acc[NRHS];
+    row_gather<VT, NRHS, G>(F.col, lvl_val<VT>(F), F.row_ptr[i], F.row_ptr[i + 1], x, acc, sub);
+    const VT d = w * lvl_dinv<VT>(F)[i];
+    const VT *b = (const VT *)F.b;
+    VT o[NRHS];
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) {
+        const size_t q = (size_t)i * NRHS + k;
+        o[k] = x[q] + d * (b[q] - acc[k]);   // every lane of the group holds the row sum
+    }
+    if (sub == 0) {
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) xo[(size_t)i * NRHS + k] = o[k];
+    }
+    if constexpr (PRO) {
+        for (int c = F.child_ptr[i] + sub; c < F.child_ptr[i + 1]; c += G) {
+            const int f = F.child[c];
+#pragma unroll
+            for (int k = 0; k < NRHS; ++k) xf[(size_t)f * NRHS + k] += al * o[k];
+        }
+    }
+}
+
 // The small levels lt..L-1 of one V(1,1) cycle (one sweep each way), one
 // workgroup per instance: the same arithmetic as k_vcc_down + k_restrict,
 // k_coarse_solve and k_vcc_up1, with workgroup barriers between the steps
 // instead of kernel launches.  Reads lev[lt].b, writes lev[l].x for l >= lt.
 template <typename VT, int NRHS>
 __global__ __launch_bounds__(1024) void k_vcycle_tail(Gate gt, const DevAmg *__restrict__ Ap, int lt, VT al,
-                                                      VT als) {
+                                                      VT als,
+                                                      VT *__restrict__ xf) {
     const DevAmg &A = *Ap;
     const int s = blockIdx.x;
     if (gate_off<NRHS>(gt, s)) return;
@@ -1963,6 +2008,20 @@ __global__ __launch_bounds__(1024) void k_vcycle_tail(Gate gt, const DevAmg *__r
             }
         }
         __syncthreads();
+    }
+    if (xf) {
+        // the transfer from level lt to the finer level is tentative and
+        // covers it: prolong lev[lt].x into that level's iterate in place,
+        // x~_f = w D^-1 b + al x_lt[agg_f] (the down-sweep wrote w D^-1 b)
+        // (one thread per finer row: every row has its aggregate)
+        const DevAmgLevel &C = A.lev[lt];
+        const DevAmgLevel &F = A.lev[lt - 1];
+        const VT *xc = (const VT *)C.x;
+        for (int f = F.inst_ptr[s] + tid; f < F.inst_ptr[s + 1]; f += nt) {
+            const int J = C.agg[f];
+#pragma unroll
+            for (int k = 0; k < NRHS; ++k) xf[(size_t)f * NRHS + k] += al * xc[(size_t)J * NRHS + k];
+        }
     }
 }
 
@@ -2903,7 +2962,10 @@ hipError_t launch_amg_setup(hipStream_t st, const DevTopo &T, const DevAmg &A, c
                                                             mixed ? L.dinv32 : nullptr);
         if (L.sa_out && L.n > 0) {
             DevAmgLevel Lw = L;
-            if (!mixed) Lw.p_val32 = nullptr;
+            if (!mixed) {
+                Lw.p_val32 = nullptr;
+                Lw.pt_val32 = nullptr;
+            }
             k_sa_p<<<(L.n + 255) / 256, 256, 0, st>>>(Lw, A.omega_p);
             if (L.t_nnz > 0)
                 with_lanes(lanes_per_row(L.t_npair, L.t_nnz), [&](auto g) {
@@ -3078,24 +3140,49 @@ static void vcycle_impl(hipStream_t st, const Gate &gt, const DevTopo &T, const 
                 });
             }
         }
-        k_vcycle_tail<VT, NR><<<A.ninst, 1024, 0, st>>>(gt, A.dev, lt, al, als);
+        // Tentative transfers l-1 <- l, from the tail down, whose coarse result
+        // the producing kernel prolongs in place into level l-1's iterate; the
+        // up-sweep of level l-1 then gathers one value per entry.
+        bool fuse[kMaxLevels] = {};
+        for (int l = lt; l >= 2; --l) {
+            if (A.lev[l - 1].sa_out || !A.covers[l] || (l < lt && !fuse[l + 1])) break;
+            fuse[l] = true;
+        }
+        k_vcycle_tail<VT, NR><<<A.ninst, 1024, 0, st>>>(gt, A.dev, lt, al, als,
+                                                        fuse[lt] ? (VT *)A.lev[lt - 1].x : nullptr);
+        VT *xres[kMaxLevels];   // each level's result after its up-sweep
+        xres[lt] = (VT *)A.lev[lt].x;
         for (int l = lt - 1; l >= 1; --l) {
             const DevAmgLevel &F = A.lev[l];
-            const VT *xc = (const VT *)A.lev[l + 1].x;
+            const VT *xc = xres[l + 1];
             if (F.sa_out) {
                 with_lanes(lanes_per_row(std::max(F.p_nnz, F.t_nnz), F.n), [&](auto g) {
                     constexpr int Gv = decltype(g)::value;
                     k_vcc_up_sa<VT, NR, Gv><<<row_blocks(F.n, Gv), 256, 0, st>>>(F, xc, wl(l), als);
                 });
+                xres[l] = (VT *)F.x;
+            } else if (fuse[l + 1]) {
+                VT *xf = fuse[l] ? (VT *)A.lev[l - 1].x : nullptr;
+                with_lanes(lanes_per_row(F.nnz, F.n), [&](auto g) {
+                    constexpr int Gv = decltype(g)::value;
+                    if (xf)
+                        k_vcc_up_plain<VT, NR, Gv, true><<<row_blocks(F.n, Gv), 256, 0, st>>>(
+                            F, (const VT *)F.x, (VT *)F.xt, xf, wl(l), al);
+                    else
+                        k_vcc_up_plain<VT, NR, Gv, false><<<row_blocks(F.n, Gv), 256, 0, st>>>(
+                            F, (const VT *)F.x, (VT *)F.xt, nullptr, wl(l), al);
+                });
+                xres[l] = (VT *)F.xt;
             } else {
                 with_lanes(lanes_per_row(F.nnz, F.n), [&](auto g) {
                     constexpr int Gv = decltype(g)::value;
                     k_vcc_up1<VT, NR, Gv><<<row_blocks(F.n, Gv), 256, 0, st>>>(F, A.lev[l + 1].agg, xc, (VT *)F.x,
                                                                              wl(l), al);
                 });
+                xres[l] = (VT *)F.x;
             }
         }
-        xcur[1] = (VT *)A.lev[1].x;
+        xcur[1] = xres[1];
     } else {
         // ---- coarse levels down
         for (int l = 1; l < L - 1; ++l) {
