@@ -107,12 +107,15 @@ typedef struct {
     int tail_rows;             /* -1: from autotune (0 when autotune is 0); >= 0: coarse
                                   levels whose largest instance has at most this many
                                   rows run in one workgroup per instance */
-    int element_operator;      /* mixed precision: 1 (default) applies the face
-                                  operator of the PCG product, its FP64 residual
-                                  replacement and the multigrid pre-smoothing residual
-                                  cell by cell from the local Schur complements, when
-                                  the structure has no periodic faces and no instance
-                                  is regularised; 0: the assembled FP32 face matrix */
+    int element_operator;      /* mixed precision: apply the face operator of the PCG
+                                  product, its FP64 residual replacement and the
+                                  multigrid pre-smoothing residual cell by cell from the
+                                  local Schur complements (only when the structure has
+                                  no periodic faces and no instance is regularised).
+                                  1 (default): when the autotuner finds it faster than
+                                  the assembled FP32 matrix (timed on the first solve of
+                                  each nrhs; always with autotune 0); 2: whenever it
+                                  applies; 0: never (the assembled FP32 face matrix) */
 } mhs_solve_options;
 
 typedef struct {

@@ -112,6 +112,9 @@ struct mhs_solver {
     std::string ebe_why;                // else why not
     DevEbe dE{};
     bool last_ebe = false;              // the last solve used it
+    // per nrhs (1..4): whether the autotuner found it faster than the
+    // assembled FP32 matrix on this structure (-1: not tried yet)
+    int ebe_pref[5] = {-1, -1, -1, -1, -1};
 
     // multigrid hierarchy
     std::vector<int> ijk;
@@ -215,6 +218,7 @@ void mhs_solver::free_amg() {
     H = AmgHierarchy();
     rho.clear();
     shapes.clear();
+    std::fill(ebe_pref, ebe_pref + 5, -1);
 }
 
 void mhs_solver::free_topology() {
@@ -229,6 +233,7 @@ void mhs_solver::free_topology() {
     an_lam = an_rlam = an_lamK = an_dynKg = an_binv = an_gflux = an_ones = nullptr;
     dT = DevTopo{};
     shapes.clear();
+    std::fill(ebe_pref, ebe_pref + 5, -1);
     dG = DevGeom{};
 }
 
@@ -1032,10 +1037,14 @@ int estimate_rho(mhs_solver *s) {
 // stream, 2 untimed + 10 timed launches each) and keep the fastest.  Done
 // once per structure and (nrhs, precision, preconditioner); the choice only
 // changes the order of floating-point partial sums.
-int autotune(mhs_solver *s, int nrhs, bool mixed, bool amg, double *tune_ms) {
+int autotune(mhs_solver *s, int nrhs, bool mixed, bool amg, bool choose_ebe, double *tune_ms) {
     const DevEbe *E = mixed ? cur_ebe(s) : nullptr;
-    const int key = (nrhs * 4 + (mixed ? 2 : 0) + (amg ? 1 : 0)) * 2 + (E ? 1 : 0);
-    auto it = s->shapes.find(key);
+    auto key_of = [&]() { return (nrhs * 4 + (mixed ? 2 : 0) + (amg ? 1 : 0)) * 2 + (E ? 1 : 0); };
+    auto it = s->shapes.find(key_of());
+    // first solve of this nrhs with the element-by-element operator allowed
+    // but not forced: time it against the assembled FP32 matrix and keep the
+    // faster (many colours or irregular cells can make it the slower one)
+    const bool trial = E && choose_ebe && s->ebe_pref[nrhs] < 0;
     *tune_ms = 0.0;
     if (it == s->shapes.end()) {
         auto t0 = std::chrono::steady_clock::now();
@@ -1081,7 +1090,7 @@ int autotune(mhs_solver *s, int nrhs, bool mixed, bool amg, double *tune_ms) {
         // the assembled-matrix SpMV (not run when the operator is applied cell by cell)
         for (int tb : {256, 512, 1024})
             for (int xcd : {1, 0}) {
-                if (E) break;
+                if (E && !trial) break;
                 s->dT.spmv_tb = tb;
                 s->dT.spmv_xcd = xcd;
                 const float ms = time_it([&]() {
@@ -1092,6 +1101,17 @@ int autotune(mhs_solver *s, int nrhs, bool mixed, bool amg, double *tune_ms) {
                 });
                 if (ms < bs) { bs = ms; best.spmv_tb = tb; best.spmv_xcd = xcd; }
             }
+        if (trial) {
+            const float te = time_it([&]() {
+                return launch_ebe_spmv(s->stream, s->T.max_ncf, nrhs, *E, s->p32, s->q32, s->S);
+            });
+            const bool use = te < bs;
+            s->ebe_pref[nrhs] = use ? 1 : 0;
+            if (!use) {      // the rest of the tuning and the solve use the assembled matrix
+                s->last_ebe = false;
+                E = nullptr;
+            }
+        }
         if (amg)
             for (int tb : {256, 512, 1024})
                 for (int xcd : {1, 0}) {
@@ -1132,7 +1152,7 @@ int autotune(mhs_solver *s, int nrhs, bool mixed, bool amg, double *tune_ms) {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         if (err != hipSuccess) return fail(s, MHS_ERR_HIP, std::string("autotune: ") + hipGetErrorString(err));
-        it = s->shapes.emplace(key, best).first;
+        it = s->shapes.emplace(key_of(), best).first;
         *tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     s->dT.spmv_tb = it->second.spmv_tb;
@@ -1344,8 +1364,11 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
     // the level-0 operator of the FP32 iteration: cell by cell when the
     // structure allows it and no instance is regularised (the doubled entry
     // of :1398-1400 is not a cell contribution)
+    // element_operator 2 always, 1 when the autotuner found it the faster
+    // operator for this nrhs on this structure (not yet tried: try it)
     s->last_ebe = mixed && s->ebe_ok && opt.element_operator != 0 &&
-                  std::none_of(reg.begin(), reg.end(), [](int v) { return v != 0; });
+                  std::none_of(reg.begin(), reg.end(), [](int v) { return v != 0; }) &&
+                  (opt.element_operator == 2 || !opt.autotune || s->ebe_pref[nrhs] != 0);
     // --- assembly -----------------------------------------------------------
     s->last_aniso = binv_use != nullptr;
     const double *binv = binv_use ? binv_use : s->binv, *gflux = gflux_use ? gflux_use : s->gflux;
@@ -1415,7 +1438,7 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
 
         // --- launch shapes -------------------------------------------------------
         if (opt.autotune) {
-            rc = autotune(s, nrhs, mixed, amg, &tune_ms);
+            rc = autotune(s, nrhs, mixed, amg, opt.element_operator == 1, &tune_ms);
             if (rc) return rc;
         } else {
             const Shape d;

@@ -24,8 +24,9 @@ def _check(pkg, orc, g, perm, tm, rm, src, bt, bv, grav=(0.0, 0.0, 0.0), inst=No
     s.init(g, perm, grav, bt, None, inst_cellpos=inst)
     out = {}
     for eo in (True, False):
+        # 2: cell by cell whenever it applies (1 would let the autotuner pick)
         rep = s.solve(tm, rm, src, bt, bv, residual_tolerance=tol, precision="mixed", precond=precond,
-                      element_operator=eo)
+                      element_operator=2 if eo else 0)
         assert rep.converged and rep.reduction <= tol
         assert rep.element_operator == int(eo)
         out[eo] = (s.getSolution(), rep.iterations, rep.colours)
@@ -109,3 +110,27 @@ def test_falls_back_where_it_does_not_apply(pkg, orc):
     assert rep.element_operator == 0 and rep.converged
     ref = orc.solve(g, perm, tm, rm, np.zeros(nc), bt, bv, tol=1e-14)
     assert rel_err(s.getSolution()[2], ref.outflux) < 1e-8
+
+
+def test_autotuner_chooses_the_operator(pkg, orc):
+    """element_operator 1 (the default): the first solve times the cell-by-cell
+    product against the assembled FP32 SpMV and keeps the faster for the
+    structure; either way the solution is the oracle's."""
+    for g in (cases.case_c1()[0], pkg.grids.corner_point(14, 12, 6, 10.0, 10.0, 2.0, seed=5)):
+        nc = g.num_cells
+        perm = pkg.fields.perm_c4(nc, seed=5)
+        bt, bv = cases.xdrop_bc(g, 2.0e5, 1.0e5)
+        tm, rm = cases.unit_fluid(nc)
+        s = pkg.HybridSolver(0)
+        s.init(g, perm, (0, 0, 0), bt, None)
+        ref = orc.solve(g, perm, tm, rm, np.zeros(nc), bt, bv, tol=1e-14)
+        chosen = None
+        for _ in range(2):
+            rep = s.solve(tm, rm, np.zeros(nc), bt, bv, residual_tolerance=1e-12, precision="mixed")
+            assert rep.converged and rep.element_operator in (0, 1)
+            if chosen is None:
+                assert rep.tune_ms > 0.0
+                chosen = rep.element_operator
+            else:
+                assert rep.tune_ms == 0.0 and rep.element_operator == chosen   # kept for the structure
+            assert rel_err(s.getSolution()[2], ref.outflux) < 1e-8
