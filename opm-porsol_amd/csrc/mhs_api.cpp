@@ -545,7 +545,8 @@ int mhs_init_system_structure(mhs_solver *s, const int *hf_bc_type, const int *h
         if (s->nb[h] < 0 && hf_bc_type[h] == MHS_BC_PERIODIC) periodic = true;
     std::vector<int> colour;
     int ncol = 1;
-    if (!periodic) ncol = greedy_colours(s->nc, s->facepos.data(), s->nb.data(), colour);
+    if (!periodic)
+        ncol = greedy_colours(s->nc, s->facepos.data(), s->nb.data(), colour, s->ijk.empty() ? nullptr : s->ijk.data());
     const bool by_parity = !s->ijk.empty();
     for (int c = 0; c < s->nc; ++c)
         for (int h = s->facepos[c]; h < s->facepos[c + 1]; ++h) {
@@ -596,7 +597,7 @@ int mhs_init_system_structure(mhs_solver *s, const int *hf_bc_type, const int *h
     if (he == hipSuccess) s->binv = dalloc<double>(s, L, (size_t)T.max_ncf * (T.max_ncf + 1) / 2 * T.nc, &he);
     if (he == hipSuccess) s->gflux = dalloc<double>(s, L, (size_t)T.nhf, &he);
     if (he != hipSuccess) return fail(s, MHS_ERR_ALLOC, std::string("device allocation: ") + hipGetErrorString(he));
-    s->ebe_why = build_ebe(s->EL, T);
+    s->ebe_why = build_ebe(s->EL, T, s->ijk.empty() ? nullptr : s->ijk.data());
     s->ebe_ok = s->ebe_why.empty() && s->EL.ncolours <= kEbeColours && T.max_ncf <= kMaxFaces;
     if (s->ebe_ok) {
         const EbeLayout &E = s->EL;
@@ -605,6 +606,7 @@ int mhs_init_system_structure(mhs_solver *s, const int *hf_bc_type, const int *h
         D.ncolours = E.ncolours;
         D.nblk = E.nblk;
         for (int k = 0; k <= E.ncolours; ++k) D.colour_blk[k] = E.colour_blk[k];
+        for (int k = 0; k < E.ncolours; ++k) D.colour_split[k] = E.colour_split[k];
         D.blk_info = reinterpret_cast<const int4 *>(dupload(s, L, E.blk_info, &he));
         if (he == hipSuccess) D.blk_base = reinterpret_cast<const longlong2 *>(dupload(s, L, E.blk_base, &he));
         if (he == hipSuccess) D.seg_val = dupload(s, L, E.seg_val, &he);

@@ -8,7 +8,29 @@
 
 namespace mhs {
 
-int greedy_colours(int nc, const int *facepos, const int *nb, std::vector<int> &colour) {
+int greedy_colours(int nc, const int *facepos, const int *nb, std::vector<int> &colour, const int *ijk) {
+    if (ijk) {
+        colour.assign(nc, 0);
+        for (int c = 0; c < nc; ++c) colour[c] = (ijk[3 * (size_t)c] + ijk[3 * (size_t)c + 1] + ijk[3 * (size_t)c + 2]) & 1;
+        int ncol = nc > 0 ? 1 : 0;
+        for (int c = 0; c < nc; ++c) {
+            bool clash = false;
+            unsigned long long used = 0;
+            for (int h = facepos[c]; h < facepos[c + 1]; ++h) {
+                const int o = nb[h];
+                if (o < 0) continue;
+                if (o < c && colour[o] == colour[c]) clash = true;
+                if (colour[o] < 64) used |= 1ull << colour[o];
+            }
+            if (clash) {
+                int k = 2;
+                while (k < 64 && (used & (1ull << k))) ++k;
+                colour[c] = k;
+            }
+            ncol = std::max(ncol, colour[c] + 1);
+        }
+        return ncol;
+    }
     colour.assign(nc, -1);
     int ncol = 0;
     for (int c = 0; c < nc; ++c) {
@@ -25,21 +47,24 @@ int greedy_colours(int nc, const int *facepos, const int *nb, std::vector<int> &
     return ncol;
 }
 
-std::string build_ebe(EbeLayout &E, const Topology &T) {
+std::string build_ebe(EbeLayout &E, const Topology &T, const int *ijk) {
     E = EbeLayout();
     if (T.has_periodic) return "periodic faces";
     if ((long long)T.nfp > (long long)kEbeRow) return "too many face rows";
     const int nc = T.nc;
     std::vector<int> colour;
-    const int ncol = greedy_colours(nc, T.facepos.data(), T.nb.data(), colour);
+    const int ncol = greedy_colours(nc, T.facepos.data(), T.nb.data(), colour, ijk);
     if (ncol > kMaxColours) return "more than 8 colours";
     // instance of every cell
     std::vector<int> inst(nc);
     for (int t = 0; t < T.ninst; ++t)
         for (int c = T.inst_cellpos[t]; c < T.inst_cellpos[t + 1]; ++c) inst[c] = t;
-    // segments: cells sorted by (colour, instance, face count, index)
+    // segments: cells sorted by (colour, hexahedron or not, instance, face
+    // count, index), so that each colour's hexahedra form one block range
+    // (launched with the 6-face kernel) followed by the other cells
     auto key = [&](int c) {
-        return ((long long)colour[c] * T.ninst + inst[c]) * (kMaxFaces + 1) + (T.facepos[c + 1] - T.facepos[c]);
+        const int n = T.facepos[c + 1] - T.facepos[c];
+        return (((long long)colour[c] * 2 + (n == 6 ? 0 : 1)) * T.ninst + inst[c]) * (kMaxFaces + 1) + n;
     };
     std::vector<int> order(nc);
     std::iota(order.begin(), order.end(), 0);
@@ -79,6 +104,12 @@ std::string build_ebe(EbeLayout &E, const Topology &T) {
     for (int k = 0, b = 0; k <= ncol; ++k) {
         while (b < E.nblk && seg_colour[E.blk_seg[b]] < k) ++b;
         E.colour_blk[k] = b;
+    }
+    E.colour_split.assign(ncol, 0);
+    for (int k = 0; k < ncol; ++k) {
+        int b = E.colour_blk[k];
+        while (b < E.colour_blk[k + 1] && E.seg_n[E.blk_seg[b]] == 6) ++b;
+        E.colour_split[k] = b;
     }
     // face rows with the first/last-writer flags, in the 64-cell tiles of the values
     E.dof.assign((size_t)E.ndof, 0u);

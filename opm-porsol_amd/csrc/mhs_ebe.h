@@ -32,6 +32,8 @@ struct EbeLayout {
     int ncolours = 0, nseg = 0, nblk = 0, nslots = 0;
     long long nval = 0, ndof = 0;
     std::vector<int> colour_blk;             // [ncolours + 1] first block of each colour
+    std::vector<int> colour_split;           // [ncolours] first block of each colour whose cells
+                                             // are not hexahedra (6 faces); hexahedra come first
     std::vector<int> blk_seg, blk_slot;      // [nblk] segment, partial-sum slot
     std::vector<int> seg_n, seg_inst, seg_count, seg_len, seg_blk0;
     std::vector<long long> seg_val, seg_dof; // value and dof bases of each segment
@@ -46,14 +48,21 @@ struct EbeLayout {
     std::vector<uint32_t> dof;               // [ndof] face row | kEbeFirst | kEbeLast
 };
 
-// Greedy colouring of the cells in index order: each cell takes the smallest
-// colour that none of its already coloured face neighbours has (the
-// checkerboard on box grids).  Returns the number of colours.
-int greedy_colours(int nc, const int *facepos, const int *nb, std::vector<int> &colour);
+// Cell colouring (no two cells sharing a face have one colour); returns the
+// number of colours.  Without logical indices: greedy in index order, each
+// cell taking the smallest colour none of its already coloured neighbours has
+// (the checkerboard on box grids).  With logical indices ijk[3 * nc]: the
+// parity of i + j + k first, and in index order a cell that shares its colour
+// with a lower-numbered neighbour (across a fault) takes the smallest colour
+// from 2 up that no neighbour has.  Most cells of a faulted grid then keep
+// their checkerboard colour, which is also how the face rows are classed.
+int greedy_colours(int nc, const int *facepos, const int *nb, std::vector<int> &colour,
+                   const int *ijk = nullptr);
 
 // Builds the layout for a structure; returns an empty string, or the reason
 // why the element-by-element operator does not apply (periodic faces, whose
-// constraint rows are not a sum of cell matrices; too many colours).
-std::string build_ebe(EbeLayout &E, const Topology &T);
+// constraint rows are not a sum of cell matrices; too many colours).  ijk:
+// logical cell indices or null (see greedy_colours).
+std::string build_ebe(EbeLayout &E, const Topology &T, const int *ijk = nullptr);
 
 }  // namespace mhs

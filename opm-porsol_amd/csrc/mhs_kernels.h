@@ -82,6 +82,7 @@ __host__ __device__ constexpr long long ebe_slot(int local, int q, int ncomp) {
 struct DevEbe {
     int ncolours = 0, nblk = 0;
     int colour_blk[kEbeColours + 1] = {};
+    int colour_split[kEbeColours] = {};     // first block of each colour that is not hexahedra
     // per block: (face count, instance, cells in the block, partial slot) and
     // (value base, face-row base) of its first tile
     const int4 *blk_info = nullptr;

@@ -818,14 +818,14 @@ __device__ __forceinline__ void ebe_cell(const DevEbe &E, int nrt, longlong2 bb,
 }
 
 template <int MAXN, int NRHS, int MODE, bool REL>
-__global__ __launch_bounds__(256) void k_ebe(DevEbe E, int colour, Gate gt, DevScal S,
+__global__ __launch_bounds__(256) void k_ebe(DevEbe E, int blk0, Gate gt, DevScal S,
                                              const float *__restrict__ x, float *__restrict__ out,
                                              const float *__restrict__ rin, RelArgs L,
                                              double *__restrict__ r64) {
     // XCD-contiguous block ranges: every colour pass gives an XCD the same
     // cells' neighbourhood, so the second colour finds the rows the first
     // one wrote in that XCD's L2
-    const int b = E.colour_blk[colour] + xcd_block(gridDim.x);
+    const int b = blk0 + xcd_block(gridDim.x);
     // The block's two records first; then the instance's flags and the
     // cells' face rows in one round trip.  The rows are loaded before the
     // gate is known, so a block of a finished instance reads them in vain.
@@ -2673,15 +2673,22 @@ static hipError_t with_maxn(int maxn, Fn fn) {
 
 static inline unsigned cell_blocks(int nc) { return (unsigned)((nc + 63) / 64); }
 
-// The element-by-element operator: one launch per colour.
+// The element-by-element operator: per colour, the hexahedra with the 6-face
+// kernel (few registers, full occupancy), then the other cells with the
+// kernel for the largest face count.
 template <int NR, int MODE, bool REL>
 static void ebe_apply(hipStream_t st, int maxn, const DevEbe &E, const Gate &gt, const DevScal &S, const float *x,
                       float *out, const float *rin, const RelArgs &L, double *r64) {
     with_maxn(maxn, [&](auto mn) {
         constexpr int MN = decltype(mn)::value;
         for (int c = 0; c < E.ncolours; ++c) {
-            const int nb = E.colour_blk[c + 1] - E.colour_blk[c];
-            if (nb > 0) k_ebe<MN, NR, MODE, REL><<<nb, 256, 0, st>>>(E, c, gt, S, x, out, rin, L, r64);
+            const int b0 = E.colour_blk[c], b1 = E.colour_split[c], b2 = E.colour_blk[c + 1];
+            if (MN == 6) {
+                if (b2 > b0) k_ebe<6, NR, MODE, REL><<<b2 - b0, 256, 0, st>>>(E, b0, gt, S, x, out, rin, L, r64);
+                continue;
+            }
+            if (b1 > b0) k_ebe<6, NR, MODE, REL><<<b1 - b0, 256, 0, st>>>(E, b0, gt, S, x, out, rin, L, r64);
+            if (b2 > b1) k_ebe<MN, NR, MODE, REL><<<b2 - b1, 256, 0, st>>>(E, b1, gt, S, x, out, rin, L, r64);
         }
     });
 }
