@@ -572,6 +572,9 @@ int mhs_init_system_structure(mhs_solver *s, const int *hf_bc_type, const int *h
     d.nc = T.nc; d.nhf = T.nhf; d.nfp = T.nfp; d.ninst = T.ninst; d.nslices = T.nslices;
     d.nblk = T.nfp / kRowBlock;
     d.max_ncf = T.max_ncf;
+    d.uniform_ncf = 1;
+    for (int c = 0; c < T.nc && d.uniform_ncf; ++c)
+        if (T.facepos[c + 1] - T.facepos[c] != T.max_ncf) d.uniform_ncf = 0;
     d.max_w = 0;
     for (int sl = 0; sl < T.nslices; ++sl)
         d.max_w = std::max(d.max_w, (int)((T.slice_ptr[sl + 1] - T.slice_ptr[sl]) / kSlice));
@@ -1223,7 +1226,7 @@ int linear_solve(mhs_solver *s, const mhs_solve_options &opt, int nrhs, bool mix
                                               s->z64, s->p64, s->R, s->S, nullptr, s->x64));
         else
             HIPCHECK(launch_amg_start<float>(s->stream, s->dT, s->dA, nrhs, s->val32, s->dinv32, s->r32,
-                                             s->z32, s->p32, s->R, s->S, nullptr, s->x32));
+                                             s->z32, s->p32, s->R, s->S, nullptr, s->x32, cur_ebe(s)));
     }
     int rc = read_state(s, nsys);
     if (rc) return rc;
@@ -1377,8 +1380,13 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
     HIPCHECK(launch_assemble(s->stream, s->dT, T.max_ncf, nrhs, binv, gflux, d_tm, d_rm, d_src,
                              s->ftype, d_bv, s->C, cur_ebe(s)));
     const int dguess = opt.linsolver_type != 0;  // AMG types start from the Dirichlet guess (:1521-1532)
+    // The FP32 face matrix is not written when the FP32 iteration applies the
+    // operator cell by cell for certain (no trial against the assembled matrix
+    // due, one smoothing sweep: more sweeps run on the assembled matrix).
+    const bool ebe_settled = s->last_ebe && (opt.element_operator == 2 || !opt.autotune || s->ebe_pref[nrhs] == 1);
+    float *fill32 = (mixed && !(ebe_settled && opt.smooth_steps <= 1)) ? s->val32 : nullptr;
     HIPCHECK(launch_fill(s->stream, s->dT, nrhs, s->C, s->ftype, d_bv, s->inst_reg, dguess, s->val64,
-                         mixed ? s->val32 : nullptr, s->b, s->dinv64, s->dinv32, s->x64));
+                         fill32, s->b, s->dinv64, s->dinv32, s->x64));
     HIPCHECK(hipStreamSynchronize(s->stream));
     auto t1 = std::chrono::steady_clock::now();
 
@@ -1409,7 +1417,7 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
             // initial guess are written again (the failed attempt changed x64)
             s->rho.clear();
             HIPCHECK(launch_fill(s->stream, s->dT, nrhs, s->C, s->ftype, d_bv, s->inst_reg, dguess, s->val64,
-                                 mixed ? s->val32 : nullptr, s->b, s->dinv64, s->dinv32, s->x64));
+                                 fill32, s->b, s->dinv64, s->dinv32, s->x64));
         }
         if (amg) {
             s->dA.alpha = opt.prolongate_factor;

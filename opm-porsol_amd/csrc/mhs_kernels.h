@@ -16,6 +16,7 @@ struct DevTopo {
     int nc, nhf, nfp, ninst, nslices, nblk;
     int max_w;                      // widest SELL slice (stored entries per row)
     int max_ncf;                    // most faces of a cell (stride of the slot map)
+    int uniform_ncf;                // 1: every cell has max_ncf faces (per-cell loops of fixed length)
     const int *facepos, *hf_dof, *hf_cell, *hf_col, *hf_ppart, *hf_twin;
     const uint8_t *hf_redirect;
     const int *row_hf, *dof_hf, *ptn;
@@ -254,7 +255,7 @@ hipError_t launch_pcg_iteration_amg(hipStream_t st, const DevTopo &T, const DevA
 template <typename VT>
 hipError_t launch_amg_start(hipStream_t st, const DevTopo &T, const DevAmg &A, int nrhs, const VT *val0,
                             const VT *dinv0, const VT *r, VT *z, VT *p, DevRed R, DevScal S,
-                            double *x64, VT *x);
+                            double *x64, VT *x, const DevEbe *E = nullptr);
 
 // Recovery and post-processing.
 // out[k*ngroups + g] = sum of v[k*nhf + ghf[e]] over e in [gptr[g], gptr[g+1]).

@@ -378,7 +378,7 @@ __global__ __launch_bounds__(64) void k_static_ip(DevTopo T, DevGeom G,
 // folds in the column rules of addCellContrib (:1880-1896): Dirichlet columns
 // and periodic columns redirected to the smaller partner dof move S(r,c)*value
 // to the right-hand side.  Dirichlet rows carry S(r,r)*p (:1832-1835).
-template <int MAXN, int NRHS>
+template <int MAXN, int NRHS, bool UNIF>
 __global__ __launch_bounds__(64) void k_assemble(DevTopo T, const double *__restrict__ binv,
                                                  const double *__restrict__ gfs,
                                                  const double *__restrict__ totmob,
@@ -390,7 +390,7 @@ __global__ __launch_bounds__(64) void k_assemble(DevTopo T, const double *__rest
     constexpr int NP = MAXN * (MAXN + 1) / 2;
     const int c = blockIdx.x * 64 + threadIdx.x;
     if (c >= T.nc) return;
-    const int f0 = T.facepos[c], n = T.facepos[c + 1] - f0;
+    const int f0 = T.facepos[c], n = UNIF ? MAXN : T.facepos[c + 1] - f0;   // UNIF: every cell has MAXN faces
     const double lam = totmob[c], rl = rhomob[c];
     double S[NP];
 #pragma unroll
@@ -577,7 +577,7 @@ __global__ __launch_bounds__(TB) void k_fill(DevTopo T, DevCellSys C,
         if (ROW_LDS) val64[base + 64LL * e] = v;
         if (val32) val32[base + 64LL * e] = (float)v;
     }
-    if (val32) dinv32[f] = (float)di;
+    if (dinv32) dinv32[f] = (float)di;
 }
 
 // ---------------------------------------------------------------------------
@@ -2073,7 +2073,7 @@ __global__ __launch_bounds__(256) void k_update_pz(DevTopo T, const VT *__restri
 
 // ---------------------------------------------------------------------------
 // Recovery, per cell: p = (g + F.pi)/L;  v = totmob B^-1 (p - pi) + rhomob gflux.
-template <int MAXN, int NRHS>
+template <int MAXN, int NRHS, bool UNIF>
 __global__ __launch_bounds__(64) void k_recover(DevTopo T, const double *__restrict__ binv,
                                                 const double *__restrict__ gfs,
                                                 const double *__restrict__ totmob,
@@ -2084,7 +2084,7 @@ __global__ __launch_bounds__(64) void k_recover(DevTopo T, const double *__restr
     constexpr int NP = MAXN * (MAXN + 1) / 2;
     const int c = blockIdx.x * 64 + threadIdx.x;
     if (c >= T.nc) return;
-    const int f0 = T.facepos[c], n = T.facepos[c + 1] - f0;
+    const int f0 = T.facepos[c], n = UNIF ? MAXN : T.facepos[c + 1] - f0;
     const double lam = totmob[c], rl = rhomob[c];
     double B[NP];
 #pragma unroll
@@ -2817,8 +2817,13 @@ hipError_t launch_assemble(hipStream_t st, const DevTopo &T, int maxn, int nrhs,
         constexpr int MN = decltype(mn)::value;
         e = with_nrhs(nrhs, [&](auto nr) {
             constexpr int NR = decltype(nr)::value;
-            k_assemble<MN, NR><<<cell_blocks(T.nc), 64, 0, st>>>(T, binv_pk, gflux, totmob, rhomob,
-                                                                 src, ftype, bcval, C, Ev);
+            // every cell with MN faces: loops of fixed length (immediate offsets)
+            if (T.uniform_ncf && T.max_ncf == MN)
+                k_assemble<MN, NR, true><<<cell_blocks(T.nc), 64, 0, st>>>(T, binv_pk, gflux, totmob, rhomob,
+                                                                           src, ftype, bcval, C, Ev);
+            else
+                k_assemble<MN, NR, false><<<cell_blocks(T.nc), 64, 0, st>>>(T, binv_pk, gflux, totmob, rhomob,
+                                                                            src, ftype, bcval, C, Ev);
         });
     });
     return e != hipSuccess ? e : e2;
@@ -3341,20 +3346,20 @@ template hipError_t launch_pcg_iteration_amg<float, float>(hipStream_t, const De
 template <typename VT>
 hipError_t launch_amg_start(hipStream_t st, const DevTopo &T, const DevAmg &A, int nrhs, const VT *val0,
                             const VT *dinv0, const VT *r, VT *z, VT *p, DevRed R, DevScal S,
-                            double *x64, VT *x) {
+                            double *x64, VT *x, const DevEbe *E) {
     return with_nrhs(nrhs, [&](auto nr) {
         constexpr int NR = decltype(nr)::value;
-        vcycle_impl<VT, NR>(st, Gate{}, T, A, val0, dinv0, r, z, R, false);
+        vcycle_impl<VT, NR>(st, Gate{}, T, A, val0, dinv0, r, z, R, false, E);
         k_finish<NR, FinRZ<NR>><<<T.ninst, 256, 0, st>>>(R, S, FinArgs{0.0, 1});
         k_update_pz<VT, NR><<<T.nblk, 256, 0, st>>>(T, z, p, x, nullptr, S);
     });
 }
 template hipError_t launch_amg_start<double>(hipStream_t, const DevTopo &, const DevAmg &, int, const double *,
                                              const double *, const double *, double *, double *, DevRed,
-                                             DevScal, double *, double *);
+                                             DevScal, double *, double *, const DevEbe *);
 template hipError_t launch_amg_start<float>(hipStream_t, const DevTopo &, const DevAmg &, int, const float *,
                                             const float *, const float *, float *, float *, DevRed, DevScal,
-                                            double *, float *);
+                                            double *, float *, const DevEbe *);
 
 hipError_t launch_recover(hipStream_t st, const DevTopo &T, int maxn, int nrhs,
                           const double *binv_pk, const double *gflux, const double *totmob,
@@ -3366,8 +3371,12 @@ hipError_t launch_recover(hipStream_t st, const DevTopo &T, int maxn, int nrhs,
         constexpr int MN = decltype(mn)::value;
         e = with_nrhs(nrhs, [&](auto nr) {
             constexpr int NR = decltype(nr)::value;
-            k_recover<MN, NR><<<cell_blocks(T.nc), 64, 0, st>>>(T, binv_pk, gflux, totmob, rhomob, C,
-                                                                x64, press, outflux);
+            if (T.uniform_ncf && T.max_ncf == MN)
+                k_recover<MN, NR, true><<<cell_blocks(T.nc), 64, 0, st>>>(T, binv_pk, gflux, totmob, rhomob, C,
+                                                                          x64, press, outflux);
+            else
+                k_recover<MN, NR, false><<<cell_blocks(T.nc), 64, 0, st>>>(T, binv_pk, gflux, totmob, rhomob, C,
+                                                                           x64, press, outflux);
         });
     });
     return e != hipSuccess ? e : e2;
