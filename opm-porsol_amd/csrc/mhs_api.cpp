@@ -586,6 +586,7 @@ int mhs_init_system_structure(mhs_solver *s, const int *hf_bc_type, const int *h
     UP(facepos, facepos); UP(hf_dof, hf_dof); UP(hf_cell, hf_cell); UP(hf_col, hf_col);
     UP(hf_ppart, hf_ppart); UP(hf_twin, hf_twin); UP(hf_redirect, hf_redirect);
     UP(row_hf, row_hf); UP(dof_hf, dof_hf); UP(ptn, ptn); UP(dslot, dslot); UP(pslot, pslot);
+    UP(row_cell, row_cell); UP(row_loc, row_loc);
     UP(slice_ptr, slice_ptr); UP(col, col); UP(slotmap, slotmap); UP(sq_pos, sq_pos);
     UP(block_seg, block_seg); UP(seg_blk, seg_blk); UP(inst_rowbase, inst_rowbase);
     UP(inst_regrow, inst_regrow);
@@ -773,7 +774,7 @@ int ensure_solve_buffers(mhs_solver *s, int nrhs, bool mixed) {
     // component-major: packed entry q of cell c at q * nc + c (q below the
     // largest cell's packed size), so lanes on consecutive cells coalesce
     s->C.schur = dalloc<double>(s, L, (size_t)T.max_ncf * (T.max_ncf + 1) / 2 * T.nc, &e);
-    if (ok()) s->C.rr = dalloc<double>(s, L, (size_t)nrhs * T.nhf, &e);
+    if (ok()) s->C.rr = dalloc<double>(s, L, (size_t)nrhs * T.max_ncf * T.nc, &e);
     if (ok()) s->C.F = dalloc<double>(s, L, (size_t)T.nhf, &e);
     if (ok()) s->C.L = dalloc<double>(s, L, (size_t)T.nc, &e);
     if (ok()) s->C.g = dalloc<double>(s, L, (size_t)nrhs * T.nc, &e);

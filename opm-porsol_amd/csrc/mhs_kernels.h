@@ -20,6 +20,8 @@ struct DevTopo {
     const int *facepos, *hf_dof, *hf_cell, *hf_col, *hf_ppart, *hf_twin;
     const uint8_t *hf_redirect;
     const int *row_hf, *dof_hf, *ptn;
+    const int *row_cell;
+    const uint8_t *row_loc;
     const uint8_t *dslot, *pslot;
     const long long *slice_ptr;
     const int *col;
@@ -39,7 +41,7 @@ struct DevTopo {
 // Per-solve cell data produced by the assembly kernel.
 struct DevCellSys {
     double *schur;     // packed local Schur complements (FP64), component-major: entry q of cell c at q*nc + c
-    double *rr;        // [nrhs][nhf] row right-hand-side contributions
+    double *rr;        // row right-hand-side contributions, component-major: (rhs k, local face r, cell c) at (k*max_ncf + r)*nc + c
     double *F;         // [nhf]
     double *L;         // [nc]
     double *g;         // [nrhs][nc]

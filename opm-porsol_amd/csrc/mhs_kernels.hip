@@ -470,7 +470,7 @@ __global__ __launch_bounds__(64) void k_assemble(DevTopo T, const double *__rest
                         rr -= ((i <= j) ? S[pidx(i, j, MAXN)] : S[pidx(j, i, MAXN)]) * val[j];
                 }
             }
-            C.rr[(size_t)k * T.nhf + f0 + i] = rr;
+            C.rr[((size_t)k * T.max_ncf + i) * T.nc + c] = rr;
         }
     }
 }
@@ -518,15 +518,17 @@ __global__ __launch_bounds__(TB) void k_fill(DevTopo T, DevCellSys C,
         diag = a;
 #pragma unroll
         for (int k = 0; k < NRHS; ++k) {
-            bk[k] = C.rr[(size_t)k * T.nhf + h0];
+            bk[k] = C.rr[((size_t)k * T.max_ncf + r) * T.nc + c];
             xk[k] = dguess ? bk[k] / a : 0.0;
         }
     } else {
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
-            const int h = T.row_hf[2 * (size_t)f + t];
-            if (h < 0) continue;
-            const int c = T.hf_cell[h], f0 = T.facepos[c], n = T.facepos[c + 1] - f0, r = h - f0;
+            // the contribution's cell and local face stored per row (coalesced),
+            // not gathered through the half-face
+            const int c = T.row_cell[2 * (size_t)f + t];
+            if (c < 0) continue;
+            const int f0 = T.facepos[c], n = T.facepos[c + 1] - f0, r = T.row_loc[2 * (size_t)f + t];
             const uint8_t *sm = T.slotmap + (size_t)r * T.max_ncf * T.nc + c;   // entry j at sm[j * nc]
             // the cell's row r of S_c, 8 faces at a time: all loads of a batch
             // are issued before the LDS sums
@@ -546,7 +548,7 @@ __global__ __launch_bounds__(TB) void k_fill(DevTopo T, DevCellSys C,
                     if (j0 + u < n) row[rs * slot[u]] += v[u];
             }
 #pragma unroll
-            for (int k = 0; k < NRHS; ++k) bk[k] += C.rr[(size_t)k * T.nhf + h];
+            for (int k = 0; k < NRHS; ++k) bk[k] += C.rr[((size_t)k * T.max_ncf + r) * T.nc + c];
         }
         const int fp = T.ptn[f];
         if (fp >= 0) {  // periodic constraint a(x_f - x_partner) = a_f pd_f - a_p pd_p (:1838-1869)

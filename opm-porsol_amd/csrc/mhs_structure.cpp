@@ -200,6 +200,15 @@ std::string build_topology(Topology &T, int ninst, const int *inst_cellpos, int 
         int *s = &T.row_hf[2 * (size_t)f];
         if (s[0] >= 0 && s[1] >= 0 && s[1] < s[0]) std::swap(s[0], s[1]);
     }
+    T.row_cell.assign(2 * (size_t)nfp, -1);
+    T.row_loc.assign(2 * (size_t)nfp, 0);
+    for (size_t q = 0; q < T.row_hf.size(); ++q) {
+        const int h = T.row_hf[q];
+        if (h < 0) continue;
+        const int c = T.hf_cell[h];
+        T.row_cell[q] = c;
+        T.row_loc[q] = (uint8_t)(h - facepos[c]);
+    }
 
     // --- SELL-64 pattern --------------------------------------------------
     T.nslices = nfp / kSlice;

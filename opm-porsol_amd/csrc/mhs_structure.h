@@ -36,6 +36,8 @@ struct Topology {
     std::vector<uint8_t> hf_periodic;     // periodic at structure time
     // rows
     std::vector<int> row_hf;              // [2*nfp] contributing half-faces
+    std::vector<int> row_cell;            // [2*nfp] their cells (-1: none), read per row without a gather
+    std::vector<uint8_t> row_loc;         // [2*nfp] their local face index in the cell
     std::vector<int> dof_hf;              // first half-face of the face, -1 for padding
     std::vector<int> ptn;                 // periodic partner row, -1
     std::vector<uint8_t> row_len, dslot, pslot;
