@@ -1235,24 +1235,29 @@ int linear_solve(mhs_solver *s, const mhs_solve_options &opt, int nrhs, bool mix
         int mx = 0;
         for (size_t i = 0; i < nsys; ++i) mx = std::max(mx, s->h_iters[i]);
         if (mx >= maxit) break;
-        // after the first chunk, launch about as many iterations as the slowest
-        // system still needs (rounded down to a power of two: no iterations
-        // after convergence, a few more state reads at the end)
+        // after the first chunk, launch as many iterations as the slowest
+        // system still needs (no iterations after convergence), as graphs of
+        // power-of-two lengths back to back (one cached graph per length) with
+        // one state read after them
         int n = std::min(chunk, maxit - mx);
         const int need = remaining_iterations(s, nsys, opt.tol, mixed);
-        if (need > 0) {
-            int p2 = 1;
-            while (2 * p2 <= need) p2 *= 2;
-            n = std::min(n, p2);
+        if (need > 0) n = std::min(n, need);
+        while (n > 0) {
+            int len = n;
+            if (n != chunk) {
+                len = 1;
+                while (2 * len <= n) len *= 2;
+            }
+            const bool graph = opt.use_graph;
+            if (!mixed)
+                rc = run_chunk<double, double>(s, nrhs, len, graph, s->val64, s->dinv64, s->x64, s->r64, s->p64,
+                                               s->q64, s->z64, amg, MHS_FP64, tol2);
+            else
+                rc = run_chunk<float, float>(s, nrhs, len, graph, s->val32, s->dinv32, s->x32, s->r32, s->p32,
+                                             s->q32, s->z32, amg, MHS_MIXED, tol2);
+            if (rc) return rc;
+            n -= len;
         }
-        const bool graph = opt.use_graph && (n == chunk || (n & (n - 1)) == 0);
-        if (!mixed)
-            rc = run_chunk<double, double>(s, nrhs, n, graph, s->val64, s->dinv64, s->x64, s->r64, s->p64,
-                                           s->q64, s->z64, amg, MHS_FP64, tol2);
-        else
-            rc = run_chunk<float, float>(s, nrhs, n, graph, s->val32, s->dinv32, s->x32, s->r32, s->p32,
-                                         s->q32, s->z32, amg, MHS_MIXED, tol2);
-        if (rc) return rc;
         rc = read_state(s, nsys);
         if (rc) return rc;
         if (opt.verbosity >= 2)
