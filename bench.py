@@ -64,9 +64,9 @@ def _vcycle_kernels(nf, v, lev_rows, lev_nnz, p_nnz, t_nnz, ap_nnz, sa_level):
         if l == sa_level:
             # P' rows (pointers, columns, values), res gathered, b_c written
             out.append((f"restriction P' level {l}", 8 * p_nnz + nc * (4 + v) + n * v))
-            # up-sweep across the smoothed transfer: P and T = A P rows, res, x,
-            # D^-1 read, x written, x_c gathered
-            out.append((f"up-sweep (P, T) level {l}", 8 * (p_nnz + t_nnz) + n * (8 + 4 * v) + nc * v))
+            # up-sweep across the smoothed transfer: the rows of M = P - w D^-1 T
+            # (T = A P's pattern), b, res, D^-1 read, x written, x_c gathered
+            out.append((f"up-sweep (M = P - w D^-1 A P) level {l}", (4 + v) * t_nnz + n * (4 + 4 * v) + nc * v))
         else:
             out.append((f"restriction level {l}", n * (4 + v) + nc * (4 + v)))
             # up-sweep: matrix, aggregate map, b, D^-1 read, x written, x_c gathered

@@ -24,8 +24,9 @@ void children(AmgLevel &C, int nf) {
 // Sparsity of P'AP and the gather lists, coarse row by coarse row: the
 // entries (agg(j), pos) of the coarse row's children are sorted locally.
 // row_entries(i, emit) must call emit(j, pos) for every stored entry of
-// finer row i.  Requires children() to have run.
-template <class RowEntries>
+// finer row i (COARSE: j is already the coarse column, the entries being
+// those of A P).  Requires children() to have run.
+template <bool COARSE = false, class RowEntries>
 void galerkin_pattern(AmgLevel &C, RowEntries row_entries) {
     C.row_ptr.assign(C.n + 1, 0);
     C.col.clear();
@@ -37,7 +38,7 @@ void galerkin_pattern(AmgLevel &C, RowEntries row_entries) {
         buf.clear();
         for (int k = C.child_ptr[I]; k < C.child_ptr[I + 1]; ++k)
             row_entries(C.child[k], [&](int j, int pos) {
-                const int J = C.agg[j];
+                const int J = COARSE ? j : C.agg[j];
                 if (J >= 0) buf.emplace_back(J, pos);
             });
         std::sort(buf.begin(), buf.end());
@@ -180,6 +181,7 @@ void sa_transfer(AmgLevel &F, AmgLevel &C) {
         if (!tb.empty()) F.t_pair_ptr.push_back(int(F.t_pa.size()));
         F.t_ptr[i + 1] = int(F.t_col.size());
     }
+
     // A_C = P' T
     C.sa_in = true;
     C.row_ptr.assign(C.n + 1, 0);
@@ -303,14 +305,18 @@ std::string build_amg(AmgHierarchy &H, const Topology &T, const std::vector<int>
             L1.agg[f] = L1.inst_ptr[t] + I + bd[0] * (J + bd[1] * K);
         }
         children(L1, T.nfp);
-        galerkin_pattern(L1, [&](int f, auto emit) {
-            const long long base = T.slice_ptr[f / kSlice] + f % kSlice;
-            for (int e = 0; e < T.row_len[f]; ++e) {
-                const long long pos = base + (long long)e * kSlice;
-                emit(T.col[pos], int(pos));
+        // A_1 = P' (A P): each entry sums the A P slots of its row's faces
+        // (about 4 per face row) instead of the face matrix's entries (11)
+        build_ap(H.ap, T, L1.agg);
+        const ApPattern &ap = H.ap;
+        galerkin_pattern<true>(L1, [&](int f, auto emit) {
+            const long long base = ap.slice_ptr[f / kSlice] + f % kSlice;
+            const int w = int((ap.slice_ptr[f / kSlice + 1] - ap.slice_ptr[f / kSlice]) / kSlice);
+            for (int e = 0; e < w; ++e) {
+                const long long q = base + (long long)e * kSlice;
+                if (ap.src_ptr[q + 1] > ap.src_ptr[q]) emit(ap.col[q], int(q));   // padding slots sum nothing
             }
         });
-        build_ap(H.ap, T, L1.agg);
         H.lev.push_back(std::move(L1));
     }
     // ---- coarser levels: group the blocks again until every instance is small
@@ -349,6 +355,18 @@ std::string build_amg(AmgHierarchy &H, const Topology &T, const std::vector<int>
         if (smoothed && !sa_done && 8LL * Fr.n <= (long long)T.nf_total) {
             sa_done = true;
             sa_transfer(Fr, C);
+            // the up-sweep stores M = P - w D^-1 T on T's pattern, which holds
+            // P's (A holds its diagonal; both rows sorted)
+            Fr.t_p.assign(Fr.t_col.size(), -1);
+            for (int i = 0; i < Fr.n; ++i) {
+                int t = Fr.t_ptr[i];
+                for (int q = Fr.p_ptr[i]; q < Fr.p_ptr[i + 1]; ++q) {
+                    while (t < Fr.t_ptr[i + 1] && Fr.t_col[t] < Fr.p_col[q]) ++t;
+                    if (t == Fr.t_ptr[i + 1] || Fr.t_col[t] != Fr.p_col[q])
+                        return "smoothed transfer: a row of P outside the pattern of A P";
+                    Fr.t_p[t] = q;
+                }
+            }
         } else {
             galerkin_pattern(C, [&](int r, auto emit) {
                 for (int k = Fr.row_ptr[r]; k < Fr.row_ptr[r + 1]; ++k) emit(Fr.col[k], k);

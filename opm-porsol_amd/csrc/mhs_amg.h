@@ -47,6 +47,7 @@ struct AmgLevel {
     std::vector<int> p_tpos;                   // per P entry: its position in the P' rows (inverse of pt_map)
     std::vector<int> t_ptr, t_col;             // T = A P
     std::vector<int> t_pair_ptr, t_pa, t_pb;   // per T entry: (value position, P position)
+    std::vector<int> t_p;                      // per T entry: the P entry of the same row and column, or -1
 };
 
 // A P with the level-1 aggregates: the fine-row by aggregate matrix of the

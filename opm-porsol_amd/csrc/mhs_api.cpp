@@ -462,6 +462,9 @@ int build_amg_device(mhs_solver *s, const Strength &strength) {
                 if (he == hipSuccess) D.p_val32 = dalloc<float>(s, L, (size_t)D.p_nnz, &he);
                 if (he == hipSuccess) D.t_val64 = dalloc<double>(s, L, (size_t)D.t_nnz, &he);
                 if (he == hipSuccess) D.t_val32 = dalloc<float>(s, L, (size_t)D.t_nnz, &he);
+                if (he == hipSuccess) D.t_p = dupload(s, L, H.t_p, &he);
+                if (he == hipSuccess) D.m_val64 = dalloc<double>(s, L, (size_t)D.t_nnz, &he);
+                if (he == hipSuccess) D.m_val32 = dalloc<float>(s, L, (size_t)D.t_nnz, &he);
             }
             if (he == hipSuccess) D.val64 = dalloc<double>(s, L, (size_t)D.nnz, &he);
             if (he == hipSuccess) D.val32 = dalloc<float>(s, L, (size_t)D.nnz, &he);
@@ -1175,9 +1178,6 @@ struct LinearResult {
     double reduction = 0.0;
 };
 
-// PCG on the assembled face system from the right-hand side in s->b and the
-// initial iterate in s->x64, to opt.tol relative FP64 residual (Jacobi or
-// multigrid preconditioner, FP64 or FP32 with FP64 residual replacement).
 // PCG iterations the slowest active system still needs, from its mean
 // residual reduction per iteration so far (the mixed iteration needs one more
 // for the FP64 replacement that confirms convergence); 0 while unknown.
@@ -1200,6 +1200,11 @@ int remaining_iterations(const mhs_solver *s, size_t nsys, double tol, bool mixe
     return need;
 }
 
+// PCG on the assembled face system from the right-hand side in s->b, the
+// initial iterate in s->x64 and its residual b - A x64 in s->r64 (the fill and
+// the adjoint right-hand side write all three), to opt.tol relative FP64
+// residual (Jacobi or multigrid preconditioner, FP64 or FP32 with FP64
+// residual replacement).
 int linear_solve(mhs_solver *s, const mhs_solve_options &opt, int nrhs, bool mixed, bool amg,
                  LinearResult &out) {
     const Topology &T = s->T;
@@ -1215,11 +1220,11 @@ int linear_solve(mhs_solver *s, const mhs_solve_options &opt, int nrhs, bool mix
     HIPCHECK(hipMemsetAsync(s->S.upd, 0, nsys * sizeof(int), s->stream));
     if (!mixed) {
         HIPCHECK(launch_pcg_start64(s->stream, s->dT, nrhs, s->val64, s->dinv64, s->b, s->x64, s->r64,
-                                    s->p64, tol2, s->R, s->S));
+                                    s->p64, tol2, s->R, s->S, true));
     } else {
         HIPCHECK(hipMemsetAsync(s->x32, 0, sizeof(float) * T.nfp * nrhs, s->stream));
         HIPCHECK(launch_reliable(s->stream, s->dT, nrhs, s->val64, s->b, s->x64, s->dinv32, s->r32,
-                                 s->p32, 1, tol2, s->R, s->S));
+                                 s->p32, 1, tol2, s->R, s->S, s->r64));
     }
     if (amg) {
         if (!mixed)
@@ -1274,9 +1279,14 @@ int linear_solve(mhs_solver *s, const mhs_solve_options &opt, int nrhs, bool mix
     out.iterations = 0;
     for (size_t i = 0; i < nsys; ++i) out.iterations = std::max(out.iterations, s->h_iters[i]);
     if (mixed) HIPCHECK(launch_merge_all(s->stream, s->dT, nrhs, s->x64, s->x32));
-    HIPCHECK(launch_resnorm(s->stream, s->dT, nrhs, s->val64, s->b, s->x64, s->R, s->S));
-    rc = read_state(s, nsys);
-    if (rc) return rc;
+    // A converged FP32 iteration ended on an FP64 residual replacement of the
+    // final x64 (a system stops only there), whose |b - A x64| is already in
+    // the state; otherwise the true residual is formed once more.
+    if (!(mixed && out.converged)) {
+        HIPCHECK(launch_resnorm(s->stream, s->dT, nrhs, s->val64, s->b, s->x64, s->R, s->S));
+        rc = read_state(s, nsys);
+        if (rc) return rc;
+    }
     out.replacements = s->h_status[2];
     double red = 0.0;
     for (size_t i = 0; i < nsys; ++i) {
@@ -1392,7 +1402,7 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
     const bool ebe_settled = s->last_ebe && (opt.element_operator == 2 || !opt.autotune || s->ebe_pref[nrhs] == 1);
     float *fill32 = (mixed && !(ebe_settled && opt.smooth_steps <= 1)) ? s->val32 : nullptr;
     HIPCHECK(launch_fill(s->stream, s->dT, nrhs, s->C, s->ftype, d_bv, s->inst_reg, dguess, s->val64,
-                         fill32, s->b, s->dinv64, s->dinv32, s->x64));
+                         fill32, s->b, s->dinv64, s->dinv32, s->x64, s->r64));
     HIPCHECK(hipStreamSynchronize(s->stream));
     auto t1 = std::chrono::steady_clock::now();
 
@@ -1423,7 +1433,7 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
             // initial guess are written again (the failed attempt changed x64)
             s->rho.clear();
             HIPCHECK(launch_fill(s->stream, s->dT, nrhs, s->C, s->ftype, d_bv, s->inst_reg, dguess, s->val64,
-                                 fill32, s->b, s->dinv64, s->dinv32, s->x64));
+                                 fill32, s->b, s->dinv64, s->dinv32, s->x64, s->r64));
         }
         if (amg) {
             s->dA.alpha = opt.prolongate_factor;
@@ -1450,6 +1460,7 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
             // the previous solve is still running: it ended with a synchronise;
             // the set-up kernels above do not read it)
             HIPCHECK(hipMemcpy(const_cast<DevAmg *>(s->dA.dev), &s->dA, sizeof(DevAmg), hipMemcpyHostToDevice));
+            HIPCHECK(launch_amg_combine(s->stream, s->dA, mixed));
         }
 
         // --- launch shapes -------------------------------------------------------
@@ -1603,7 +1614,7 @@ int adjoint_impl(mhs_solver *s, const double *dJ_dv, const mhs_solve_options *op
     HIPCHECK(hipMemcpyAsync(s->adj_pi, s->x64, sizeof(double) * nv, d2d, s->stream));
     HIPCHECK(hipMemcpyAsync(s->adj_b, s->b, sizeof(double) * nv, d2d, s->stream));
     HIPCHECK(launch_adjoint_rhs(s->stream, s->dT, T.max_ncf, nrhs, s->binv, s->in_totmob, s->C, s->ftype,
-                                s->adj_w, s->adj_y, s->b, s->x64));
+                                s->adj_w, s->adj_y, s->b, s->x64, s->r64));
     LinearResult lr;
     int rc = linear_solve(s, opt, nrhs, mixed, amg, lr);
     if (rc == MHS_OK && lr.converged)

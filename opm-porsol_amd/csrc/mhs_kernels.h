@@ -124,6 +124,11 @@ struct DevAmgLevel {
     const int *t_ptr, *t_col, *t_pair_ptr, *t_pa, *t_pb;
     double *p_val64, *t_val64;
     float *p_val32, *t_val32;
+    // M = P - w D^-1 T on T's pattern: the up-sweep's prolongation and
+    // post-smoothing in one product (refilled once the weight w is known)
+    const int *t_p;                 // per T entry: the P entry at its place, or -1
+    double *m_val64;
+    float *m_val32;
 };
 struct DevAmg {
     int nlev;                       // levels including level 0
@@ -185,7 +190,7 @@ hipError_t launch_ebe_spmv(hipStream_t st, int maxn, int nrhs, const DevEbe &E, 
 hipError_t launch_fill(hipStream_t st, const DevTopo &T, int nrhs, const DevCellSys &C,
                        const uint8_t *ftype, const double *bcval, const int *inst_reg,
                        int dirichlet_guess, double *val64, float *val32, double *b,
-                       double *dinv64, float *dinv32, double *x0);
+                       double *dinv64, float *dinv32, double *x0, double *r0);
 
 // One Jacobi-PCG iteration (spmv_pq, update_xr, update_p; in FP32 also the
 // fused FP64 residual replacement, which needs val64, b and x64).
@@ -202,22 +207,28 @@ template <typename MT, typename VT>
 hipError_t launch_spmv(hipStream_t st, const DevTopo &T, int nrhs, const MT *val, const VT *p,
                        VT *q, DevRed R, DevScal S);
 
-// FP64 PCG start: r = b - A x, p = dinv r, rz, rr0, thresholds, activation.
+// FP64 PCG start: r = b - A x (r_given: r holds it already), p = dinv r, rz,
+// rr0, thresholds, activation.
 hipError_t launch_pcg_start64(hipStream_t st, const DevTopo &T, int nrhs, const double *val,
                               const double *dinv, const double *b, const double *x, double *r,
-                              double *p, double tol2, DevRed R, DevScal S);
+                              double *p, double tol2, DevRed R, DevScal S, bool r_given);
 
-// FP32 start / FP64 residual replacement (see k_reliable), final fold of x32
-// into x64, and the FP64 residual norm for the report.
+// FP32 start / FP64 residual replacement (see k_reliable; r0 non-null: the
+// residual b - A64 x64 is given there), final fold of x32 into x64, and the
+// FP64 residual norm for the report.
 hipError_t launch_reliable(hipStream_t st, const DevTopo &T, int nrhs, const double *val64,
                            const double *b, const double *x64, const float *dinv32, float *r32,
-                           float *p32, int init, double tol2, DevRed R, DevScal S);
+                           float *p32, int init, double tol2, DevRed R, DevScal S,
+                           const double *r0 = nullptr);
 hipError_t launch_merge_all(hipStream_t st, const DevTopo &T, int nrhs, double *x64, float *x32);
 hipError_t launch_resnorm(hipStream_t st, const DevTopo &T, int nrhs, const double *val64,
                           const double *b, const double *x64, DevRed R, DevScal S);
 
 // Multigrid: per-solve set-up (Galerkin products, diagonals, coarsest inverses)
 // and the preconditioned iteration pieces.
+// The up-sweep operator M = P - w D^-1 T of each smoothed transfer, once the
+// smoother weights are known (after launch_amg_setup).
+hipError_t launch_amg_combine(hipStream_t st, const DevAmg &A, bool mixed);
 hipError_t launch_amg_setup(hipStream_t st, const DevTopo &T, const DevAmg &A, const double *val64,
                             bool mixed);
 // Lanczos on D^-1/2 A_l D^-1/2 (FP64) for the Jacobi weight of level l:
@@ -265,10 +276,11 @@ hipError_t launch_group_sum(hipStream_t st, int nrhs, int ngroups, int nhf, cons
                             const double *v, double *out);
 
 // Adjoint of the recovered fluxes (see k_adj_cell / k_adj_grad): the adjoint
-// right-hand side into b (x zeroed), and the gradients after the adjoint solve.
+// right-hand side into b and r (x zeroed), and the gradients after the adjoint
+// solve.
 hipError_t launch_adjoint_rhs(hipStream_t st, const DevTopo &T, int maxn, int nrhs, const double *binv_pk,
                               const double *totmob, const DevCellSys &C, const uint8_t *ftype,
-                              const double *w, double *y, double *b, double *x);
+                              const double *w, double *y, double *b, double *x, double *r);
 hipError_t launch_adjoint_grad(hipStream_t st, const DevTopo &T, const DevGeom &G, int maxn, int nrhs,
                                const double *binv_pk, const double *gflux, const double *totmob,
                                const double *rhomob, const double *grav, const DevCellSys &C,

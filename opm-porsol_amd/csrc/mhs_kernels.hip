@@ -476,8 +476,11 @@ __global__ __launch_bounds__(64) void k_assemble(DevTopo T, const double *__rest
 }
 
 // ---------------------------------------------------------------------------
-// Face-row gather: fills SELL values, right-hand sides and the Jacobi diagonal.
-// With ROW_LDS each thread sums its row in its own column of LDS (max_w
+// Face-row gather: fills SELL values, right-hand sides, the Jacobi diagonal,
+// the initial iterate and its residual r0 = b - A x0.  x0 is nonzero only on
+// Dirichlet rows, which hold nothing but their diagonal, and Dirichlet columns
+// are dropped from every other row, so (A x0)_f = diag_f x0_f: the PCG start
+// needs no product with the face matrix.  With ROW_LDS each thread sums its row in its own column of LDS (max_w
 // doubles, stride TB) and writes every stored entry once; otherwise the row
 // is summed in place in val64 (rows too wide for LDS).
 template <int NRHS, int TB, bool ROW_LDS>
@@ -490,7 +493,8 @@ __global__ __launch_bounds__(TB) void k_fill(DevTopo T, DevCellSys C,
                                              double *__restrict__ b,
                                              double *__restrict__ dinv64,
                                              float *__restrict__ dinv32,
-                                             double *__restrict__ x0) {
+                                             double *__restrict__ x0,
+                                             double *__restrict__ r0) {
     extern __shared__ double rowbuf[];
     // contiguous row ranges per XCD: the cells a row range touches (up to a
     // layer of cells away) stay in that XCD's L2 for the rows that reuse them
@@ -571,6 +575,7 @@ __global__ __launch_bounds__(TB) void k_fill(DevTopo T, DevCellSys C,
     for (int k = 0; k < NRHS; ++k) {
         b[(size_t)f * NRHS + k] = bk[k];
         x0[(size_t)f * NRHS + k] = xk[k];
+        r0[(size_t)f * NRHS + k] = bk[k] - diag * xk[k];
     }
     const double di = 1.0 / diag;
     if (dinv64) dinv64[f] = di;
@@ -977,8 +982,9 @@ __global__ __launch_bounds__(256) void k_update_p(DevTopo T, const VT *__restric
     }
 }
 
-// r = b - A x (FP64), p = dinv r; rz = r.dinv.r, rr0 = r.r; activate systems.
-template <int NRHS>
+// r = b - A x (FP64; GIVEN: r holds it already), p = dinv r; rz = r.dinv.r,
+// rr0 = r.r; activate systems.
+template <int NRHS, bool GIVEN>
 __global__ __launch_bounds__(256) void k_start64(DevTopo T, const double *__restrict__ val,
                                                  const double *__restrict__ dinv,
                                                  const double *__restrict__ bb,
@@ -993,12 +999,12 @@ __global__ __launch_bounds__(256) void k_start64(DevTopo T, const double *__rest
     for (int j = 0; j < 4; ++j) {
         const int f = b * kRowBlock + j * 256 + threadIdx.x;
         double acc[NRHS];
-        sell_row<double, double, double, NRHS>(T, val, x, f, acc);
+        if constexpr (!GIVEN) sell_row<double, double, double, NRHS>(T, val, x, f, acc);
         const double d = dinv[f];
 #pragma unroll
         for (int k = 0; k < NRHS; ++k) {
             const size_t i = (size_t)f * NRHS + k;
-            const double ri = bb[i] - acc[k];
+            const double ri = GIVEN ? r[i] : bb[i] - acc[k];
             r[i] = ri;
             p[i] = d * ri;
             part[2 * k] += ri * (d * ri);
@@ -1010,13 +1016,15 @@ __global__ __launch_bounds__(256) void k_start64(DevTopo T, const double *__rest
 
 // FP64 residual replacement for the FP32 iteration, fused into one pass.
 // For systems with upd != 0 (all systems when init): r64 = b - A64 x64 row by
-// row, r32 = (float)r64, and on restart (upd == 2, or init) p32 = dinv r32.
+// row (GIVEN: the start residual r0 the fill wrote), r32 = (float)r64, and on
+// restart (upd == 2, or init) p32 = dinv r32.
 // The last workgroup of the instance decides: |r64| <= tol |r0| ends the system,
 // otherwise it continues from the replaced residual.
-template <int NRHS>
+template <int NRHS, bool GIVEN>
 __global__ __launch_bounds__(256) void k_reliable(DevTopo T, const double *__restrict__ val,
                                                   const double *__restrict__ bb,
                                                   const double *__restrict__ x,
+                                                  const double *__restrict__ r0,
                                                   const float *__restrict__ dinv,
                                                   float *__restrict__ r32, float *__restrict__ p32,
                                                   int init, double tol2, DevRed R, DevScal S) {
@@ -1036,13 +1044,13 @@ __global__ __launch_bounds__(256) void k_reliable(DevTopo T, const double *__res
     for (int j = 0; j < 4; ++j) {
         const int f = b * kRowBlock + j * 256 + threadIdx.x;
         double acc[NRHS];
-        sell_row<double, double, double, NRHS>(T, val, x, f, acc);
+        if constexpr (!GIVEN) sell_row<double, double, double, NRHS>(T, val, x, f, acc);
         const float d = dinv[f];
 #pragma unroll
         for (int k = 0; k < NRHS; ++k) {
             if (!u[k]) continue;
             const size_t i = (size_t)f * NRHS + k;
-            const double ri = bb[i] - acc[k];
+            const double ri = GIVEN ? r0[i] : bb[i] - acc[k];
             const float rf = (float)ri;
             r32[i] = rf;
             if (u[k] == 2) p32[i] = d * rf;
@@ -1169,13 +1177,15 @@ __global__ __launch_bounds__(256) void k_coarse_diag(int n, const int *__restric
     if (dinv32) dinv32[i] = (float)d;
 }
 
-// Smoothed prolongation of a coarse transfer, row by row:
+// Smoothed prolongation of a coarse transfer, G lanes per row:
 // P_iJ = [J = agg_i] - w_p d_i sum_{j in row i, agg_j = J} a_ij.
+template <int G>
 __global__ __launch_bounds__(256) void k_sa_p(DevAmgLevel F, double wp) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int i = t / G, sub = t % G;
     if (i >= F.n) return;
     const double d = F.dinv64[i];
-    for (int q = F.p_ptr[i]; q < F.p_ptr[i + 1]; ++q) {
+    for (int q = F.p_ptr[i] + sub; q < F.p_ptr[i + 1]; q += G) {
         double s = 0.0;
         for (int k = F.p_src_ptr[q]; k < F.p_src_ptr[q + 1]; ++k) s += F.val64[F.p_src[k]];
         const double v = (q == F.p_own[i] ? 1.0 : 0.0) - wp * d * s;
@@ -1185,6 +1195,26 @@ __global__ __launch_bounds__(256) void k_sa_p(DevAmgLevel F, double wp) {
         const int e = F.p_tpos[q];
         F.pt_val64[e] = v;
         if (F.pt_val32) F.pt_val32[e] = (float)v;
+    }
+}
+
+// Combined up-sweep operator of a smoothed transfer, row by row:
+// M = P - w d_i T (T = A P holds P's pattern), so that the prolongation
+// x~ = w D^-1 b + al P xc followed by x = x~ + w D^-1 (b - A x~) reads one
+// matrix: x = w D^-1 (b + res) + al M xc, with res = b - A (w D^-1 b) from the
+// down-sweep.
+// G lanes per row.
+template <int G>
+__global__ __launch_bounds__(256) void k_sa_m(DevAmgLevel F, double w) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int i = t / G, sub = t % G;
+    if (i >= F.n) return;
+    const double d = w * F.dinv64[i];
+    for (int e = F.t_ptr[i] + sub; e < F.t_ptr[i + 1]; e += G) {
+        const int q = F.t_p[e];
+        const double v = (q >= 0 ? F.p_val64[q] : 0.0) - d * F.t_val64[e];
+        F.m_val64[e] = v;
+        if (F.m_val32) F.m_val32[e] = (float)v;
     }
 }
 
@@ -1381,8 +1411,8 @@ __device__ __forceinline__ const VT *lvl_pt(const DevAmgLevel &L) {
     if constexpr (std::is_same<VT, float>::value) return L.pt_val32; else return L.pt_val64;
 }
 template <typename VT>
-__device__ __forceinline__ const VT *lvl_t(const DevAmgLevel &L) {
-    if constexpr (std::is_same<VT, float>::value) return L.t_val32; else return L.t_val64;
+__device__ __forceinline__ const VT *lvl_m(const DevAmgLevel &L) {
+    if constexpr (std::is_same<VT, float>::value) return L.m_val32; else return L.m_val64;
 }
 
 // acc = (P' res)_I across a smoothed transfer (P' row I: fine rows and the P
@@ -1458,16 +1488,14 @@ __device__ __forceinline__ void up_sa_row(const DevAmgLevel &F, const VT *__rest
     const VT *b = (const VT *)F.b;
     const VT *res = (const VT *)F.res;
     VT *xo = (VT *)F.x;
-    VT px[NRHS], tx[NRHS];
-    row_gather<VT, NRHS, G>(F.p_col, lvl_p<VT>(F), F.p_ptr[i], F.p_ptr[i + 1], xc, px, sub);
-    row_gather<VT, NRHS, G>(F.t_col, lvl_t<VT>(F), F.t_ptr[i], F.t_ptr[i + 1], xc, tx, sub);
+    VT mx[NRHS];
+    row_gather<VT, NRHS, G>(F.t_col, lvl_m<VT>(F), F.t_ptr[i], F.t_ptr[i + 1], xc, mx, sub);
     if (sub != 0) return;
     const VT d = w * dinv[i];
 #pragma unroll
     for (int k = 0; k < NRHS; ++k) {
         const size_t q = (size_t)i * NRHS + k;
-        const VT xt = d * b[q] + al * px[k];
-        xo[q] = xt + d * (res[q] - al * tx[k]);
+        xo[q] = d * (b[q] + res[q]) + al * mx[k];
     }
 }
 
@@ -2206,7 +2234,7 @@ __global__ __launch_bounds__(64) void k_adj_cell(DevTopo T, const double *__rest
 template <int NRHS>
 __global__ __launch_bounds__(256) void k_adj_rows(DevTopo T, const uint8_t *__restrict__ ftype,
                                                   const double *__restrict__ y, double *__restrict__ b,
-                                                  double *__restrict__ x) {
+                                                  double *__restrict__ x, double *__restrict__ r) {
     const int f = blockIdx.x * 256 + threadIdx.x;
     if (f >= T.nfp) return;
     const int h0 = T.dof_hf[f];
@@ -2226,6 +2254,7 @@ __global__ __launch_bounds__(256) void k_adj_rows(DevTopo T, const uint8_t *__re
     for (int k = 0; k < NRHS; ++k) {
         b[(size_t)f * NRHS + k] = acc[k];
         x[(size_t)f * NRHS + k] = 0.0;
+        r[(size_t)f * NRHS + k] = acc[k];   // start residual b - A 0
     }
 }
 
@@ -2834,7 +2863,7 @@ hipError_t launch_assemble(hipStream_t st, const DevTopo &T, int maxn, int nrhs,
 hipError_t launch_fill(hipStream_t st, const DevTopo &T, int nrhs, const DevCellSys &C,
                        const uint8_t *ftype, const double *bcval, const int *inst_reg,
                        int dguess, double *val64, float *val32, double *b, double *dinv64,
-                       float *dinv32, double *x0) {
+                       float *dinv32, double *x0, double *r0) {
     // rows of up to 24 entries: 256 threads and 48 KB of LDS; up to 128: 64
     // threads and 64 KB; wider rows are summed in place in val64
     const size_t w = (size_t)std::max(T.max_w, 1);
@@ -2842,13 +2871,13 @@ hipError_t launch_fill(hipStream_t st, const DevTopo &T, int nrhs, const DevCell
         constexpr int NR = decltype(nr)::value;
         if (w <= 24)
             k_fill<NR, 256, true><<<T.nfp / 256, 256, w * 256 * sizeof(double), st>>>(
-                T, C, ftype, bcval, inst_reg, dguess, val64, val32, b, dinv64, dinv32, x0);
+                T, C, ftype, bcval, inst_reg, dguess, val64, val32, b, dinv64, dinv32, x0, r0);
         else if (w <= 128)
             k_fill<NR, 64, true><<<T.nfp / 64, 64, w * 64 * sizeof(double), st>>>(
-                T, C, ftype, bcval, inst_reg, dguess, val64, val32, b, dinv64, dinv32, x0);
+                T, C, ftype, bcval, inst_reg, dguess, val64, val32, b, dinv64, dinv32, x0, r0);
         else
             k_fill<NR, 256, false><<<T.nfp / 256, 256, 0, st>>>(T, C, ftype, bcval, inst_reg, dguess, val64,
-                                                                 val32, b, dinv64, dinv32, x0);
+                                                                 val32, b, dinv64, dinv32, x0, r0);
     });
 }
 
@@ -2920,20 +2949,27 @@ template hipError_t launch_spmv<float, float>(hipStream_t, const DevTopo &, int,
 
 hipError_t launch_pcg_start64(hipStream_t st, const DevTopo &T, int nrhs, const double *val,
                               const double *dinv, const double *b, const double *x, double *r,
-                              double *p, double tol2, DevRed R, DevScal S) {
+                              double *p, double tol2, DevRed R, DevScal S, bool r_given) {
     return with_nrhs(nrhs, [&](auto nr) {
         constexpr int NR = decltype(nr)::value;
-        k_start64<NR><<<T.nblk, 256, 0, st>>>(T, val, dinv, b, x, r, p, tol2, R, S);
+        if (r_given)
+            k_start64<NR, true><<<T.nblk, 256, 0, st>>>(T, val, dinv, b, x, r, p, tol2, R, S);
+        else
+            k_start64<NR, false><<<T.nblk, 256, 0, st>>>(T, val, dinv, b, x, r, p, tol2, R, S);
         k_finish<2 * NR, FinStart<NR>><<<T.ninst, 256, 0, st>>>(R, S, FinArgs{tol2, 0});
     });
 }
 
 hipError_t launch_reliable(hipStream_t st, const DevTopo &T, int nrhs, const double *val64,
                            const double *b, const double *x64, const float *dinv32, float *r32,
-                           float *p32, int init, double tol2, DevRed R, DevScal S) {
+                           float *p32, int init, double tol2, DevRed R, DevScal S, const double *r0) {
     return with_nrhs(nrhs, [&](auto nr) {
         constexpr int NR = decltype(nr)::value;
-        k_reliable<NR><<<T.nblk, 256, 0, st>>>(T, val64, b, x64, dinv32, r32, p32, init, tol2, R, S);
+        if (r0)
+            k_reliable<NR, true><<<T.nblk, 256, 0, st>>>(T, val64, b, x64, r0, dinv32, r32, p32, init, tol2, R, S);
+        else
+            k_reliable<NR, false><<<T.nblk, 256, 0, st>>>(T, val64, b, x64, r0, dinv32, r32, p32, init, tol2, R,
+                                                          S);
         k_finish<3 * NR, FinRel<NR>><<<T.ninst, 256, 0, st>>>(R, S, FinArgs{tol2, init});
     });
 }
@@ -2957,6 +2993,13 @@ hipError_t launch_resnorm(hipStream_t st, const DevTopo &T, int nrhs, const doub
 
 hipError_t launch_amg_setup(hipStream_t st, const DevTopo &T, const DevAmg &A, const double *val64,
                             bool mixed) {
+    // A P of the level-1 aggregates first: level 1's Galerkin sums its slots
+    if (A.ap_slots > 0)
+        with_lanes(lanes_per_row(A.ap_nsrc, (int)A.ap_slots), [&](auto g) {
+            constexpr int Gv = decltype(g)::value;
+            k_galerkin<Gv><<<row_blocks((int)A.ap_slots, Gv), 256, 0, st>>>(
+                (int)A.ap_slots, A.ap_src_ptr, A.ap_src, val64, A.ap64, mixed ? A.ap32 : nullptr);
+        });
     for (int l = 1; l < A.nlev; ++l) {
         const DevAmgLevel &L = A.lev[l];
         const DevAmgLevel &F = A.lev[l - 1];
@@ -2968,7 +3011,7 @@ hipError_t launch_amg_setup(hipStream_t st, const DevTopo &T, const DevAmg &A, c
                         L.nnz, L.src_ptr, L.src, L.src2, F.p_val64, F.t_val64, L.val64, mixed ? L.val32 : nullptr);
                 else
                     k_galerkin<Gv><<<row_blocks(L.nnz, Gv), 256, 0, st>>>(
-                        L.nnz, L.src_ptr, L.src, (l == 1) ? val64 : F.val64, L.val64, mixed ? L.val32 : nullptr);
+                        L.nnz, L.src_ptr, L.src, (l == 1) ? A.ap64 : F.val64, L.val64, mixed ? L.val32 : nullptr);
             });
         }
         if (L.n > 0)
@@ -2980,7 +3023,11 @@ hipError_t launch_amg_setup(hipStream_t st, const DevTopo &T, const DevAmg &A, c
                 Lw.p_val32 = nullptr;
                 Lw.pt_val32 = nullptr;
             }
-            k_sa_p<<<(L.n + 255) / 256, 256, 0, st>>>(Lw, A.omega_p);
+            // a few entries per lane: the set-up is latency-bound
+            with_lanes(L.p_nnz > 2LL * L.n ? 4 : 1, [&](auto g) {
+                constexpr int Gv = decltype(g)::value;
+                k_sa_p<Gv><<<row_blocks(L.n, Gv), 256, 0, st>>>(Lw, A.omega_p);
+            });
             if (L.t_nnz > 0)
                 with_lanes(lanes_per_row(L.t_npair, L.t_nnz), [&](auto g) {
                     constexpr int Gv = decltype(g)::value;
@@ -2990,15 +3037,22 @@ hipError_t launch_amg_setup(hipStream_t st, const DevTopo &T, const DevAmg &A, c
                 });
         }
     }
-    if (A.ap_slots > 0)
-        with_lanes(lanes_per_row(A.ap_nsrc, (int)A.ap_slots), [&](auto g) {
-            constexpr int Gv = decltype(g)::value;
-            k_galerkin<Gv><<<row_blocks((int)A.ap_slots, Gv), 256, 0, st>>>(
-                (int)A.ap_slots, A.ap_src_ptr, A.ap_src, val64, A.ap64, mixed ? A.ap32 : nullptr);
-        });
     const DevAmgLevel &C = A.lev[A.nlev - 1];
     k_coarse_inverse<<<A.ninst, 1024, 0, st>>>(C, C.inst_ptr, A.coarse_off, A.cinv64,
                                               mixed ? A.cinv32 : nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_amg_combine(hipStream_t st, const DevAmg &A, bool mixed) {
+    for (int l = 1; l + 1 < A.nlev; ++l) {
+        DevAmgLevel F = A.lev[l];
+        if (!F.sa_out || F.n == 0) continue;
+        if (!mixed) F.m_val32 = nullptr;
+        with_lanes(lanes_per_row(F.t_nnz, F.n), [&](auto g) {
+            constexpr int Gv = decltype(g)::value;
+            k_sa_m<Gv><<<row_blocks(F.n, Gv), 256, 0, st>>>(F, A.omega_l[l]);
+        });
+    }
     return hipGetLastError();
 }
 
@@ -3170,7 +3224,7 @@ static void vcycle_impl(hipStream_t st, const Gate &gt, const DevTopo &T, const 
             const DevAmgLevel &F = A.lev[l];
             const VT *xc = xres[l + 1];
             if (F.sa_out) {
-                with_lanes(lanes_per_row(std::max(F.p_nnz, F.t_nnz), F.n), [&](auto g) {
+                with_lanes(lanes_per_row(F.t_nnz, F.n), [&](auto g) {
                     constexpr int Gv = decltype(g)::value;
                     k_vcc_up_sa<VT, NR, Gv><<<row_blocks(F.n, Gv), 256, 0, st>>>(F, xc, wl(l), als);
                 });
@@ -3401,14 +3455,14 @@ hipError_t launch_group_sum(hipStream_t st, int nrhs, int ngroups, int nhf, cons
 
 hipError_t launch_adjoint_rhs(hipStream_t st, const DevTopo &T, int maxn, int nrhs, const double *binv_pk,
                               const double *totmob, const DevCellSys &C, const uint8_t *ftype,
-                              const double *w, double *y, double *b, double *x) {
+                              const double *w, double *y, double *b, double *x, double *r) {
     hipError_t e = hipSuccess;
     hipError_t e2 = with_maxn(maxn, [&](auto mn) {
         constexpr int MN = decltype(mn)::value;
         e = with_nrhs(nrhs, [&](auto nr) {
             constexpr int NR = decltype(nr)::value;
             if (T.nc > 0) k_adj_cell<MN, NR><<<cell_blocks(T.nc), 64, 0, st>>>(T, binv_pk, totmob, C, w, y);
-            k_adj_rows<NR><<<(unsigned)((T.nfp + 255) / 256), 256, 0, st>>>(T, ftype, y, b, x);
+            k_adj_rows<NR><<<(unsigned)((T.nfp + 255) / 256), 256, 0, st>>>(T, ftype, y, b, x, r);
         });
     });
     return e != hipSuccess ? e : e2;

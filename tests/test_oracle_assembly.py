@@ -207,7 +207,9 @@ def test_threaded_cg_agrees_and_one_thread_is_unchanged(pkg, orc):
     again = [orc.solve(g, perm, tm, rm, src, bt, bv, tol=1e-14, precond=pc) for pc in (0, 1)]
     for b, t, a in zip(base, thr, again):
         assert cases.rel_err(t.face_pressure, b.face_pressure) < 1e-11
-        assert abs(t.iterations - b.iterations) <= 2
+        # at 1e-14 the count depends on round-off, and the threads' partial
+        # sums are combined in no fixed order
+        assert abs(t.iterations - b.iterations) <= max(2, b.iterations // 20)
         np.testing.assert_array_equal(a.face_pressure, b.face_pressure)
         assert a.iterations == b.iterations
 
