@@ -73,6 +73,8 @@ struct mhs_solver {
     hipStream_t stream = nullptr;       // our own stream (default priority)
     hipStream_t user_stream = nullptr;  // caller's stream, ordered against ours
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    hipEvent_t ev_ph[4] = {};       // phase boundaries of a solve on its stream (report times)
+    std::vector<unsigned char> dA_copied;   // the multigrid descriptor last copied to the device
     std::string err;
     // this device's scratch array (mhs_sum_outflux_noalloc); null when the
     // address lookup at creation failed, with the reason in scratch_err
@@ -153,6 +155,7 @@ struct mhs_solver {
     // boundary half-faces of the structure (their instance), and the face
     // types last uploaded for them; internal half-faces are always FT_INTERNAL
     std::vector<int> bnd_hf, bnd_inst;
+    std::vector<uint8_t> bnd_per;   // periodic flag of each, from the structure
     long long bnd_gen = -1;
     std::vector<uint8_t> last_bft;
     int last_nrhs = 0;
@@ -214,6 +217,7 @@ void mhs_solver::free_amg() {
     amg_bufs.clear();
     amg_planned = false;
     dA = DevAmg{};
+    dA_copied.clear();      // a new hierarchy has a new device descriptor
     has_amg = false;
     H = AmgHierarchy();
     rho.clear();
@@ -264,6 +268,8 @@ mhs_solver::~mhs_solver() {
     free_topology();
     if (ev_in) hipEventDestroy(ev_in);
     if (ev_out) hipEventDestroy(ev_out);
+    for (hipEvent_t e : ev_ph)
+        if (e) hipEventDestroy(e);
     if (stream) hipStreamDestroy(stream);
 }
 
@@ -302,6 +308,8 @@ int mhs_create(mhs_solver **out, int device, void *stream) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_in, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_out, hipEventDisableTiming);
+    for (hipEvent_t &ev : s->ev_ph)
+        if (e == hipSuccess) e = hipEventCreate(&ev);
     if (e != hipSuccess) {
         delete s;
         return MHS_ERR_HIP;
@@ -1215,14 +1223,11 @@ int linear_solve(mhs_solver *s, const mhs_solve_options &opt, int nrhs, bool mix
     s->S.mixed = mixed ? 1 : 0;
     s->S.amg = amg ? 1 : 0;
     s->S.rf2 = opt.replace_factor * opt.replace_factor;
-    HIPCHECK(hipMemsetAsync(s->S.status, 0, 4 * sizeof(int), s->stream));
-    HIPCHECK(hipMemsetAsync(s->S.iters, 0, nsys * sizeof(int), s->stream));
-    HIPCHECK(hipMemsetAsync(s->S.upd, 0, nsys * sizeof(int), s->stream));
+    HIPCHECK(launch_clear_state(s->stream, s->S, (int)nsys, mixed ? s->x32 : nullptr, (long long)T.nfp * nrhs));
     if (!mixed) {
         HIPCHECK(launch_pcg_start64(s->stream, s->dT, nrhs, s->val64, s->dinv64, s->b, s->x64, s->r64,
                                     s->p64, tol2, s->R, s->S, true));
     } else {
-        HIPCHECK(hipMemsetAsync(s->x32, 0, sizeof(float) * T.nfp * nrhs, s->stream));
         HIPCHECK(launch_reliable(s->stream, s->dT, nrhs, s->val64, s->b, s->x64, s->dinv32, s->r32,
                                  s->p32, 1, tol2, s->R, s->S, s->r64));
     }
@@ -1234,8 +1239,16 @@ int linear_solve(mhs_solver *s, const mhs_solve_options &opt, int nrhs, bool mix
             HIPCHECK(launch_amg_start<float>(s->stream, s->dT, s->dA, nrhs, s->val32, s->dinv32, s->r32,
                                              s->z32, s->p32, s->R, s->S, nullptr, s->x32, cur_ebe(s)));
     }
-    int rc = read_state(s, nsys);
-    if (rc) return rc;
+    // The first chunk follows the start without a host round trip (its
+    // kernels skip the systems the start found converged); the state is read
+    // after it.
+    for (size_t i = 0; i < nsys; ++i) {
+        s->h_act[i] = 1;
+        s->h_iters[i] = 0;
+    }
+    s->h_status[0] = 1;
+    s->h_status[1] = 0;
+    int rc = MHS_OK;
     while (s->h_status[0] > 0) {
         int mx = 0;
         for (size_t i = 0; i < nsys; ++i) mx = std::max(mx, s->h_iters[i]);
@@ -1329,6 +1342,7 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
     if (s->bnd_gen != s->struct_gen) {
         s->bnd_hf.clear();
         s->bnd_inst.clear();
+        s->bnd_per.clear();
         int t = 0;
         for (int c = 0; c < T.nc; ++c) {
             while (c >= T.inst_cellpos[t + 1]) ++t;
@@ -1336,6 +1350,7 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
                 if (T.nb[h] < 0) {
                     s->bnd_hf.push_back(h);
                     s->bnd_inst.push_back(t);
+                    s->bnd_per.push_back(T.hf_periodic[h]);
                 }
         }
         s->bnd_gen = s->struct_gen;
@@ -1344,21 +1359,27 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
     const size_t nbnd = s->bnd_hf.size();
     std::vector<uint8_t> bft(nbnd);
     std::vector<int> reg(T.ninst, 1);
+    // the caller's array is read at scattered positions: keep a few dozen
+    // loads in flight
+    constexpr size_t kAhead = 32;
     for (size_t e = 0; e < nbnd; ++e) {
+        if (e + kAhead < nbnd) __builtin_prefetch(hf_bc_type + s->bnd_hf[e + kAhead]);
         const int h = s->bnd_hf[e];
         const int bt = hf_bc_type[h];
         if (bt == MHS_BC_DIRICHLET) { bft[e] = FT_DIRICHLET; reg[s->bnd_inst[e]] = 0; }
         else if (bt == MHS_BC_PERIODIC) bft[e] = FT_PERIODIC;
         else if (bt == MHS_BC_NEUMANN) bft[e] = FT_NEUMANN;
         else return fail(s, MHS_ERR_INVALID, "solve: unknown boundary condition type");
-        if ((bt == MHS_BC_PERIODIC) != (T.hf_periodic[h] != 0))
+        if ((bt == MHS_BC_PERIODIC) != (s->bnd_per[e] != 0))
             return fail(s, MHS_ERR_INVALID, "solve: periodic faces differ from those given to init_system_structure");
     }
-    auto t0 = std::chrono::steady_clock::now();
     if (s->user_stream) {
         HIPCHECK(hipEventRecord(s->ev_in, s->user_stream));
         HIPCHECK(hipStreamWaitEvent(s->stream, s->ev_in, 0));
     }
+    // the report's phase times come from events on the solver's stream: no
+    // host synchronisation between the phases
+    HIPCHECK(hipEventRecord(s->ev_ph[0], s->stream));
     if (bft != s->last_bft) {
         std::vector<uint8_t> ft(T.nhf, FT_INTERNAL);
         for (size_t e = 0; e < nbnd; ++e) ft[s->bnd_hf[e]] = bft[e];
@@ -1403,8 +1424,7 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
     float *fill32 = (mixed && !(ebe_settled && opt.smooth_steps <= 1)) ? s->val32 : nullptr;
     HIPCHECK(launch_fill(s->stream, s->dT, nrhs, s->C, s->ftype, d_bv, s->inst_reg, dguess, s->val64,
                          fill32, s->b, s->dinv64, s->dinv32, s->x64, s->r64));
-    HIPCHECK(hipStreamSynchronize(s->stream));
-    auto t1 = std::chrono::steady_clock::now();
+    HIPCHECK(hipEventRecord(s->ev_ph[1], s->stream));
 
     // --- preconditioner ---------------------------------------------------------
     // The smoother weights come from rho(D^-1 A) of an earlier solve of this
@@ -1459,7 +1479,13 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
             // the kernels that walk the whole hierarchy read this copy (nothing of
             // the previous solve is still running: it ended with a synchronise;
             // the set-up kernels above do not read it)
-            HIPCHECK(hipMemcpy(const_cast<DevAmg *>(s->dA.dev), &s->dA, sizeof(DevAmg), hipMemcpyHostToDevice));
+            // (copied only when it changed: a blocking copy stalls the stream)
+            const unsigned char *da = reinterpret_cast<const unsigned char *>(&s->dA);
+            if (s->dA_copied.size() != sizeof(DevAmg) ||
+                std::memcmp(s->dA_copied.data(), da, sizeof(DevAmg)) != 0) {
+                HIPCHECK(hipMemcpy(const_cast<DevAmg *>(s->dA.dev), &s->dA, sizeof(DevAmg), hipMemcpyHostToDevice));
+                s->dA_copied.assign(da, da + sizeof(DevAmg));
+            }
             HIPCHECK(launch_amg_combine(s->stream, s->dA, mixed));
         }
 
@@ -1498,17 +1524,17 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
     const bool converged = lr.converged;
     const double red = lr.reduction;
     s->last_amg = amg;
-    auto t2 = std::chrono::steady_clock::now();
+    HIPCHECK(hipEventRecord(s->ev_ph[2], s->stream));
 
     // --- recovery -------------------------------------------------------------
     HIPCHECK(launch_recover(s->stream, s->dT, T.max_ncf, nrhs, binv, gflux, d_tm, d_rm, s->C,
                             s->x64, s->press, s->outflux));
+    HIPCHECK(hipEventRecord(s->ev_ph[3], s->stream));
     HIPCHECK(hipStreamSynchronize(s->stream));
     if (s->user_stream) {
         HIPCHECK(hipEventRecord(s->ev_out, s->stream));
         HIPCHECK(hipStreamWaitEvent(s->user_stream, s->ev_out, 0));
     }
-    auto t3 = std::chrono::steady_clock::now();
     s->solved = true;
     s->last_nrhs = nrhs;
     ++s->solve_count;
@@ -1526,9 +1552,11 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
                 rep->mg_rows += s->H.lev[l].n;
                 rep->mg_nnz += (long long)s->H.lev[l].col.size();
             }
-        rep->assemble_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-        rep->solve_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
-        rep->recover_ms = std::chrono::duration<double, std::milli>(t3 - t2).count();
+        float ph[3] = {0.0f, 0.0f, 0.0f};
+        for (int k = 0; k < 3; ++k) HIPCHECK(hipEventElapsedTime(&ph[k], s->ev_ph[k], s->ev_ph[k + 1]));
+        rep->assemble_ms = ph[0];
+        rep->solve_ms = ph[1];
+        rep->recover_ms = ph[2];
         rep->spmv_threads = s->dT.spmv_tb;
         rep->spmv_xcd = s->dT.spmv_xcd;
         for (int l = 0; l < 8; ++l) {

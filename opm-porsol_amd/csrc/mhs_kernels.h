@@ -221,6 +221,9 @@ hipError_t launch_reliable(hipStream_t st, const DevTopo &T, int nrhs, const dou
                            float *p32, int init, double tol2, DevRed R, DevScal S,
                            const double *r0 = nullptr);
 hipError_t launch_merge_all(hipStream_t st, const DevTopo &T, int nrhs, double *x64, float *x32);
+// Clears the status words, iteration counts and replacement flags of nsys
+// systems and zeroes x32[0, nx) (x32 may be null).
+hipError_t launch_clear_state(hipStream_t st, const DevScal &S, int nsys, float *x32, long long nx);
 hipError_t launch_resnorm(hipStream_t st, const DevTopo &T, int nrhs, const double *val64,
                           const double *b, const double *x64, DevRed R, DevScal S);
 

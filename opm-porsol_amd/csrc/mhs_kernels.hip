@@ -1062,6 +1062,18 @@ __global__ __launch_bounds__(256) void k_reliable(DevTopo T, const double *__res
     block_partials<3 * NRHS>(part, b, R);
 }
 
+// Start of a linear solve: status words, iteration counts and replacement
+// flags cleared and the FP32 correction x32 (may be null) zeroed, in one launch.
+__global__ __launch_bounds__(256) void k_clear_state(DevScal S, int nsys, float *__restrict__ x32, long long nx) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < 4) S.status[i] = 0;
+    if (i < nsys) {
+        S.iters[i] = 0;
+        S.upd[i] = 0;
+    }
+    if (x32 && i < nx) x32[i] = 0.0f;
+}
+
 // x64 += x32, x32 = 0 for every system (final fold after the iteration).
 template <int NRHS>
 __global__ __launch_bounds__(256) void k_merge_all(DevTopo T, double *__restrict__ x64,
@@ -2972,6 +2984,12 @@ hipError_t launch_reliable(hipStream_t st, const DevTopo &T, int nrhs, const dou
                                                           S);
         k_finish<3 * NR, FinRel<NR>><<<T.ninst, 256, 0, st>>>(R, S, FinArgs{tol2, init});
     });
+}
+
+hipError_t launch_clear_state(hipStream_t st, const DevScal &S, int nsys, float *x32, long long nx) {
+    const long long n = std::max<long long>({4LL, (long long)nsys, x32 ? nx : 0LL});
+    k_clear_state<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(S, nsys, x32, nx);
+    return hipGetLastError();
 }
 
 hipError_t launch_merge_all(hipStream_t st, const DevTopo &T, int nrhs, double *x64, float *x32) {
