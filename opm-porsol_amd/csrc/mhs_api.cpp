@@ -158,6 +158,7 @@ struct mhs_solver {
     std::vector<uint8_t> bnd_per;   // periodic flag of each, from the structure
     long long bnd_gen = -1;
     std::vector<uint8_t> last_bft;
+    std::vector<int> last_reg;      // per-instance regularisation flags last uploaded
     int last_nrhs = 0;
     bool last_amg = false;
     long long solve_count = 0;
@@ -261,6 +262,7 @@ void mhs_solver::free_solve_buffers() {
     h_act = h_iters = h_upd = h_status = nullptr;
     h_nsys = 0;
     last_bft.clear();
+    last_reg.clear();
 }
 
 mhs_solver::~mhs_solver() {
@@ -1338,7 +1340,9 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
     // Face types (FaceType, :378) and per-instance regularisation flags.
     // Only boundary half-faces can change type between solves: their list is
     // built once per structure, and the device types are rewritten only when
-    // a boundary type changed.
+    // a boundary type changed.  Once types are on the device, the scan of the
+    // caller's array (scattered reads of a large host array) runs while the
+    // GPU assembles with the previous types; a change repeats the assembly.
     if (s->bnd_gen != s->struct_gen) {
         s->bnd_hf.clear();
         s->bnd_inst.clear();
@@ -1355,23 +1359,45 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
         }
         s->bnd_gen = s->struct_gen;
         s->last_bft.clear();
+        s->last_reg.clear();
     }
     const size_t nbnd = s->bnd_hf.size();
     std::vector<uint8_t> bft(nbnd);
     std::vector<int> reg(T.ninst, 1);
-    // the caller's array is read at scattered positions: keep a few dozen
-    // loads in flight
-    constexpr size_t kAhead = 32;
-    for (size_t e = 0; e < nbnd; ++e) {
-        if (e + kAhead < nbnd) __builtin_prefetch(hf_bc_type + s->bnd_hf[e + kAhead]);
-        const int h = s->bnd_hf[e];
-        const int bt = hf_bc_type[h];
-        if (bt == MHS_BC_DIRICHLET) { bft[e] = FT_DIRICHLET; reg[s->bnd_inst[e]] = 0; }
-        else if (bt == MHS_BC_PERIODIC) bft[e] = FT_PERIODIC;
-        else if (bt == MHS_BC_NEUMANN) bft[e] = FT_NEUMANN;
-        else return fail(s, MHS_ERR_INVALID, "solve: unknown boundary condition type");
-        if ((bt == MHS_BC_PERIODIC) != (s->bnd_per[e] != 0))
-            return fail(s, MHS_ERR_INVALID, "solve: periodic faces differ from those given to init_system_structure");
+    auto scan = [&]() -> int {
+        std::fill(reg.begin(), reg.end(), 1);
+        // the caller's array is read at scattered positions: keep a few dozen
+        // loads in flight
+        constexpr size_t kAhead = 32;
+        for (size_t e = 0; e < nbnd; ++e) {
+            if (e + kAhead < nbnd) __builtin_prefetch(hf_bc_type + s->bnd_hf[e + kAhead]);
+            const int h = s->bnd_hf[e];
+            const int bt = hf_bc_type[h];
+            if (bt == MHS_BC_DIRICHLET) { bft[e] = FT_DIRICHLET; reg[s->bnd_inst[e]] = 0; }
+            else if (bt == MHS_BC_PERIODIC) bft[e] = FT_PERIODIC;
+            else if (bt == MHS_BC_NEUMANN) bft[e] = FT_NEUMANN;
+            else return fail(s, MHS_ERR_INVALID, "solve: unknown boundary condition type");
+            if ((bt == MHS_BC_PERIODIC) != (s->bnd_per[e] != 0))
+                return fail(s, MHS_ERR_INVALID, "solve: periodic faces differ from those given to init_system_structure");
+        }
+        return MHS_OK;
+    };
+    auto upload_types = [&]() -> int {
+        std::vector<uint8_t> ft(T.nhf, FT_INTERNAL);
+        for (size_t e = 0; e < nbnd; ++e) ft[s->bnd_hf[e]] = bft[e];
+        HIPCHECK(hipMemcpyAsync(s->ftype, ft.data(), T.nhf, hipMemcpyHostToDevice, s->stream));
+        HIPCHECK(hipMemcpyAsync(s->inst_reg, reg.data(), sizeof(int) * T.ninst, hipMemcpyHostToDevice, s->stream));
+        HIPCHECK(hipStreamSynchronize(s->stream));   // the host vectors end here
+        s->last_bft = bft;
+        s->last_reg = reg;
+        return MHS_OK;
+    };
+    const bool speculate = !s->last_bft.empty() && s->last_reg.size() == (size_t)T.ninst;
+    if (speculate) {
+        reg = s->last_reg;
+    } else {
+        rc = scan();
+        if (rc) return rc;
     }
     if (s->user_stream) {
         HIPCHECK(hipEventRecord(s->ev_in, s->user_stream));
@@ -1380,14 +1406,10 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
     // the report's phase times come from events on the solver's stream: no
     // host synchronisation between the phases
     HIPCHECK(hipEventRecord(s->ev_ph[0], s->stream));
-    if (bft != s->last_bft) {
-        std::vector<uint8_t> ft(T.nhf, FT_INTERNAL);
-        for (size_t e = 0; e < nbnd; ++e) ft[s->bnd_hf[e]] = bft[e];
-        HIPCHECK(hipMemcpyAsync(s->ftype, ft.data(), T.nhf, hipMemcpyHostToDevice, s->stream));
-        HIPCHECK(hipStreamSynchronize(s->stream));
-        s->last_bft = bft;
+    if (!speculate) {
+        rc = upload_types();
+        if (rc) return rc;
     }
-    HIPCHECK(hipMemcpyAsync(s->inst_reg, reg.data(), sizeof(int) * T.ninst, hipMemcpyHostToDevice, s->stream));
     const double *d_tm = totmob, *d_rm = rhomob, *d_src = src, *d_bv = bcval;
     if (!device_inputs) {
         auto kind = hipMemcpyHostToDevice;
@@ -1403,27 +1425,50 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
         HIPCHECK(hipMemcpyAsync(s->in_rhomob, rhomob, sizeof(double) * T.nc, kind, s->stream));
         d_tm = s->in_totmob; d_rm = s->in_rhomob;
     }
-    // the level-0 operator of the FP32 iteration: cell by cell when the
-    // structure allows it and no instance is regularised (the doubled entry
-    // of :1398-1400 is not a cell contribution)
-    // element_operator 2 always, 1 when the autotuner found it the faster
-    // operator for this nrhs on this structure (not yet tried: try it)
-    s->last_ebe = mixed && s->ebe_ok && opt.element_operator != 0 &&
-                  std::none_of(reg.begin(), reg.end(), [](int v) { return v != 0; }) &&
-                  (opt.element_operator == 2 || !opt.autotune || s->ebe_pref[nrhs] != 0);
     // --- assembly -----------------------------------------------------------
+    // (it overwrites the state the solution accessors read: no solution until
+    // this solve has recovered one)
+    s->solved = false;
     s->last_aniso = binv_use != nullptr;
     const double *binv = binv_use ? binv_use : s->binv, *gflux = gflux_use ? gflux_use : s->gflux;
-    HIPCHECK(launch_assemble(s->stream, s->dT, T.max_ncf, nrhs, binv, gflux, d_tm, d_rm, d_src,
-                             s->ftype, d_bv, s->C, cur_ebe(s)));
     const int dguess = opt.linsolver_type != 0;  // AMG types start from the Dirichlet guess (:1521-1532)
-    // The FP32 face matrix is not written when the FP32 iteration applies the
-    // operator cell by cell for certain (no trial against the assembled matrix
-    // due, one smoothing sweep: more sweeps run on the assembled matrix).
-    const bool ebe_settled = s->last_ebe && (opt.element_operator == 2 || !opt.autotune || s->ebe_pref[nrhs] == 1);
-    float *fill32 = (mixed && !(ebe_settled && opt.smooth_steps <= 1)) ? s->val32 : nullptr;
-    HIPCHECK(launch_fill(s->stream, s->dT, nrhs, s->C, s->ftype, d_bv, s->inst_reg, dguess, s->val64,
-                         fill32, s->b, s->dinv64, s->dinv32, s->x64, s->r64));
+    float *fill32 = nullptr;
+    auto assemble = [&]() -> int {
+        // the level-0 operator of the FP32 iteration: cell by cell when the
+        // structure allows it and no instance is regularised (the doubled
+        // entry of :1398-1400 is not a cell contribution); element_operator 2
+        // always, 1 when the autotuner found it the faster operator for this
+        // nrhs on this structure (not yet tried: try it)
+        s->last_ebe = mixed && s->ebe_ok && opt.element_operator != 0 &&
+                      std::none_of(reg.begin(), reg.end(), [](int v) { return v != 0; }) &&
+                      (opt.element_operator == 2 || !opt.autotune || s->ebe_pref[nrhs] != 0);
+        HIPCHECK(launch_assemble(s->stream, s->dT, T.max_ncf, nrhs, binv, gflux, d_tm, d_rm, d_src,
+                                 s->ftype, d_bv, s->C, cur_ebe(s)));
+        // The FP32 face matrix is not written when the FP32 iteration applies
+        // the operator cell by cell for certain (no trial against the
+        // assembled matrix due, one smoothing sweep: more sweeps run on the
+        // assembled matrix).
+        const bool ebe_settled =
+            s->last_ebe && (opt.element_operator == 2 || !opt.autotune || s->ebe_pref[nrhs] == 1);
+        fill32 = (mixed && !(ebe_settled && opt.smooth_steps <= 1)) ? s->val32 : nullptr;
+        HIPCHECK(launch_fill(s->stream, s->dT, nrhs, s->C, s->ftype, d_bv, s->inst_reg, dguess, s->val64,
+                             fill32, s->b, s->dinv64, s->dinv32, s->x64, s->r64));
+        return MHS_OK;
+    };
+    rc = assemble();
+    if (rc) return rc;
+    if (speculate) {
+        rc = scan();
+        if (rc) {
+            (void)hipStreamSynchronize(s->stream);
+            return rc;
+        }
+        if (bft != s->last_bft) {   // the types changed: assemble again with them
+            rc = upload_types();
+            if (rc == MHS_OK) rc = assemble();
+            if (rc) return rc;
+        }
+    }
     HIPCHECK(hipEventRecord(s->ev_ph[1], s->stream));
 
     // --- preconditioner ---------------------------------------------------------

@@ -173,6 +173,42 @@ def test_multiple_rhs_periodic(pkg, orc):
             assert rel_err(p[d], refs[d].cell_pressure) < TOL_SOL, (prec, d)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_boundary_types_change_between_solves(pkg, orc, precision):
+    """One handle, boundary types changed between solves (x drop, then a y
+    drop with Neumann inflow on z-, then an invalid type, then the x drop
+    again): every solve matches the oracle for its own conditions, the invalid
+    type raises and leaves no solution behind."""
+    g, perm, tm, rm, src, bt_x, bv_x = cases.case_c1()
+    bt_y = np.zeros_like(bt_x)
+    bv_y = np.zeros_like(bv_x)
+    bt_y[g.side == 2] = DIR
+    bv_y[g.side == 2] = 2.0e5
+    bt_y[g.side == 3] = DIR
+    bt_y[g.side == 4] = NEU
+    bv_y[g.side == 4] = -1.0e-9
+    s = _solver(pkg, g, perm, bt=bt_x)
+    for case in ((bt_x, bv_x), (bt_y, bv_y), None, (bt_x, bv_x), (bt_x, bv_x)):
+        if case is None:
+            bad = bt_y.copy()
+            bad[np.flatnonzero(g.side == 4)[0]] = 7
+            with pytest.raises(RuntimeError) as e:
+                s.solve(tm, rm, src, bad, bv_y, residual_tolerance=1e-12, precision=precision)
+            assert e.value.status == pkg.mhs.MHS_ERR_INVALID
+            with pytest.raises(RuntimeError):
+                s.getSolution()
+            continue
+        bt, bv = case
+        ref = orc.solve(g, perm, tm, rm, src, bt, bv, tol=1e-14)
+        rep = s.solve(tm, rm, src, bt, bv, residual_tolerance=1e-12, precision=precision)
+        assert rep.converged
+        p, pi, v = s.getSolution()
+        assert rel_err(pi, ref.face_pressure) < TOL_SOL
+        assert rel_err(p, ref.cell_pressure) < TOL_SOL
+        assert rel_err(v, ref.outflux) < TOL_SOL
+        assert _mass_balance(g, v, src) < 1e-12
+
+
 def test_unknown_linsolver_type_raises(pkg):
     g, perm, tm, rm, src, bt, bv = cases.case_c1()
     s = _solver(pkg, g, perm)
