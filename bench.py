@@ -213,6 +213,8 @@ def _forced_shape(args):
         out["sweep_threads"] = args.sweep_threads
     if args.tail_rows >= 0:
         out["tail_rows"] = args.tail_rows
+    if args.element_operator >= 0:
+        out["element_operator"] = args.element_operator
     return out
 
 
@@ -896,6 +898,9 @@ def main():
                     help="force the post-smoothing sweep's workgroup size (default: autotuned)")
     ap.add_argument("--tail-rows", type=int, default=-1,
                     help="force the one-workgroup tail's level size limit (default: autotuned)")
+    ap.add_argument("--element-operator", type=int, default=-1, choices=[-1, 0, 1, 2],
+                    help="level-0 operator of the FP32 iteration: 0 assembled matrix, 1 autotuned, "
+                         "2 element by element (default: the solver's, 1)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch PCG kernels one by one instead of replaying captured graphs")
     ap.add_argument("--no-compare", action="store_true",
