@@ -148,6 +148,10 @@ typedef struct {
     long long transfer_nnz[2];    /* entries of the smoothed transfer P and of T = A P
                                      (0 when no transfer is smoothed) */
     int smoothed_level;           /* level whose transfer to the next is smoothed, -1: none */
+    int face_matrix;              /* 1: the solve filled the assembled face matrix; 0: it
+                                     applied the operator cell by cell with known smoother
+                                     weights and read none (filled when asked for later,
+                                     e.g. by mhs_get_system) */
 } mhs_solve_report;
 
 typedef struct {

@@ -117,6 +117,10 @@ struct mhs_solver {
     // per nrhs (1..4): whether the autotuner found it faster than the
     // assembled FP32 matrix on this structure (-1: not tried yet)
     int ebe_pref[5] = {-1, -1, -1, -1, -1};
+    // whether val64 (and val32) hold the face matrix of the last assembly: a
+    // solve that applies the operator element by element and needs no face
+    // matrix fills only the vectors; the matrix is filled when asked for
+    bool mat_valid = false;
 
     // multigrid hierarchy
     std::vector<int> ijk;
@@ -263,6 +267,7 @@ void mhs_solver::free_solve_buffers() {
     h_nsys = 0;
     last_bft.clear();
     last_reg.clear();
+    mat_valid = false;
 }
 
 mhs_solver::~mhs_solver() {
@@ -954,6 +959,37 @@ int run_chunk(mhs_solver *s, int nrhs, int chunk, bool use_graph, const MT *val,
     return MHS_OK;
 }
 
+// The face matrix of the last assembly into val64 (and val32), if the solve
+// did not fill it.
+int ensure_matrix(mhs_solver *s) {
+    if (s->mat_valid) return MHS_OK;
+    HIPCHECK(launch_fill(s->stream, s->dT, 1, s->C, s->ftype, nullptr, s->inst_reg, 0, s->val64, s->val32, nullptr,
+                         nullptr, nullptr, nullptr, nullptr, FILL_MAT));
+    // callers may read it with copies outside the solver's stream
+    HIPCHECK(hipStreamSynchronize(s->stream));
+    s->mat_valid = true;
+    return MHS_OK;
+}
+
+// Gather lists of the level-1 A P slots over the FP64 element values (built
+// on first need, kept with the hierarchy).
+int ensure_ap_elements(mhs_solver *s) {
+    DevAmg &A = s->dA;
+    if (A.ap_esrc_ptr) return MHS_OK;
+    if (!s->ebe_ok || A.nlev < 2) return fail(s, MHS_ERR_STATE, "element A P: no element layout or hierarchy");
+    std::vector<int> ptr, src;
+    const std::string e = ebe_ap_lists(s->EL, s->T, s->H.ap, s->H.lev[1].agg, ptr, src);
+    if (!e.empty()) return fail(s, MHS_ERR_INVALID, "element A P: " + e);
+    hipError_t he = hipSuccess;
+    const int *dp = dupload(s, s->amg_bufs, ptr, &he);
+    const int *ds = he == hipSuccess ? dupload(s, s->amg_bufs, src, &he) : nullptr;
+    if (he != hipSuccess) return fail(s, MHS_ERR_ALLOC, std::string("element A P: ") + hipGetErrorString(he));
+    A.ap_esrc_ptr = dp;
+    A.ap_esrc = ds;
+    A.ap_ensrc = (long long)src.size();
+    return MHS_OK;
+}
+
 int read_state(mhs_solver *s, size_t nsys) {
     // sc, act, iters and status: one block on both sides, one copy
     HIPCHECK(hipMemcpyAsync(s->h_sc, s->S.sc, state_bytes(nsys), hipMemcpyDeviceToHost, s->stream));
@@ -993,6 +1029,7 @@ double tridiag_max_eig(const std::vector<double> &a, const std::vector<double> &
 // start), which approaches rho from below.  Done once per hierarchy.
 int estimate_rho(mhs_solver *s) {
     constexpr int kSteps = 12;
+    if (const int rm = ensure_matrix(s)) return rm;
     const DevAmg &A = s->dA;
     const int nl = A.nlev - 1;
     size_t nmax = (size_t)s->T.nfp;
@@ -1298,6 +1335,8 @@ int linear_solve(mhs_solver *s, const mhs_solve_options &opt, int nrhs, bool mix
     // final x64 (a system stops only there), whose |b - A x64| is already in
     // the state; otherwise the true residual is formed once more.
     if (!(mixed && out.converged)) {
+        const int rm = ensure_matrix(s);
+        if (rm) return rm;
         HIPCHECK(launch_resnorm(s->stream, s->dT, nrhs, s->val64, s->b, s->x64, s->R, s->S));
         rc = read_state(s, nsys);
         if (rc) return rc;
@@ -1433,6 +1472,7 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
     const double *binv = binv_use ? binv_use : s->binv, *gflux = gflux_use ? gflux_use : s->gflux;
     const int dguess = opt.linsolver_type != 0;  // AMG types start from the Dirichlet guess (:1521-1532)
     float *fill32 = nullptr;
+    bool filled_matrix = true;
     auto assemble = [&]() -> int {
         // the level-0 operator of the FP32 iteration: cell by cell when the
         // structure allows it and no instance is regularised (the doubled
@@ -1451,8 +1491,23 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
         const bool ebe_settled =
             s->last_ebe && (opt.element_operator == 2 || !opt.autotune || s->ebe_pref[nrhs] == 1);
         fill32 = (mixed && !(ebe_settled && opt.smooth_steps <= 1)) ? s->val32 : nullptr;
+        // No face matrix at all when nothing of this solve reads it: the
+        // operator is applied element by element, the smoother weights are
+        // known (no Lanczos estimate) and the multigrid's A P is summed from
+        // the element values.  It is filled when asked for afterwards.
+        const bool will_amg =
+            s->has_amg && (opt.precond == 1 || (opt.precond == -1 && opt.linsolver_type != 0));
+        const bool rho_known = opt.smoother_weight > 0.0 ||
+                               (!s->rho.empty() && s->rho_gen == s->perm_gen &&
+                                s->rho_aniso == (binv_use != nullptr));
+        const bool no_matrix = mixed && ebe_settled && !fill32 && (!will_amg || rho_known);
+        if (no_matrix && will_amg) {
+            const int ra = ensure_ap_elements(s);
+            if (ra) return ra;
+        }
         HIPCHECK(launch_fill(s->stream, s->dT, nrhs, s->C, s->ftype, d_bv, s->inst_reg, dguess, s->val64,
-                             fill32, s->b, s->dinv64, s->dinv32, s->x64, s->r64));
+                             fill32, s->b, s->dinv64, s->dinv32, s->x64, s->r64, no_matrix ? FILL_VEC : FILL_ALL));
+        s->mat_valid = filled_matrix = !no_matrix;
         return MHS_OK;
     };
     rc = assemble();
@@ -1499,11 +1554,13 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
             s->rho.clear();
             HIPCHECK(launch_fill(s->stream, s->dT, nrhs, s->C, s->ftype, d_bv, s->inst_reg, dguess, s->val64,
                                  fill32, s->b, s->dinv64, s->dinv32, s->x64, s->r64));
+            s->mat_valid = filled_matrix = true;
         }
         if (amg) {
             s->dA.alpha = opt.prolongate_factor;
             s->dA.nu = std::max(1, opt.smooth_steps);
-            HIPCHECK(launch_amg_setup(s->stream, s->dT, s->dA, s->val64, mixed));
+            // A P from the element values when no face matrix was filled
+            HIPCHECK(launch_amg_setup(s->stream, s->dT, s->dA, s->val64, mixed, s->mat_valid ? nullptr : s->dE.val64));
             // Jacobi weights: the caller's, or 4 / (3 rho(D^-1 A)) per level with
             // rho estimated on the first multigrid solve of this hierarchy
             if (opt.smoother_weight > 0.0) {
@@ -1626,6 +1683,7 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
         }
         rep->transfer_nnz[0] = rep->transfer_nnz[1] = 0;
         rep->smoothed_level = -1;
+        rep->face_matrix = filled_matrix ? 1 : 0;
         if (amg)
             for (int l = 0; l < s->H.nlev(); ++l)
                 if (s->H.lev[l].sa_out) {
@@ -2050,6 +2108,8 @@ int mhs_time_kernel(mhs_solver *s, int which, int reps, double *avg_ms) {
     if (which != 0 && which != 1) return fail(s, MHS_ERR_INVALID, "time_kernel: unknown kernel id");
     const DevEbe *E = (which == 0 && s->cap_mixed) ? cur_ebe(s) : nullptr;
     HIPCHECK(hipSetDevice(s->device));
+    if (!E)
+        if (const int rm = ensure_matrix(s)) return rm;
     const size_t nsys = (size_t)s->T.ninst * s->cap_nrhs;
     std::vector<int> ones(nsys, 1);
     HIPCHECK(hipMemcpyAsync(s->S.act, ones.data(), nsys * sizeof(int), hipMemcpyHostToDevice, s->stream));
@@ -2096,6 +2156,7 @@ int mhs_get_system(mhs_solver *s, int k, int *row_ptr, int *col, double *val, do
     for (int f = 0; f < T.nf_total; ++f) count += T.row_len[T.face_row[f]];
     if (nnz) *nnz = count;
     if (!col) return MHS_OK;
+    if (const int rm = ensure_matrix(s)) return rm;
     std::vector<double> v(T.slice_ptr[T.nslices]);
     HIPCHECK(hipMemcpy(v.data(), s->val64, sizeof(double) * v.size(), hipMemcpyDeviceToHost));
     std::vector<double> bb((size_t)T.nfp * nrhs);

@@ -47,6 +47,54 @@ int greedy_colours(int nc, const int *facepos, const int *nb, std::vector<int> &
     return ncol;
 }
 
+std::string ebe_ap_lists(const EbeLayout &E, const Topology &T, const ApPattern &ap, const std::vector<int> &agg,
+                         std::vector<int> &ptr, std::vector<int> &src) {
+    const long long slots = (long long)ap.col.size();
+    if (slots >= 0x7fffffffLL || E.nval >= 0x7fffffffLL) return "too many values for 32-bit lists";
+    // slot of (row f, aggregate J): the row's slots are few (about 4)
+    auto slot_of = [&](int f, int J) -> long long {
+        const int sl = f / kSlice;
+        const long long base = ap.slice_ptr[sl] + f % kSlice;
+        const int w = int((ap.slice_ptr[sl + 1] - ap.slice_ptr[sl]) / kSlice);
+        for (int e = 0; e < w; ++e) {
+            const long long q = base + (long long)e * kSlice;
+            if (ap.col[q] == J && ap.src_ptr[q + 1] > ap.src_ptr[q]) return q;
+        }
+        return -1;
+    };
+    ptr.assign(slots + 1, 0);
+    // two passes: counts, then the lists in slot order
+    for (int pass = 0; pass < 2; ++pass) {
+        std::vector<int> fill;
+        if (pass == 1) {
+            for (long long q = 0; q < slots; ++q) ptr[q + 1] += ptr[q];
+            src.assign(ptr[slots], 0);
+            fill.assign(ptr.begin(), ptr.end() - 1);
+        }
+        for (int c = 0; c < T.nc; ++c) {
+            const int s = E.cell_seg[c], l = E.cell_local[c];
+            const int f0 = T.facepos[c], n = T.facepos[c + 1] - f0, np = n * (n + 1) / 2;
+            for (int i = 0; i < n; ++i) {
+                const int f = T.hf_dof[f0 + i];
+                if (agg[f] < 0) continue;
+                for (int j = 0; j < n; ++j) {
+                    const int J = agg[T.hf_dof[f0 + j]];
+                    if (J < 0) continue;
+                    const long long q = slot_of(f, J);
+                    if (q < 0) return "a cell entry outside the A P pattern";
+                    if (pass == 0) {
+                        ++ptr[q + 1];
+                    } else {
+                        const int pq = i <= j ? pidx(i, j, n) : pidx(j, i, n);
+                        src[fill[q]++] = int(E.seg_val[s] + ebe_slot(l, pq, np));
+                    }
+                }
+            }
+        }
+    }
+    return "";
+}
+
 std::string build_ebe(EbeLayout &E, const Topology &T, const int *ijk) {
     E = EbeLayout();
     if (T.has_periodic) return "periodic faces";

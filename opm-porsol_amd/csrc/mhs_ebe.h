@@ -18,6 +18,7 @@
 #include <string>
 #include <vector>
 
+#include "mhs_amg.h"
 #include "mhs_structure.h"
 
 namespace mhs {
@@ -58,6 +59,15 @@ struct EbeLayout {
 // their checkerboard colour, which is also how the face rows are classed.
 int greedy_colours(int nc, const int *facepos, const int *nb, std::vector<int> &colour,
                    const int *ijk = nullptr);
+
+// Gather lists of the A P slots (face rows by level-1 aggregates agg, pattern
+// ap) over the element values: slot (f, J) sums the entries (i, j) of every
+// cell whose local face i is row f and whose local face j has its row in
+// aggregate J.  With the Dirichlet rules inside the element values this is
+// the face matrix's A P.  ptr[slots + 1], src: positions in the packed values.
+// Returns "" or why the layout does not cover the pattern.
+std::string ebe_ap_lists(const EbeLayout &E, const Topology &T, const ApPattern &ap, const std::vector<int> &agg,
+                         std::vector<int> &ptr, std::vector<int> &src);
 
 // Builds the layout for a structure; returns an empty string, or the reason
 // why the element-by-element operator does not apply (periodic faces, whose

@@ -79,6 +79,10 @@ constexpr int kEbeColours = 8;
 // component is 64 consecutive elements, and all components of its cells lie
 // in one contiguous ncomp * 64-element range, so the loads of a wavefront stay
 // within a page instead of striding across the whole segment.
+// Packed upper-triangle index of entry (i, j), i <= j, of an n x n local matrix.
+__host__ __device__ constexpr int pidx(int i, int j, int n) {
+    return i * n - (i * (i - 1)) / 2 + (j - i);
+}
 __host__ __device__ constexpr long long ebe_slot(int local, int q, int ncomp) {
     return (long long)(local >> 6) * ((long long)ncomp << 6) + ((long long)q << 6) + (local & 63);
 }
@@ -154,6 +158,10 @@ struct DevAmg {
     long long ap_slots, ap_nsrc;
     double *ap64;
     float *ap32;
+    // the same slots as sums of the FP64 element values S'_c (element-by-
+    // element layout), for solves that fill no face matrix (null: none)
+    const int *ap_esrc_ptr, *ap_esrc;
+    long long ap_ensrc;
 };
 // DevAmg is larger than the kernel-argument limit: kernels that need the
 // whole hierarchy read a copy in device memory.
@@ -187,10 +195,14 @@ hipError_t launch_assemble(hipStream_t st, const DevTopo &T, int maxn, int nrhs,
 hipError_t launch_ebe_spmv(hipStream_t st, int maxn, int nrhs, const DevEbe &E, const float *p, float *q,
                            DevScal S);
 
+// Face rows from the cell systems (k_fill).  FILL_ALL: matrix values and the
+// vectors (right-hand side, Jacobi diagonal, initial iterate and its
+// residual); FILL_VEC: the vectors only; FILL_MAT: the matrix values only.
+enum FillMode { FILL_ALL = 0, FILL_VEC = 1, FILL_MAT = 2 };
 hipError_t launch_fill(hipStream_t st, const DevTopo &T, int nrhs, const DevCellSys &C,
                        const uint8_t *ftype, const double *bcval, const int *inst_reg,
                        int dirichlet_guess, double *val64, float *val32, double *b,
-                       double *dinv64, float *dinv32, double *x0, double *r0);
+                       double *dinv64, float *dinv32, double *x0, double *r0, int mode = FILL_ALL);
 
 // One Jacobi-PCG iteration (spmv_pq, update_xr, update_p; in FP32 also the
 // fused FP64 residual replacement, which needs val64, b and x64).
@@ -233,7 +245,7 @@ hipError_t launch_resnorm(hipStream_t st, const DevTopo &T, int nrhs, const doub
 // smoother weights are known (after launch_amg_setup).
 hipError_t launch_amg_combine(hipStream_t st, const DevAmg &A, bool mixed);
 hipError_t launch_amg_setup(hipStream_t st, const DevTopo &T, const DevAmg &A, const double *val64,
-                            bool mixed);
+                            bool mixed, const double *elem64 = nullptr);
 // Lanczos on D^-1/2 A_l D^-1/2 (FP64) for the Jacobi weight of level l:
 // start vector w with |w|^2 partials; one step v = scale w, y = D^-1/2 A
 // D^-1/2 v with <v, y> partials; w = y - a v - b vp with |w|^2 partials.

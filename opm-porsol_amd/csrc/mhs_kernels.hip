@@ -29,10 +29,6 @@ namespace mhs {
 
 namespace {
 
-__host__ __device__ constexpr int pidx(int i, int j, int n) {  // i <= j
-    return i * n - (i * (i - 1)) / 2 + (j - i);
-}
-
 struct FinArgs {
     double tol2;
     int init;
@@ -480,10 +476,14 @@ __global__ __launch_bounds__(64) void k_assemble(DevTopo T, const double *__rest
 // the initial iterate and its residual r0 = b - A x0.  x0 is nonzero only on
 // Dirichlet rows, which hold nothing but their diagonal, and Dirichlet columns
 // are dropped from every other row, so (A x0)_f = diag_f x0_f: the PCG start
-// needs no product with the face matrix.  With ROW_LDS each thread sums its row in its own column of LDS (max_w
-// doubles, stride TB) and writes every stored entry once; otherwise the row
-// is summed in place in val64 (rows too wide for LDS).
-template <int NRHS, int TB, bool ROW_LDS>
+// needs no product with the face matrix.  With ROW_LDS each thread sums its
+// row in its own column of LDS (max_w doubles, stride TB) and writes every
+// stored entry once; otherwise the row is summed in place in val64 (rows too
+// wide for LDS).  FM: FILL_ALL writes the vectors and the matrix, FILL_VEC the
+// vectors only (the diagonal is summed alone; a solve that applies the
+// operator element by element reads no face matrix), FILL_MAT the matrix only
+// (a later request for the face matrix of that solve).
+template <int NRHS, int TB, bool ROW_LDS, int FM>
 __global__ __launch_bounds__(TB) void k_fill(DevTopo T, DevCellSys C,
                                              const uint8_t *__restrict__ ftype,
                                              const double *__restrict__ bcval,
@@ -495,9 +495,10 @@ __global__ __launch_bounds__(TB) void k_fill(DevTopo T, DevCellSys C,
                                              float *__restrict__ dinv32,
                                              double *__restrict__ x0,
                                              double *__restrict__ r0) {
+    constexpr bool MAT = FM != FILL_VEC, VEC = FM != FILL_MAT;
     extern __shared__ double rowbuf[];
     // contiguous row ranges per XCD: the cells a row range touches (up to a
-    // layer of cells away) stay in that XCD's L2 for the rows that reuse them
+    // layer of cells away) stay in that XCD's L2 until their other rows come
     const int f = xcd_block(gridDim.x) * TB + threadIdx.x;
     if (f >= T.nfp) return;
     const int slice = f >> 6, lane = f & 63;
@@ -505,7 +506,8 @@ __global__ __launch_bounds__(TB) void k_fill(DevTopo T, DevCellSys C,
     const int w = (int)((T.slice_ptr[slice + 1] - T.slice_ptr[slice]) >> 6);
     double *row = ROW_LDS ? rowbuf + threadIdx.x : val64 + base;
     constexpr long long rs = ROW_LDS ? TB : 64;    // stride between a row's entries
-    for (int e = 0; e < w; ++e) row[rs * e] = 0.0;
+    if constexpr (MAT)
+        for (int e = 0; e < w; ++e) row[rs * e] = 0.0;
     const int h0 = T.dof_hf[f];
     const long long dpos = rs * T.dslot[f];
     double bk[NRHS], xk[NRHS];
@@ -513,19 +515,22 @@ __global__ __launch_bounds__(TB) void k_fill(DevTopo T, DevCellSys C,
     for (int k = 0; k < NRHS; ++k) bk[k] = xk[k] = 0.0;
     double diag;
     if (h0 < 0) {  // padding row: identity
-        row[dpos] = 1.0;
+        if constexpr (MAT) row[dpos] = 1.0;
         diag = 1.0;
     } else if (ftype[h0] == FT_DIRICHLET) {
         const int c = T.hf_cell[h0], f0 = T.facepos[c], n = T.facepos[c + 1] - f0, r = h0 - f0;
         const double a = C.schur[(size_t)pidx(r, r, n) * T.nc + c];
-        row[dpos] = a;
+        if constexpr (MAT) row[dpos] = a;
         diag = a;
+        if constexpr (VEC) {
 #pragma unroll
-        for (int k = 0; k < NRHS; ++k) {
-            bk[k] = C.rr[((size_t)k * T.max_ncf + r) * T.nc + c];
-            xk[k] = dguess ? bk[k] / a : 0.0;
+            for (int k = 0; k < NRHS; ++k) {
+                bk[k] = C.rr[((size_t)k * T.max_ncf + r) * T.nc + c];
+                xk[k] = dguess ? bk[k] / a : 0.0;
+            }
         }
     } else {
+        double dsum = 0.0;   // the diagonal alone, in the order the row sums it
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
             // the contribution's cell and local face stored per row (coalesced),
@@ -533,26 +538,33 @@ __global__ __launch_bounds__(TB) void k_fill(DevTopo T, DevCellSys C,
             const int c = T.row_cell[2 * (size_t)f + t];
             if (c < 0) continue;
             const int f0 = T.facepos[c], n = T.facepos[c + 1] - f0, r = T.row_loc[2 * (size_t)f + t];
-            const uint8_t *sm = T.slotmap + (size_t)r * T.max_ncf * T.nc + c;   // entry j at sm[j * nc]
-            // the cell's row r of S_c, 8 faces at a time: all loads of a batch
-            // are issued before the LDS sums
-            for (int j0 = 0; j0 < n; j0 += 8) {
-                double v[8];
-                int slot[8];
+            if constexpr (MAT) {
+                const uint8_t *sm = T.slotmap + (size_t)r * T.max_ncf * T.nc + c;   // entry j at sm[j * nc]
+                // the cell's row r of S_c, 8 faces at a time: all loads of a
+                // batch are issued before the LDS sums
+                for (int j0 = 0; j0 < n; j0 += 8) {
+                    double v[8];
+                    int slot[8];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int j = j0 + u;
-                    const bool in = j < n;
-                    const bool dir = in && ftype[f0 + j] == FT_DIRICHLET;
-                    slot[u] = in ? (int)sm[(size_t)j * T.nc] : 0;
-                    v[u] = (in && !dir) ? C.schur[(size_t)(r <= j ? pidx(r, j, n) : pidx(j, r, n)) * T.nc + c] : 0.0;
+                    for (int u = 0; u < 8; ++u) {
+                        const int j = j0 + u;
+                        const bool in = j < n;
+                        const bool dir = in && ftype[f0 + j] == FT_DIRICHLET;
+                        slot[u] = in ? (int)sm[(size_t)j * T.nc] : 0;
+                        v[u] = (in && !dir) ? C.schur[(size_t)(r <= j ? pidx(r, j, n) : pidx(j, r, n)) * T.nc + c]
+                                            : 0.0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
+                        if (j0 + u < n) row[rs * slot[u]] += v[u];
                 }
-#pragma unroll
-                for (int u = 0; u < 8; ++u)
-                    if (j0 + u < n) row[rs * slot[u]] += v[u];
+            } else {
+                dsum += C.schur[(size_t)pidx(r, r, n) * T.nc + c];
             }
+            if constexpr (VEC) {
 #pragma unroll
-            for (int k = 0; k < NRHS; ++k) bk[k] += C.rr[((size_t)k * T.max_ncf + r) * T.nc + c];
+                for (int k = 0; k < NRHS; ++k) bk[k] += C.rr[((size_t)k * T.max_ncf + r) * T.nc + c];
+            }
         }
         const int fp = T.ptn[f];
         if (fp >= 0) {  // periodic constraint a(x_f - x_partner) = a_f pd_f - a_p pd_p (:1838-1869)
@@ -561,30 +573,44 @@ __global__ __launch_bounds__(TB) void k_fill(DevTopo T, DevCellSys C,
             const int c1 = T.hf_cell[hp], f01 = T.facepos[c1], n1 = T.facepos[c1 + 1] - f01;
             const double a0 = C.schur[(size_t)pidx(h0 - f00, h0 - f00, n0) * T.nc + c0];
             const double a1 = C.schur[(size_t)pidx(hp - f01, hp - f01, n1) * T.nc + c1];
-            row[dpos] += a0 + a1;
-            row[rs * T.pslot[f]] -= a0 + a1;
+            if constexpr (MAT) {
+                row[dpos] += a0 + a1;
+                row[rs * T.pslot[f]] -= a0 + a1;
+            } else {
+                dsum += a0 + a1;
+            }
+            if constexpr (VEC) {
 #pragma unroll
-            for (int k = 0; k < NRHS; ++k)
-                bk[k] += a0 * bcval[(size_t)k * T.nhf + h0] - a1 * bcval[(size_t)k * T.nhf + hp];
+                for (int k = 0; k < NRHS; ++k)
+                    bk[k] += a0 * bcval[(size_t)k * T.nhf + h0] - a1 * bcval[(size_t)k * T.nhf + hp];
+            }
         }
         const int s = T.block_seg[f / kRowBlock];
-        if (inst_reg[s] && f == T.inst_regrow[s]) row[dpos] *= 2.0;  // S[0][0] *= 2 (:1398-1400)
-        diag = row[dpos];
+        if (inst_reg[s] && f == T.inst_regrow[s]) {  // S[0][0] *= 2 (:1398-1400)
+            if constexpr (MAT) row[dpos] *= 2.0;
+            else dsum *= 2.0;
+        }
+        if constexpr (MAT) diag = row[dpos];
+        else diag = dsum;
     }
+    if constexpr (VEC) {
 #pragma unroll
-    for (int k = 0; k < NRHS; ++k) {
-        b[(size_t)f * NRHS + k] = bk[k];
-        x0[(size_t)f * NRHS + k] = xk[k];
-        r0[(size_t)f * NRHS + k] = bk[k] - diag * xk[k];
+        for (int k = 0; k < NRHS; ++k) {
+            b[(size_t)f * NRHS + k] = bk[k];
+            x0[(size_t)f * NRHS + k] = xk[k];
+            r0[(size_t)f * NRHS + k] = bk[k] - diag * xk[k];
+        }
+        const double di = 1.0 / diag;
+        if (dinv64) dinv64[f] = di;
+        if (dinv32) dinv32[f] = (float)di;
     }
-    const double di = 1.0 / diag;
-    if (dinv64) dinv64[f] = di;
-    for (int e = 0; e < w; ++e) {
-        const double v = row[rs * e];
-        if (ROW_LDS) val64[base + 64LL * e] = v;
-        if (val32) val32[base + 64LL * e] = (float)v;
+    if constexpr (MAT) {
+        for (int e = 0; e < w; ++e) {
+            const double v = row[rs * e];
+            if (ROW_LDS) val64[base + 64LL * e] = v;
+            if (val32) val32[base + 64LL * e] = (float)v;
+        }
     }
-    if (dinv32) dinv32[f] = (float)di;
 }
 
 // ---------------------------------------------------------------------------
@@ -2875,21 +2901,30 @@ hipError_t launch_assemble(hipStream_t st, const DevTopo &T, int maxn, int nrhs,
 hipError_t launch_fill(hipStream_t st, const DevTopo &T, int nrhs, const DevCellSys &C,
                        const uint8_t *ftype, const double *bcval, const int *inst_reg,
                        int dguess, double *val64, float *val32, double *b, double *dinv64,
-                       float *dinv32, double *x0, double *r0) {
+                       float *dinv32, double *x0, double *r0, int mode) {
     // rows of up to 24 entries: 256 threads and 48 KB of LDS; up to 128: 64
     // threads and 64 KB; wider rows are summed in place in val64
     const size_t w = (size_t)std::max(T.max_w, 1);
     return with_nrhs(nrhs, [&](auto nr) {
         constexpr int NR = decltype(nr)::value;
-        if (w <= 24)
-            k_fill<NR, 256, true><<<T.nfp / 256, 256, w * 256 * sizeof(double), st>>>(
-                T, C, ftype, bcval, inst_reg, dguess, val64, val32, b, dinv64, dinv32, x0, r0);
-        else if (w <= 128)
-            k_fill<NR, 64, true><<<T.nfp / 64, 64, w * 64 * sizeof(double), st>>>(
-                T, C, ftype, bcval, inst_reg, dguess, val64, val32, b, dinv64, dinv32, x0, r0);
-        else
-            k_fill<NR, 256, false><<<T.nfp / 256, 256, 0, st>>>(T, C, ftype, bcval, inst_reg, dguess, val64,
-                                                                 val32, b, dinv64, dinv32, x0, r0);
+        auto go = [&](auto fm) {
+            constexpr int FM = decltype(fm)::value;
+            if (FM == FILL_VEC)
+                k_fill<NR, 256, false, FM><<<T.nfp / 256, 256, 0, st>>>(
+                    T, C, ftype, bcval, inst_reg, dguess, val64, val32, b, dinv64, dinv32, x0, r0);
+            else if (w <= 24)
+                k_fill<NR, 256, true, FM><<<T.nfp / 256, 256, w * 256 * sizeof(double), st>>>(
+                    T, C, ftype, bcval, inst_reg, dguess, val64, val32, b, dinv64, dinv32, x0, r0);
+            else if (w <= 128)
+                k_fill<NR, 64, true, FM><<<T.nfp / 64, 64, w * 64 * sizeof(double), st>>>(
+                    T, C, ftype, bcval, inst_reg, dguess, val64, val32, b, dinv64, dinv32, x0, r0);
+            else
+                k_fill<NR, 256, false, FM><<<T.nfp / 256, 256, 0, st>>>(
+                    T, C, ftype, bcval, inst_reg, dguess, val64, val32, b, dinv64, dinv32, x0, r0);
+        };
+        if (mode == FILL_VEC) go(IC<FILL_VEC>{});
+        else if (mode == FILL_MAT) go(IC<FILL_MAT>{});
+        else go(IC<FILL_ALL>{});
     });
 }
 
@@ -3010,14 +3045,18 @@ hipError_t launch_resnorm(hipStream_t st, const DevTopo &T, int nrhs, const doub
 
 
 hipError_t launch_amg_setup(hipStream_t st, const DevTopo &T, const DevAmg &A, const double *val64,
-                            bool mixed) {
-    // A P of the level-1 aggregates first: level 1's Galerkin sums its slots
-    if (A.ap_slots > 0)
-        with_lanes(lanes_per_row(A.ap_nsrc, (int)A.ap_slots), [&](auto g) {
+                            bool mixed, const double *elem64) {
+    // A P of the level-1 aggregates first: level 1's Galerkin sums its slots.
+    // elem64: from the FP64 element values (no face matrix filled)
+    if (A.ap_slots > 0) {
+        const bool el = elem64 != nullptr;
+        const int *sp = el ? A.ap_esrc_ptr : A.ap_src_ptr, *sl = el ? A.ap_esrc : A.ap_src;
+        with_lanes(lanes_per_row(el ? A.ap_ensrc : A.ap_nsrc, (int)A.ap_slots), [&](auto g) {
             constexpr int Gv = decltype(g)::value;
             k_galerkin<Gv><<<row_blocks((int)A.ap_slots, Gv), 256, 0, st>>>(
-                (int)A.ap_slots, A.ap_src_ptr, A.ap_src, val64, A.ap64, mixed ? A.ap32 : nullptr);
+                (int)A.ap_slots, sp, sl, el ? elem64 : val64, A.ap64, mixed ? A.ap32 : nullptr);
         });
+    }
     for (int l = 1; l < A.nlev; ++l) {
         const DevAmgLevel &L = A.lev[l];
         const DevAmgLevel &F = A.lev[l - 1];

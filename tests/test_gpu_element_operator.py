@@ -134,3 +134,58 @@ def test_autotuner_chooses_the_operator(pkg, orc):
             else:
                 assert rep.tune_ms == 0.0 and rep.element_operator == chosen   # kept for the structure
             assert rel_err(s.getSolution()[2], ref.outflux) < 1e-8
+
+
+@pytest.mark.parametrize("precond", ["amg", "jacobi"])
+def test_solve_without_face_matrix(pkg, orc, precond):
+    """Once the element operator is settled and the smoother weights are
+    known, a mixed solve fills no face matrix (multigrid A P from the element
+    values): it matches the oracle and a solve that filled the matrix, the
+    matrix asked for afterwards equals the one an FP64 solve assembles, and a
+    solve stopped early still reports its true FP64 residual."""
+    g, perm, tm, rm, src, bt, bv = cases.case_c1()
+    ref = orc.solve(g, perm, tm, rm, src, bt, bv, tol=1e-14)
+    s = pkg.HybridSolver(0)
+    s.init(g, perm, (0.0, 0.0, 0.0), bt, None)
+    kw = dict(residual_tolerance=1e-12, precision="mixed", precond=precond, element_operator=2)
+    first = s.solve(tm, rm, src, bt, bv, **kw)
+    assert first.converged and first.element_operator == 1
+    v_first = s.getSolution()[2].copy()
+    # different mobilities, then the original ones again: no matrix either time
+    rep = s.solve(2.0 * tm, rm, src, bt, bv, **kw)
+    assert rep.converged and rep.face_matrix == 0
+    rep = s.solve(tm, rm, src, bt, bv, **kw)
+    assert rep.converged and rep.face_matrix == 0 and rep.reduction <= 1e-12
+    p, pi, v = s.getSolution()
+    assert rel_err(v, ref.outflux) < 1e-8 and rel_err(p, ref.cell_pressure) < 1e-8
+    assert rel_err(pi, ref.face_pressure) < 1e-8
+    assert abs(rep.iterations - first.iterations) <= 1
+    assert rel_err(v, v_first) < 1e-9
+    # the face matrix on request: the one a fresh FP64 solve fills
+    rp, cols, vals, rhs = s.system(0)
+    s64 = pkg.HybridSolver(0)
+    s64.init(g, perm, (0.0, 0.0, 0.0), bt, None)
+    r64 = s64.solve(tm, rm, src, bt, bv, residual_tolerance=1e-12, precision="fp64", precond=precond)
+    assert r64.face_matrix == 1
+    rp64, cols64, vals64, rhs64 = s64.system(0)
+    np.testing.assert_array_equal(rp, rp64)
+    np.testing.assert_array_equal(cols, cols64)
+    np.testing.assert_array_equal(vals, vals64)
+    np.testing.assert_array_equal(rhs, rhs64)
+    # stopped early: the report's reduction is the true FP64 residual of the
+    # iterate, ||b - A x|| / ||b - A x0|| with the Dirichlet guess x0
+    with pytest.raises(RuntimeError):
+        s.solve(tm, rm, src, bt, bv, linsolver_maxit=3, **kw)
+    red = s.report.reduction
+    x = s.getSolution()[1]
+    import scipy.sparse as sp
+    n = len(rp64) - 1
+    A = sp.csr_matrix((vals64, cols64, rp64), shape=(n, n))
+    nz = np.diff(sp.csr_matrix(A).indptr)
+    Ad = sp.csr_matrix(A)
+    Ad.eliminate_zeros()
+    dirichlet = np.diff(Ad.indptr) == 1
+    x0 = np.where(dirichlet, rhs64 / A.diagonal(), 0.0)
+    true = np.linalg.norm(rhs64 - A @ x) / np.linalg.norm(rhs64 - A @ x0)
+    assert len(nz) == n
+    assert red > 1e-12 and abs(red - true) <= 1e-6 * true
