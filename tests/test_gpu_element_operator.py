@@ -181,11 +181,9 @@ def test_solve_without_face_matrix(pkg, orc, precond):
     import scipy.sparse as sp
     n = len(rp64) - 1
     A = sp.csr_matrix((vals64, cols64, rp64), shape=(n, n))
-    nz = np.diff(sp.csr_matrix(A).indptr)
-    Ad = sp.csr_matrix(A)
+    Ad = A.copy()
     Ad.eliminate_zeros()
-    dirichlet = np.diff(Ad.indptr) == 1
+    dirichlet = np.diff(Ad.indptr) == 1       # rows that hold only their diagonal
     x0 = np.where(dirichlet, rhs64 / A.diagonal(), 0.0)
     true = np.linalg.norm(rhs64 - A @ x) / np.linalg.norm(rhs64 - A @ x0)
-    assert len(nz) == n
     assert red > 1e-12 and abs(red - true) <= 1e-6 * true
