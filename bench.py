@@ -1063,7 +1063,15 @@ def main():
                                              "tail_rows")},
         "level0_operator": ("element by element (FP32 local matrices, %d colours)" % rep["colours"]) if ebe
                            else "assembled face matrix",
+        "face_matrix_filled": bool(rep["face_matrix"]),
     }
+    if rep["precond"] == 1:
+        config["multigrid"] = {"level_rows": [int(x) for x in rep["level_rows"] if x],
+                               "level_nnz": [int(x) for x in rep["level_nnz"] if x],
+                               "ap_nnz": int(rep["ap_nnz"]), "smoothed_level": int(rep["smoothed_level"]),
+                               "transfer_nnz": [int(x) for x in rep["transfer_nnz"]],
+                               "smoother_weights": rep["smoother_weights"],
+                               "coarsening": rep["coarsening"]}
     kkey = "kernels_ebe" if (ebe and rep["precond"] == 1) else ("kernels" + tag)
     if kkey in B:
         # the per-iteration part of alg_bytes_per_solve, kernel by kernel
