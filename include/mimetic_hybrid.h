@@ -77,7 +77,10 @@ typedef struct {
                                   logical cell indices).  All converge to the
                                   reference's discrete solution. */
     int verbosity;             /* linsolver_verbosity (:663) */
-    int same_matrix;           /* (:665) accepted; the matrix is always rebuilt */
+    int same_matrix;           /* (:665, :1491) the caller's promise that the matrix is the
+                                  previous solve's: the multigrid coarse levels, smoothed
+                                  transfer and coarsest inverses are kept (the face system
+                                  is still assembled; the PCG uses it) */
     double prolongate_factor;  /* (:667) multigrid coarse-correction factor (default 1.6) */
     int smooth_steps;          /* (:668) multigrid smoothing sweeps before and after
                                   the coarse correction */
@@ -152,6 +155,7 @@ typedef struct {
                                      applied the operator cell by cell with known smoother
                                      weights and read none (filled when asked for later,
                                      e.g. by mhs_get_system) */
+    int mg_setup_reused;          /* 1: same_matrix kept the previous multigrid set-up */
 } mhs_solve_report;
 
 typedef struct {

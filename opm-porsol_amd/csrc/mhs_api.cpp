@@ -121,6 +121,11 @@ struct mhs_solver {
     // solve that applies the operator element by element and needs no face
     // matrix fills only the vectors; the matrix is filled when asked for
     bool mat_valid = false;
+    // multigrid numeric set-up last done for this permeability generation, in
+    // mixed precision (FP32 copies written too) or not, with tensor mobilities
+    // or not (-1: none since the hierarchy was built)
+    long long setup_gen = -1;
+    bool setup_mixed = false, setup_aniso = false;
 
     // multigrid hierarchy
     std::vector<int> ijk;
@@ -223,6 +228,7 @@ void mhs_solver::free_amg() {
     amg_planned = false;
     dA = DevAmg{};
     dA_copied.clear();      // a new hierarchy has a new device descriptor
+    setup_gen = -1;
     has_amg = false;
     H = AmgHierarchy();
     rho.clear();
@@ -1545,6 +1551,7 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
         return fail(s, MHS_ERR_INVALID, "solve: unknown preconditioner");
     }
     bool rho_fresh = false;   // the weights were estimated on this very system
+    bool mg_reused = false;   // same_matrix kept the multigrid set-up of the previous solve
     double tune_ms = 0.0;
     LinearResult lr;
     for (int attempt = 0;; ++attempt) {
@@ -1559,8 +1566,20 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
         if (amg) {
             s->dA.alpha = opt.prolongate_factor;
             s->dA.nu = std::max(1, opt.smooth_steps);
+            // same_matrix (the caller's promise, :1491): the coarse levels,
+            // the smoothed transfer and the coarsest inverses of the previous
+            // solve are kept, as the reference keeps its AMG hierarchy; A P,
+            // the face level's part of the cycle, follows the new assembly
+            mg_reused = opt.same_matrix && attempt == 0 && s->setup_gen == s->perm_gen &&
+                        (s->setup_mixed || !mixed) && s->setup_aniso == (binv_use != nullptr);
             // A P from the element values when no face matrix was filled
-            HIPCHECK(launch_amg_setup(s->stream, s->dT, s->dA, s->val64, mixed, s->mat_valid ? nullptr : s->dE.val64));
+            HIPCHECK(launch_amg_setup(s->stream, s->dT, s->dA, s->val64, mixed, s->mat_valid ? nullptr : s->dE.val64,
+                                      mg_reused));
+            if (!mg_reused) {
+                s->setup_gen = s->perm_gen;
+                s->setup_mixed = mixed;
+                s->setup_aniso = binv_use != nullptr;
+            }
             // Jacobi weights: the caller's, or 4 / (3 rho(D^-1 A)) per level with
             // rho estimated on the first multigrid solve of this hierarchy
             if (opt.smoother_weight > 0.0) {
@@ -1588,7 +1607,7 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
                 HIPCHECK(hipMemcpy(const_cast<DevAmg *>(s->dA.dev), &s->dA, sizeof(DevAmg), hipMemcpyHostToDevice));
                 s->dA_copied.assign(da, da + sizeof(DevAmg));
             }
-            HIPCHECK(launch_amg_combine(s->stream, s->dA, mixed));
+            if (!mg_reused) HIPCHECK(launch_amg_combine(s->stream, s->dA, mixed));
         }
 
         // --- launch shapes -------------------------------------------------------
@@ -1684,6 +1703,7 @@ int solve_impl(mhs_solver *s, const double *totmob, const double *rhomob, const 
         rep->transfer_nnz[0] = rep->transfer_nnz[1] = 0;
         rep->smoothed_level = -1;
         rep->face_matrix = filled_matrix ? 1 : 0;
+        rep->mg_setup_reused = mg_reused ? 1 : 0;
         if (amg)
             for (int l = 0; l < s->H.nlev(); ++l)
                 if (s->H.lev[l].sa_out) {

@@ -245,7 +245,7 @@ hipError_t launch_resnorm(hipStream_t st, const DevTopo &T, int nrhs, const doub
 // smoother weights are known (after launch_amg_setup).
 hipError_t launch_amg_combine(hipStream_t st, const DevAmg &A, bool mixed);
 hipError_t launch_amg_setup(hipStream_t st, const DevTopo &T, const DevAmg &A, const double *val64,
-                            bool mixed, const double *elem64 = nullptr);
+                            bool mixed, const double *elem64 = nullptr, bool ap_only = false);
 // Lanczos on D^-1/2 A_l D^-1/2 (FP64) for the Jacobi weight of level l:
 // start vector w with |w|^2 partials; one step v = scale w, y = D^-1/2 A
 // D^-1/2 v with <v, y> partials; w = y - a v - b vp with |w|^2 partials.

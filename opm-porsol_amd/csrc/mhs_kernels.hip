@@ -3045,7 +3045,7 @@ hipError_t launch_resnorm(hipStream_t st, const DevTopo &T, int nrhs, const doub
 
 
 hipError_t launch_amg_setup(hipStream_t st, const DevTopo &T, const DevAmg &A, const double *val64,
-                            bool mixed, const double *elem64) {
+                            bool mixed, const double *elem64, bool ap_only) {
     // A P of the level-1 aggregates first: level 1's Galerkin sums its slots.
     // elem64: from the FP64 element values (no face matrix filled)
     if (A.ap_slots > 0) {
@@ -3057,6 +3057,7 @@ hipError_t launch_amg_setup(hipStream_t st, const DevTopo &T, const DevAmg &A, c
                 (int)A.ap_slots, sp, sl, el ? elem64 : val64, A.ap64, mixed ? A.ap32 : nullptr);
         });
     }
+    if (ap_only) return hipGetLastError();   // the coarse levels are kept
     for (int l = 1; l < A.nlev; ++l) {
         const DevAmgLevel &L = A.lev[l];
         const DevAmgLevel &F = A.lev[l - 1];

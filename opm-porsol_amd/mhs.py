@@ -65,7 +65,8 @@ class Report(ctypes.Structure):
                 ("element_operator", ctypes.c_int), ("colours", ctypes.c_int),
                 ("element_values", ctypes.c_longlong), ("level_rows", ctypes.c_longlong * 8),
                 ("level_nnz", ctypes.c_longlong * 8), ("transfer_nnz", ctypes.c_longlong * 2),
-                ("smoothed_level", ctypes.c_int), ("face_matrix", ctypes.c_int)]
+                ("smoothed_level", ctypes.c_int), ("face_matrix", ctypes.c_int),
+                ("mg_setup_reused", ctypes.c_int)]
 
     def as_dict(self):
         out = {}

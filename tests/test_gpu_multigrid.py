@@ -247,3 +247,36 @@ def test_new_permeability_reestimates_smoother_weights(pkg, orc):
     assert cases.rel_err(s.getSolution()[2], f.getSolution()[2]) < 1e-8
     ref = orc.solve(g, p2, tm, rm, np.zeros(nc), bt, bv, tol=1e-14)
     assert cases.rel_err(s.getSolution()[2], ref.outflux) < 1e-8
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_same_matrix_keeps_the_multigrid_setup(pkg, orc, precision):
+    """same_matrix (IncompFlowSolverHybrid.hpp:1491): the second solve keeps
+    the coarse levels of the first, as the reference keeps its AMG, and
+    solves its own system (new sources and boundary values) to the oracle's
+    solution.  A caller whose matrix did change (new mobilities) still gets
+    the new system's solution: the PCG runs on the new assembly and only the
+    preconditioner is the old one."""
+    g, perm, tm, rm, src, bt, bv = cases.case_c1()
+    s = pkg.HybridSolver(0)
+    s.init(g, perm, (0.0, 0.0, 0.0), bt, None)
+    kw = dict(residual_tolerance=1e-12, precision=precision, precond="amg")
+    first = s.solve(tm, rm, src, bt, bv, **kw)
+    assert first.converged and first.mg_setup_reused == 0
+    rng = np.random.default_rng(3)
+    src2 = np.zeros(g.num_cells)
+    src2[rng.integers(0, g.num_cells, 5)] = 1e-6
+    src2[rng.integers(0, g.num_cells, 5)] -= 1e-6
+    bv2 = 0.5 * bv
+    for tm_k, src_k, bv_k in ((tm, src2, bv2), (3.0 * tm, src2, bv2)):
+        rep = s.solve(tm_k, rm, src_k, bt, bv_k, same_matrix=True, **kw)
+        assert rep.converged and rep.mg_setup_reused == 1 and rep.reduction <= 1e-12
+        ref = orc.solve(g, perm, tm_k, rm, src_k, bt, bv_k, tol=1e-14)
+        p, pi, v = s.getSolution()
+        assert cases.rel_err(pi, ref.face_pressure) < 1e-8
+        assert cases.rel_err(p, ref.cell_pressure) < 1e-8
+        assert cases.rel_err(v, ref.outflux) < 1e-8
+    # without the promise the set-up is done again
+    rep = s.solve(tm, rm, src, bt, bv, **kw)
+    assert rep.converged and rep.mg_setup_reused == 0
+    assert abs(rep.iterations - first.iterations) <= 1
