@@ -1901,13 +1901,40 @@ __global__ __launch_bounds__(256) void k_restrict_sa(DevAmgLevel F, int nc, VT *
     for (int k = 0; k < NRHS; ++k) bc[(size_t)I * NRHS + k] = acc[k];
 }
 
-template <typename VT, int NRHS, int G = 1>
+// PRO: the transfer from this level to the finer one is tentative and covers
+// it, so the launch also prolongs its result into the finer level's iterate
+// in place, xf[c] += alf xo[i] for the children c of row i (the finer level's
+// up-sweep then gathers one value per entry).
+template <typename VT, int NRHS, int G, bool PRO>
 __global__ __launch_bounds__(256) void k_vcc_up_sa(DevAmgLevel F, const VT *__restrict__ xc, VT w,
-                                                   VT al) {
+                                                   VT al, VT *__restrict__ xf, VT alf) {
     const int t = blockIdx.x * 256 + threadIdx.x;
-    const int i = t / G;
+    const int i = t / G, sub = t % G;
     if (i >= F.n) return;
-    up_sa_row<VT, NRHS, G>(F, xc, w, al, i, t % G);
+    if constexpr (!PRO) {
+        up_sa_row<VT, NRHS, G>(F, xc, w, al, i, sub);
+    } else {
+        const VT *b = (const VT *)F.b;
+        const VT *res = (const VT *)F.res;
+        VT mx[NRHS];
+        row_gather<VT, NRHS, G>(F.t_col, lvl_m<VT>(F), F.t_ptr[i], F.t_ptr[i + 1], xc, mx, sub);
+        const VT d = w * lvl_dinv<VT>(F)[i];
+        VT o[NRHS];
+#pragma unroll
+        for (int k = 0; k < NRHS; ++k) {   // every lane of the group holds the row sum
+            const size_t q = (size_t)i * NRHS + k;
+            o[k] = d * (b[q] + res[q]) + al * mx[k];
+        }
+        if (sub == 0) {
+#pragma unroll
+            for (int k = 0; k < NRHS; ++k) ((VT *)F.x)[(size_t)i * NRHS + k] = o[k];
+        }
+        for (int c = F.child_ptr[i] + sub; c < F.child_ptr[i + 1]; c += G) {
+            const int f = F.child[c];
+#pragma unroll
+            for (int k = 0; k < NRHS; ++k) xf[(size_t)f * NRHS + k] += alf * o[k];
+        }
+    }
 }
 
 // Restriction b_c = P' res: sum over the coarse row's children.
@@ -3266,13 +3293,15 @@ static void vcycle_impl(hipStream_t st, const Gate &gt, const DevTopo &T, const 
                 });
             }
         }
-        // Tentative transfers l-1 <- l, from the tail down, whose coarse result
-        // the producing kernel prolongs in place into level l-1's iterate; the
-        // up-sweep of level l-1 then gathers one value per entry.
+        // Tentative transfers l-1 <- l whose coarse result the producing
+        // kernel prolongs in place into level l-1's iterate; the up-sweep of
+        // level l-1 then gathers one value per entry.  The producing kernel
+        // can: the tail, the smoothed-transfer up-sweep, and the up-sweep of a
+        // level that was itself prolonged in place.
         bool fuse[kMaxLevels] = {};
         for (int l = lt; l >= 2; --l) {
-            if (A.lev[l - 1].sa_out || !A.covers[l] || (l < lt && !fuse[l + 1])) break;
-            fuse[l] = true;
+            const bool can = l == lt || A.lev[l].sa_out || fuse[l + 1];
+            fuse[l] = can && !A.lev[l - 1].sa_out && A.covers[l];
         }
         k_vcycle_tail<VT, NR><<<A.ninst, 1024, 0, st>>>(gt, A.dev, lt, al, als,
                                                         fuse[lt] ? (VT *)A.lev[lt - 1].x : nullptr);
@@ -3282,9 +3311,15 @@ static void vcycle_impl(hipStream_t st, const Gate &gt, const DevTopo &T, const 
             const DevAmgLevel &F = A.lev[l];
             const VT *xc = xres[l + 1];
             if (F.sa_out) {
+                VT *xf = fuse[l] ? (VT *)A.lev[l - 1].x : nullptr;
                 with_lanes(lanes_per_row(F.t_nnz, F.n), [&](auto g) {
                     constexpr int Gv = decltype(g)::value;
-                    k_vcc_up_sa<VT, NR, Gv><<<row_blocks(F.n, Gv), 256, 0, st>>>(F, xc, wl(l), als);
+                    if (xf)
+                        k_vcc_up_sa<VT, NR, Gv, true><<<row_blocks(F.n, Gv), 256, 0, st>>>(F, xc, wl(l), als, xf,
+                                                                                          al);
+                    else
+                        k_vcc_up_sa<VT, NR, Gv, false><<<row_blocks(F.n, Gv), 256, 0, st>>>(F, xc, wl(l), als,
+                                                                                           nullptr, VT(0));
                 });
                 xres[l] = (VT *)F.x;
             } else if (fuse[l + 1]) {
