@@ -471,6 +471,86 @@ __global__ __launch_bounds__(64) void k_assemble(DevTopo T, const double *__rest
     }
 }
 
+// The same for grids whose cells have up to 10 or 16 faces of varying count
+// (faulted corner-point grids): held in registers, the packed local matrix of
+// the largest cell takes every VGPR (one wave per SIMD).  Here nothing is held
+// per entry: the row sums go to C.F and are read back, S_c = lam B^-1 is
+// recomputed from B^-1 where needed and the right-hand side reads the Schur
+// entries this thread has just written, with runtime loops over the cell's own
+// face count (few registers, full occupancy).  Same operations in the same
+// order per entry as k_assemble.
+template <int NRHS>
+__global__ __launch_bounds__(64) void k_assemble_stream(DevTopo T, const double *__restrict__ binv,
+                                                        const double *__restrict__ gfs,
+                                                        const double *__restrict__ totmob,
+                                                        const double *__restrict__ rhomob,
+                                                        const double *__restrict__ src,
+                                                        const uint8_t *__restrict__ ftype,
+                                                        const double *__restrict__ bcval,
+                                                        DevCellSys C, DevEbe E) {
+    const int c = blockIdx.x * 64 + threadIdx.x;
+    if (c >= T.nc) return;
+    const int f0 = T.facepos[c], n = T.facepos[c + 1] - f0;
+    const double lam = totmob[c], rl = rhomob[c];
+    const size_t nc = (size_t)T.nc;
+    auto Bs = [&](int i, int j) {   // S_c(i, j) = lam B^-1(i, j), either triangle
+        return lam * binv[(size_t)(i <= j ? pidx(i, j, n) : pidx(j, i, n)) * nc + c];
+    };
+    double L = 0.0, gsum = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double sum = 0.0;
+        for (int j = 0; j < n; ++j) sum += Bs(i, j);
+        C.F[f0 + i] = sum;
+        L += sum;
+        gsum += gfs[f0 + i] * rl;
+    }
+    const double rL = 1.0 / L;
+    C.L[c] = L;
+    const int sg = E.val64 ? E.cell_seg[c] : 0, local = E.val64 ? E.cell_local[c] : 0, np = n * (n + 1) / 2;
+    const size_t vb = E.val64 ? (size_t)E.seg_val[sg] : 0;
+    for (int i = 0; i < n; ++i) {
+        const double Fi = C.F[f0 + i];
+        const bool di = ftype[f0 + i] == FT_DIRICHLET;
+        for (int j = i; j < n; ++j) {
+            const double v = -rL * Fi * C.F[f0 + j] + Bs(i, j);
+            C.schur[(size_t)pidx(i, j, n) * nc + c] = v;
+            if (E.val64) {
+                // S'_c: a Dirichlet row keeps its diagonal only, a Dirichlet column is dropped
+                const bool drop = i != j && (di || ftype[f0 + j] == FT_DIRICHLET);
+                const double w = drop ? 0.0 : v;
+                const size_t o = vb + (size_t)ebe_slot(local, pidx(i, j, n), np);
+                E.val64[o] = w;
+                E.val32[o] = (float)w;
+            }
+        }
+    }
+    auto Sc = [&](int i, int j) { return C.schur[(size_t)(i <= j ? pidx(i, j, n) : pidx(j, i, n)) * nc + c]; };
+#pragma unroll
+    for (int k = 0; k < NRHS; ++k) {
+        const double gk = src[(size_t)k * nc + c] - gsum;
+        C.g[(size_t)k * nc + c] = gk;
+        const double *bv = bcval + (size_t)k * T.nhf + f0;
+        for (int i = 0; i < n; ++i) {
+            const uint8_t fi = ftype[f0 + i];
+            const double vi = fi != FT_INTERNAL ? bv[i] : 0.0;
+            double rr;
+            if (fi == FT_DIRICHLET) {
+                rr = Sc(i, i) * vi;
+            } else {
+                double lr = gfs[f0 + i] * rl - (fi == FT_NEUMANN ? vi : 0.0);
+                lr = 1.0 * lr + (gk * rL) * C.F[f0 + i];
+                rr = lr;
+                for (int j = 0; j < n; ++j) {
+                    const uint8_t fj = ftype[f0 + j];
+                    if (fj == FT_DIRICHLET || T.hf_redirect[f0 + j])
+                        rr -= Sc(i, j) * (fj != FT_INTERNAL ? bv[j] : 0.0);
+                }
+            }
+            C.rr[((size_t)k * T.max_ncf + i) * nc + c] = rr;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Face-row gather: fills SELL values, right-hand sides, the Jacobi diagonal,
 // the initial iterate and its residual r0 = b - A x0.  x0 is nonzero only on
@@ -2917,6 +2997,9 @@ hipError_t launch_assemble(hipStream_t st, const DevTopo &T, int maxn, int nrhs,
             if (T.uniform_ncf && T.max_ncf == MN)
                 k_assemble<MN, NR, true><<<cell_blocks(T.nc), 64, 0, st>>>(T, binv_pk, gflux, totmob, rhomob,
                                                                            src, ftype, bcval, C, Ev);
+            else if (MN >= 10)   // varying counts up to 10 or 16 faces: no per-entry registers
+                k_assemble_stream<NR><<<cell_blocks(T.nc), 64, 0, st>>>(T, binv_pk, gflux, totmob, rhomob, src,
+                                                                        ftype, bcval, C, Ev);
             else
                 k_assemble<MN, NR, false><<<cell_blocks(T.nc), 64, 0, st>>>(T, binv_pk, gflux, totmob, rhomob,
                                                                             src, ftype, bcval, C, Ev);
