@@ -603,7 +603,8 @@ def bench_faulted(args, pkg, torch, dev):
 
     def step(precision):
         return s.solve_device(tm.data_ptr(), rm.data_ptr(), rm.data_ptr(), bt, bvd.data_ptr(), nrhs=1,
-                              tol=TOL, precision=precision, chunk=args.chunk, use_graph=not args.no_graph)
+                              tol=TOL, precision=precision, chunk=args.chunk, use_graph=not args.no_graph,
+                              **_forced_shape(args))
 
     cmp = None
     if not args.no_compare:
